@@ -1,0 +1,100 @@
+/* C ABI of the MI355X screened Poisson reconstruction library
+ * (libwarp_psr_amd.so).  Plain pointers and sizes only; every entry point
+ * returns 0 on success, 2 for invalid input (the reference raises ValueError,
+ * message text identical), 3 for a type mismatch (TypeError), 1 otherwise;
+ * psr_last_error() returns the calling thread's last message.
+ *
+ * Device pointers are HIP device pointers on the current device; streams are
+ * hipStream_t passed as void* (NULL = default stream).  A PsrResult is used
+ * from one stream at a time; independent results may run on distinct streams.
+ *
+ * Reference interfaces replaced (alecjacobson/warp-psr, Python over Warp):
+ *   psr_params_default  -> keyword defaults of reconstruct()      src/warp_psr/reconstruction.py:159-187
+ *   psr_reconstruct     -> reconstruct(positions, normals, ...)   src/warp_psr/reconstruction.py:159-371
+ *   psr_evaluate        -> Reconstruction.evaluate / LevelField.evaluate
+ *                                                                 src/warp_psr/reconstruction.py:151-152, field.py:150-184
+ *   psr_result_iso      -> Reconstruction.iso_value               src/warp_psr/reconstruction.py:147-149
+ *   psr_result_field    -> Reconstruction.device_field()          src/warp_psr/reconstruction.py:154-156, device_field.py:472-499
+ *   psr_result_metadata_json -> Reconstruction.metadata          src/warp_psr/reconstruction.py:325-366
+ *   psr_result_array    -> Reconstruction.diagnostics (stage arrays)
+ *                                                                 src/warp_psr/reconstruction.py:367-371
+ *   psr_relax_sliced    -> relax_sliced(A, b, x, schedule, depth, iters)
+ *                                                                 src/warp_psr/relaxation.py:189-210
+ *   psr_result_free     -> (Python garbage collection of Reconstruction)
+ */
+#ifndef WARP_PSR_AMD_H
+#define WARP_PSR_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct PsrParams {
+  int depth;               /* default 6 */
+  int full_depth;          /* default 3 */
+  int base_depth;          /* -1 = full_depth */
+  int solve_depth;         /* -1 = depth */
+  int kernel_depth;        /* -1 = max(depth - 2, 0) */
+  double samples_per_node; /* default 1.5 */
+  double scale;            /* default 1.1 */
+  double point_weight;     /* default 2.0; 0 selects PSR */
+  int exact_interpolation; /* default 0 */
+  int iters;               /* default 8 */
+  int base_v_cycles;       /* default 1 */
+  double cg_accuracy;      /* default 1e-3 */
+  int keep_intermediates;  /* keep per-stage arrays for psr_result_array (default 1) */
+} PsrParams;
+
+typedef struct PsrResult PsrResult;
+
+/* Borrowed device view of a reconstructed field (zero-copy; valid while the
+ * result lives and is not mutated).  Mirrors the reference DeviceField struct. */
+typedef struct PsrField {
+  const int64_t* keys;        /* sorted basis keys d<<48|x<<32|y<<16|z, all levels */
+  const void* coefficients;   /* per-key level corrections (float or double) */
+  const void* cumulative;     /* per-key cumulative coefficients */
+  const int32_t* level_offsets; /* device, depth+2 entries: keys of level d in [off[d], off[d+1]) */
+  int64_t count;
+  int depth;
+  int dtype;                  /* 0 float32, 1 float64 */
+  double linear[9];           /* model -> cube, row major */
+  double translation[3];
+  double iso_value;
+} PsrField;
+
+void psr_params_default(PsrParams* p);
+
+/* points, normals: device arrays of n x 3 values (dtype 0 = float32, 1 = float64). */
+int psr_reconstruct(const void* points, const void* normals, int64_t n, int dtype, const PsrParams* params,
+                    void* stream, PsrResult** out);
+
+/* queries: device m x 3 in model space; values m, grads m x 3 (may be NULL), valid m int32. */
+int psr_evaluate(const PsrResult* result, const void* queries, int64_t m, void* values, void* grads, int32_t* valid,
+                 void* stream);
+
+int psr_result_iso(const PsrResult* result, double* iso);
+int psr_result_field(const PsrResult* result, PsrField* field);
+/* Writes NUL-terminated JSON into buf (size bytes); *needed receives the full length + 1. */
+int psr_result_metadata_json(const PsrResult* result, char* buf, size_t size, size_t* needed);
+/* Host copy (owned by the result) of a named stage array; see DESIGN for names. */
+int psr_result_array(PsrResult* result, const char* name, int level, const void** data, int64_t* nbytes);
+void psr_result_free(PsrResult* result);
+
+/* Sliced Gauss-Seidel (reference order) on a caller CSR operator, all device
+ * pointers: keys[rows] sorted basis keys of one level, CSR offsets[rows+1],
+ * columns/values (<= 27 entries per row), b[rows], x[rows] updated in place. */
+int psr_relax_sliced(int dtype, const int64_t* keys, int64_t rows, const int32_t* offsets, const int32_t* columns,
+                     const void* values, const void* b, void* x, int depth, int iters, void* stream);
+
+const char* psr_last_error(void);
+/* Frees the library's cached device blocks. */
+void psr_release_cache(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* WARP_PSR_AMD_H */

@@ -40,6 +40,7 @@ namespace {
 
 constexpr int64_t KMAX = std::numeric_limits<int64_t>::max();
 
+template <class T> inline T wmax_(T a, T b) { return a > b ? a : b; }
 struct I3 {
   int v[3];
   int operator[](int a) const { return v[a]; }
@@ -237,7 +238,7 @@ template <class T> struct State {
 // ---------------------------------------------------------------- samples
 // src/warp_psr/samples.py:54-88 (_transform); computed once on the host.
 template <class T> int compute_transform(const T* b, double scale, T* t) {
-  T width = std::max(b[3] - b[0], std::max(b[4] - b[1], b[5] - b[2])) * T(scale);
+  T width = wmax_(b[3] - b[0], wmax_(b[4] - b[1], b[5] - b[2])) * T(scale);
   if (!std::isfinite(width) || width <= T(0)) return 1;
   T inv = T(1.0 / double(width));
   t[0] = inv;
@@ -348,7 +349,83 @@ template <class T> T density_scale() {
   return total / T(1000);
 }
 
-template <class T> inline T real_log(T v) { return T(std::log(double(v))); }
+// Deterministic natural logarithm shared with the product: double-double
+// k*ln2 + 2*atanh((m-1)/(m+1)), rounded once.  The reference calls CUDA's
+// double log (density.py:31-35); both CUDA's and glibc's can differ from the
+// correctly rounded value in the last bit, so host and device use this one.
+struct DD { double hi, lo; };
+inline DD dd_two_sum(double a, double b) {
+  double s = a + b;
+  double bb = s - a;
+  double e = (a - (s - bb)) + (b - bb);
+  return {s, e};
+}
+inline DD dd_quick(double a, double b) {
+  double s = a + b;
+  return {s, b - (s - a)};
+}
+inline DD dd_add(DD a, DD b) {
+  DD s = dd_two_sum(a.hi, b.hi);
+  DD t = dd_two_sum(a.lo, b.lo);
+  s.lo += t.hi;
+  s = dd_quick(s.hi, s.lo);
+  s.lo += t.lo;
+  return dd_quick(s.hi, s.lo);
+}
+inline DD dd_mul(DD a, DD b) {
+  double p = a.hi * b.hi;
+  double e = std::fma(a.hi, b.hi, -p);
+  e += a.hi * b.lo + a.lo * b.hi;
+  return dd_quick(p, e);
+}
+inline DD dd_div(DD a, DD b) {
+  double q1 = a.hi / b.hi;
+  DD r = dd_add(a, dd_mul(b, DD{-q1, 0.0}));
+  double q2 = r.hi / b.hi;
+  r = dd_add(r, dd_mul(b, DD{-q2, 0.0}));
+  double q3 = r.hi / b.hi;
+  DD q = dd_quick(q1, q2);
+  return dd_add(q, DD{q3, 0.0});
+}
+double det_log(double x) {
+  if (!(x > 0.0)) return x == 0.0 ? -std::numeric_limits<double>::infinity() : std::numeric_limits<double>::quiet_NaN();
+  if (std::isinf(x)) return x;
+  uint64_t bits;
+  std::memcpy(&bits, &x, 8);
+  int k = 0;
+  if ((bits >> 52) == 0) {
+    x *= 18014398509481984.0;
+    std::memcpy(&bits, &x, 8);
+    k = -54;
+  }
+  k += (int)((bits >> 52) & 0x7ff) - 1023;
+  uint64_t mbits = (bits & 0x000fffffffffffffULL) | 0x3ff0000000000000ULL;
+  double m;
+  std::memcpy(&m, &mbits, 8);
+  if (m > 1.4142135623730951) { m *= 0.5; k += 1; }
+  DD num = dd_two_sum(m, -1.0);
+  DD den = dd_two_sum(m, 1.0);
+  DD s = dd_div(num, den);
+  DD s2 = dd_mul(s, s);
+  DD acc = {1.0 / 49.0, 0.0};
+  for (int j = 23; j >= 0; --j) {
+    DD inv = dd_div(DD{1.0, 0.0}, DD{double(2 * j + 1), 0.0});
+    acc = dd_add(dd_mul(acc, s2), inv);
+  }
+  DD at = dd_mul(acc, s);
+  at = dd_add(at, at);
+  const DD ln2 = {6.93147180559945286227e-01, 2.31904681384629955842e-17};
+  DD kl = dd_mul(ln2, DD{double(k), 0.0});
+  DD r = dd_add(kl, at);
+  return r.hi + r.lo;
+}
+constexpr double LN2_D = 6.93147180559945286227e-01;
+
+// Warp's scalar min/max/clamp (a<b?a:b forms): same ties and NaN behaviour.
+template <class T> inline T wmax(T a, T b) { return a > b ? a : b; }
+template <class T> inline T wmin(T a, T b) { return a < b ? a : b; }
+
+template <class T> inline T real_log(T v) { return T(det_log(double(v))); }
 
 // density.py:108-136 (density_value)
 template <class T> T density_value(const Records<T>& lvl, V3<T> p, int depth) {
@@ -433,7 +510,7 @@ template <class T> void build_density(State<T>& s) {
   // density.py:152-182 (_depths)
   const T target = T(s.prm.samples_per_node);
   s.dw.assign(2 * G, T(0));
-  const double two_ln2 = std::log(2.0) * 2.0;
+  const double two_ln2 = LN2_D * 2.0;
   for (size_t i = 0; i < G; ++i) {
     int d = kd;
     T density = s.densities[i * (kd + 1) + d];
@@ -456,7 +533,7 @@ template <class T> void build_density(State<T>& s) {
 }
 
 // ---------------------------------------------------------------- normals
-template <class T> inline T clampT(T x, T a, T b) { return std::min(std::max(a, x), b); }
+template <class T> inline T clampT(T x, T a, T b) { return wmin(wmax(a, x), b); }
 
 template <class T> void build_normal_field(State<T>& s) {
   const int minimum = s.prm.base_depth, maximum = s.prm.depth;
@@ -737,7 +814,7 @@ template <class T> inline T hat(const V3<T>& p, I3 c, int depth) {
   T h[3];
   for (int a = 0; a < 3; ++a) {
     T t = p[a] * T(1 << depth) - T(c[a]);
-    h[a] = std::max(T(1) - std::fabs(t), T(0));
+    h[a] = wmax(T(1) - std::fabs(t), T(0));
   }
   return h[0] * h[1] * h[2];
 }

@@ -1,0 +1,371 @@
+// C ABI (include/warp_psr_amd.h): parameter checks with the reference's
+// messages (src/warp_psr/reconstruction.py:181-234), stage orchestration with
+// HIP-event stage timings (reconstruction.py:241-250), result ownership and
+// the named stage-array export used by parity tests.
+#include "../../include/warp_psr_amd.h"
+#include "psr_result.h"
+
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <sstream>
+
+using namespace psr;
+
+struct PsrResult {
+  int dtype = 0;
+  void* impl = nullptr;
+  std::map<std::string, std::vector<char>> cache;
+  DArray<int> level_offsets_dev;
+  std::string metadata;
+};
+
+namespace {
+
+thread_local std::string g_error;
+
+int fail(const PsrError& e) {
+  g_error = e.what();
+  return e.code;
+}
+
+void check_params(const PsrParams& p) {
+  const int depth = p.depth, full = p.full_depth, base = p.base_depth, solve = p.solve_depth, kd = p.kernel_depth;
+  if (!(0 <= base && base <= full && full <= solve && solve <= depth && depth <= 15) || depth < 1 || !(0 <= kd && kd <= depth))
+    throw PsrError(2, "Require 0 <= base_depth <= full_depth <= solve_depth <= depth <= 15, depth >= 1, and 0 <= kernel_depth <= depth");
+  if (depth - full > 7)
+    throw PsrError(2, "Warp 1.19 AdaptiveNanogrid supports at most 8 integration levels; require depth - full_depth <= 7");
+  if (p.iters < 1 || p.base_v_cycles < 1) throw PsrError(2, "iters and base_v_cycles must be positive integers");
+  if (!std::isfinite(p.point_weight) || p.point_weight < 0 || !std::isfinite(p.cg_accuracy) || p.cg_accuracy <= 0)
+    throw PsrError(2, "Require finite nonnegative point_weight and positive cg_accuracy");
+  if (!std::isfinite(p.samples_per_node) || p.samples_per_node <= 0 || !std::isfinite(p.scale) || p.scale <= 1)
+    throw PsrError(2, "Require finite positive samples_per_node and scale > 1");
+}
+
+Params resolve(const PsrParams& in) {
+  PsrParams p = in;
+  if (p.base_depth < 0) p.base_depth = p.full_depth;
+  if (p.solve_depth < 0) p.solve_depth = p.depth;
+  if (p.kernel_depth < 0) p.kernel_depth = std::max(p.depth - 2, 0);
+  check_params(p);
+  Params q;
+  q.depth = p.depth; q.full_depth = p.full_depth; q.base_depth = p.base_depth; q.solve_depth = p.solve_depth;
+  q.kernel_depth = p.kernel_depth; q.samples_per_node = p.samples_per_node; q.scale = p.scale;
+  q.point_weight = p.point_weight; q.exact_interpolation = p.exact_interpolation ? 1 : 0; q.iters = p.iters;
+  q.base_v_cycles = p.base_v_cycles; q.cg_accuracy = p.cg_accuracy; q.keep_intermediates = p.keep_intermediates;
+  return q;
+}
+
+template <class T> Result<T>* run(const T* pts, const T* nrm, int64_t n, const Params& prm, hipStream_t s) {
+  auto r = std::make_unique<Result<T>>();
+  r->prm = prm;
+  r->s = s;
+  r->n = n;
+  hipEvent_t ev[7];
+  for (auto& e : ev) HIP_CHECK(hipEventCreate(&e));
+  const char* names[6] = {"samples", "density", "normal_field", "tree", "assembly_and_solve", "field_and_isovalue"};
+  try {
+    HIP_CHECK(hipEventRecord(ev[0], s));
+    stage_samples(*r, pts, nrm);
+    HIP_CHECK(hipEventRecord(ev[1], s));
+    stage_density(*r);
+    HIP_CHECK(hipEventRecord(ev[2], s));
+    stage_normals(*r);
+    HIP_CHECK(hipEventRecord(ev[3], s));
+    stage_tree(*r);
+    HIP_CHECK(hipEventRecord(ev[4], s));
+    stage_solve(*r);
+    HIP_CHECK(hipEventRecord(ev[5], s));
+    stage_field(*r);
+    HIP_CHECK(hipEventRecord(ev[6], s));
+    HIP_CHECK(hipEventSynchronize(ev[6]));
+    for (int k = 0; k < 6; ++k) {
+      float ms = 0.f;
+      HIP_CHECK(hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
+      r->timings.push_back({names[k], ms / 1000.f});
+    }
+  } catch (...) {
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    throw;
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  return r.release();
+}
+
+template <class X> void put(std::vector<char>& buf, const X* p, size_t n) {
+  buf.resize(n * sizeof(X));
+  if (n) std::memcpy(buf.data(), p, n * sizeof(X));
+}
+template <class X> void put_dev(std::vector<char>& buf, const DArray<X>& a) {
+  std::vector<X> h = a.to_host();
+  put(buf, h.data(), h.size());
+}
+
+// stencil -> compact CSR (present slots in slot order)
+template <class V>
+void stencil_csr(const Stencil<V>& m, int slots, const char* part, std::vector<char>& out) {
+  std::vector<int> cols = m.cols.to_host();
+  std::vector<V> vals = m.vals.to_host();
+  const int R = m.rows;
+  std::vector<int> offs(R + 1, 0), cc;
+  std::vector<V> vv;
+  for (int i = 0; i < R; ++i) {
+    for (int k = 0; k < slots; ++k) {
+      int c = cols[(size_t)k * R + i];
+      if (c >= 0) { cc.push_back(c); vv.push_back(vals[(size_t)k * R + i]); }
+    }
+    offs[i + 1] = (int)cc.size();
+  }
+  if (!std::strcmp(part, "offsets")) put(out, offs.data(), offs.size());
+  else if (!std::strcmp(part, "columns")) put(out, cc.data(), cc.size());
+  else put(out, vv.data(), vv.size());
+}
+
+template <class T> bool export_array(Result<T>& r, const std::string& name, int level, std::vector<char>& out) {
+  const bool lvl = level >= 0 && level < (int)r.levels.size();
+  auto L = [&]() -> LevelState<T>& { return r.levels[level]; };
+  if (name == "transform") { put(out, r.transform, 5); return true; }
+  if (name == "bounds") { put(out, r.bounds, 6); return true; }
+  if (name == "positions") { put_dev(out, r.q); return true; }
+  if (name == "unit_normals") { put_dev(out, r.unit_normals); return true; }
+  if (name == "input_status") { put_dev(out, r.input_status); return true; }
+  if (name == "input_group_indices") { put_dev(out, r.input_group); return true; }
+  if (name == "input_order") { put_dev(out, r.input_order); return true; }
+  if (name == "sample_keys") { put_dev(out, r.sample_keys); return true; }
+  if (name == "position_sums") { put_dev(out, r.position_sums); return true; }
+  if (name == "normal_sums") { put_dev(out, r.normal_sums); return true; }
+  if (name == "weights") { put_dev(out, r.weights); return true; }
+  if (name == "first_indices") { put_dev(out, r.first_indices); return true; }
+  if (name == "group_offsets") { put_dev(out, r.group_offsets); return true; }
+  if (name == "aggregate_keys" && lvl) { put_dev(out, L().agg_keys); return true; }
+  if (name == "aggregate_values" && lvl) { put_dev(out, L().agg_vals); return true; }
+  if (name == "density_keys" && level >= 0 && level <= r.prm.kernel_depth) { put_dev(out, L().dens_keys); return true; }
+  if (name == "density_values" && level >= 0 && level <= r.prm.kernel_depth) { put_dev(out, L().dens_vals); return true; }
+  if (name == "densities") { put_dev(out, r.densities); return true; }
+  if (name == "depths_and_weights") { put_dev(out, r.dw); return true; }
+  if (name == "density_scale") { put(out, &r.density_scale, 1); return true; }
+  if (name == "normal_keys") { put_dev(out, r.normal_keys); return true; }
+  if (name == "normal_values") { put_dev(out, r.normal_vals); return true; }
+  if (name == "normal_average") { put(out, r.normal_average_host, 3); return true; }
+  if (name == "normal_offsets") { put(out, r.normal_offsets.data(), r.normal_offsets.size()); return true; }
+  if (name == "tree_keys" && lvl) { put_dev(out, L().tree_keys); return true; }
+  if (name == "basis_keys" && lvl) { put_dev(out, L().basis_keys); return true; }
+  if (name == "normal_load" && lvl) { put_dev(out, L().normal_load); return true; }
+  if (name == "point_load" && lvl) { put_dev(out, L().point_load); return true; }
+  if (name == "rhs" && lvl) { put_dev(out, L().rhs); return true; }
+  if (name == "corrections" && lvl) { put_dev(out, L().correction); return true; }
+  if (lvl && name.rfind("matrix_", 0) == 0) { stencil_csr(L().A, 27, name.c_str() + 7, out); return true; }
+  if (lvl && name.rfind("cross_", 0) == 0) { stencil_csr(L().cross, 27, name.c_str() + 6, out); return true; }
+  if (lvl && name.rfind("prolong_", 0) == 0) { stencil_csr(L().P, 8, name.c_str() + 8, out); return true; }
+  if (name == "keys") { put_dev(out, r.keys); return true; }
+  if (name == "coefficients") { put_dev(out, r.coefficients); return true; }
+  if (name == "cumulative") { put_dev(out, r.cumulative); return true; }
+  if (name == "centroid_values") { put_dev(out, r.centroid_values); return true; }
+  if (name == "iso") { put(out, &r.iso, 1); return true; }
+  if (name == "cube_to_model") { put(out, r.cube_to_model, 16); return true; }
+  if (name == "model_to_cube") { put(out, r.model_to_cube, 16); return true; }
+  if (name == "cg_runs") { put(out, r.cg_runs.data(), r.cg_runs.size()); return true; }
+  return false;
+}
+
+template <class T> std::string metadata_json(const Result<T>& r) {
+  std::ostringstream o;
+  o.precision(17);
+  o << "{\"precision\": \"" << (sizeof(T) == 4 ? "float32" : "float64") << "\", \"basis_counts\": [";
+  for (size_t d = 0; d < r.levels.size(); ++d) o << (d ? ", " : "") << r.levels[d].basis_count;
+  o << "], \"aggregates\": " << r.G << ", \"normal_records\": " << r.normal_keys.size() << ", \"timings_seconds\": {";
+  for (size_t k = 0; k < r.timings.size(); ++k) o << (k ? ", " : "") << "\"" << r.timings[k].first << "\": " << r.timings[k].second;
+  o << "}, \"stage_timing_scope\": \"HIP event elapsed time on the caller's stream, including host count reads\"";
+  o << ", \"cube_to_model\": [";
+  for (int i = 0; i < 4; ++i) {
+    o << (i ? ", " : "") << "[";
+    for (int j = 0; j < 4; ++j) o << (j ? ", " : "") << double(r.cube_to_model[4 * i + j]);
+    o << "]";
+  }
+  const Params& p = r.prm;
+  o << "], \"parameters\": {\"depth\": " << p.depth << ", \"full_depth\": " << p.full_depth << ", \"base_depth\": " << p.base_depth
+    << ", \"solve_depth\": " << p.solve_depth << ", \"kernel_depth\": " << p.kernel_depth
+    << ", \"samples_per_node\": " << p.samples_per_node << ", \"scale\": " << p.scale << ", \"point_weight\": " << p.point_weight
+    << ", \"exact_interpolation\": " << (p.exact_interpolation ? "true" : "false") << ", \"iters\": " << p.iters
+    << ", \"base_v_cycles\": " << p.base_v_cycles << ", \"cg_accuracy\": " << p.cg_accuracy << "}";
+  o << ", \"root_cg_cycles\": [";
+  for (size_t k = 0; k < r.cg_runs.size(); ++k) o << (k ? ", " : "") << r.cg_runs[k];
+  o << "], \"profile\": \"pinned cascadic degree-1 Neumann\", \"constant_mode\": \""
+    << ((p.base_depth == 0 && p.point_weight == 0) ? "reference root mean term" : "reference solve without an added mean term")
+    << "\", \"exact_interpolation_behavior\": \""
+    << (p.exact_interpolation ? "empty constraints in pinned reference (issue #16)" : "level-clustered approximate constraints")
+    << "\", \"iso_value\": " << double(r.iso) << "}";
+  return o.str();
+}
+
+}  // namespace
+
+extern "C" {
+
+void psr_params_default(PsrParams* p) {
+  p->depth = 6;
+  p->full_depth = 3;
+  p->base_depth = -1;
+  p->solve_depth = -1;
+  p->kernel_depth = -1;
+  p->samples_per_node = 1.5;
+  p->scale = 1.1;
+  p->point_weight = 2.0;
+  p->exact_interpolation = 0;
+  p->iters = 8;
+  p->base_v_cycles = 1;
+  p->cg_accuracy = 1e-3;
+  p->keep_intermediates = 1;
+}
+
+int psr_reconstruct(const void* points, const void* normals, int64_t n, int dtype, const PsrParams* params, void* stream,
+                    PsrResult** out) {
+  try {
+    if (!params || !out) throw PsrError(2, "params and out must be non-null");
+    Params prm = resolve(*params);
+    if (dtype != 0 && dtype != 1)
+      throw PsrError(2, "Reconstruction requires matching float32/float64 3D vectors on the same CUDA device");
+    if (n <= 0) throw PsrError(2, "points and normals must have equal nonempty 1D shapes");
+    hipStream_t s = (hipStream_t)stream;
+    auto res = std::make_unique<PsrResult>();
+    res->dtype = dtype;
+    if (dtype == 0) {
+      Result<float>* r = run<float>((const float*)points, (const float*)normals, n, prm, s);
+      res->impl = r;
+      res->metadata = metadata_json(*r);
+      res->level_offsets_dev.from_host(r->level_offsets.data(), r->level_offsets.size(), s);
+    } else {
+      Result<double>* r = run<double>((const double*)points, (const double*)normals, n, prm, s);
+      res->impl = r;
+      res->metadata = metadata_json(*r);
+      res->level_offsets_dev.from_host(r->level_offsets.data(), r->level_offsets.size(), s);
+    }
+    *out = res.release();
+    return 0;
+  } catch (const PsrError& e) {
+    return fail(e);
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return 1;
+  }
+}
+
+int psr_evaluate(const PsrResult* result, const void* queries, int64_t m, void* values, void* grads, int32_t* valid,
+                 void* stream) {
+  try {
+    if (!result) throw PsrError(2, "null result");
+    hipStream_t s = (hipStream_t)stream;
+    if (result->dtype == 0) {
+      auto* r = (Result<float>*)result->impl;
+      float L[9], t[3];
+      for (int i = 0; i < 3; ++i) {
+        for (int j = 0; j < 3; ++j) L[3 * i + j] = r->model_to_cube[4 * i + j];
+        t[i] = r->model_to_cube[4 * i + 3];
+      }
+      evaluate_field(*r, (const float*)queries, m, L, t, (float*)values, (float*)grads, valid, s);
+    } else {
+      auto* r = (Result<double>*)result->impl;
+      double L[9], t[3];
+      for (int i = 0; i < 3; ++i) {
+        for (int j = 0; j < 3; ++j) L[3 * i + j] = r->model_to_cube[4 * i + j];
+        t[i] = r->model_to_cube[4 * i + 3];
+      }
+      evaluate_field(*r, (const double*)queries, m, L, t, (double*)values, (double*)grads, valid, s);
+    }
+    return 0;
+  } catch (const PsrError& e) {
+    return fail(e);
+  }
+}
+
+int psr_result_iso(const PsrResult* result, double* iso) {
+  if (!result || !iso) return 2;
+  *iso = result->dtype == 0 ? double(((Result<float>*)result->impl)->iso) : ((Result<double>*)result->impl)->iso;
+  return 0;
+}
+
+int psr_result_field(const PsrResult* result, PsrField* f) {
+  if (!result || !f) return 2;
+  auto fill = [&](auto* r) {
+    f->keys = r->keys.data();
+    f->coefficients = r->coefficients.data();
+    f->cumulative = r->cumulative.data();
+    f->level_offsets = result->level_offsets_dev.data();
+    f->count = (int64_t)r->keys.size();
+    f->depth = r->prm.depth;
+    f->dtype = result->dtype;
+    for (int i = 0; i < 3; ++i) {
+      for (int j = 0; j < 3; ++j) f->linear[3 * i + j] = double(r->model_to_cube[4 * i + j]);
+      f->translation[i] = double(r->model_to_cube[4 * i + 3]);
+    }
+    f->iso_value = double(r->iso);
+  };
+  if (result->dtype == 0) fill((Result<float>*)result->impl);
+  else fill((Result<double>*)result->impl);
+  return 0;
+}
+
+int psr_result_metadata_json(const PsrResult* result, char* buf, size_t size, size_t* needed) {
+  if (!result) return 2;
+  const std::string& m = result->metadata;
+  if (needed) *needed = m.size() + 1;
+  if (buf && size) {
+    size_t k = std::min(size - 1, m.size());
+    std::memcpy(buf, m.data(), k);
+    buf[k] = 0;
+  }
+  return 0;
+}
+
+int psr_result_array(PsrResult* result, const char* name, int level, const void** data, int64_t* nbytes) {
+  try {
+    if (!result || !name || !data || !nbytes) throw PsrError(2, "null argument");
+    std::string key = std::string(name) + "#" + std::to_string(level);
+    auto it = result->cache.find(key);
+    if (it == result->cache.end()) {
+      std::vector<char> buf;
+      bool ok = result->dtype == 0 ? export_array(*(Result<float>*)result->impl, name, level, buf)
+                                   : export_array(*(Result<double>*)result->impl, name, level, buf);
+      if (!ok) throw PsrError(2, std::string("unknown stage array ") + name);
+      it = result->cache.emplace(key, std::move(buf)).first;
+    }
+    *data = it->second.data();
+    *nbytes = (int64_t)it->second.size();
+    return 0;
+  } catch (const PsrError& e) {
+    return fail(e);
+  }
+}
+
+void psr_result_free(PsrResult* result) {
+  if (!result) return;
+  if (result->dtype == 0) delete (Result<float>*)result->impl;
+  else delete (Result<double>*)result->impl;
+  delete result;
+}
+
+int psr_relax_sliced(int dtype, const int64_t* keys, int64_t rows, const int32_t* offsets, const int32_t* columns,
+                     const void* values, const void* b, void* x, int depth, int iters, void* stream) {
+  try {
+    if (depth < 0 || depth > 15 || iters < 1) throw PsrError(2, "Require 0 <= depth <= 15 and iters >= 1");
+    if (rows < 0 || rows > INT32_MAX) throw PsrError(2, "row count out of range");
+    hipStream_t s = (hipStream_t)stream;
+    if (dtype == 0)
+      relax_sliced_csr<float>(keys, (int)rows, offsets, columns, (const float*)values, (const float*)b, (float*)x, depth, iters, s);
+    else if (dtype == 1)
+      relax_sliced_csr<double>(keys, (int)rows, offsets, columns, (const double*)values, (const double*)b, (double*)x, depth,
+                               iters, s);
+    else
+      throw PsrError(3, "dtype must be 0 (float32) or 1 (float64)");
+    HIP_CHECK(hipStreamSynchronize(s));
+    return 0;
+  } catch (const PsrError& e) {
+    return fail(e);
+  }
+}
+
+const char* psr_last_error(void) { return g_error.c_str(); }
+
+void psr_release_cache(void) { Pool::instance().release_all(); }
+
+}  // extern "C"

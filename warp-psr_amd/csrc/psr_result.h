@@ -1,0 +1,116 @@
+// Device-resident reconstruction state.  One Result owns every device array
+// of one reconstruction (keys, coefficients, cumulative coefficients and the
+// per-stage intermediates kept for diagnostics and parity checks).
+#pragma once
+#include "psr_runtime.h"
+
+namespace psr {
+
+struct Params {
+  int depth = 6, full_depth = 3, base_depth = 3, solve_depth = 6, kernel_depth = 4;
+  double samples_per_node = 1.5, scale = 1.1, point_weight = 2.0;
+  int exact_interpolation = 0, iters = 8, base_v_cycles = 1;
+  double cg_accuracy = 1e-3;
+  int keep_intermediates = 1;  // keep per-stage arrays for diagnostics
+};
+
+// 27-slot stencil matrix (slot k <-> neighbour offset (k/9-1, k/3%3-1, k%3-1)
+// around the row node or its parent), slot-major: cols[k*rows + i].  Slot
+// order equals the ascending-column order of the reference's sorted CSR rows.
+template <class V> struct Stencil {
+  int rows = 0, cols_count = 0;
+  DArray<int> cols;
+  DArray<V> vals;
+};
+
+// Gauss-Seidel operator in schedule order: position j holds row order[j];
+// values/columns are slot-major over positions; inv_diag = 1/A_ii (0 = skip).
+template <class T> struct RelaxOperator {
+  int rows = 0;
+  DArray<int> order;        // schedule position -> row
+  DArray<int> group_offsets;
+  std::vector<int> host_offsets;
+  int max_group = 0;
+  DArray<int> cols;
+  DArray<T> vals;
+  DArray<T> b;              // right-hand side in schedule order
+  DArray<T> inv_diag;
+  DArray<unsigned char> has_diag;
+};
+
+template <class T> struct LevelState {
+  DArray<int64_t> tree_keys, basis_keys;
+  int basis_count = 0;
+  // aggregates (density.py:228-240): keys and (sum x, sum y, sum z, count)
+  DArray<int64_t> agg_keys;
+  DArray<T> agg_vals;  // 4 per record
+  // density splat level (density.py:245-271)
+  DArray<int64_t> dens_keys;
+  DArray<T> dens_vals;
+  // operators and solve
+  Stencil<T> A, cross;      // cast to run precision
+  Stencil<T> P;             // prolongation rows (8 slots, x-major corner order)
+  DArray<double> normal_load, point_load;
+  DArray<T> rhs, correction;
+  bool solved = false;
+};
+
+template <class T> struct Result {
+  Params prm;
+  hipStream_t s = nullptr;
+  int64_t n = 0;
+  int G = 0;
+  // samples (samples.py)
+  T bounds[6];
+  T transform[5];
+  DArray<T> q, unit_normals;
+  DArray<int> input_status, input_group, input_order;
+  DArray<int64_t> sample_keys;
+  DArray<T> position_sums, normal_sums, weights;
+  DArray<int> first_indices, group_offsets;
+  // density (density.py)
+  T density_scale;
+  DArray<T> densities, dw;
+  // normals (normals.py)
+  DArray<int64_t> normal_keys;
+  DArray<T> normal_vals;  // 3 per record (already negated)
+  std::vector<int> normal_offsets;
+  DArray<T> normal_average;  // 3 values on device
+  T normal_average_host[3];
+  DArray<int> normal_order;
+  // tree + solve per level
+  std::vector<LevelState<T>> levels;
+  std::vector<int> cg_runs;
+  // field
+  DArray<int64_t> keys;
+  DArray<T> coefficients, cumulative, centroid_values;
+  std::vector<int> level_offsets;  // concatenated key offsets per depth
+  T iso;
+  T cube_to_model[16], model_to_cube[16];
+  // timings (seconds) per stage, from HIP events
+  std::vector<std::pair<std::string, float>> timings;
+};
+
+// stage entry points (one translation unit each)
+template <class T> void stage_samples(Result<T>& r, const T* pts, const T* nrm);
+template <class T> void stage_density(Result<T>& r);
+template <class T> void stage_normals(Result<T>& r);
+template <class T> void stage_tree(Result<T>& r);
+template <class T> void stage_solve(Result<T>& r);
+template <class T> void stage_field(Result<T>& r);
+template <class T>
+void evaluate_field(const Result<T>& r, const T* queries, int64_t m, const T* linear, const T* translation, T* values,
+                    T* grads, int* valid, hipStream_t s);
+
+template <class T>
+void relax_sliced_csr(const int64_t* keys, int R, const int* offsets, const int* columns, const T* values, const T* b, T* x,
+                      int depth, int iters, hipStream_t s);
+
+// shared device helpers used by several stages
+template <class T> void sort_records_u64(DArray<uint64_t>& keys, DArray<int>& ids, int64_t n, int bits, hipStream_t s);
+
+// Host-side transform (samples.py:54-88) -- scalar work done once per call.
+template <class T> int compute_transform(const T* b, double scale, T* t);
+template <class T> void reference_inverse(const T* m, T* out);
+
+}  // namespace psr

@@ -1,0 +1,173 @@
+// Device memory and launch plumbing: a stream-ordered caching allocator and a
+// move-only device array.  All work of one reconstruction is issued on one
+// stream, so a block returned to the cache may be handed out again on the same
+// stream without synchronisation (stream order protects it).
+#pragma once
+#include "psr_common.h"
+
+#include <map>
+#include <memory>
+
+namespace psr {
+
+class Pool {
+ public:
+  static Pool& instance() {
+    static Pool pool;
+    return pool;
+  }
+  void* allocate(size_t bytes, hipStream_t stream) {
+    size_t cls = size_class(bytes);
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      auto& lst = free_[{stream, cls}];
+      if (!lst.empty()) {
+        void* p = lst.back();
+        lst.pop_back();
+        return p;
+      }
+    }
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, cls);
+    if (e != hipSuccess) {
+      // release cached blocks of every stream and retry once
+      (void)hipGetLastError();
+      release_all();
+      HIP_CHECK(hipMalloc(&p, cls));
+    }
+    std::lock_guard<std::mutex> g(mu_);
+    sizes_[p] = cls;
+    total_ += cls;
+    return p;
+  }
+  void release(void* p, hipStream_t stream) {
+    if (!p) return;
+    std::lock_guard<std::mutex> g(mu_);
+    free_[{stream, sizes_.at(p)}].push_back(p);
+  }
+  // Frees every cached block (after synchronising the device).
+  void release_all() {
+    std::lock_guard<std::mutex> g(mu_);
+    (void)hipDeviceSynchronize();
+    for (auto& kv : free_)
+      for (void* p : kv.second) {
+        total_ -= sizes_.at(p);
+        sizes_.erase(p);
+        (void)hipFree(p);
+      }
+    free_.clear();
+  }
+  size_t reserved_bytes() {
+    std::lock_guard<std::mutex> g(mu_);
+    return total_;
+  }
+
+ private:
+  static size_t size_class(size_t bytes) {
+    if (bytes < 256) bytes = 256;
+    // powers of two up to 1 MiB, then 1/8-octave steps (bounded waste)
+    size_t c = 256;
+    while (c < bytes && c < (1u << 20)) c <<= 1;
+    if (c >= bytes) return c;
+    size_t step = 1;
+    while ((step << 4) <= bytes) step <<= 1;  // step = highest power of two / 8
+    return (bytes + step - 1) / step * step;
+  }
+  std::mutex mu_;
+  std::map<std::pair<hipStream_t, size_t>, std::vector<void*>> free_;
+  std::unordered_map<void*, size_t> sizes_;
+  size_t total_ = 0;
+};
+
+template <class T> class DArray {
+ public:
+  DArray() = default;
+  DArray(size_t n, hipStream_t s) { reset(n, s); }
+  ~DArray() { free(); }
+  DArray(const DArray&) = delete;
+  DArray& operator=(const DArray&) = delete;
+  DArray(DArray&& o) noexcept : p_(o.p_), n_(o.n_), s_(o.s_) { o.p_ = nullptr; o.n_ = 0; }
+  DArray& operator=(DArray&& o) noexcept {
+    if (this != &o) {
+      free();
+      p_ = o.p_; n_ = o.n_; s_ = o.s_;
+      o.p_ = nullptr; o.n_ = 0;
+    }
+    return *this;
+  }
+  void reset(size_t n, hipStream_t s) {
+    free();
+    s_ = s;
+    n_ = n;
+    p_ = n ? (T*)Pool::instance().allocate(n * sizeof(T), s) : nullptr;
+  }
+  void free() {
+    if (p_) Pool::instance().release(p_, s_);
+    p_ = nullptr;
+    n_ = 0;
+  }
+  T* data() const { return p_; }
+  size_t size() const { return n_; }
+  bool empty() const { return n_ == 0; }
+  hipStream_t stream() const { return s_; }
+  void zero() {
+    if (n_) HIP_CHECK(hipMemsetAsync(p_, 0, n_ * sizeof(T), s_));
+  }
+  std::vector<T> to_host() const {
+    std::vector<T> h(n_);
+    if (n_) {
+      HIP_CHECK(hipMemcpyAsync(h.data(), p_, n_ * sizeof(T), hipMemcpyDeviceToHost, s_));
+      HIP_CHECK(hipStreamSynchronize(s_));
+    }
+    return h;
+  }
+  void from_host(const T* h, size_t n, hipStream_t s) {
+    reset(n, s);
+    if (n) HIP_CHECK(hipMemcpyAsync(p_, h, n * sizeof(T), hipMemcpyHostToDevice, s));
+  }
+
+ private:
+  T* p_ = nullptr;
+  size_t n_ = 0;
+  hipStream_t s_ = nullptr;
+};
+
+// Small pinned host staging buffer for scalar reads that size the next stage.
+class HostScalars {
+ public:
+  explicit HostScalars(size_t bytes = 4096) { HIP_CHECK(hipHostMalloc(&p_, bytes, hipHostMallocDefault)); }
+  ~HostScalars() { if (p_) (void)hipHostFree(p_); }
+  template <class T> T* as() { return (T*)p_; }
+
+ private:
+  void* p_ = nullptr;
+};
+
+inline unsigned blocks_for(size_t n, unsigned threads) { return (unsigned)((n + threads - 1) / threads); }
+
+// ------------------------------------------------------------------ primitives
+// Implemented in psr_primitives.hip.
+// Exclusive scan of n int32 values (out may alias in); returns nothing, total
+// (sum of all inputs) is written to *total_dev when non-null.
+void scan_exclusive_i32(const int* in, int* out, int64_t n, int* total_dev, hipStream_t s);
+// Stable LSD radix sort of (key, value) pairs on the low `bits` key bits.
+// keys/vals are sorted in place (scratch from the pool).
+void sort_pairs_u32(uint32_t* keys, int* vals, int64_t n, int bits, hipStream_t s);
+void sort_pairs_u64(uint64_t* keys, int* vals, int64_t n, int bits, hipStream_t s);
+// heads[i] = 1 where sorted keys start a new run and are not the sentinel.
+void mark_heads_u32(const uint32_t* keys, int64_t n, uint32_t sentinel, int* heads, hipStream_t s);
+void mark_heads_u64(const uint64_t* keys, int64_t n, uint64_t sentinel, int* heads, hipStream_t s);
+// Reads one int from device memory (synchronises the stream).
+int read_int(const int* dev, hipStream_t s);
+void read_bytes(void* host, const void* dev, size_t bytes, hipStream_t s);
+// iota
+void fill_iota(int* out, int64_t n, hipStream_t s);
+// After heads + exclusive scan: compacted positions of heads (head_pos[g] = i).
+void compact_heads(const int* heads, const int* prefix, int64_t n, int* head_pos, hipStream_t s);
+inline int bits_for(uint64_t maxval) {
+  int b = 0;
+  while (b < 64 && (maxval >> b) != 0) ++b;
+  return b;
+}
+
+}  // namespace psr

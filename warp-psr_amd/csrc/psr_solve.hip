@@ -1,0 +1,1094 @@
+// Stage 5: operators and the cascadic solve (src/warp_psr/solver.py:320-466).
+//
+// Matrices are 27-slot stencils: slot k of row i holds the neighbour at
+// offset (k/9-1, k/3%3-1, k%3-1) around the row node (same level) or around its
+// parent (coarser trial level); absent neighbours have column -1.  Iterating
+// present slots in k order is exactly the reference's ascending-column CSR
+// order, so every row sum keeps the reference order.
+//
+// Gauss-Seidel runs on a copy of the operator gathered into schedule order
+// (rows of one colour/slice group contiguous, slot-major), so the loads of one
+// phase are coalesced; x stays in canonical order for the neighbour gathers.
+#include "psr_result.h"
+
+#include <algorithm>
+#include <cmath>
+
+namespace psr {
+
+namespace {
+
+struct D2 { double v0, v1; };
+
+// stiffness.py:14-35
+PSR_HD D2 hat_integrals(int depth, int index, int trial_depth, int trial_index) {
+  const int resolution = 1 << depth;
+  const int ratio = 1 << (depth - trial_depth);
+  const int first = imax(0, imax(index - 1, (trial_index - 1) * ratio));
+  const int end = imin(resolution, imin(index + 1, (trial_index + 1) * ratio));
+  D2 r = {0.0, 0.0};
+  for (int cell = first; cell < end; ++cell) {
+    const double t0 = double(cell - index);
+    const double c0 = double(cell) / double(ratio) - double(trial_index);
+    const double ts = cell >= index ? -1.0 : 1.0;
+    double cs = 1.0 / double(ratio);
+    if (cell >= trial_index * ratio) cs = -cs;
+    const double tv = 1.0 - fabs(t0);
+    const double cv = 1.0 - fabs(c0);
+    r.v0 += (tv * cv + (tv * cs + ts * cv) * 0.5 + ts * cs / 3.0) / double(resolution);
+    r.v1 += ts * cs * double(resolution);
+  }
+  return r;
+}
+
+// ordered_normal_loads.py:15-84
+PSR_HD void quadratic_piece(double t0, double step, int offset, double* b) {
+  b[0] = b[1] = b[2] = 0.0;
+  if (offset == -1) {
+    b[0] = 0.5 * (1.0 - t0) * (1.0 - t0);
+    b[1] = -(1.0 - t0) * step;
+    b[2] = 0.5 * step * step;
+  } else if (offset == 0) {
+    b[0] = 0.5 + t0 - t0 * t0;
+    b[1] = (1.0 - 2.0 * t0) * step;
+    b[2] = -step * step;
+  } else if (offset == 1) {
+    b[0] = 0.5 * t0 * t0;
+    b[1] = t0 * step;
+    b[2] = 0.5 * step * step;
+  }
+}
+
+PSR_HD D2 normal_integral(int test_depth, int test_index, int normal_depth, int normal_index) {
+  const int nt = 1 << test_depth, nn = 1 << normal_depth;
+  const int resolution = imax(nt, nn);
+  const double test_step = double(nt) / double(resolution);
+  const double normal_step = double(nn) / double(resolution);
+  const int first = imax(0, imax((test_index - 1) * (resolution / nt), (normal_index - 1) * (resolution / nn)));
+  const int end = imin(resolution, imin((test_index + 1) * (resolution / nt), (normal_index + 2) * (resolution / nn)));
+  D2 out = {0.0, 0.0};
+  for (int cell = first; cell < end; ++cell) {
+    const double tx = double(cell) * test_step - double(test_index);
+    double sign = -1.0;
+    if (tx + 0.5 * test_step < 0.0) sign = 1.0;
+    const double h0 = 1.0 + sign * tx;
+    const double h1 = sign * test_step;
+    const int normal_cell = (int)floor((double(cell) + 0.5) * normal_step);
+    const double t0 = double(cell) * normal_step - double(normal_cell);
+    double b[3], rr[3];
+    quadratic_piece(t0, normal_step, normal_index - normal_cell, b);
+    quadratic_piece(t0, normal_step, -1 - normal_index - normal_cell, rr);
+    for (int a = 0; a < 3; ++a) b[a] = b[a] - rr[a];
+    quadratic_piece(t0, normal_step, 2 * nn - 1 - normal_index - normal_cell, rr);
+    for (int a = 0; a < 3; ++a) b[a] = b[a] - rr[a];
+    out.v0 += (h0 * b[0] + (h0 * b[1] + h1 * b[0]) * 0.5 + (h0 * b[2] + h1 * b[1]) / 3.0 + h1 * b[2] * 0.25) /
+              double(resolution);
+    out.v1 += (b[0] + b[1] * 0.5 + b[2] / 3.0) * sign * test_step;
+  }
+  return out;
+}
+
+PSR_HD int normal_center(int index, int test_depth, int normal_depth) {
+  if (test_depth > normal_depth) return index >> 1;
+  if (test_depth < normal_depth) return index << 1;
+  return index;
+}
+
+// ---------------------------------------------------------------- tables
+__global__ void k_hat_table(int depth, int trial_depth, D2* __restrict__ table) {
+  int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  const int res = 1 << depth;
+  if (idx >= (res + 1) * 3) return;
+  const int i = idx / 3, k = idx % 3;
+  table[idx] = hat_integrals(depth, i, trial_depth, (i >> (depth - trial_depth)) + k - 1);
+}
+
+__global__ void k_normal_table(int test_depth, int normal_depth, int first, int width, D2* __restrict__ table) {
+  int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  const int rows = (1 << test_depth) + 1;
+  if (idx >= rows * width) return;
+  const int i = idx / width, k = idx % width;
+  table[idx] = normal_integral(test_depth, i, normal_depth, normal_center(i, test_depth, normal_depth) + first + k);
+}
+
+// ---------------------------------------------------------------- stiffness (stiffness.py:38-109)
+__global__ void k_stiffness(const int64_t* __restrict__ rows, int R, const int64_t* __restrict__ cols, int64_t C, int depth,
+                            int trial_depth, const D2* __restrict__ table, int* __restrict__ out_cols,
+                            double* __restrict__ out_vals) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= R) return;
+  const int64_t key = rows[i];
+  const int cx = key_x(key), cy = key_y(key), cz = key_z(key);
+  const int shift = depth - trial_depth;
+  const int px = cx >> shift, py = cy >> shift, pz = cz >> shift;
+  const int lim = 1 << trial_depth;
+  for (int k = 0; k < 27; ++k) {
+    const int qx = px + k / 9 - 1, qy = py + (k / 3) % 3 - 1, qz = pz + k % 3 - 1;
+    int64_t index = -1;
+    if (imin(imin(qx, qy), qz) >= 0 && imax(imax(qx, qy), qz) <= lim)
+      index = find_key(cols, C, basis_key(trial_depth, qx, qy, qz));
+    double v = 0.0;
+    if (index >= 0) {
+      const D2 x = table[cx * 3 + k / 9], y = table[cy * 3 + (k / 3) % 3], z = table[cz * 3 + k % 3];
+      v = x.v1 * y.v0 * z.v0 + x.v0 * y.v1 * z.v0 + x.v0 * y.v0 * z.v1;
+    }
+    out_cols[(int64_t)k * R + i] = (int)index;
+    out_vals[(int64_t)k * R + i] = v;
+  }
+}
+
+// ---------------------------------------------------------------- constraints (constraints.py:20-84)
+template <class T>
+__global__ void k_points(const int64_t* __restrict__ akeys, const T* __restrict__ avals, int P, const int64_t* __restrict__ active,
+                         int64_t nactive, const T* __restrict__ average, T point_weight, int depth, T* __restrict__ pos,
+                         T* __restrict__ w, uint64_t* __restrict__ bins, int* __restrict__ ids) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P) return;
+  const T* v = avals + 4 * (int64_t)i;
+  const T p0 = v[0] / v[3], p1 = v[1] / v[3], p2 = v[2] / v[3];
+  pos[3 * i] = p0; pos[3 * i + 1] = p1; pos[3 * i + 2] = p2;
+  const int64_t key = akeys[i];
+  T weight = v[3] * T(1 << depth);
+  weight = weight * (point_weight * average[1]);
+  if (depth == 0 || find_key(active, nactive, tree_key(key_x(key), key_y(key), key_z(key))) < 0) weight = T(0);
+  w[i] = weight;
+  // screening_stencils.py:16-26 (bin_points: floor cell, clamped)
+  const int n = 1 << depth;
+  const int bx = imin(imax((int)floor(p0 * T(n)), 0), n - 1);
+  const int by = imin(imax((int)floor(p1 * T(n)), 0), n - 1);
+  const int bz = imin(imax((int)floor(p2 * T(n)), 0), n - 1);
+  bins[i] = ((uint64_t)bx << (2 * depth)) | ((uint64_t)by << depth) | (uint64_t)bz;
+  ids[i] = i;
+}
+
+// screening_stencils.py:28-32 (Warp max(a, 0) = a > 0 ? a : 0)
+template <class T> PSR_HD T hat3(T p0, T p1, T p2, int cx, int cy, int cz, int depth) {
+  const T s = T(1 << depth);
+  T t0 = p0 * s - T(cx), t1 = p1 * s - T(cy), t2 = p2 * s - T(cz);
+  T h0 = T(1) - fabs(t0), h1 = T(1) - fabs(t1), h2 = T(1) - fabs(t2);
+  h0 = h0 > T(0) ? h0 : T(0);
+  h1 = h1 > T(0) ? h1 : T(0);
+  h2 = h2 > T(0) ? h2 : T(0);
+  return h0 * h1 * h2;
+}
+
+// screening_stencils.py:34-70 (add): per row, points binned in the 8 cells at its vertex
+template <class T>
+__global__ void k_screening(const int64_t* __restrict__ rows, int R, const int64_t* __restrict__ cols_keys, int depth,
+                            int trial_depth, const uint64_t* __restrict__ bin_keys, const int* __restrict__ bin_ids, int P,
+                            const T* __restrict__ pos, const T* __restrict__ w, const int* __restrict__ scols,
+                            double* __restrict__ svals, double* __restrict__ load) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= R) return;
+  const int64_t key = rows[i];
+  const int cx = key_x(key), cy = key_y(key), cz = key_z(key);
+  const int n = 1 << depth;
+  double acc[27];
+  int qx[27], qy[27], qz[27];
+  bool present[27];
+  for (int k = 0; k < 27; ++k) {
+    acc[k] = 0.0;
+    const int col = scols[(int64_t)k * R + i];
+    present[k] = col >= 0;
+    if (present[k]) {
+      const int64_t ck = cols_keys[col];
+      qx[k] = key_x(ck); qy[k] = key_y(ck); qz[k] = key_z(ck);
+    } else {
+      qx[k] = qy[k] = qz[k] = 0;
+    }
+  }
+  double rhs = 0.0;
+  for (int kk = 0; kk < 8; ++kk) {
+    const int bx = cx - kk / 4, by = cy - (kk / 2) % 2, bz = cz - kk % 2;
+    if (imin(imin(bx, by), bz) < 0 || imax(imax(bx, by), bz) >= n) continue;
+    const uint64_t bk = ((uint64_t)bx << (2 * depth)) | ((uint64_t)by << depth) | (uint64_t)bz;
+    int lo = 0, hi = P;
+    while (lo < hi) {
+      int mid = (lo + hi) >> 1;
+      if (bin_keys[mid] < bk) lo = mid + 1; else hi = mid;
+    }
+    for (int j = lo; j < P && bin_keys[j] == bk; ++j) {
+      const int index = bin_ids[j];
+      const T p0 = pos[3 * index], p1 = pos[3 * index + 1], p2 = pos[3 * index + 2];
+      const T weight = w[index];
+      if (weight != T(0)) {
+        const T phi = hat3(p0, p1, p2, cx, cy, cz, depth);
+        rhs += double(phi * weight);
+        for (int k = 0; k < 27; ++k) {
+          if (!present[k]) continue;
+          const T trial = hat3(p0, p1, p2, qx[k], qy[k], qz[k], trial_depth);
+          acc[k] += double(phi * trial * weight);
+        }
+      }
+    }
+  }
+  if (load) load[i] = rhs;
+  for (int k = 0; k < 27; ++k)
+    if (present[k]) svals[(int64_t)k * R + i] += acc[k];
+}
+
+template <class T>
+__global__ void k_cast(const double* __restrict__ in, T* __restrict__ out, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = T(in[i]);
+}
+
+// solver.py:119-126 (_rhs)
+template <class T>
+__global__ void k_rhs(const double* __restrict__ load, const double* __restrict__ point_load, int R, T* __restrict__ b) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < R) b[i] = T(load[i]) + T(0.5) * T(point_load[i]);
+}
+
+// ---------------------------------------------------------------- prolongation (transfer_stencils.py:15-66)
+template <class T>
+__global__ void k_prolongation(const int64_t* __restrict__ fine, int R, const int64_t* __restrict__ coarse, int64_t C, int depth,
+                               int* __restrict__ cols, T* __restrict__ vals) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= R) return;
+  const int64_t key = fine[i];
+  const int f[3] = {key_x(key), key_y(key), key_z(key)};
+  for (int k = 0; k < 8; ++k) {
+    const int c[3] = {f[0] / 2 + k / 4, f[1] / 2 + (k / 2) % 2, f[2] / 2 + k % 2};
+    T h[3];
+    for (int a = 0; a < 3; ++a) h[a] = T(1) - fabs(T(f[a]) * T(0.5) - T(c[a]));
+    const T weight = h[0] * h[1] * h[2];
+    int64_t column = -1;
+    if (weight > T(0)) column = find_key(coarse, C, basis_key(depth - 1, c[0], c[1], c[2]));
+    cols[(int64_t)k * R + i] = (int)column;
+    vals[(int64_t)k * R + i] = column >= 0 ? weight : T(0);
+  }
+}
+
+// Restriction P^T as a gather: coarse row r takes the fine nodes 2r+o,
+// o in {-1,0,1}^3 (ascending fine index = lexicographic o), weight 1/2 per
+// nonzero axis offset; exactly the rows of the transposed prolongation.
+__global__ void k_restriction(const int64_t* __restrict__ coarse, int C, const int64_t* __restrict__ fine, int64_t F,
+                              int fine_depth, int* __restrict__ cols) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= C) return;
+  const int64_t key = coarse[r];
+  const int x = 2 * key_x(key), y = 2 * key_y(key), z = 2 * key_z(key);
+  const int lim = 1 << fine_depth;
+  for (int k = 0; k < 27; ++k) {
+    const int fx = x + k / 9 - 1, fy = y + (k / 3) % 3 - 1, fz = z + k % 3 - 1;
+    int64_t idx = -1;
+    if (imin(imin(fx, fy), fz) >= 0 && imax(imax(fx, fy), fz) <= lim) idx = find_key(fine, F, basis_key(fine_depth, fx, fy, fz));
+    cols[(int64_t)k * C + r] = (int)idx;
+  }
+}
+
+PSR_HD double restriction_weight(int k) {
+  double w = 1.0;
+  if (k / 9 != 1) w *= 0.5;
+  if ((k / 3) % 3 != 1) w *= 0.5;
+  if (k % 3 != 1) w *= 0.5;
+  return w;
+}
+
+// y[r] = (sum_k R_rk x[k]) (*alpha) (+ beta*y[r]) in warp.sparse.bsr_mv order
+template <class V>
+__global__ void k_restrict_apply(const int* __restrict__ cols, int C, const V* __restrict__ x, V* __restrict__ y, int accumulate) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= C) return;
+  V v = V(0);
+  for (int k = 0; k < 27; ++k) {
+    const int c = cols[(int64_t)k * C + r];
+    if (c >= 0) v = v + V(restriction_weight(k)) * x[c];
+  }
+  if (accumulate) v = v + y[r];
+  y[r] = v;
+}
+
+// y = alpha*(A x) + beta*y with A a stencil (slots in column order)
+template <class T>
+__global__ void k_spmv(const int* __restrict__ cols, const T* __restrict__ vals, int R, int slots, const T* __restrict__ x,
+                       T* __restrict__ y, T alpha, T beta) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= R) return;
+  T v = T(0);
+  for (int k = 0; k < slots; ++k) {
+    const int c = cols[(int64_t)k * R + i];
+    if (c >= 0) v = v + vals[(int64_t)k * R + i] * x[c];
+  }
+  v = v * alpha;
+  if (beta != T(0)) v = v + beta * y[i];
+  y[i] = v;
+}
+
+// ---------------------------------------------------------------- normal loads (normal_stencils.py, normal_loads.py)
+template <class V>
+__global__ void k_normal_load(const int64_t* __restrict__ keys, int R, const int64_t* __restrict__ nkeys, int64_t ncount,
+                              const V* __restrict__ nvals, int nstride, int test_depth, int normal_depth, int first, int width,
+                              const D2* __restrict__ table, double* __restrict__ out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= R) return;
+  const int64_t key = keys[i];
+  const int c0 = key_x(key), c1 = key_y(key), c2 = key_z(key);
+  const int e0 = normal_center(c0, test_depth, normal_depth), e1 = normal_center(c1, test_depth, normal_depth),
+            e2 = normal_center(c2, test_depth, normal_depth);
+  const int lim = 1 << normal_depth;
+  double value = 0.0;
+  const int w3 = width * width * width;
+  for (int k = 0; k < w3; ++k) {
+    const int x = k / (width * width), y = (k / width) % width, z = k % width;
+    const int nx = e0 + x + first, ny = e1 + y + first, nz = e2 + z + first;
+    if (imin(imin(nx, ny), nz) < 0 || imax(imax(nx, ny), nz) >= lim) continue;
+    const int64_t j = find_key(nkeys, ncount, basis_key(normal_depth, nx, ny, nz));
+    if (j < 0) continue;
+    const D2 ix = table[c0 * width + x], iy = table[c1 * width + y], iz = table[c2 * width + z];
+    const V* nv = nvals + (int64_t)nstride * j;
+    value += double(nv[0]) * ix.v1 * iy.v0 * iz.v0;
+    value += double(nv[1]) * ix.v0 * iy.v1 * iz.v0;
+    value += double(nv[2]) * ix.v0 * iy.v0 * iz.v1;
+  }
+  out[i] = value;
+}
+
+// normal_loads.py:27-63 (prolong degree-2 coefficients with degree-1 weights, issue #35)
+template <class T>
+__global__ void k_normal_prolong(const int64_t* __restrict__ keys, int R, const int64_t* __restrict__ own_keys, int64_t own_count,
+                                 const T* __restrict__ own_vals, const int64_t* __restrict__ prev_keys, int64_t prev_count,
+                                 const T* __restrict__ prev_vals, int depth, T* __restrict__ out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= R) return;
+  const int64_t key = keys[i];
+  const int c[3] = {key_x(key), key_y(key), key_z(key)};
+  T v0 = T(0), v1 = T(0), v2 = T(0);
+  const int64_t own = find_key(own_keys, own_count, key);
+  if (own >= 0) { v0 = own_vals[3 * own]; v1 = own_vals[3 * own + 1]; v2 = own_vals[3 * own + 2]; }
+  if (depth > 0) {
+    for (int corner = 0; corner < 8; ++corner) {
+      int p[3] = {c[0] / 2, c[1] / 2, c[2] / 2};
+      T weight = T(1);
+      for (int axis = 0; axis < 3; ++axis) {
+        if ((corner >> axis) & 1) p[axis] += 1;
+        weight = weight * (T(1) - fabs(T(c[axis]) * T(0.5) - T(p[axis])));
+      }
+      if (weight > T(0)) {
+        const int64_t j = find_key(prev_keys, prev_count, basis_key(depth - 1, p[0], p[1], p[2]));
+        if (j >= 0) {
+          v0 = v0 + prev_vals[3 * j] * weight;
+          v1 = v1 + prev_vals[3 * j + 1] * weight;
+          v2 = v2 + prev_vals[3 * j + 2] * weight;
+        }
+      }
+    }
+  }
+  out[3 * i] = v0; out[3 * i + 1] = v1; out[3 * i + 2] = v2;
+}
+
+__global__ void k_add_f64(double* __restrict__ y, const double* __restrict__ x, int n) {  // y = x + y
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = x[i] + y[i];
+}
+
+// ---------------------------------------------------------------- Galerkin R*A*P (base levels)
+// Row r of R*A*P with the reference spgemm order: contributions to an entry
+// arrive in ascending middle index (R row order, then (RA) row order).
+template <class T>
+__global__ void k_galerkin(const int* __restrict__ rcols, int C, const int* __restrict__ acols, const T* __restrict__ avals, int F,
+                           const int* __restrict__ pcols, const T* __restrict__ pvals, int* __restrict__ out_cols,
+                           T* __restrict__ out_vals, const int64_t* __restrict__ ckeys, int64_t Ccount, int coarse_depth) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= C) return;
+  T ra[125];
+  bool rap[125];
+  int rak[125];
+  for (int t = 0; t < 125; ++t) { ra[t] = T(0); rap[t] = false; rak[t] = -1; }
+  for (int kr = 0; kr < 27; ++kr) {  // fine k = 2r + o_kr, ascending
+    const int k = rcols[(int64_t)kr * C + r];
+    if (k < 0) continue;
+    const T rw = T(restriction_weight(kr));
+    const int ox = kr / 9 - 1, oy = (kr / 3) % 3 - 1, oz = kr % 3 - 1;
+    for (int j = 0; j < 27; ++j) {
+      const int c = acols[(int64_t)j * F + k];
+      if (c < 0) continue;
+      const int tx = ox + j / 9 - 1 + 2, ty = oy + (j / 3) % 3 - 1 + 2, tz = oz + j % 3 - 1 + 2;
+      const int t = tx * 25 + ty * 5 + tz;
+      ra[t] = ra[t] + rw * avals[(int64_t)j * F + k];
+      rap[t] = true;
+      rak[t] = c;
+    }
+  }
+  T out[27];
+  bool outp[27];
+  for (int t = 0; t < 27; ++t) { out[t] = T(0); outp[t] = false; }
+  const int64_t rkey = ckeys[r];
+  const int rx = key_x(rkey), ry = key_y(rkey), rz = key_z(rkey);
+  for (int t = 0; t < 125; ++t) {  // (RA) row entries in ascending fine column
+    if (!rap[t]) continue;
+    const int c = rak[t];
+    for (int kp = 0; kp < 8; ++kp) {
+      const int pc = pcols[(int64_t)kp * F + c];
+      if (pc < 0) continue;
+      const int64_t pkey = ckeys[pc];
+      const int ox = key_x(pkey) - rx + 1, oy = key_y(pkey) - ry + 1, oz = key_z(pkey) - rz + 1;
+      if ((unsigned)ox > 2u || (unsigned)oy > 2u || (unsigned)oz > 2u) continue;  // never for trilinear P
+      const int s = ox * 9 + oy * 3 + oz;
+      out[s] = out[s] + ra[t] * pvals[(int64_t)kp * F + c];
+      outp[s] = true;
+    }
+  }
+  for (int s = 0; s < 27; ++s) {
+    int col = -1;
+    if (outp[s]) col = (int)find_key(ckeys, Ccount, basis_key(coarse_depth, rx + s / 9 - 1, ry + (s / 3) % 3 - 1, rz + s % 3 - 1));
+    out_cols[(int64_t)s * C + r] = col;
+    out_vals[(int64_t)s * C + r] = out[s];
+  }
+}
+
+// ---------------------------------------------------------------- relaxation
+// relaxation.py:38-64 (_relax) on the schedule-ordered operator.
+template <class T>
+__device__ __forceinline__ void gs_update(int j, int R, const int* __restrict__ order, const int* __restrict__ cols,
+                                          const T* __restrict__ vals, const T* __restrict__ b, const T* __restrict__ inv_diag,
+                                          const unsigned char* __restrict__ has_diag, T* __restrict__ x) {
+  T r = b[j];
+#pragma unroll
+  for (int k = 0; k < 27; ++k) {
+    const int c = cols[(int64_t)k * R + j];
+    if (c >= 0) r = r - x[c] * vals[(int64_t)k * R + j];
+  }
+  if (has_diag[j]) {
+    const int i = order[j];
+    x[i] = x[i] + r * inv_diag[j];
+  }
+}
+
+// One launch of relaxation.py:100-116 (wavefront_update): `slices` slices two
+// apart, same colour.
+template <class T>
+__global__ void __launch_bounds__(256) k_relax_wave(const int* __restrict__ order, const int* __restrict__ goff,
+                                                    const int* __restrict__ cols, const T* __restrict__ vals,
+                                                    const T* __restrict__ b, const T* __restrict__ inv_diag,
+                                                    const unsigned char* __restrict__ has_diag, T* __restrict__ x, int R,
+                                                    int first_z, int color) {
+  const int group = 4 * (first_z + 2 * (int)blockIdx.y) + color;
+  const int j = goff[group] + blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < goff[group + 1]) gs_update(j, R, order, cols, vals, b, inv_diag, has_diag, x);
+}
+
+// relaxation.py:261-271 (8-colour, all sweeps) inside one workgroup: the
+// barrier between colour groups replaces the kernel boundary.
+template <class T>
+__global__ void __launch_bounds__(1024) k_relax_block(const int* __restrict__ order, const int* __restrict__ goff,
+                                                      const int* __restrict__ cols, const T* __restrict__ vals,
+                                                      const T* __restrict__ b, const T* __restrict__ inv_diag,
+                                                      const unsigned char* __restrict__ has_diag, T* __restrict__ x, int R,
+                                                      int iters, int forward) {
+  for (int it = 0; it < iters; ++it) {
+    for (int c = 0; c < 8; ++c) {
+      const int g = forward ? c : 7 - c;
+      for (int j = goff[g] + threadIdx.x; j < goff[g + 1]; j += blockDim.x)
+        gs_update(j, R, order, cols, vals, b, inv_diag, has_diag, x);
+      __syncthreads();
+    }
+  }
+}
+
+// schedule: group per row
+__global__ void k_groups_of(const int64_t* __restrict__ keys, int R, int sliced, uint64_t* __restrict__ g, int* __restrict__ ids) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= R) return;
+  const int64_t k = keys[i];
+  const int x = key_x(k), y = key_y(k), z = key_z(k);
+  g[i] = sliced ? (uint64_t)((x & 1) + 2 * (y & 1) + 4 * z) : (uint64_t)((x & 1) + 2 * (y & 1) + 4 * (z & 1));
+  ids[i] = i;
+}
+
+__global__ void k_group_offsets(const uint64_t* __restrict__ g, int R, int groups, int* __restrict__ off) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t > groups) return;
+  int lo = 0, hi = R;
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (g[mid] < (uint64_t)t) lo = mid + 1; else hi = mid;
+  }
+  off[t] = lo;
+}
+
+template <class T>
+__global__ void k_gather_operator(const int* __restrict__ order, int R, const int* __restrict__ cols, const T* __restrict__ vals,
+                                  const T* __restrict__ b, int* __restrict__ scols, T* __restrict__ svals, T* __restrict__ sb,
+                                  T* __restrict__ inv_diag, unsigned char* __restrict__ has_diag) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= R) return;
+  const int i = order[j];
+  T diagonal = T(0);
+  for (int k = 0; k < 27; ++k) {
+    const int c = cols[(int64_t)k * R + i];
+    const T v = vals[(int64_t)k * R + i];
+    scols[(int64_t)k * R + j] = c;
+    svals[(int64_t)k * R + j] = v;
+    if (c == i) diagonal = v;
+  }
+  sb[j] = b[i];
+  has_diag[j] = diagonal != T(0);
+  inv_diag[j] = diagonal != T(0) ? T(1) / diagonal : T(0);
+}
+
+// Root solve on one lane: skip when |r|^2 <= eps^2, else unpreconditioned CG
+// (solver.py:276-297; Warp's cg restated, bitwise parity unpinned).
+template <class T>
+__global__ void k_root_solve(const int* __restrict__ cols, const T* __restrict__ vals, int R, const int64_t* __restrict__ keys,
+                             const T* __restrict__ rhs, T* __restrict__ x, T* __restrict__ pad_x, T* __restrict__ work,
+                             int dc, int base, int cycle, double acc, int maxiter, int* __restrict__ ran) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const int m = dc ? 64 : R;
+  T* b = work;
+  T* r = work + 64;
+  T* p = work + 128;
+  T* ap = work + 192;
+  T* xx = dc ? pad_x : x;
+  auto row_of = [&](int i) {
+    const int64_t key = keys[i];
+    const int cx = key_x(key) + 2, cy = key_y(key) + 2, cz = key_z(key) + 2;
+    return cz * 16 + (cx & 1) + 2 * (cy & 1) + 2 * (cx & 2) + 4 * (cy & 2);  // solver.py:30-33
+  };
+  for (int i = 0; i < m; ++i) b[i] = T(0);
+  for (int i = 0; i < R; ++i) b[dc ? row_of(i) : i] = rhs[i];
+  auto apply = [&](const T* v, T* out) {
+    if (!dc) {
+      for (int i = 0; i < R; ++i) {
+        T a = T(0);
+        for (int k = 0; k < 27; ++k) {
+          const int c = cols[k * R + i];
+          if (c >= 0) a = a + vals[k * R + i] * v[c];
+        }
+        out[i] = a;
+      }
+      return;
+    }
+    // padded rows: entries sorted by padded column
+    for (int i = 0; i < 64; ++i) out[i] = T(0);
+    for (int i = 0; i < R; ++i) {
+      int pc[27];
+      T pv[27];
+      int cnt = 0;
+      for (int k = 0; k < 27; ++k) {
+        const int c = cols[k * R + i];
+        if (c < 0) continue;
+        int q = row_of(c);
+        int pos = cnt++;
+        while (pos > 0 && pc[pos - 1] > q) { pc[pos] = pc[pos - 1]; pv[pos] = pv[pos - 1]; --pos; }
+        pc[pos] = q;
+        pv[pos] = vals[k * R + i];
+      }
+      T a = T(0);
+      for (int e = 0; e < cnt; ++e) a = a + pv[e] * v[pc[e]];
+      out[row_of(i)] = a;
+    }
+    T mean = T(0);
+    for (int i = 0; i < 64; ++i) mean = mean + v[i];
+    mean = mean / T(64);
+    for (int i = 0; i < 64; ++i) out[i] = out[i] + T(1) * mean;
+  };
+  for (int i = 0; i < m; ++i) r[i] = b[i];
+  if (base == 0 && cycle > 0) {
+    apply(xx, ap);
+    for (int i = 0; i < m; ++i) r[i] = b[i] - ap[i];
+  }
+  double norm2 = 0.0;
+  for (int i = 0; i < m; ++i) norm2 += double(r[i]) * double(r[i]);
+  *ran = 0;
+  if (!(norm2 > acc * acc)) return;
+  *ran = 1;
+  const double tol = base == 0 ? 0.0 : acc;
+  const double atol = base == 0 ? acc * sqrt(norm2) : 0.0;
+  apply(xx, ap);
+  for (int i = 0; i < m; ++i) r[i] = b[i] - ap[i];
+  double bnorm = 0.0;
+  for (int i = 0; i < m; ++i) bnorm += double(b[i]) * double(b[i]);
+  const double threshold = fmax(tol * sqrt(bnorm), atol);
+  double rz = 0.0;
+  for (int i = 0; i < m; ++i) rz += double(r[i]) * double(r[i]);
+  for (int i = 0; i < m; ++i) p[i] = r[i];
+  for (int it = 0; it < maxiter; ++it) {
+    if (sqrt(rz) <= threshold) break;
+    apply(p, ap);
+    double pap = 0.0;
+    for (int i = 0; i < m; ++i) pap += double(p[i]) * double(ap[i]);
+    if (pap == 0.0) break;
+    const T alpha = T(rz / pap);
+    for (int i = 0; i < m; ++i) { xx[i] = xx[i] + alpha * p[i]; r[i] = r[i] - alpha * ap[i]; }
+    double rz_new = 0.0;
+    for (int i = 0; i < m; ++i) rz_new += double(r[i]) * double(r[i]);
+    const T beta = T(rz_new / rz);
+    rz = rz_new;
+    for (int i = 0; i < m; ++i) p[i] = r[i] + beta * p[i];
+  }
+}
+
+template <class T>
+__global__ void k_root_extract(const int64_t* __restrict__ keys, int R, const T* __restrict__ pad_x, T* __restrict__ x) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= R) return;
+  const int64_t key = keys[i];
+  const int cx = key_x(key) + 2, cy = key_y(key) + 2, cz = key_z(key) + 2;
+  x[i] = pad_x[cz * 16 + (cx & 1) + 2 * (cy & 1) + 2 * (cx & 2) + 4 * (cy & 2)];
+}
+
+inline int floordiv(int a, int b) { return (a >= 0) ? a / b : -((-a + b - 1) / b); }
+
+}  // namespace
+
+// ---------------------------------------------------------------- host side
+template <class T> struct Schedule {
+  DArray<int> order, offsets;
+  std::vector<int> host_offsets;
+  int max_group = 0;
+};
+
+template <class T> static Schedule<T> make_schedule(const DArray<int64_t>& keys, int R, int depth, bool sliced, hipStream_t s) {
+  Schedule<T> sc;
+  const int groups = sliced ? 4 * ((1 << depth) + 1) : 8;
+  DArray<uint64_t> g(R, s);
+  sc.order.reset(R, s);
+  if (R) {
+    hipLaunchKernelGGL(k_groups_of, dim3(blocks_for(R, 256)), dim3(256), 0, s, (const int64_t*)keys.data(), R, sliced ? 1 : 0,
+                       g.data(), sc.order.data());
+    KERNEL_CHECK();
+    sort_pairs_u64(g.data(), sc.order.data(), R, bits_for((uint64_t)groups), s);
+  }
+  sc.offsets.reset(groups + 1, s);
+  hipLaunchKernelGGL(k_group_offsets, dim3(blocks_for(groups + 1, 256)), dim3(256), 0, s, (const uint64_t*)g.data(), R, groups,
+                     sc.offsets.data());
+  KERNEL_CHECK();
+  sc.host_offsets.resize(groups + 1);
+  read_bytes(sc.host_offsets.data(), sc.offsets.data(), sizeof(int) * (groups + 1), s);
+  for (int t = 0; t < groups; ++t) sc.max_group = std::max(sc.max_group, sc.host_offsets[t + 1] - sc.host_offsets[t]);
+  return sc;
+}
+
+template <class T> struct GsOperator {
+  int R = 0;
+  DArray<int> cols;
+  DArray<T> vals, b, inv_diag;
+  DArray<unsigned char> has_diag;
+};
+
+template <class T>
+static GsOperator<T> gather_operator(const Schedule<T>& sc, const DArray<int>& cols, const DArray<T>& vals, const T* b, int R,
+                                     hipStream_t s) {
+  GsOperator<T> op;
+  op.R = R;
+  op.cols.reset(27 * (size_t)R, s);
+  op.vals.reset(27 * (size_t)R, s);
+  op.b.reset(R, s);
+  op.inv_diag.reset(R, s);
+  op.has_diag.reset(R, s);
+  if (R) {
+    hipLaunchKernelGGL(k_gather_operator<T>, dim3(blocks_for(R, 256)), dim3(256), 0, s, (const int*)sc.order.data(), R,
+                       (const int*)cols.data(), (const T*)vals.data(), b, op.cols.data(), op.vals.data(), op.b.data(),
+                       op.inv_diag.data(), op.has_diag.data());
+    KERNEL_CHECK();
+  }
+  return op;
+}
+
+template <class T> static void refresh_rhs(GsOperator<T>& op, const Schedule<T>& sc, const T* b, hipStream_t s);
+
+template <class T>
+__global__ void k_gather_b(const int* __restrict__ order, int R, const T* __restrict__ b, T* __restrict__ sb) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < R) sb[j] = b[order[j]];
+}
+
+template <class T> static void refresh_rhs(GsOperator<T>& op, const Schedule<T>& sc, const T* b, hipStream_t s) {
+  if (!op.R) return;
+  hipLaunchKernelGGL(k_gather_b<T>, dim3(blocks_for(op.R, 256)), dim3(256), 0, s, (const int*)sc.order.data(), op.R, b,
+                     op.b.data());
+  KERNEL_CHECK();
+}
+
+// relaxation.py:128-143 (wavefront_groups) + :189-210
+template <class T>
+void relax_sliced(const GsOperator<T>& op, const Schedule<T>& sc, T* x, int depth, int iters, hipStream_t s) {
+  const int n = 1 << depth;
+  const unsigned bx = (unsigned)std::max(1, (sc.max_group + 255) / 256);
+  for (int q = 0; q <= n + 2 * (iters - 1); ++q) {
+    const int first = std::max(0, floordiv(q - n + 1, 2));
+    const int last = std::min(iters - 1, q / 2);
+    if (last < first) continue;
+    const int z0 = n - q + 2 * first;
+    const unsigned count = (unsigned)(last - first + 1);
+    for (int color = 0; color < 4; ++color) {
+      hipLaunchKernelGGL(k_relax_wave<T>, dim3(bx, count), dim3(256), 0, s, (const int*)sc.order.data(),
+                         (const int*)sc.offsets.data(), (const int*)op.cols.data(), (const T*)op.vals.data(),
+                         (const T*)op.b.data(), (const T*)op.inv_diag.data(), (const unsigned char*)op.has_diag.data(), x,
+                         op.R, z0, color);
+    }
+  }
+  KERNEL_CHECK();
+}
+
+template <class T>
+void relax_colored(const GsOperator<T>& op, const Schedule<T>& sc, T* x, int iters, bool forward, hipStream_t s) {
+  if (!op.R) return;
+  hipLaunchKernelGGL(k_relax_block<T>, dim3(1), dim3(1024), 0, s, (const int*)sc.order.data(), (const int*)sc.offsets.data(),
+                     (const int*)op.cols.data(), (const T*)op.vals.data(), (const T*)op.b.data(), (const T*)op.inv_diag.data(),
+                     (const unsigned char*)op.has_diag.data(), x, op.R, iters, forward ? 1 : 0);
+  KERNEL_CHECK();
+}
+
+template <class T> static Stencil<double> assemble_stiffness(const LevelState<T>& rowsL, const LevelState<T>& colsL, int depth,
+                                                             int trial_depth, hipStream_t s) {
+  Stencil<double> m;
+  const int R = rowsL.basis_count;
+  m.rows = R;
+  m.cols_count = colsL.basis_count;
+  m.cols.reset(27 * (size_t)R, s);
+  m.vals.reset(27 * (size_t)R, s);
+  const int res = 1 << depth;
+  DArray<D2> table((res + 1) * 3, s);
+  hipLaunchKernelGGL(k_hat_table, dim3(blocks_for((res + 1) * 3, 256)), dim3(256), 0, s, depth, trial_depth, table.data());
+  KERNEL_CHECK();
+  if (R) {
+    hipLaunchKernelGGL(k_stiffness, dim3(blocks_for(R, 256)), dim3(256), 0, s, (const int64_t*)rowsL.basis_keys.data(), R,
+                       (const int64_t*)colsL.basis_keys.data(), (int64_t)colsL.basis_count, depth, trial_depth,
+                       (const D2*)table.data(), m.cols.data(), m.vals.data());
+    KERNEL_CHECK();
+  }
+  return m;
+}
+
+template <class T> struct PointSet {
+  int P = 0;
+  DArray<T> pos, w;
+  DArray<uint64_t> bins;
+  DArray<int> ids;
+};
+
+template <class T> static PointSet<T> point_constraints(Result<T>& r, int d, hipStream_t s) {
+  PointSet<T> ps;
+  LevelState<T>& L = r.levels[d];
+  ps.P = (int)L.agg_keys.size();
+  ps.pos.reset(3 * (size_t)ps.P, s);
+  ps.w.reset(ps.P, s);
+  ps.bins.reset(ps.P, s);
+  ps.ids.reset(ps.P, s);
+  if (ps.P) {
+    hipLaunchKernelGGL(k_points<T>, dim3(blocks_for(ps.P, 256)), dim3(256), 0, s, (const int64_t*)L.agg_keys.data(),
+                       (const T*)L.agg_vals.data(), ps.P, (const int64_t*)L.tree_keys.data(), (int64_t)L.tree_keys.size(),
+                       (const T*)r.normal_average.data(), T(r.prm.point_weight), d, ps.pos.data(), ps.w.data(),
+                       ps.bins.data(), ps.ids.data());
+    KERNEL_CHECK();
+    sort_pairs_u64(ps.bins.data(), ps.ids.data(), ps.P, 3 * d, s);
+  }
+  return ps;
+}
+
+template <class T>
+static void add_screening(Stencil<double>& m, const DArray<int64_t>& rows, const DArray<int64_t>& cols, const PointSet<T>& ps,
+                          int depth, int trial_depth, double* load, hipStream_t s) {
+  if (!m.rows) return;
+  hipLaunchKernelGGL(k_screening<T>, dim3(blocks_for(m.rows, 128)), dim3(128), 0, s, (const int64_t*)rows.data(), m.rows,
+                     (const int64_t*)cols.data(), depth, trial_depth, (const uint64_t*)ps.bins.data(),
+                     (const int*)ps.ids.data(), ps.P, (const T*)ps.pos.data(), (const T*)ps.w.data(),
+                     (const int*)m.cols.data(), m.vals.data(), load);
+  KERNEL_CHECK();
+}
+
+template <class T> static Stencil<T> cast_stencil(Stencil<double>&& m, hipStream_t s) {
+  Stencil<T> out;
+  out.rows = m.rows;
+  out.cols_count = m.cols_count;
+  out.cols = std::move(m.cols);
+  out.vals.reset(m.vals.size(), s);
+  if (m.vals.size()) {
+    hipLaunchKernelGGL(k_cast<T>, dim3(blocks_for(m.vals.size(), 256)), dim3(256), 0, s, (const double*)m.vals.data(),
+                       out.vals.data(), (int64_t)m.vals.size());
+    KERNEL_CHECK();
+  }
+  return out;
+}
+
+template <class T>
+static void normal_stencil_load(const LevelState<T>& testL, int test_depth, const int64_t* nkeys, int64_t ncount, const T* nvals,
+                                int normal_depth, double* out, hipStream_t s) {
+  const int R = testL.basis_count;
+  if (!R) return;
+  if (!ncount) {
+    HIP_CHECK(hipMemsetAsync(out, 0, sizeof(double) * R, s));
+    return;
+  }
+  const int width = test_depth < normal_depth ? 6 : 4;
+  const int first = test_depth < normal_depth ? -3 : -2;
+  const int rows = (1 << test_depth) + 1;
+  DArray<D2> table((size_t)rows * width, s);
+  hipLaunchKernelGGL(k_normal_table, dim3(blocks_for(rows * width, 256)), dim3(256), 0, s, test_depth, normal_depth, first, width,
+                     table.data());
+  KERNEL_CHECK();
+  hipLaunchKernelGGL(k_normal_load<T>, dim3(blocks_for(R, 128)), dim3(128), 0, s, (const int64_t*)testL.basis_keys.data(), R,
+                     nkeys, ncount, nvals, 3, test_depth, normal_depth, first, width, (const D2*)table.data(), out);
+  KERNEL_CHECK();
+}
+
+// normal_loads.py:66-190 (stencil path)
+template <class T> static void normal_loads(Result<T>& r, std::vector<DArray<int>>& restr, hipStream_t s) {
+  const int D = r.prm.depth, base = r.prm.base_depth;
+  std::vector<DArray<double>> same(D + 1), from_child(D + 1), from_parent(D + 1);
+  DArray<T> previous;
+  bool has_cumulative = false;
+  for (int d = 0; d <= D; ++d) {
+    const int64_t r0 = r.normal_offsets[d], r1 = r.normal_offsets[d + 1];
+    const int64_t* nk = r.normal_keys.data() + r0;
+    const T* nv = r.normal_vals.data() + 3 * r0;
+    has_cumulative = has_cumulative || (r1 > r0);
+    LevelState<T>& L = r.levels[d];
+    if (d >= base) {
+      same[d].reset(L.basis_count, s);
+      normal_stencil_load(L, d, nk, r1 - r0, nv, d, same[d].data(), s);
+      if (d > base) {
+        from_child[d - 1].reset(r.levels[d - 1].basis_count, s);
+        normal_stencil_load(r.levels[d - 1], d - 1, nk, r1 - r0, nv, d, from_child[d - 1].data(), s);
+      }
+    }
+    if (d < D) {
+      DArray<T> cumulative(3 * (size_t)L.basis_count, s);
+      if (L.basis_count) {
+        const LevelState<T>* P = d > 0 ? &r.levels[d - 1] : nullptr;
+        hipLaunchKernelGGL(k_normal_prolong<T>, dim3(blocks_for(L.basis_count, 256)), dim3(256), 0, s,
+                           (const int64_t*)L.basis_keys.data(), L.basis_count, nk, r1 - r0, nv,
+                           P ? (const int64_t*)P->basis_keys.data() : (const int64_t*)nullptr,
+                           (int64_t)(P ? P->basis_count : 0), (const T*)previous.data(), d, cumulative.data());
+        KERNEL_CHECK();
+      }
+      if (d >= base && has_cumulative) {
+        from_parent[d + 1].reset(r.levels[d + 1].basis_count, s);
+        normal_stencil_load(r.levels[d + 1], d + 1, (const int64_t*)L.basis_keys.data(), (int64_t)L.basis_count,
+                            (const T*)cumulative.data(), d, from_parent[d + 1].data(), s);
+      }
+      previous = std::move(cumulative);
+    }
+  }
+  DArray<double> finer(r.levels[D].basis_count, s);
+  finer.zero();
+  for (int d = D; d >= base; --d) {
+    LevelState<T>& L = r.levels[d];
+    DArray<double> load = std::move(same[d]);
+    const int R = L.basis_count;
+    if (R) {
+      hipLaunchKernelGGL(k_add_f64, dim3(blocks_for(R, 256)), dim3(256), 0, s, load.data(), (const double*)finer.data(), R);
+      KERNEL_CHECK();
+      if (!from_parent[d].empty()) {
+        hipLaunchKernelGGL(k_add_f64, dim3(blocks_for(R, 256)), dim3(256), 0, s, load.data(),
+                           (const double*)from_parent[d].data(), R);
+        KERNEL_CHECK();
+      }
+    }
+    L.normal_load = std::move(load);
+    if (d > base) {
+      DArray<double> next = std::move(from_child[d - 1]);
+      const int C = r.levels[d - 1].basis_count;
+      if (C) {
+        hipLaunchKernelGGL(k_restrict_apply<double>, dim3(blocks_for(C, 256)), dim3(256), 0, s, (const int*)restr[d - 1].data(),
+                           C, (const double*)finer.data(), next.data(), 1);
+        KERNEL_CHECK();
+      }
+      finer = std::move(next);
+    }
+  }
+}
+
+// solver.py:241-317 (regular_multigrid)
+template <class T> static void regular_multigrid(Result<T>& r, std::vector<DArray<int>>& restr, hipStream_t s) {
+  const int base = r.prm.base_depth;
+  LevelState<T>& B = r.levels[base];
+  std::vector<DArray<int>> mcols(base + 1);
+  std::vector<DArray<T>> mvals(base + 1);
+  std::vector<int> Rn(base + 1);
+  for (int d = 0; d <= base; ++d) Rn[d] = r.levels[d].basis_count;
+  mcols[base].reset(27 * (size_t)Rn[base], s);
+  mvals[base].reset(27 * (size_t)Rn[base], s);
+  if (Rn[base]) {
+    HIP_CHECK(hipMemcpyAsync(mcols[base].data(), B.A.cols.data(), sizeof(int) * 27 * Rn[base], hipMemcpyDeviceToDevice, s));
+    HIP_CHECK(hipMemcpyAsync(mvals[base].data(), B.A.vals.data(), sizeof(T) * 27 * Rn[base], hipMemcpyDeviceToDevice, s));
+  }
+  for (int d = base; d > 0; --d) {
+    const int C = Rn[d - 1], F = Rn[d];
+    mcols[d - 1].reset(27 * (size_t)C, s);
+    mvals[d - 1].reset(27 * (size_t)C, s);
+    if (C) {
+      hipLaunchKernelGGL(k_galerkin<T>, dim3(blocks_for(C, 64)), dim3(64), 0, s, (const int*)restr[d - 1].data(), C,
+                         (const int*)mcols[d].data(), (const T*)mvals[d].data(), F, (const int*)r.levels[d].P.cols.data(),
+                         (const T*)r.levels[d].P.vals.data(), mcols[d - 1].data(), mvals[d - 1].data(),
+                         (const int64_t*)r.levels[d - 1].basis_keys.data(), (int64_t)C, d - 1);
+      KERNEL_CHECK();
+    }
+  }
+  std::vector<DArray<T>> rhs(base + 1), x(base + 1);
+  for (int d = 0; d <= base; ++d) {
+    rhs[d].reset(Rn[d], s);
+    rhs[d].zero();
+    x[d].reset(Rn[d], s);
+    x[d].zero();
+  }
+  if (Rn[base]) HIP_CHECK(hipMemcpyAsync(rhs[base].data(), B.rhs.data(), sizeof(T) * Rn[base], hipMemcpyDeviceToDevice, s));
+  std::vector<Schedule<T>> sched;
+  std::vector<GsOperator<T>> ops;
+  for (int d = 0; d <= base; ++d) {
+    sched.push_back(make_schedule<T>(r.levels[d].basis_keys, Rn[d], d, false, s));
+    ops.push_back(gather_operator<T>(sched[d], mcols[d], mvals[d], rhs[d].data(), Rn[d], s));
+  }
+  const int dc = (base == 0 && r.prm.point_weight == 0.0) ? 1 : 0;
+  DArray<T> pad_x(64, s), work(256, s), residual;
+  pad_x.zero();
+  DArray<int> ran(1, s);
+  const int iters = r.prm.iters;
+  for (int cycle = 0; cycle < r.prm.base_v_cycles; ++cycle) {
+    for (int d = base; d > 0; --d) {
+      refresh_rhs(ops[d], sched[d], rhs[d].data(), s);
+      relax_colored(ops[d], sched[d], x[d].data(), iters, true, s);
+      residual.reset(Rn[d], s);
+      HIP_CHECK(hipMemcpyAsync(residual.data(), rhs[d].data(), sizeof(T) * Rn[d], hipMemcpyDeviceToDevice, s));
+      hipLaunchKernelGGL(k_spmv<T>, dim3(blocks_for(Rn[d], 256)), dim3(256), 0, s, (const int*)mcols[d].data(),
+                         (const T*)mvals[d].data(), Rn[d], 27, (const T*)x[d].data(), residual.data(), T(-1), T(1));
+      KERNEL_CHECK();
+      if (Rn[d - 1]) {
+        hipLaunchKernelGGL(k_restrict_apply<T>, dim3(blocks_for(Rn[d - 1], 256)), dim3(256), 0, s,
+                           (const int*)restr[d - 1].data(), Rn[d - 1], (const T*)residual.data(), rhs[d - 1].data(), 0);
+        KERNEL_CHECK();
+      }
+      x[d - 1].zero();
+    }
+    hipLaunchKernelGGL(k_root_solve<T>, dim3(1), dim3(64), 0, s, (const int*)mcols[0].data(), (const T*)mvals[0].data(), Rn[0],
+                       (const int64_t*)r.levels[0].basis_keys.data(), (const T*)rhs[0].data(), x[0].data(), pad_x.data(),
+                       work.data(), dc, base, cycle, r.prm.cg_accuracy, Rn[0], ran.data());
+    KERNEL_CHECK();
+    if (read_int(ran.data(), s)) r.cg_runs.push_back(cycle);
+    for (int d = 1; d <= base; ++d) {
+      hipLaunchKernelGGL(k_spmv<T>, dim3(blocks_for(Rn[d], 256)), dim3(256), 0, s, (const int*)r.levels[d].P.cols.data(),
+                         (const T*)r.levels[d].P.vals.data(), Rn[d], 8, (const T*)x[d - 1].data(), x[d].data(), T(1), T(1));
+      KERNEL_CHECK();
+      x[d - 1].zero();
+      relax_colored(ops[d], sched[d], x[d].data(), iters, false, s);
+    }
+  }
+  if (dc && Rn[0]) {
+    hipLaunchKernelGGL(k_root_extract<T>, dim3(1), dim3(64), 0, s, (const int64_t*)r.levels[0].basis_keys.data(), Rn[0],
+                       (const T*)pad_x.data(), x[0].data());
+    KERNEL_CHECK();
+  }
+  B.correction = std::move(x[base]);
+}
+
+template <class T> void stage_solve(Result<T>& r) {
+  hipStream_t s = r.s;
+  const int D = r.prm.depth, base = r.prm.base_depth;
+  // prolongation P_d and restriction gathers (solver.py:340-342)
+  std::vector<DArray<int>> restr(D + 1);
+  for (int d = 1; d <= D; ++d) {
+    LevelState<T>& F = r.levels[d];
+    LevelState<T>& C = r.levels[d - 1];
+    F.P.rows = F.basis_count;
+    F.P.cols_count = C.basis_count;
+    F.P.cols.reset(8 * (size_t)F.basis_count, s);
+    F.P.vals.reset(8 * (size_t)F.basis_count, s);
+    if (F.basis_count) {
+      hipLaunchKernelGGL(k_prolongation<T>, dim3(blocks_for(F.basis_count, 256)), dim3(256), 0, s,
+                         (const int64_t*)F.basis_keys.data(), F.basis_count, (const int64_t*)C.basis_keys.data(),
+                         (int64_t)C.basis_count, d, F.P.cols.data(), F.P.vals.data());
+      KERNEL_CHECK();
+    }
+    restr[d - 1].reset(27 * (size_t)C.basis_count, s);
+    if (C.basis_count) {
+      hipLaunchKernelGGL(k_restriction, dim3(blocks_for(C.basis_count, 256)), dim3(256), 0, s,
+                         (const int64_t*)C.basis_keys.data(), C.basis_count, (const int64_t*)F.basis_keys.data(),
+                         (int64_t)F.basis_count, d, restr[d - 1].data());
+      KERNEL_CHECK();
+    }
+  }
+  for (int d = 0; d <= D; ++d) {
+    LevelState<T>& L = r.levels[d];
+    L.correction.reset(L.basis_count, s);
+    L.correction.zero();
+  }
+  normal_loads(r, restr, s);
+  const bool screened = r.prm.point_weight > 0 && !r.prm.exact_interpolation;
+  DArray<T> cumulative;
+  for (int d = base; d <= r.prm.solve_depth; ++d) {
+    LevelState<T>& L = r.levels[d];
+    const int R = L.basis_count;
+    if (!R) break;
+    Stencil<double> A = assemble_stiffness(L, L, d, d, s);
+    PointSet<T> pts;
+    L.point_load.reset(R, s);
+    L.point_load.zero();
+    if (screened) {
+      pts = point_constraints(r, d, s);
+      add_screening(A, L.basis_keys, L.basis_keys, pts, d, d, L.point_load.data(), s);
+    }
+    L.rhs.reset(R, s);
+    hipLaunchKernelGGL(k_rhs<T>, dim3(blocks_for(R, 256)), dim3(256), 0, s, (const double*)L.normal_load.data(),
+                       (const double*)L.point_load.data(), R, L.rhs.data());
+    KERNEL_CHECK();
+    L.A = cast_stencil<T>(std::move(A), s);
+    if (d == base) {
+      regular_multigrid(r, restr, s);
+      cumulative.reset(R, s);
+      HIP_CHECK(hipMemcpyAsync(cumulative.data(), L.correction.data(), sizeof(T) * R, hipMemcpyDeviceToDevice, s));
+    } else {
+      Stencil<double> cross = assemble_stiffness(L, r.levels[d - 1], d, d - 1, s);
+      if (screened) add_screening(cross, L.basis_keys, r.levels[d - 1].basis_keys, pts, d, d - 1, (double*)nullptr, s);
+      L.cross = cast_stencil<T>(std::move(cross), s);
+      hipLaunchKernelGGL(k_spmv<T>, dim3(blocks_for(R, 256)), dim3(256), 0, s, (const int*)L.cross.cols.data(),
+                         (const T*)L.cross.vals.data(), R, 27, (const T*)cumulative.data(), L.rhs.data(), T(-1), T(1));
+      KERNEL_CHECK();
+      Schedule<T> sc = make_schedule<T>(L.basis_keys, R, d, true, s);
+      GsOperator<T> op = gather_operator<T>(sc, L.A.cols, L.A.vals, L.rhs.data(), R, s);
+      relax_sliced(op, sc, L.correction.data(), d, r.prm.iters, s);
+      DArray<T> next(R, s);
+      HIP_CHECK(hipMemcpyAsync(next.data(), L.correction.data(), sizeof(T) * R, hipMemcpyDeviceToDevice, s));
+      hipLaunchKernelGGL(k_spmv<T>, dim3(blocks_for(R, 256)), dim3(256), 0, s, (const int*)L.P.cols.data(),
+                         (const T*)L.P.vals.data(), R, 8, (const T*)cumulative.data(), next.data(), T(1), T(1));
+      KERNEL_CHECK();
+      cumulative = std::move(next);
+    }
+    L.solved = true;
+  }
+}
+
+// CSR -> slot layout in entry order (GS only iterates present slots in order)
+template <class T>
+__global__ void k_csr_to_slots(const int* __restrict__ offsets, const int* __restrict__ columns, const T* __restrict__ values,
+                               int R, int* __restrict__ cols, T* __restrict__ vals, int* __restrict__ too_long) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= R) return;
+  const int o0 = offsets[i], o1 = offsets[i + 1];
+  if (o1 - o0 > 27) *too_long = 1;
+  for (int k = 0; k < 27; ++k) {
+    const int e = o0 + k;
+    cols[(int64_t)k * R + i] = e < o1 ? columns[e] : -1;
+    vals[(int64_t)k * R + i] = e < o1 ? values[e] : T(0);
+  }
+}
+
+template <class T>
+void relax_sliced_csr(const int64_t* keys, int R, const int* offsets, const int* columns, const T* values, const T* b, T* x,
+                      int depth, int iters, hipStream_t s) {
+  if (R <= 0) return;
+  DArray<int64_t> k(R, s);
+  HIP_CHECK(hipMemcpyAsync(k.data(), keys, sizeof(int64_t) * R, hipMemcpyDeviceToDevice, s));
+  DArray<int> cols(27 * (size_t)R, s), flag(1, s);
+  DArray<T> vals(27 * (size_t)R, s);
+  flag.zero();
+  hipLaunchKernelGGL(k_csr_to_slots<T>, dim3(blocks_for(R, 256)), dim3(256), 0, s, offsets, columns, values, R, cols.data(),
+                     vals.data(), flag.data());
+  KERNEL_CHECK();
+  if (read_int(flag.data(), s)) throw PsrError(2, "relaxation rows hold at most 27 entries");
+  Schedule<T> sc = make_schedule<T>(k, R, depth, true, s);
+  GsOperator<T> op = gather_operator<T>(sc, cols, vals, b, R, s);
+  relax_sliced(op, sc, x, depth, iters, s);
+}
+
+template void relax_sliced_csr<float>(const int64_t*, int, const int*, const int*, const float*, const float*, float*, int, int,
+                                      hipStream_t);
+template void relax_sliced_csr<double>(const int64_t*, int, const int*, const int*, const double*, const double*, double*, int,
+                                       int, hipStream_t);
+
+template void stage_solve<float>(Result<float>&);
+template void stage_solve<double>(Result<double>&);
+
+}  // namespace psr

@@ -1,0 +1,12 @@
+"""MI355X screened Poisson surface reconstruction with warp-psr's API.
+
+``reconstruct(points, normals, depth=..., ...)`` takes (n, 3) float32/float64
+PyTorch tensors on one ROCm device and returns a :class:`Reconstruction` with
+``iso_value``, ``evaluate(queries)`` and ``device_field()``, like
+``warp_psr.reconstruct`` (src/warp_psr/reconstruction.py:159-371).  All compute
+runs in hand-written HIP kernels of ``libwarp_psr_amd.so`` through its C ABI;
+there is no CPU fallback.
+"""
+from .reconstruction import reconstruct, Reconstruction, LevelField, DeviceField
+
+__all__ = ["reconstruct", "Reconstruction", "LevelField", "DeviceField"]

@@ -1,0 +1,88 @@
+"""ctypes binding of libwarp_psr_amd.so (the C ABI in include/warp_psr_amd.h).
+
+The library is required: importing the compute entry points without it raises,
+there is no CPU fallback.
+"""
+import ctypes
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libwarp_psr_amd.so"
+
+EXPORTED = (
+    "psr_params_default", "psr_reconstruct", "psr_evaluate", "psr_result_iso", "psr_result_field",
+    "psr_result_metadata_json", "psr_result_array", "psr_result_free", "psr_relax_sliced", "psr_last_error",
+    "psr_release_cache",
+)
+
+
+class PsrParams(ctypes.Structure):
+    _fields_ = [
+        ("depth", ctypes.c_int), ("full_depth", ctypes.c_int), ("base_depth", ctypes.c_int),
+        ("solve_depth", ctypes.c_int), ("kernel_depth", ctypes.c_int), ("samples_per_node", ctypes.c_double),
+        ("scale", ctypes.c_double), ("point_weight", ctypes.c_double), ("exact_interpolation", ctypes.c_int),
+        ("iters", ctypes.c_int), ("base_v_cycles", ctypes.c_int), ("cg_accuracy", ctypes.c_double),
+        ("keep_intermediates", ctypes.c_int),
+    ]
+
+
+class PsrField(ctypes.Structure):
+    _fields_ = [
+        ("keys", ctypes.c_void_p), ("coefficients", ctypes.c_void_p), ("cumulative", ctypes.c_void_p),
+        ("level_offsets", ctypes.c_void_p), ("count", ctypes.c_int64), ("depth", ctypes.c_int),
+        ("dtype", ctypes.c_int), ("linear", ctypes.c_double * 9), ("translation", ctypes.c_double * 3),
+        ("iso_value", ctypes.c_double),
+    ]
+
+
+class LibraryMissing(RuntimeError):
+    pass
+
+
+_LIB = None
+
+
+def load():
+    """Load the HIP library; raise LibraryMissing when it has not been built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not LIB_PATH.exists():
+        raise LibraryMissing(f"{LIB_PATH} is not built; run __graft_entry__.build() (no CPU fallback exists)")
+    lib = ctypes.CDLL(str(LIB_PATH))
+    vp, i64, c_int, c_double = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_double
+    lib.psr_params_default.argtypes = [ctypes.POINTER(PsrParams)]
+    lib.psr_params_default.restype = None
+    lib.psr_reconstruct.argtypes = [vp, vp, i64, c_int, ctypes.POINTER(PsrParams), vp, ctypes.POINTER(vp)]
+    lib.psr_reconstruct.restype = c_int
+    lib.psr_evaluate.argtypes = [vp, vp, i64, vp, vp, vp, vp]
+    lib.psr_evaluate.restype = c_int
+    lib.psr_result_iso.argtypes = [vp, ctypes.POINTER(c_double)]
+    lib.psr_result_iso.restype = c_int
+    lib.psr_result_field.argtypes = [vp, ctypes.POINTER(PsrField)]
+    lib.psr_result_field.restype = c_int
+    lib.psr_result_metadata_json.argtypes = [vp, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
+    lib.psr_result_metadata_json.restype = c_int
+    lib.psr_result_array.argtypes = [vp, ctypes.c_char_p, c_int, ctypes.POINTER(vp), ctypes.POINTER(i64)]
+    lib.psr_result_array.restype = c_int
+    lib.psr_result_free.argtypes = [vp]
+    lib.psr_result_free.restype = None
+    lib.psr_relax_sliced.argtypes = [c_int, vp, i64, vp, vp, vp, vp, vp, c_int, c_int, vp]
+    lib.psr_relax_sliced.restype = c_int
+    lib.psr_last_error.argtypes = []
+    lib.psr_last_error.restype = ctypes.c_char_p
+    lib.psr_release_cache.argtypes = []
+    lib.psr_release_cache.restype = None
+    _LIB = lib
+    return lib
+
+
+def check(rc):
+    """Map C return codes to the reference's exception types."""
+    if rc == 0:
+        return
+    message = load().psr_last_error().decode()
+    if rc == 2:
+        raise ValueError(message)
+    if rc == 3:
+        raise TypeError(message)
+    raise RuntimeError(message)

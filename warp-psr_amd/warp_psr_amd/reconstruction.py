@@ -1,0 +1,224 @@
+"""Host-side mirror of warp-psr's public API over the C ABI.
+
+Reference: src/warp_psr/reconstruction.py:141-371 (reconstruct, Reconstruction),
+src/warp_psr/field.py:122-184 (LevelField), src/warp_psr/device_field.py:18-142
+(device handles).  Argument names, defaults, validation order and ValueError
+messages follow the reference; inputs are PyTorch tensors instead of Warp arrays.
+"""
+import ctypes
+import json
+import math
+from dataclasses import dataclass, field as dc_field
+
+import numpy as np
+
+from . import _lib
+
+_ARRAY_TYPES = {
+    "input_status": np.int32, "input_group_indices": np.int32, "input_order": np.int32, "first_indices": np.int32,
+    "group_offsets": np.int32, "normal_offsets": np.int32, "matrix_offsets": np.int32, "matrix_columns": np.int32,
+    "cross_offsets": np.int32, "cross_columns": np.int32, "prolong_offsets": np.int32, "prolong_columns": np.int32,
+    "cg_runs": np.int32, "sample_keys": np.int64, "aggregate_keys": np.int64, "density_keys": np.int64,
+    "normal_keys": np.int64, "tree_keys": np.int64, "basis_keys": np.int64, "keys": np.int64,
+    "normal_load": np.float64, "point_load": np.float64,
+}
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _stream_handle(device):
+    torch = _torch()
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+@dataclass
+class DeviceField:
+    """Zero-copy device view of a field (reference device_field.py:375-386).
+
+    ``handle`` is the C ``PsrField`` struct; HIP kernels can evaluate it with
+    ``psr_evaluate_field`` from include/warp_psr_amd_field.h.  Keep the owning
+    Reconstruction alive and unmodified while kernels use it.
+    """
+    handle: _lib.PsrField
+    owner: object = dc_field(repr=False)
+
+    @property
+    def depth(self):
+        return self.handle.depth
+
+    @property
+    def count(self):
+        return self.handle.count
+
+
+class LevelField:
+    """Per-level sparse degree-1 field (reference field.py:122-184)."""
+
+    def __init__(self, result, dtype, device):
+        self._result = result
+        self.dtype = dtype
+        self.device = device
+        field = _lib.PsrField()
+        _lib.check(_lib.load().psr_result_field(result._handle, ctypes.byref(field)))
+        self._field = field
+        self.depth = field.depth
+        self.iso_value = field.iso_value
+        self.linear = tuple(field.linear)
+        self.translation = tuple(field.translation)
+
+    def evaluate(self, queries, *, requires_grad=False):
+        """Values, model-space gradients and validity; NaN outside the cube."""
+        torch = _torch()
+        if requires_grad:
+            raise ValueError("tracked cumulative fields require reconstruct_differentiable")
+        if (not isinstance(queries, torch.Tensor) or queries.dtype != self.dtype or queries.device != self.device
+                or queries.ndim != 2 or queries.shape[1] != 3):
+            raise ValueError("Query precision/device must match coefficients")
+        q = queries.contiguous()
+        m = q.shape[0]
+        values = torch.empty(m, dtype=self.dtype, device=self.device)
+        gradients = torch.empty((m, 3), dtype=self.dtype, device=self.device)
+        valid = torch.empty(m, dtype=torch.int32, device=self.device)
+        if m:
+            _lib.check(_lib.load().psr_evaluate(self._result._handle, q.data_ptr(), m, values.data_ptr(),
+                                                gradients.data_ptr(), valid.data_ptr(), _stream_handle(self.device)))
+        return values, gradients, valid
+
+    def device_field(self):
+        return DeviceField(self._field, self._result)
+
+
+class _ResultHandle:
+    def __init__(self, handle):
+        self._handle = handle
+
+    def __del__(self):
+        if getattr(self, "_handle", None):
+            try:
+                _lib.load().psr_result_free(self._handle)
+            except Exception:
+                pass
+            self._handle = None
+
+
+class Reconstruction:
+    """Reference Reconstruction (reconstruction.py:141-156)."""
+
+    def __init__(self, handle, dtype, device, params):
+        self._owner = _ResultHandle(handle)
+        self.field = LevelField(self._owner, dtype, device)
+        lib = _lib.load()
+        need = ctypes.c_size_t()
+        lib.psr_result_metadata_json(handle, None, 0, ctypes.byref(need))
+        buf = ctypes.create_string_buffer(need.value)
+        lib.psr_result_metadata_json(handle, buf, need.value, None)
+        self.metadata = json.loads(buf.value.decode())
+        self.metadata["device"] = str(device)
+        self.diagnostics = _Diagnostics(self._owner, dtype, params)
+
+    @property
+    def iso_value(self):
+        return self.field.iso_value
+
+    def evaluate(self, queries):
+        return self.field.evaluate(queries)
+
+    def device_field(self):
+        """Zero-copy handle for psr_evaluate_field in HIP kernels."""
+        return self.field.device_field()
+
+
+class _Diagnostics:
+    """Host copies of stage arrays (reference ``Reconstruction.diagnostics``)."""
+
+    def __init__(self, owner, dtype, params):
+        self._owner = owner
+        self._np_dtype = np.float32 if str(dtype).endswith("float32") else np.float64
+        self.params = params
+
+    def array(self, name, level=-1):
+        data = ctypes.c_void_p()
+        nbytes = ctypes.c_int64()
+        _lib.check(_lib.load().psr_result_array(self._owner._handle, name.encode(), level, ctypes.byref(data),
+                                                ctypes.byref(nbytes)))
+        dt = _ARRAY_TYPES.get(name, self._np_dtype)
+        count = nbytes.value // np.dtype(dt).itemsize
+        if not count:
+            return np.zeros(0, dtype=dt)
+        buf = (ctypes.c_char * nbytes.value).from_address(data.value)
+        return np.frombuffer(buf, dtype=dt, count=count).copy()
+
+
+def reconstruct(positions, normals, *, depth=6, full_depth=3, base_depth=None, solve_depth=None, kernel_depth=None,
+                samples_per_node=1.5, scale=1.1, point_weight=2.0, exact_interpolation=False, iters=8, base_v_cycles=1,
+                cg_accuracy=1e-3):
+    """Reconstruct oriented (n, 3) float32/float64 ROCm tensors; point_weight=0 selects PSR.
+
+    Degree one, Neumann boundaries, confidence disabled, library alignDir=0.
+    The surface is ``result.evaluate(q)[0] == result.iso_value``.
+    """
+    torch = _torch()
+    if base_depth is None:
+        base_depth = full_depth
+    if solve_depth is None:
+        solve_depth = depth
+    if kernel_depth is None:
+        kernel_depth = max(depth - 2, 0)
+    depths = (depth, full_depth, base_depth, solve_depth, kernel_depth)
+    if any(not isinstance(v, int) or isinstance(v, bool) for v in depths):
+        raise ValueError("Depth parameters must be integers")
+    if not 0 <= base_depth <= full_depth <= solve_depth <= depth <= 15 or depth < 1 or not 0 <= kernel_depth <= depth:
+        raise ValueError("Require 0 <= base_depth <= full_depth <= solve_depth <= depth <= 15, depth >= 1, "
+                         "and 0 <= kernel_depth <= depth")
+    if depth - full_depth > 7:
+        raise ValueError("Warp 1.19 AdaptiveNanogrid supports at most 8 integration levels; require depth - full_depth <= 7")
+    if not isinstance(iters, int) or iters < 1 or not isinstance(base_v_cycles, int) or base_v_cycles < 1:
+        raise ValueError("iters and base_v_cycles must be positive integers")
+    if not math.isfinite(point_weight) or point_weight < 0 or not math.isfinite(cg_accuracy) or cg_accuracy <= 0:
+        raise ValueError("Require finite nonnegative point_weight and positive cg_accuracy")
+    if not math.isfinite(samples_per_node) or samples_per_node <= 0 or not math.isfinite(scale) or scale <= 1:
+        raise ValueError("Require finite positive samples_per_node and scale > 1")
+    if (not isinstance(positions, torch.Tensor) or not isinstance(normals, torch.Tensor)
+            or positions.dtype not in (torch.float32, torch.float64) or normals.dtype != positions.dtype
+            or positions.device.type != "cuda" or normals.device != positions.device
+            or positions.ndim != 2 or positions.shape[1] != 3 or normals.shape != positions.shape):
+        raise ValueError("Reconstruction requires matching float32/float64 3D vectors on the same CUDA device")
+    if not positions.shape[0]:
+        raise ValueError("points and normals must have equal nonempty 1D shapes")
+    p = positions.contiguous()
+    n = normals.contiguous()
+    params = _lib.PsrParams()
+    lib = _lib.load()
+    lib.psr_params_default(ctypes.byref(params))
+    params.depth, params.full_depth, params.base_depth = depth, full_depth, base_depth
+    params.solve_depth, params.kernel_depth = solve_depth, kernel_depth
+    params.samples_per_node, params.scale, params.point_weight = samples_per_node, scale, point_weight
+    params.exact_interpolation = int(bool(exact_interpolation))
+    params.iters, params.base_v_cycles, params.cg_accuracy = iters, base_v_cycles, cg_accuracy
+    handle = ctypes.c_void_p()
+    dtype_code = 1 if p.dtype == torch.float64 else 0
+    with torch.cuda.device(p.device):
+        _lib.check(lib.psr_reconstruct(p.data_ptr(), n.data_ptr(), p.shape[0], dtype_code, ctypes.byref(params),
+                                       _stream_handle(p.device), ctypes.byref(handle)))
+    resolved = dict(depth=depth, full_depth=full_depth, base_depth=base_depth, solve_depth=solve_depth,
+                    kernel_depth=kernel_depth, samples_per_node=samples_per_node, scale=scale,
+                    point_weight=point_weight, exact_interpolation=bool(exact_interpolation), iters=iters,
+                    base_v_cycles=base_v_cycles, cg_accuracy=cg_accuracy)
+    return Reconstruction(handle.value, p.dtype, p.device, resolved)
+
+
+def relax_sliced(keys, offsets, columns, values, b, x, depth, iters):
+    """Sliced Gauss-Seidel on device CSR tensors in the reference order (relaxation.py:189-210); updates x."""
+    torch = _torch()
+    for t in (keys, offsets, columns, values, b, x):
+        if not isinstance(t, torch.Tensor) or t.device.type != "cuda" or not t.is_contiguous():
+            raise ValueError("relax_sliced requires contiguous ROCm tensors")
+    dtype = 1 if values.dtype == torch.float64 else 0
+    with torch.cuda.device(x.device):
+        _lib.check(_lib.load().psr_relax_sliced(dtype, keys.data_ptr(), keys.shape[0], offsets.data_ptr(),
+                                                columns.data_ptr(), values.data_ptr(), b.data_ptr(), x.data_ptr(),
+                                                depth, iters, _stream_handle(x.device)))
+    return x
