@@ -1,0 +1,169 @@
+#!/usr/bin/env python3
+"""Depth-9 screened Poisson reconstruction throughput on one MI355X.
+
+Workload: 800,000 points on a smooth union of six seeded random ellipsoids
+(semi-axes 0.15-0.45), sampling density varying 10x, position noise 0.001,
+exact analytic normals; depth 9, full_depth 4, point_weight 2.0, iters 8.
+A step is one complete reconstruct() (inputs already resident in HBM) of the
+float32 path (float64 operator accumulation, float64 isovalue); the float64
+path is timed the same way and both are evaluated at 812,000 queries (every
+input plus 12,000 points straddling refinement interfaces) for the field error.
+
+Prints one JSON line (metric/value per the benchmark contract) with a
+``roofline`` object for the sliced Gauss-Seidel relaxation of the finest level
+and a ``cpu_baseline`` object (the serial CPU oracle on a bounded sample).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "warp-psr_amd"))
+
+METRIC = "Mpts/s SPSR depth-9 mixed vs FP64 at field err<=1e-6; relax HBM GB/s"
+HBM_PEAK_GBS = 8000.0
+PARAMS = dict(depth=9, full_depth=4, point_weight=2.0)
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def relax_roofline(metadata, value_bytes):
+    """Algorithmic bytes of the finest sliced relaxation over its launch span.
+
+    Per sweep: nnz*(4 + s) (column index + value) + (R+1)*4 (row extent) +
+    R*s (b) + 2*R*s (x read and written), s = value bytes; times the sweeps.
+    """
+    stats = metadata["relaxation"]
+    fine = max(stats, key=lambda s: s["rows"])
+    s = value_bytes
+    per_sweep = fine["nnz"] * (4 + s) + (fine["rows"] + 1) * 4 + fine["rows"] * s + 2 * fine["rows"] * s
+    total = per_sweep * fine["sweeps"]
+    achieved = total / fine["seconds"] / 1e9
+    return {
+        "kernel": "k_relax_wave (sliced Gauss-Seidel, finest level)",
+        "bound": "hbm",
+        "achieved": round(achieved, 2),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 5),
+        "traffic": None,
+        "depth": fine["depth"],
+        "rows": fine["rows"],
+        "nnz": fine["nnz"],
+        "launches": fine["launches"],
+        "algorithmic_bytes": total,
+        "seconds": fine["seconds"],
+        "avg_launch_us": round(fine["seconds"] / fine["launches"] * 1e6, 3),
+    }
+
+
+def cpu_baseline(points, normals, sample):
+    import oracle  # the checker; timed here as the CPU reference path
+    p = points[:sample]
+    n = normals[:sample]
+    t = time.perf_counter()
+    oracle.reconstruct(p, n, **PARAMS)
+    dt = time.perf_counter() - t
+    return {
+        "value": round(sample / dt / 1e6, 6),
+        "unit": "Mpts/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"first {sample:,} of the {len(points):,}-point cloud, same depth-9 parameters, "
+                  f"serial C++ oracle, {dt:.2f} s",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--points", type=int, default=800000)
+    ap.add_argument("--cpu-sample", type=int, default=50000)
+    ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--skip-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the build targets single-GPU "
+              "reconstruction; see DESIGN for how the path would shard)", file=sys.stderr)
+        return 2
+
+    import torch
+    from warp_psr_amd import reconstruct, synthetic
+
+    t0 = time.perf_counter()
+    p32, n32 = synthetic.workload_ellipsoids(args.points, dtype=np.float32)
+    log(f"workload generated in {time.perf_counter() - t0:.1f}s")
+    p64, n64 = p32.astype(np.float64), n32.astype(np.float64)
+    dev = torch.device("cuda:0")
+    tp32, tn32 = torch.from_numpy(p32).to(dev), torch.from_numpy(n32).to(dev)
+    tp64, tn64 = torch.from_numpy(p64).to(dev), torch.from_numpy(n64).to(dev)
+    torch.cuda.synchronize()
+
+    def timed(tp, tn, steps, warmup, label):
+        for _ in range(warmup):
+            r = reconstruct(tp, tn, **PARAMS)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(steps):
+            r = reconstruct(tp, tn, **PARAMS)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t) / steps
+        log(f"{label}: {dt * 1e3:.1f} ms/step, {len(tp) / dt / 1e6:.3f} Mpts/s; stages {r.metadata['timings_seconds']}")
+        return r, dt
+
+    r32, dt32 = timed(tp32, tn32, args.steps, args.warmup, "float32")
+    out = {
+        "metric": METRIC,
+        "value": round(args.points / dt32 / 1e6, 4),
+        "unit": "Mpts/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt32 * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "six-ellipsoid union, 800,000 points, 10x density variation, noise 0.001, "
+                               "depth 9, full_depth 4, point_weight 2.0, iters 8 (SPSR)",
+                   "points": args.points, "depth": 9, "full_depth": 4, "point_weight": 2.0,
+                   "basis_counts": r32.metadata["basis_counts"]},
+    }
+    out["roofline"] = relax_roofline(r32.metadata, 4)
+    out["stage_seconds"] = r32.metadata["timings_seconds"]
+    if not args.skip_fp64:
+        r64, dt64 = timed(tp64, tn64, max(1, args.steps // 2), 1, "float64")
+        keys = r32.diagnostics.array("keys")
+        iq = synthetic.interface_queries(keys, 9, r32.diagnostics.array("model_to_cube"),
+                                         r32.diagnostics.array("cube_to_model"), p32)
+        q = np.concatenate([p32, iq])
+        v32, _, ok32 = r32.evaluate(torch.from_numpy(q).to(dev))
+        v64, _, ok64 = r64.evaluate(torch.from_numpy(q.astype(np.float64)).to(dev))
+        both = (ok32.bool() & ok64.bool()).cpu().numpy()
+        err = np.abs(v32.cpu().numpy().astype(np.float64) - v64.cpu().numpy())[both]
+        iso_err = abs(r32.iso_value - r64.iso_value)
+        out["fp64"] = {"Mpts_s": round(args.points / dt64 / 1e6, 4), "ms_per_step": round(dt64 * 1e3, 3),
+                       "speedup_f32_over_f64": round(dt64 / dt32, 3),
+                       "relax_roofline": relax_roofline(r64.metadata, 8)}
+        out["field_error_vs_fp64"] = {"queries": int(len(q)), "max_abs": float(err.max()),
+                                      "centered_max_abs": float(np.abs((v32.cpu().numpy().astype(np.float64) - r32.iso_value)
+                                                                       - (v64.cpu().numpy() - r64.iso_value))[both].max()),
+                                      "iso_abs": iso_err, "within_1e-6": bool(err.max() <= 1e-6)}
+    if not args.skip_cpu:
+        out["cpu_baseline"] = cpu_baseline(p32, n32, args.cpu_sample)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1720,3 +1720,5 @@ void oracle_normal_integral(int test_depth, int test_index, int normal_depth, in
 }
 
 }  // extern "C"
+
+extern "C" double oracle_det_log(double x) { return det_log(x); }

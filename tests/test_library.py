@@ -1,0 +1,101 @@
+"""Host-side checks of the C ABI and the Python mirror (CPU only, no GPU compute).
+
+* libwarp_psr_amd.so loads and exports every function include/*.h declares.
+* Parameter validation happens before any device work and reproduces the
+  reference's ValueError messages (src/warp_psr/reconstruction.py:181-234,
+  tests/test_reconstruction.py:36-39).
+"""
+import ctypes
+import re
+from pathlib import Path
+
+import pytest
+
+from warp_psr_amd import _lib
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def declared_functions():
+    names = []
+    for header in sorted((ROOT / "include").glob("*.h")):
+        text = re.sub(r"/\*.*?\*/", "", header.read_text(), flags=re.S)
+        text = re.sub(r"//[^\n]*", "", text)
+        names += re.findall(r"\b(psr_[a-z_]+)\s*\(", text)
+    return sorted(set(n for n in names if not n.startswith("psr_evaluate_field")))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    declared = declared_functions()
+    assert "psr_reconstruct" in declared and "psr_relax_sliced" in declared
+    for name in declared:
+        assert hasattr(lib, name), f"{name} declared in include/ but not exported"
+    assert set(declared) == set(_lib.EXPORTED)
+
+
+def _params(**kw):
+    p = _lib.PsrParams()
+    _lib.load().psr_params_default(ctypes.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def test_params_default():
+    p = _params()
+    assert (p.depth, p.full_depth, p.base_depth, p.solve_depth, p.kernel_depth) == (6, 3, -1, -1, -1)
+    assert (p.samples_per_node, p.scale, p.point_weight, p.iters, p.base_v_cycles, p.cg_accuracy) == (1.5, 1.1, 2.0, 8, 1, 1e-3)
+
+
+@pytest.mark.parametrize("kw,message", [
+    (dict(depth=16), "Require 0 <= base_depth <= full_depth"),
+    (dict(depth=6, full_depth=7), "Require 0 <= base_depth <= full_depth"),
+    (dict(depth=12, full_depth=3), "at most 8 integration levels"),
+    (dict(iters=0), "iters and base_v_cycles must be positive integers"),
+    (dict(point_weight=-1.0), "Require finite nonnegative point_weight"),
+    (dict(cg_accuracy=0.0), "positive cg_accuracy"),
+    (dict(scale=1.0), "scale > 1"),
+    (dict(samples_per_node=float("nan")), "samples_per_node"),
+])
+def test_c_abi_rejects_invalid_parameters_before_device_work(kw, message):
+    lib = _lib.load()
+    out = ctypes.c_void_p()
+    rc = lib.psr_reconstruct(None, None, 1, 0, ctypes.byref(_params(**kw)), None, ctypes.byref(out))
+    assert rc == 2
+    assert message in lib.psr_last_error().decode()
+    with pytest.raises(ValueError, match=re.escape(message)):
+        _lib.check(rc)
+
+
+def test_c_abi_rejects_empty_and_bad_dtype():
+    lib = _lib.load()
+    out = ctypes.c_void_p()
+    assert lib.psr_reconstruct(None, None, 0, 0, ctypes.byref(_params()), None, ctypes.byref(out)) == 2
+    assert "nonempty" in lib.psr_last_error().decode()
+    assert lib.psr_reconstruct(None, None, 4, 7, ctypes.byref(_params()), None, ctypes.byref(out)) == 2
+    assert lib.psr_relax_sliced(5, None, 1, None, None, None, None, None, 3, 8, None) == 3
+
+
+def test_python_validation_messages():
+    import torch
+    from warp_psr_amd import reconstruct
+    p = torch.zeros((4, 3))
+    with pytest.raises(ValueError, match="Depth parameters must be integers"):
+        reconstruct(p, p, depth=6.0)
+    with pytest.raises(ValueError, match="Require 0 <= base_depth"):
+        reconstruct(p, p, depth=6, full_depth=7)
+    with pytest.raises(ValueError, match="iters and base_v_cycles"):
+        reconstruct(p, p, iters=0)
+    # CPU tensors are rejected like non-CUDA Warp arrays (reconstruction.py:226-234)
+    with pytest.raises(ValueError, match="matching float32/float64 3D vectors on the same CUDA device"):
+        reconstruct(p, p)
+    with pytest.raises(ValueError, match="matching float32/float64"):
+        reconstruct(p, p.double())
+
+
+def test_metadata_parser_shapes():
+    """The C ABI's metadata JSON keys used by bench.py exist in the source (no device needed)."""
+    text = (ROOT / "warp-psr_amd" / "csrc" / "psr_capi.hip").read_text()
+    for key in ("basis_counts", "timings_seconds", "relaxation", "cube_to_model", "parameters", "iso_value"):
+        assert f'\\"{key}\\"' in text

@@ -84,6 +84,11 @@ template <class T> Result<T>* run(const T* pts, const T* nrm, int64_t n, const P
       HIP_CHECK(hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
       r->timings.push_back({names[k], ms / 1000.f});
     }
+    for (auto& st : r->relax_stats) {
+      float ms = 0.f;
+      HIP_CHECK(hipEventElapsedTime(&ms, st.start, st.end));
+      st.seconds = ms / 1000.f;
+    }
   } catch (...) {
     for (auto& e : ev) (void)hipEventDestroy(e);
     throw;
@@ -190,6 +195,12 @@ template <class T> std::string metadata_json(const Result<T>& r) {
     << ", \"base_v_cycles\": " << p.base_v_cycles << ", \"cg_accuracy\": " << p.cg_accuracy << "}";
   o << ", \"root_cg_cycles\": [";
   for (size_t k = 0; k < r.cg_runs.size(); ++k) o << (k ? ", " : "") << r.cg_runs[k];
+  o << "], \"relaxation\": [";
+  for (size_t k = 0; k < r.relax_stats.size(); ++k) {
+    const auto& st = r.relax_stats[k];
+    o << (k ? ", " : "") << "{\"depth\": " << st.depth << ", \"rows\": " << st.rows << ", \"nnz\": " << st.nnz
+      << ", \"launches\": " << st.launches << ", \"sweeps\": " << p.iters << ", \"seconds\": " << st.seconds << "}";
+  }
   o << "], \"profile\": \"pinned cascadic degree-1 Neumann\", \"constant_mode\": \""
     << ((p.base_depth == 0 && p.point_weight == 0) ? "reference root mean term" : "reference solve without an added mean term")
     << "\", \"exact_interpolation_behavior\": \""

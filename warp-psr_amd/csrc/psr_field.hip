@@ -236,6 +236,9 @@ void evaluate_field(const Result<T>& r, const T* queries, int64_t m, const T* li
 template <class T> void stage_field(Result<T>& r) {
   hipStream_t s = r.s;
   const int D = r.prm.depth;
+  r.join_average();
+  read_bytes(r.normal_average_host, r.normal_average.data(), sizeof(T) * 3, s);
+  r.average_contrib.free();
   r.level_offsets.assign(D + 2, 0);
   for (int d = 0; d <= D; ++d) r.level_offsets[d + 1] = r.level_offsets[d] + r.levels[d].basis_count;
   const int64_t K = r.level_offsets[D + 1];

@@ -78,6 +78,17 @@ template <class T> struct Result {
   DArray<T> normal_average;  // 3 values on device
   T normal_average_host[3];
   DArray<int> normal_order;
+  // The serial normal average runs on a side stream, overlapped with the
+  // tree stage; its input stays alive until the main stream joins it.
+  hipStream_t side = nullptr;
+  hipEvent_t average_done = nullptr;
+  bool average_pending = false;
+  DArray<T> average_contrib;
+  void join_average() {
+    if (!average_pending) return;
+    HIP_CHECK(hipStreamWaitEvent(s, average_done, 0));
+    average_pending = false;
+  }
   // tree + solve per level
   std::vector<LevelState<T>> levels;
   std::vector<int> cg_runs;
@@ -89,6 +100,30 @@ template <class T> struct Result {
   T cube_to_model[16], model_to_cube[16];
   // timings (seconds) per stage, from HIP events
   std::vector<std::pair<std::string, float>> timings;
+  // sliced relaxation per finer level: rows, stored nonzeros, kernel launches
+  // and the HIP-event span of its launch sequence on the stream
+  struct RelaxStat {
+    int depth, rows;
+    int64_t nnz;
+    int launches;
+    hipEvent_t start, end;
+    float seconds;
+  };
+  std::vector<RelaxStat> relax_stats;
+  ~Result() {
+    if (average_done) {
+      (void)hipEventSynchronize(average_done);
+      (void)hipEventDestroy(average_done);
+    }
+    if (side) {
+      (void)hipStreamSynchronize(side);
+      (void)hipStreamDestroy(side);
+    }
+    for (auto& st : relax_stats) {
+      (void)hipEventDestroy(st.start);
+      (void)hipEventDestroy(st.end);
+    }
+  }
 };
 
 // stage entry points (one translation unit each)

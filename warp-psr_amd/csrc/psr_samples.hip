@@ -589,25 +589,36 @@ __global__ void k_average_contributions(const int* __restrict__ order, int G, co
   out[3 * i] = v0; out[3 * i + 1] = v1; out[3 * i + 2] = v2;
 }
 
+// One component chain per wave (three waves on three SIMDs).  A skipped
+// contribution is exactly (0,0,0); the running sums start at +0 and never
+// become -0, so adding it is the identity and the reference's `> 0` test
+// (normals.py:160-163) needs no branch.  Each wave loads 64 contributions
+// and lane 0's chain reads them with readlane, in order.
+template <class T> __device__ __forceinline__ T lane_value(T v, int lane);
+template <> __device__ __forceinline__ float lane_value<float>(float v, int lane) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+template <> __device__ __forceinline__ double lane_value<double>(double v, int lane) {
+  const long long bits = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(bits & 0xffffffff), lane);
+  const int hi = __builtin_amdgcn_readlane((int)(bits >> 32), lane);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
 template <class T>
-__global__ void __launch_bounds__(256) k_average_serial(const T* __restrict__ contrib, int G, T* __restrict__ out) {
-  constexpr int CHUNK = 1024;
-  __shared__ T buf[3 * CHUNK];
-  T t0 = T(0), t1 = T(0), t2 = T(0);
-  for (int begin = 0; begin < G; begin += CHUNK) {
-    const int cnt = imin(CHUNK, G - begin);
-    for (int k = threadIdx.x; k < 3 * cnt; k += blockDim.x) buf[k] = contrib[3 * (int64_t)begin + k];
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      for (int j = 0; j < cnt; ++j) {
-        if (buf[3 * j + 2] > T(0)) {
-          t0 = t0 + buf[3 * j]; t1 = t1 + buf[3 * j + 1]; t2 = t2 + buf[3 * j + 2];
-        }
-      }
-    }
-    __syncthreads();
+__global__ void __launch_bounds__(192) k_average_serial(const T* __restrict__ contrib, int G, T* __restrict__ out) {
+  __shared__ T totals[3];
+  const int comp = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  T acc = T(0);
+  for (int base = 0; base < G; base += 64) {
+    const T v = base + lane < G ? contrib[3 * (int64_t)(base + lane) + comp] : T(0);
+#pragma unroll
+    for (int j = 0; j < 64; ++j) acc = acc + lane_value(v, j);
   }
+  if (lane == 0) totals[comp] = acc;
+  __syncthreads();
   if (threadIdx.x == 0) {
+    const T t0 = totals[0], t1 = totals[1], t2 = totals[2];
     out[0] = out[1] = out[2] = T(0);
     if (t2 > T(0)) {
       out[0] = t0 / t2;
@@ -660,15 +671,28 @@ template <class T> void stage_normals(Result<T>& r) {
                        maximum, r.normal_keys.data(), r.normal_vals.data());
     KERNEL_CHECK();
   }
-  // average (normals.py:221-227)
-  DArray<T> contrib(3 * (size_t)G, s);
+  // average (normals.py:221-227): contributions here, the serial chain on a
+  // side stream (joined before the first use of the average)
+  r.average_contrib.reset(3 * (size_t)G, s);
   r.normal_average.reset(3, s);
   hipLaunchKernelGGL(k_average_contributions<T>, dim3(blocks_for(G, 256)), dim3(256), 0, s,
                      (const int*)r.normal_order.data(), G, (const T*)r.normal_sums.data(), (const T*)r.weights.data(),
-                     (const T*)r.dw.data(), contrib.data());
+                     (const T*)r.dw.data(), r.average_contrib.data());
   KERNEL_CHECK();
-  hipLaunchKernelGGL(k_average_serial<T>, dim3(1), dim3(256), 0, s, (const T*)contrib.data(), G, r.normal_average.data());
+  if (!r.side) HIP_CHECK(hipStreamCreateWithFlags(&r.side, hipStreamNonBlocking));
+  if (!r.average_done) HIP_CHECK(hipEventCreateWithFlags(&r.average_done, hipEventDisableTiming));
+  {
+    hipEvent_t ready;
+    HIP_CHECK(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
+    HIP_CHECK(hipEventRecord(ready, s));
+    HIP_CHECK(hipStreamWaitEvent(r.side, ready, 0));
+    HIP_CHECK(hipEventDestroy(ready));
+  }
+  hipLaunchKernelGGL(k_average_serial<T>, dim3(1), dim3(192), 0, r.side, (const T*)r.average_contrib.data(), G,
+                     r.normal_average.data());
   KERNEL_CHECK();
+  HIP_CHECK(hipEventRecord(r.average_done, r.side));
+  r.average_pending = true;
   // per-depth record offsets (normals.py:17-29, 228-234)
   DArray<int> offsets(maximum + 2, s);
   hipLaunchKernelGGL(k_depth_offsets, dim3(1), dim3(64), 0, s, (const int64_t*)r.normal_keys.data(), (int64_t)count,
@@ -676,7 +700,6 @@ template <class T> void stage_normals(Result<T>& r) {
   KERNEL_CHECK();
   r.normal_offsets.resize(maximum + 2);
   read_bytes(r.normal_offsets.data(), offsets.data(), sizeof(int) * (maximum + 2), s);
-  read_bytes(r.normal_average_host, r.normal_average.data(), sizeof(T) * 3, s);
 }
 
 // samples.py:54-88 (_transform), host scalar code shared by the stages

@@ -444,11 +444,22 @@ template <class T>
 __device__ __forceinline__ void gs_update(int j, int R, const int* __restrict__ order, const int* __restrict__ cols,
                                           const T* __restrict__ vals, const T* __restrict__ b, const T* __restrict__ inv_diag,
                                           const unsigned char* __restrict__ has_diag, T* __restrict__ x) {
+  // All column loads, then all value/x loads in flight, then the subtraction
+  // chain in slot order; absent slots are skipped by a select (no rounding).
+  int c[27];
+#pragma unroll
+  for (int k = 0; k < 27; ++k) c[k] = cols[(int64_t)k * R + j];
+  T v[27], xv[27];
+#pragma unroll
+  for (int k = 0; k < 27; ++k) {
+    v[k] = vals[(int64_t)k * R + j];
+    xv[k] = x[c[k] >= 0 ? c[k] : 0];
+  }
   T r = b[j];
 #pragma unroll
   for (int k = 0; k < 27; ++k) {
-    const int c = cols[(int64_t)k * R + j];
-    if (c >= 0) r = r - x[c] * vals[(int64_t)k * R + j];
+    const T t = r - xv[k] * v[k];
+    r = c[k] >= 0 ? t : r;
   }
   if (has_diag[j]) {
     const int i = order[j];
@@ -702,11 +713,25 @@ template <class T> static void refresh_rhs(GsOperator<T>& op, const Schedule<T>&
   KERNEL_CHECK();
 }
 
-// relaxation.py:128-143 (wavefront_groups) + :189-210
+__global__ void __launch_bounds__(256) k_count_present(const int* __restrict__ cols, int64_t n,
+                                                       unsigned long long* __restrict__ out) {
+  __shared__ unsigned wave_counts[4];
+  unsigned count = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    count += cols[i] >= 0 ? 1u : 0u;
+  for (int o = 32; o > 0; o >>= 1) count += __shfl_down(count, o, 64);
+  if ((threadIdx.x & 63) == 0) wave_counts[threadIdx.x >> 6] = count;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    atomicAdd(out, (unsigned long long)(wave_counts[0] + wave_counts[1] + wave_counts[2] + wave_counts[3]));
+}
+
+// relaxation.py:128-143 (wavefront_groups) + :189-210; returns the launch count
 template <class T>
-void relax_sliced(const GsOperator<T>& op, const Schedule<T>& sc, T* x, int depth, int iters, hipStream_t s) {
+int relax_sliced(const GsOperator<T>& op, const Schedule<T>& sc, T* x, int depth, int iters, hipStream_t s) {
   const int n = 1 << depth;
   const unsigned bx = (unsigned)std::max(1, (sc.max_group + 255) / 256);
+  int launches = 0;
   for (int q = 0; q <= n + 2 * (iters - 1); ++q) {
     const int first = std::max(0, floordiv(q - n + 1, 2));
     const int last = std::min(iters - 1, q / 2);
@@ -718,9 +743,11 @@ void relax_sliced(const GsOperator<T>& op, const Schedule<T>& sc, T* x, int dept
                          (const int*)sc.offsets.data(), (const int*)op.cols.data(), (const T*)op.vals.data(),
                          (const T*)op.b.data(), (const T*)op.inv_diag.data(), (const unsigned char*)op.has_diag.data(), x,
                          op.R, z0, color);
+      ++launches;
     }
   }
   KERNEL_CHECK();
+  return launches;
 }
 
 template <class T>
@@ -762,6 +789,7 @@ template <class T> struct PointSet {
 
 template <class T> static PointSet<T> point_constraints(Result<T>& r, int d, hipStream_t s) {
   PointSet<T> ps;
+  r.join_average();  // the weights use the serial normal average
   LevelState<T>& L = r.levels[d];
   ps.P = (int)L.agg_keys.size();
   ps.pos.reset(3 * (size_t)ps.P, s);
@@ -1038,7 +1066,24 @@ template <class T> void stage_solve(Result<T>& r) {
       KERNEL_CHECK();
       Schedule<T> sc = make_schedule<T>(L.basis_keys, R, d, true, s);
       GsOperator<T> op = gather_operator<T>(sc, L.A.cols, L.A.vals, L.rhs.data(), R, s);
-      relax_sliced(op, sc, L.correction.data(), d, r.prm.iters, s);
+      typename Result<T>::RelaxStat st{d, R, 0, 0, nullptr, nullptr, 0.f};
+      {
+        DArray<unsigned long long> cnt(1, s);
+        cnt.zero();
+        const int64_t slots = 27 * (int64_t)R;
+        const unsigned grid = (unsigned)std::min<int64_t>(1024, (slots + 255) / 256);
+        hipLaunchKernelGGL(k_count_present, dim3(grid), dim3(256), 0, s, (const int*)L.A.cols.data(), slots, cnt.data());
+        KERNEL_CHECK();
+        unsigned long long nnz = 0;
+        read_bytes(&nnz, cnt.data(), sizeof(nnz), s);
+        st.nnz = (int64_t)nnz;
+      }
+      HIP_CHECK(hipEventCreate(&st.start));
+      HIP_CHECK(hipEventCreate(&st.end));
+      HIP_CHECK(hipEventRecord(st.start, s));
+      st.launches = relax_sliced(op, sc, L.correction.data(), d, r.prm.iters, s);
+      HIP_CHECK(hipEventRecord(st.end, s));
+      r.relax_stats.push_back(st);
       DArray<T> next(R, s);
       HIP_CHECK(hipMemcpyAsync(next.data(), L.correction.data(), sizeof(T) * R, hipMemcpyDeviceToDevice, s));
       hipLaunchKernelGGL(k_spmv<T>, dim3(blocks_for(R, 256)), dim3(256), 0, s, (const int*)L.P.cols.data(),
@@ -1080,7 +1125,7 @@ void relax_sliced_csr(const int64_t* keys, int R, const int* offsets, const int*
   if (read_int(flag.data(), s)) throw PsrError(2, "relaxation rows hold at most 27 entries");
   Schedule<T> sc = make_schedule<T>(k, R, depth, true, s);
   GsOperator<T> op = gather_operator<T>(sc, cols, vals, b, R, s);
-  relax_sliced(op, sc, x, depth, iters, s);
+  (void)relax_sliced(op, sc, x, depth, iters, s);
 }
 
 template void relax_sliced_csr<float>(const int64_t*, int, const int*, const int*, const float*, const float*, float*, int, int,

@@ -127,23 +127,23 @@ class EllipsoidUnion:
         # value-noise lattice for the sampling density (Perlin-style smooth weights)
         self.lattice = rng.uniform(0, 1, size=(9, 9, 9))
 
-    def _parts(self, x):
-        local = np.einsum("kji,nkj->nki", self.rot, x[:, None, :] - self.centers[None])  # R^T (x-c)
-        scaled = local / self.axes[None]
-        rad = np.linalg.norm(scaled, axis=2)
-        f = rad - 1.0
-        # gradient of |A^-1 R^T (x-c)| = R A^-2 R^T (x-c) / rad
-        g = np.einsum("kij,nkj->nki", self.rot, scaled / self.axes[None]) / np.maximum(rad, 1e-12)[..., None]
-        return f, g
-
     def value_and_gradient(self, x):
-        f, g = self._parts(x)
+        """Soft-min (sharpness ``smooth``) of |A^-1 R^T (x-c)| - 1 over the ellipsoids, and its gradient."""
         s = self.smooth
+        f = np.empty((len(x), len(self.centers)))
+        g = np.empty((len(x), len(self.centers), 3))
+        for k in range(len(self.centers)):
+            local = (x - self.centers[k]) @ self.rot[k]  # R^T (x - c) as row vectors
+            scaled = local / self.axes[k]
+            rad = np.sqrt((scaled * scaled).sum(axis=1))
+            f[:, k] = rad - 1.0
+            # gradient of |A^-1 R^T (x-c)| = R A^-2 R^T (x-c) / rad
+            g[:, k] = ((scaled / self.axes[k]) @ self.rot[k].T) / np.maximum(rad, 1e-12)[:, None]
         m = f.min(axis=1, keepdims=True)
         w = np.exp(-(f - m) / s)
-        total = w.sum(axis=1, keepdims=True)
-        value = m[:, 0] - s * np.log(total[:, 0])
-        grad = (w[..., None] * g).sum(axis=1) / total
+        total = w.sum(axis=1)
+        value = m[:, 0] - s * np.log(total)
+        grad = np.einsum("nk,nka->na", w, g) / total[:, None]
         return value, grad
 
     def density(self, x):
@@ -161,10 +161,18 @@ class EllipsoidUnion:
         lo, hi = self.lattice.min(), self.lattice.max()
         return 0.1 + 0.9 * (out - lo) / (hi - lo)
 
-    def project(self, x, steps=12):
+    def project(self, x, steps=12, tol=1e-12):
+        """Newton projection onto the zero set; iterates only unconverged points."""
+        x = x.copy()
+        active = np.arange(len(x))
         for _ in range(steps):
-            v, g = self.value_and_gradient(x)
-            x = x - (v / np.maximum((g * g).sum(axis=1), 1e-20))[:, None] * g
+            v, g = self.value_and_gradient(x[active])
+            done = np.abs(v) < tol
+            step = (v / np.maximum((g * g).sum(axis=1), 1e-20))[:, None] * g
+            x[active[~done]] -= step[~done]
+            active = active[~done]
+            if not len(active):
+                break
         v, g = self.value_and_gradient(x)
         return x, v, g
 
@@ -212,3 +220,47 @@ def workload_displaced_sphere(count=100000, seed=SEED, dtype=np.float64, amplitu
     n = r[:, None] * d - dr_dtheta[:, None] * e_theta - (dr_dphi / sin_t)[:, None] * e_phi
     n /= np.linalg.norm(n, axis=1)[:, None]
     return p.astype(dtype), n.astype(dtype)
+
+
+def leaf_depths(keys, cube_points, depth):
+    """Deepest level whose cell (ceil(p*2^d)-1, clamped) is an active key (field.py:406-412)."""
+    keys = np.sort(np.asarray(keys, dtype=np.int64))
+    p = np.asarray(cube_points, dtype=np.float64)
+    leaf = np.full(len(p), -1)
+    for d in range(depth, -1, -1):
+        n = 1 << d
+        c = np.clip(np.ceil(p * n).astype(np.int64) - 1, 0, n - 1)
+        k = (np.int64(d) << 48) | (c[:, 0] << 32) | (c[:, 1] << 16) | c[:, 2]
+        pos = np.minimum(np.searchsorted(keys, k), len(keys) - 1)
+        hit = (keys[pos] == k) & (leaf < 0)
+        leaf[hit] = d
+    leaf[leaf < 0] = 0
+    return leaf
+
+
+def interface_queries(keys, depth, model_to_cube, cube_to_model, points, count=12000, seed=SEED):
+    """Pairs of points one finest cell apart whose leaf depths differ (refinement
+    interfaces), in model space; ``count`` points (count/2 pairs)."""
+    rng = np.random.default_rng(seed)
+    m2c = np.asarray(model_to_cube, dtype=np.float64).reshape(4, 4)
+    c2m = np.asarray(cube_to_model, dtype=np.float64).reshape(4, 4)
+    cube = points.astype(np.float64) @ m2c[:3, :3].T + m2c[:3, 3]
+    h = 1.0 / (1 << depth)
+    out = []
+    have = 0
+    for _ in range(64):
+        idx = rng.integers(0, len(cube), size=8 * count)
+        a = cube[idx] + rng.uniform(-3 * h, 3 * h, size=(len(idx), 3))
+        axis = rng.integers(0, 3, size=len(idx))
+        b = a.copy()
+        b[np.arange(len(idx)), axis] += h
+        ok = (a.min(axis=1) > 0) & (a.max(axis=1) < 1) & (b.min(axis=1) > 0) & (b.max(axis=1) < 1)
+        a, b = a[ok], b[ok]
+        diff = leaf_depths(keys, a, depth) != leaf_depths(keys, b, depth)
+        pairs = np.stack([a[diff], b[diff]], axis=1).reshape(-1, 3)
+        out.append(pairs)
+        have += len(pairs)
+        if have >= count:
+            break
+    cube_q = np.concatenate(out)[:count]
+    return (cube_q @ c2m[:3, :3].T + c2m[:3, 3]).astype(points.dtype)
