@@ -106,18 +106,31 @@ template <class X> void put_dev(std::vector<char>& buf, const DArray<X>& a) {
   put(buf, h.data(), h.size());
 }
 
-// stencil -> compact CSR (present slots in slot order)
+// position of canonical row i in a schedule-ordered array (identity if none)
+std::vector<int> inverse_order(const DArray<int>& order, int R) {
+  std::vector<int> inv(R);
+  if (order.empty()) {
+    for (int i = 0; i < R; ++i) inv[i] = i;
+    return inv;
+  }
+  std::vector<int> o = order.to_host();
+  for (int j = 0; j < R; ++j) inv[o[j]] = j;
+  return inv;
+}
+
+// stencil -> compact CSR in canonical row order (present slots in slot order)
 template <class V>
-void stencil_csr(const Stencil<V>& m, int slots, const char* part, std::vector<char>& out) {
+void stencil_csr(const Stencil<V>& m, int slots, const char* part, std::vector<char>& out, const std::vector<int>& inv) {
   std::vector<int> cols = m.cols.to_host();
   std::vector<V> vals = m.vals.to_host();
   const int R = m.rows;
   std::vector<int> offs(R + 1, 0), cc;
   std::vector<V> vv;
   for (int i = 0; i < R; ++i) {
+    const int j = inv.empty() ? i : inv[i];
     for (int k = 0; k < slots; ++k) {
-      int c = cols[(size_t)k * R + i];
-      if (c >= 0) { cc.push_back(c); vv.push_back(vals[(size_t)k * R + i]); }
+      int c = cols[(size_t)k * R + j];
+      if (c >= 0) { cc.push_back(c); vv.push_back(vals[(size_t)k * R + j]); }
     }
     offs[i + 1] = (int)cc.size();
   }
@@ -155,13 +168,23 @@ template <class T> bool export_array(Result<T>& r, const std::string& name, int 
   if (name == "normal_offsets") { put(out, r.normal_offsets.data(), r.normal_offsets.size()); return true; }
   if (name == "tree_keys" && lvl) { put_dev(out, L().tree_keys); return true; }
   if (name == "basis_keys" && lvl) { put_dev(out, L().basis_keys); return true; }
+  auto canonical = [&](const auto& a) {  // schedule-ordered vector -> canonical rows
+    auto h = a.to_host();
+    if (!L().scheduled) { put(out, h.data(), h.size()); return; }
+    std::vector<int> o = L().order.to_host();
+    decltype(h) c(h.size());
+    for (size_t j = 0; j < o.size(); ++j) c[o[j]] = h[j];
+    put(out, c.data(), c.size());
+  };
   if (name == "normal_load" && lvl) { put_dev(out, L().normal_load); return true; }
-  if (name == "point_load" && lvl) { put_dev(out, L().point_load); return true; }
-  if (name == "rhs" && lvl) { put_dev(out, L().rhs); return true; }
+  if (name == "point_load" && lvl) { canonical(L().point_load); return true; }
+  if (name == "rhs" && lvl) { canonical(L().rhs); return true; }
   if (name == "corrections" && lvl) { put_dev(out, L().correction); return true; }
-  if (lvl && name.rfind("matrix_", 0) == 0) { stencil_csr(L().A, 27, name.c_str() + 7, out); return true; }
-  if (lvl && name.rfind("cross_", 0) == 0) { stencil_csr(L().cross, 27, name.c_str() + 6, out); return true; }
-  if (lvl && name.rfind("prolong_", 0) == 0) { stencil_csr(L().P, 8, name.c_str() + 8, out); return true; }
+  const bool mat = lvl && (name.rfind("matrix_", 0) == 0 || name.rfind("cross_", 0) == 0);
+  const std::vector<int> inv = mat && L().scheduled ? inverse_order(L().order, L().basis_count) : std::vector<int>();
+  if (lvl && name.rfind("matrix_", 0) == 0) { stencil_csr(L().A, 27, name.c_str() + 7, out, inv); return true; }
+  if (lvl && name.rfind("cross_", 0) == 0) { stencil_csr(L().cross, 27, name.c_str() + 6, out, inv); return true; }
+  if (lvl && name.rfind("prolong_", 0) == 0) { stencil_csr(L().P, 8, name.c_str() + 8, out, {}); return true; }
   if (name == "keys") { put_dev(out, r.keys); return true; }
   if (name == "coefficients") { put_dev(out, r.coefficients); return true; }
   if (name == "cumulative") { put_dev(out, r.cumulative); return true; }

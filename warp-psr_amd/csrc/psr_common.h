@@ -70,6 +70,26 @@ PSR_HD int64_t lower_bound(const int64_t* __restrict__ keys, int64_t n, int64_t 
   return lo;
 }
 
+// Indices of keys (d, x, y, z0+k), k < len, in one sorted key array: one
+// lower-bound for the column (x, y), then a forward scan (keys are unique and
+// z is the fastest coordinate).  out[k] = index or -1.  Same results as len
+// find_key calls.
+template <int MAXLEN>
+PSR_HD void find_run(const int64_t* __restrict__ keys, int64_t n, int d, int x, int y, int z0, int len, int64_t* out) {
+  int64_t lo = lower_bound(keys, n, basis_key(d, x, y, z0));
+#pragma unroll
+  for (int k = 0; k < MAXLEN; ++k) {
+    if (k >= len) break;
+    const int64_t key = basis_key(d, x, y, z0 + k);
+    if (lo < n && keys[lo] == key) {
+      out[k] = lo;
+      ++lo;
+    } else {
+      out[k] = -1;
+    }
+  }
+}
+
 PSR_HD int imin(int a, int b) { return a < b ? a : b; }
 PSR_HD int imax(int a, int b) { return a > b ? a : b; }
 template <class T> PSR_HD T tmin(T a, T b) { return b < a ? b : a; }

@@ -28,16 +28,18 @@ __global__ void k_cumulative(const int64_t* __restrict__ keys, const int* __rest
   if (i >= off[d + 1]) return;
   const int64_t key = keys[i];
   const int x = key_x(key), y = key_y(key), z = key_z(key);
+  const int64_t lo = off[d - 1], cnt = (int64_t)off[d] - lo;
   T value = T(0);
-  for (int corner = 0; corner < 8; ++corner) {
-    const int a = corner / 4, b = (corner / 2) % 2, c = corner % 2;
-    const T h = (T(1) - fabs(T(x) * T(.5) - T(x / 2 + a))) * (T(1) - fabs(T(y) * T(.5) - T(y / 2 + b))) *
-                (T(1) - fabs(T(z) * T(.5) - T(z / 2 + c)));
-    if (h > T(0)) {
-      const int64_t idx = find_level(keys, off, d - 1, basis_key(d - 1, x / 2 + a, y / 2 + b, z / 2 + c));
-      if (idx >= 0) value = value + cum[idx] * h;
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b) {  // corners a*4 + b*2 + c in order; one lookup per (x, y) column
+      int64_t found[2];
+      find_run<2>(keys + lo, cnt, d - 1, x / 2 + a, y / 2 + b, z / 2, 2, found);
+      for (int c = 0; c < 2; ++c) {
+        const T h = (T(1) - fabs(T(x) * T(.5) - T(x / 2 + a))) * (T(1) - fabs(T(y) * T(.5) - T(y / 2 + b))) *
+                    (T(1) - fabs(T(z) * T(.5) - T(z / 2 + c)));
+        if (h > T(0) && found[c] >= 0) value = value + cum[lo + found[c]] * h;
+      }
     }
-  }
   cum[i] = value + coef[i];
 }
 
@@ -55,10 +57,15 @@ __device__ __forceinline__ void add_level(const int64_t* keys, const int* off, c
     base[a] = (int)floor(t[a]);
     f[a] = t[a] - T(base[a]);
   }
+  const int64_t lo = off[d], cnt = (int64_t)off[d + 1] - lo;
+  int64_t found[2][2][2];
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b) find_run<2>(keys + lo, cnt, d, cell[0] + a, cell[1] + b, cell[2], 2, found[a][b]);
   for (int corner = 0; corner < 8; ++corner) {
     const int node[3] = {cell[0] + corner / 4, cell[1] + (corner / 2) % 2, cell[2] + corner % 2};
     if (imin(imin(node[0] - base[0], node[1] - base[1]), node[2] - base[2]) < 0) continue;
-    const int64_t idx = find_level(keys, off, d, basis_key(d, node[0], node[1], node[2]));
+    const int64_t fidx = found[corner / 4][(corner / 2) % 2][corner % 2];
+    const int64_t idx = fidx >= 0 ? fidx + lo : -1;
     if (idx < 0) continue;
     T h[3], sg[3];
     for (int a = 0; a < 3; ++a) {

@@ -52,6 +52,10 @@ template <class T> struct LevelState {
   Stencil<T> P;             // prolongation rows (8 slots, x-major corner order)
   DArray<double> normal_load, point_load;
   DArray<T> rhs, correction;
+  // finer levels keep A, cross, rhs and point_load in sliced schedule order:
+  // position j holds row order[j]
+  DArray<int> order;
+  bool scheduled = false;
   bool solved = false;
 };
 

@@ -344,14 +344,20 @@ __global__ void k_density_eval(const int64_t* __restrict__ lkeys, const T* __res
   const Quad by = quadratic(double((p1 - T(cy) * h) / h));
   const Quad bz = quadratic(double((p2 - T(cz) * h) / h));
   T value = T(0);
-  for (int k = 0; k < 27; ++k) {
-    const int x = k / 9, y = k / 3 % 3, z = k % 3;
-    const int tx = cx + x - 1, ty = cy + y - 1, tz = cz + z - 1;
-    if (imin(imin(tx, ty), tz) >= 0 && imax(imax(tx, ty), tz) < n) {
-      int64_t idx = find_key(lkeys, lcount, basis_key(d, tx, ty, tz));
-      if (idx >= 0) value = value + T(quad_at(bx, x) * quad_at(by, y) * quad_at(bz, z) * double(lvals[idx]));
+  const int zlo = imax(cz - 1, 0), zhi = imin(cz + 1, n - 1);
+  for (int x = 0; x < 3; ++x)
+    for (int y = 0; y < 3; ++y) {  // k = 9x + 3y + z, one lookup per (x, y) column
+      const int tx = cx + x - 1, ty = cy + y - 1;
+      if (tx < 0 || ty < 0 || tx >= n || ty >= n || zlo > zhi) continue;
+      int64_t found[3];
+      find_run<3>(lkeys, lcount, d, tx, ty, zlo, zhi - zlo + 1, found);
+      for (int z = 0; z < 3; ++z) {
+        const int tz = cz + z - 1;
+        if (tz < zlo || tz > zhi) continue;
+        const int64_t idx = found[tz - zlo];
+        if (idx >= 0) value = value + T(quad_at(bx, x) * quad_at(by, y) * quad_at(bz, z) * double(lvals[idx]));
+      }
     }
-  }
   out[(int64_t)i * stride + d] = value;
 }
 

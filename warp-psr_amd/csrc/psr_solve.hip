@@ -13,6 +13,10 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
+#include <map>
+#include <mutex>
+#include <tuple>
 
 namespace psr {
 
@@ -112,28 +116,38 @@ __global__ void k_normal_table(int test_depth, int normal_depth, int first, int 
 }
 
 // ---------------------------------------------------------------- stiffness (stiffness.py:38-109)
-__global__ void k_stiffness(const int64_t* __restrict__ rows, int R, const int64_t* __restrict__ cols, int64_t C, int depth,
-                            int trial_depth, const D2* __restrict__ table, int* __restrict__ out_cols,
-                            double* __restrict__ out_vals) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= R) return;
+__global__ void k_stiffness(const int64_t* __restrict__ rows, int R, const int* __restrict__ row_order,
+                            const int64_t* __restrict__ cols, int64_t C, int depth, int trial_depth,
+                            const D2* __restrict__ table, int* __restrict__ out_cols, double* __restrict__ out_vals) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;  // output position
+  if (j >= R) return;
+  const int i = row_order ? row_order[j] : j;
   const int64_t key = rows[i];
   const int cx = key_x(key), cy = key_y(key), cz = key_z(key);
   const int shift = depth - trial_depth;
   const int px = cx >> shift, py = cy >> shift, pz = cz >> shift;
   const int lim = 1 << trial_depth;
-  for (int k = 0; k < 27; ++k) {
-    const int qx = px + k / 9 - 1, qy = py + (k / 3) % 3 - 1, qz = pz + k % 3 - 1;
-    int64_t index = -1;
-    if (imin(imin(qx, qy), qz) >= 0 && imax(imax(qx, qy), qz) <= lim)
-      index = find_key(cols, C, basis_key(trial_depth, qx, qy, qz));
-    double v = 0.0;
-    if (index >= 0) {
-      const D2 x = table[cx * 3 + k / 9], y = table[cy * 3 + (k / 3) % 3], z = table[cz * 3 + k % 3];
-      v = x.v1 * y.v0 * z.v0 + x.v0 * y.v1 * z.v0 + x.v0 * y.v0 * z.v1;
+  const int zlo = imax(pz - 1, 0), zhi = imin(pz + 1, lim);
+  for (int kx = 0; kx < 3; ++kx) {
+    const int qx = px + kx - 1;
+    for (int ky = 0; ky < 3; ++ky) {
+      const int qy = py + ky - 1;
+      int64_t found[3] = {-1, -1, -1};
+      if (qx >= 0 && qx <= lim && qy >= 0 && qy <= lim && zlo <= zhi)
+        find_run<3>(cols, C, trial_depth, qx, qy, zlo, zhi - zlo + 1, found);
+      for (int kz = 0; kz < 3; ++kz) {
+        const int qz = pz + kz - 1;
+        const int k = kx * 9 + ky * 3 + kz;
+        const int64_t index = (qz >= zlo && qz <= zhi) ? found[qz - zlo] : -1;
+        double v = 0.0;
+        if (index >= 0) {
+          const D2 x = table[cx * 3 + kx], y = table[cy * 3 + ky], z = table[cz * 3 + kz];
+          v = x.v1 * y.v0 * z.v0 + x.v0 * y.v1 * z.v0 + x.v0 * y.v0 * z.v1;
+        }
+        out_cols[(int64_t)k * R + j] = (int)index;
+        out_vals[(int64_t)k * R + j] = v;
+      }
     }
-    out_cols[(int64_t)k * R + i] = (int)index;
-    out_vals[(int64_t)k * R + i] = v;
   }
 }
 
@@ -174,57 +188,78 @@ template <class T> PSR_HD T hat3(T p0, T p1, T p2, int cx, int cy, int cz, int d
 
 // screening_stencils.py:34-70 (add): per row, points binned in the 8 cells at its vertex
 template <class T>
-__global__ void k_screening(const int64_t* __restrict__ rows, int R, const int64_t* __restrict__ cols_keys, int depth,
-                            int trial_depth, const uint64_t* __restrict__ bin_keys, const int* __restrict__ bin_ids, int P,
+__global__ void k_screening(const int64_t* __restrict__ rows, int R, const int* __restrict__ row_order,
+                            const int64_t* __restrict__ cols_keys, int depth, int trial_depth,
+                            const uint64_t* __restrict__ bin_keys, const int* __restrict__ bin_ids, int P,
                             const T* __restrict__ pos, const T* __restrict__ w, const int* __restrict__ scols,
                             double* __restrict__ svals, double* __restrict__ load) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= R) return;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;  // output position
+  if (j >= R) return;
+  const int i = row_order ? row_order[j] : j;
   const int64_t key = rows[i];
   const int cx = key_x(key), cy = key_y(key), cz = key_z(key);
   const int n = 1 << depth;
-  double acc[27];
-  int qx[27], qy[27], qz[27];
-  bool present[27];
-  for (int k = 0; k < 27; ++k) {
-    acc[k] = 0.0;
-    const int col = scols[(int64_t)k * R + i];
-    present[k] = col >= 0;
-    if (present[k]) {
-      const int64_t ck = cols_keys[col];
-      qx[k] = key_x(ck); qy[k] = key_y(ck); qz[k] = key_z(ck);
-    } else {
-      qx[k] = qy[k] = qz[k] = 0;
-    }
-  }
-  double rhs = 0.0;
-  for (int kk = 0; kk < 8; ++kk) {
-    const int bx = cx - kk / 4, by = cy - (kk / 2) % 2, bz = cz - kk % 2;
-    if (imin(imin(bx, by), bz) < 0 || imax(imax(bx, by), bz) >= n) continue;
-    const uint64_t bk = ((uint64_t)bx << (2 * depth)) | ((uint64_t)by << depth) | (uint64_t)bz;
-    int lo = 0, hi = P;
-    while (lo < hi) {
-      int mid = (lo + hi) >> 1;
-      if (bin_keys[mid] < bk) lo = mid + 1; else hi = mid;
-    }
-    for (int j = lo; j < P && bin_keys[j] == bk; ++j) {
-      const int index = bin_ids[j];
-      const T p0 = pos[3 * index], p1 = pos[3 * index + 1], p2 = pos[3 * index + 2];
-      const T weight = w[index];
-      if (weight != T(0)) {
-        const T phi = hat3(p0, p1, p2, cx, cy, cz, depth);
-        rhs += double(phi * weight);
-        for (int k = 0; k < 27; ++k) {
-          if (!present[k]) continue;
-          const T trial = hat3(p0, p1, p2, qx[k], qy[k], qz[k], trial_depth);
-          acc[k] += double(phi * trial * weight);
-        }
+  // point ranges of the 8 cells at the vertex, in the reference's cell order
+  // k = 0..7 -> cell (cx - k/4, cy - k/2%2, cz - k%2); bins are sorted, so the
+  // bins (cz-1) and (cz) of one column are adjacent and found with one search
+  int rb[8], re[8];
+  for (int a = 0; a < 2; ++a)
+    for (int bb = 0; bb < 2; ++bb) {
+      const int k0 = a * 4 + bb * 2;
+      rb[k0] = re[k0] = rb[k0 + 1] = re[k0 + 1] = 0;
+      const int bx = cx - a, by = cy - bb;
+      if (bx < 0 || by < 0 || bx >= n || by >= n) continue;
+      const uint64_t col = ((uint64_t)bx << (2 * depth)) | ((uint64_t)by << depth);
+      const int zfirst = cz >= 1 ? cz - 1 : cz;
+      int lo = 0, hi = P;
+      const uint64_t first_key = col | (uint64_t)zfirst;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (bin_keys[mid] < first_key) lo = mid + 1; else hi = mid;
+      }
+      if (cz >= 1) {
+        rb[k0 + 1] = lo;
+        while (lo < P && bin_keys[lo] == (col | (uint64_t)(cz - 1))) ++lo;
+        re[k0 + 1] = lo;
+      }
+      if (cz < n) {
+        rb[k0] = lo;
+        while (lo < P && bin_keys[lo] == (col | (uint64_t)cz)) ++lo;
+        re[k0] = lo;
       }
     }
+  double rhs = 0.0;
+  for (int kk = 0; kk < 8; ++kk)
+    for (int jj = rb[kk]; jj < re[kk]; ++jj) {
+      const int index = bin_ids[jj];
+      const T weight = w[index];
+      if (weight != T(0)) {
+        const T phi = hat3(pos[3 * index], pos[3 * index + 1], pos[3 * index + 2], cx, cy, cz, depth);
+        rhs += double(phi * weight);
+      }
+    }
+  if (load) load[j] = rhs;
+  // one accumulator at a time (slot outer): per slot the points arrive in the
+  // same (cell, point) order as the reference's point-outer loop
+  for (int k = 0; k < 27; ++k) {
+    const int c = scols[(int64_t)k * R + j];
+    if (c < 0) continue;
+    const int64_t ck = cols_keys[c];
+    const int qx = key_x(ck), qy = key_y(ck), qz = key_z(ck);
+    double acc = 0.0;
+    for (int kk = 0; kk < 8; ++kk)
+      for (int jj = rb[kk]; jj < re[kk]; ++jj) {
+        const int index = bin_ids[jj];
+        const T weight = w[index];
+        if (weight != T(0)) {
+          const T p0 = pos[3 * index], p1 = pos[3 * index + 1], p2 = pos[3 * index + 2];
+          const T phi = hat3(p0, p1, p2, cx, cy, cz, depth);
+          const T trial = hat3(p0, p1, p2, qx, qy, qz, trial_depth);
+          acc += double(phi * trial * weight);
+        }
+      }
+    svals[(int64_t)k * R + j] += acc;
   }
-  if (load) load[i] = rhs;
-  for (int k = 0; k < 27; ++k)
-    if (present[k]) svals[(int64_t)k * R + i] += acc[k];
 }
 
 template <class T>
@@ -248,16 +283,22 @@ __global__ void k_prolongation(const int64_t* __restrict__ fine, int R, const in
   if (i >= R) return;
   const int64_t key = fine[i];
   const int f[3] = {key_x(key), key_y(key), key_z(key)};
-  for (int k = 0; k < 8; ++k) {
-    const int c[3] = {f[0] / 2 + k / 4, f[1] / 2 + (k / 2) % 2, f[2] / 2 + k % 2};
-    T h[3];
-    for (int a = 0; a < 3; ++a) h[a] = T(1) - fabs(T(f[a]) * T(0.5) - T(c[a]));
-    const T weight = h[0] * h[1] * h[2];
-    int64_t column = -1;
-    if (weight > T(0)) column = find_key(coarse, C, basis_key(depth - 1, c[0], c[1], c[2]));
-    cols[(int64_t)k * R + i] = (int)column;
-    vals[(int64_t)k * R + i] = column >= 0 ? weight : T(0);
-  }
+  T h[3][2];
+  for (int a = 0; a < 3; ++a)
+    for (int o = 0; o < 2; ++o) h[a][o] = T(1) - fabs(T(f[a]) * T(0.5) - T(f[a] / 2 + o));
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b) {
+      const T wz0 = h[0][a] * h[1][b] * h[2][0], wz1 = h[0][a] * h[1][b] * h[2][1];
+      int64_t found[2] = {-1, -1};
+      if (wz0 > T(0) || wz1 > T(0)) find_run<2>(coarse, C, depth - 1, f[0] / 2 + a, f[1] / 2 + b, f[2] / 2, 2, found);
+      for (int c = 0; c < 2; ++c) {
+        const int k = a * 4 + b * 2 + c;
+        const T weight = c ? wz1 : wz0;
+        const int64_t column = weight > T(0) ? found[c] : -1;
+        cols[(int64_t)k * R + i] = (int)column;
+        vals[(int64_t)k * R + i] = column >= 0 ? weight : T(0);
+      }
+    }
 }
 
 // Restriction P^T as a gather: coarse row r takes the fine nodes 2r+o,
@@ -270,12 +311,18 @@ __global__ void k_restriction(const int64_t* __restrict__ coarse, int C, const i
   const int64_t key = coarse[r];
   const int x = 2 * key_x(key), y = 2 * key_y(key), z = 2 * key_z(key);
   const int lim = 1 << fine_depth;
-  for (int k = 0; k < 27; ++k) {
-    const int fx = x + k / 9 - 1, fy = y + (k / 3) % 3 - 1, fz = z + k % 3 - 1;
-    int64_t idx = -1;
-    if (imin(imin(fx, fy), fz) >= 0 && imax(imax(fx, fy), fz) <= lim) idx = find_key(fine, F, basis_key(fine_depth, fx, fy, fz));
-    cols[(int64_t)k * C + r] = (int)idx;
-  }
+  const int zlo = imax(z - 1, 0), zhi = imin(z + 1, lim);
+  for (int kx = 0; kx < 3; ++kx)
+    for (int ky = 0; ky < 3; ++ky) {
+      const int fx = x + kx - 1, fy = y + ky - 1;
+      int64_t found[3] = {-1, -1, -1};
+      if (fx >= 0 && fx <= lim && fy >= 0 && fy <= lim && zlo <= zhi)
+        find_run<3>(fine, F, fine_depth, fx, fy, zlo, zhi - zlo + 1, found);
+      for (int kz = 0; kz < 3; ++kz) {
+        const int fz = z + kz - 1;
+        cols[(int64_t)(kx * 9 + ky * 3 + kz) * C + r] = (fz >= zlo && fz <= zhi) ? (int)found[fz - zlo] : -1;
+      }
+    }
 }
 
 PSR_HD double restriction_weight(int k) {
@@ -328,19 +375,28 @@ __global__ void k_normal_load(const int64_t* __restrict__ keys, int R, const int
   const int e0 = normal_center(c0, test_depth, normal_depth), e1 = normal_center(c1, test_depth, normal_depth),
             e2 = normal_center(c2, test_depth, normal_depth);
   const int lim = 1 << normal_depth;
+  const int zlo = imax(e2 + first, 0), zhi = imin(e2 + first + width - 1, lim - 1);
   double value = 0.0;
-  const int w3 = width * width * width;
-  for (int k = 0; k < w3; ++k) {
-    const int x = k / (width * width), y = (k / width) % width, z = k % width;
-    const int nx = e0 + x + first, ny = e1 + y + first, nz = e2 + z + first;
-    if (imin(imin(nx, ny), nz) < 0 || imax(imax(nx, ny), nz) >= lim) continue;
-    const int64_t j = find_key(nkeys, ncount, basis_key(normal_depth, nx, ny, nz));
-    if (j < 0) continue;
-    const D2 ix = table[c0 * width + x], iy = table[c1 * width + y], iz = table[c2 * width + z];
-    const V* nv = nvals + (int64_t)nstride * j;
-    value += double(nv[0]) * ix.v1 * iy.v0 * iz.v0;
-    value += double(nv[1]) * ix.v0 * iy.v1 * iz.v0;
-    value += double(nv[2]) * ix.v0 * iy.v0 * iz.v1;
+  for (int x = 0; x < width; ++x) {
+    const int nx = e0 + x + first;
+    if (nx < 0 || nx >= lim) continue;
+    for (int y = 0; y < width; ++y) {
+      const int ny = e1 + y + first;
+      if (ny < 0 || ny >= lim || zlo > zhi) continue;
+      int64_t found[6];
+      find_run<6>(nkeys, ncount, normal_depth, nx, ny, zlo, zhi - zlo + 1, found);
+      for (int z = 0; z < width; ++z) {
+        const int nz = e2 + z + first;
+        if (nz < zlo || nz > zhi) continue;
+        const int64_t jn = found[nz - zlo];
+        if (jn < 0) continue;
+        const D2 ix = table[c0 * width + x], iy = table[c1 * width + y], iz = table[c2 * width + z];
+        const V* nv = nvals + (int64_t)nstride * jn;
+        value += double(nv[0]) * ix.v1 * iy.v0 * iz.v0;
+        value += double(nv[1]) * ix.v0 * iy.v1 * iz.v0;
+        value += double(nv[2]) * ix.v0 * iy.v0 * iz.v1;
+      }
+    }
   }
   out[i] = value;
 }
@@ -358,7 +414,10 @@ __global__ void k_normal_prolong(const int64_t* __restrict__ keys, int R, const 
   const int64_t own = find_key(own_keys, own_count, key);
   if (own >= 0) { v0 = own_vals[3 * own]; v1 = own_vals[3 * own + 1]; v2 = own_vals[3 * own + 2]; }
   if (depth > 0) {
-    for (int corner = 0; corner < 8; ++corner) {
+    int64_t idx[2][2][2];  // [x bit][y bit][z bit]
+    for (int a = 0; a < 2; ++a)
+      for (int b = 0; b < 2; ++b) find_run<2>(prev_keys, prev_count, depth - 1, c[0] / 2 + a, c[1] / 2 + b, c[2] / 2, 2, idx[a][b]);
+    for (int corner = 0; corner < 8; ++corner) {  // x is bit 0 here (normal_loads.py:285-297)
       int p[3] = {c[0] / 2, c[1] / 2, c[2] / 2};
       T weight = T(1);
       for (int axis = 0; axis < 3; ++axis) {
@@ -366,7 +425,7 @@ __global__ void k_normal_prolong(const int64_t* __restrict__ keys, int R, const 
         weight = weight * (T(1) - fabs(T(c[axis]) * T(0.5) - T(p[axis])));
       }
       if (weight > T(0)) {
-        const int64_t j = find_key(prev_keys, prev_count, basis_key(depth - 1, p[0], p[1], p[2]));
+        const int64_t j = idx[corner & 1][(corner >> 1) & 1][(corner >> 2) & 1];
         if (j >= 0) {
           v0 = v0 + prev_vals[3 * j] * weight;
           v1 = v1 + prev_vals[3 * j + 1] * weight;
@@ -462,22 +521,130 @@ __device__ __forceinline__ void gs_update(int j, int R, const int* __restrict__ 
     r = c[k] >= 0 ? t : r;
   }
   if (has_diag[j]) {
-    const int i = order[j];
+    const int i = order ? order[j] : j;  // no order: x is kept in schedule order
     x[i] = x[i] + r * inv_diag[j];
   }
 }
 
 // One launch of relaxation.py:100-116 (wavefront_update): `slices` slices two
 // apart, same colour.
+// waves_per_eu(1,4): the row update keeps all 81 loads of a row in flight
+// (needs ~100 VGPRs); occupancy is not what bounds these small launches.
 template <class T>
-__global__ void __launch_bounds__(256) k_relax_wave(const int* __restrict__ order, const int* __restrict__ goff,
-                                                    const int* __restrict__ cols, const T* __restrict__ vals,
-                                                    const T* __restrict__ b, const T* __restrict__ inv_diag,
-                                                    const unsigned char* __restrict__ has_diag, T* __restrict__ x, int R,
-                                                    int first_z, int color) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4)))
+k_relax_wave(const int* __restrict__ order, const int* __restrict__ goff, const int* __restrict__ cols,
+             const T* __restrict__ vals, const T* __restrict__ b, const T* __restrict__ inv_diag,
+             const unsigned char* __restrict__ has_diag, T* __restrict__ x, int R, int first_z, int color) {
   const int group = 4 * (first_z + 2 * (int)blockIdx.y) + color;
   const int j = goff[group] + blockIdx.x * blockDim.x + threadIdx.x;
   if (j < goff[group + 1]) gs_update(j, R, order, cols, vals, b, inv_diag, has_diag, x);
+}
+
+// Same launch with its arguments read from a device descriptor, so one
+// captured graph of the whole launch sequence serves every call of a shape.
+template <class T> struct RelaxDesc {
+  const int* order;
+  const int* goff;
+  const int* cols;
+  const T* vals;
+  const T* b;
+  const T* inv_diag;
+  const unsigned char* has_diag;
+  T* x;
+  int R;
+};
+
+template <class T>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4)))
+k_relax_wave_graph(const RelaxDesc<T>* __restrict__ desc, int first_z, int color) {
+  const RelaxDesc<T> d = *desc;
+  const int group = 4 * (first_z + 2 * (int)blockIdx.y) + color;
+  const int j = d.goff[group] + blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < d.goff[group + 1]) gs_update(j, d.R, d.order, d.cols, d.vals, d.b, d.inv_diag, d.has_diag, d.x);
+}
+
+// Dependency-driven sliced relaxation in one cooperative launch.
+//
+// Phase (z, t, c) = colour c of the t-th visit of slice z (relaxation.py:121-125).
+// Its rows are split into chunks of RELAX_CHUNK; chunk tasks are listed in the
+// reference's sequential phase order and assigned round-robin to co-resident
+// workgroups, which take their tasks in list order.  A phase waits for
+//   c > 0 : (z, t, c-1)
+//   c = 0 : (z, t-1, 3), (z+1, t, 3), (z-1, t-1, 3)
+// -- exactly the predecessors of the sequential order that touch the rows it
+// reads or writes -- so every row update sees the values the sequential order
+// gives it (same bits).  Every dependency precedes the waiting task in the list
+// and every owner takes its tasks in order, so the smallest unfinished task can
+// always run: no deadlock.  A bounded spin raises an error flag instead of
+// waiting forever.  Matrix rows are loaded before the wait (they do not depend
+// on x); x is gathered after an agent-scope acquire and published with an
+// agent-scope release before the phase counter is incremented.
+constexpr int RELAX_CHUNK = 256;
+
+template <class T>
+__global__ void __launch_bounds__(RELAX_CHUNK) k_relax_persistent(
+    const int* __restrict__ order, const int* __restrict__ cols, const T* __restrict__ vals, const T* __restrict__ b,
+    const T* __restrict__ inv_diag, const unsigned char* __restrict__ has_diag, T* __restrict__ x, int R,
+    const int4* __restrict__ tasks, int ntasks, const int4* __restrict__ deps, const int* __restrict__ chunks,
+    int* __restrict__ done, int* __restrict__ error, int spin_limit) {
+  for (int task = blockIdx.x; task < ntasks; task += gridDim.x) {
+    const int4 td = tasks[task];  // phase, first row, end row
+    const int j = td.y + (int)threadIdx.x;
+    const bool active = j < td.z;
+    int c[27];
+    T v[27];
+    T bj = T(0), inv = T(0);
+    bool hd = false;
+    if (active) {
+#pragma unroll
+      for (int k = 0; k < 27; ++k) {
+        c[k] = cols[(int64_t)k * R + j];
+        v[k] = vals[(int64_t)k * R + j];
+      }
+      bj = b[j];
+      inv = inv_diag[j];
+      hd = has_diag[j] != 0;
+    }
+    if (threadIdx.x == 0) {
+      const int4 dp = deps[td.x];
+      const int dl[3] = {dp.x, dp.y, dp.z};
+      for (int q = 0; q < 3; ++q) {
+        const int d = dl[q];
+        if (d < 0) continue;
+        const int need = chunks[d];
+        int spins = 0;
+        // once any wait has timed out, no task waits again: the grid drains
+        while (__hip_atomic_load(&done[d], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < need) {
+          if (++spins > spin_limit || ((spins & 255) == 0 && __hip_atomic_load(error, __ATOMIC_RELAXED,
+                                                                               __HIP_MEMORY_SCOPE_AGENT))) {
+            __hip_atomic_store(error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+        }
+      }
+    }
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    if (active) {
+      T xv[27];
+#pragma unroll
+      for (int k = 0; k < 27; ++k) xv[k] = x[c[k] >= 0 ? c[k] : 0];
+      T r = bj;
+#pragma unroll
+      for (int k = 0; k < 27; ++k) {
+        const T t = r - xv[k] * v[k];
+        r = c[k] >= 0 ? t : r;
+      }
+      if (hd) {
+        const int i = order ? order[j] : j;
+        x[i] = x[i] + r * inv;
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    if (threadIdx.x == 0) __hip_atomic_fetch_add(&done[td.x], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  }
 }
 
 // relaxation.py:261-271 (8-colour, all sweeps) inside one workgroup: the
@@ -640,6 +807,47 @@ __global__ void k_root_extract(const int64_t* __restrict__ keys, int R, const T*
   x[i] = pad_x[cz * 16 + (cx & 1) + 2 * (cy & 1) + 2 * (cx & 2) + 4 * (cy & 2)];
 }
 
+// solver.py:119-126 (_rhs) written in schedule order
+template <class T>
+__global__ void k_rhs_sched(const double* __restrict__ load, const double* __restrict__ point_load_sched,
+                            const int* __restrict__ order, int R, T* __restrict__ b) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < R) b[j] = T(load[order[j]]) + T(0.5) * T(point_load_sched[j]);
+}
+
+__global__ void k_inverse_permutation(const int* __restrict__ order, int R, int* __restrict__ inv) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < R) inv[order[j]] = j;
+}
+
+__global__ void k_remap_columns(const int* __restrict__ cols, const int* __restrict__ inv, int64_t n, int* __restrict__ out) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    const int c = cols[i];
+    out[i] = c >= 0 ? inv[c] : -1;
+  }
+}
+
+template <class T>
+__global__ void k_scatter_rows(const int* __restrict__ order, int R, const T* __restrict__ xs, T* __restrict__ x) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < R) x[order[j]] = xs[j];
+}
+
+// 1/A_ii of schedule-ordered rows (the entry whose column is the row itself)
+template <class T>
+__global__ void k_diag_sched(const int* __restrict__ order, int R, const int* __restrict__ cols, const T* __restrict__ vals,
+                             T* __restrict__ inv_diag, unsigned char* __restrict__ has_diag) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= R) return;
+  const int i = order[j];
+  T diagonal = T(0);
+  for (int k = 0; k < 27; ++k)
+    if (cols[(int64_t)k * R + j] == i) diagonal = vals[(int64_t)k * R + j];
+  has_diag[j] = diagonal != T(0);
+  inv_diag[j] = diagonal != T(0) ? T(1) / diagonal : T(0);
+}
+
 inline int floordiv(int a, int b) { return (a >= 0) ? a / b : -((-a + b - 1) / b); }
 
 }  // namespace
@@ -677,6 +885,9 @@ template <class T> struct GsOperator {
   DArray<int> cols;
   DArray<T> vals, b, inv_diag;
   DArray<unsigned char> has_diag;
+  // row of x updated by schedule position j: update_order[j], or j itself
+  // when x is kept in schedule order (nullptr)
+  const int* update_order = nullptr;
 };
 
 template <class T>
@@ -684,6 +895,7 @@ static GsOperator<T> gather_operator(const Schedule<T>& sc, const DArray<int>& c
                                      hipStream_t s) {
   GsOperator<T> op;
   op.R = R;
+  op.update_order = sc.order.data();
   op.cols.reset(27 * (size_t)R, s);
   op.vals.reset(27 * (size_t)R, s);
   op.b.reset(R, s);
@@ -739,7 +951,7 @@ int relax_sliced(const GsOperator<T>& op, const Schedule<T>& sc, T* x, int depth
     const int z0 = n - q + 2 * first;
     const unsigned count = (unsigned)(last - first + 1);
     for (int color = 0; color < 4; ++color) {
-      hipLaunchKernelGGL(k_relax_wave<T>, dim3(bx, count), dim3(256), 0, s, (const int*)sc.order.data(),
+      hipLaunchKernelGGL(k_relax_wave<T>, dim3(bx, count), dim3(256), 0, s, op.update_order,
                          (const int*)sc.offsets.data(), (const int*)op.cols.data(), (const T*)op.vals.data(),
                          (const T*)op.b.data(), (const T*)op.inv_diag.data(), (const unsigned char*)op.has_diag.data(), x,
                          op.R, z0, color);
@@ -750,17 +962,184 @@ int relax_sliced(const GsOperator<T>& op, const Schedule<T>& sc, T* x, int depth
   return launches;
 }
 
+// Captured launch sequence of relax_sliced, cached per (precision, depth,
+// iters, row capacity, device, caller stream): the graph reads its buffers
+// through a descriptor that each call rewrites (stream-ordered) before
+// launching it.  Capture happens on a private stream (the default stream
+// cannot be captured); the graph then runs on the caller's stream.
+struct RelaxGraph {
+  hipGraphExec_t exec = nullptr;
+  void* desc = nullptr;
+  int launches = 0;
+};
+
+template <class T>
+int relax_sliced_graph(const GsOperator<T>& op, const Schedule<T>& sc, T* x, int depth, int iters, hipStream_t s) {
+  static std::mutex mu;
+  static std::map<std::tuple<int, int, int, unsigned, int, hipStream_t>, RelaxGraph> cache;
+  static std::map<int, hipStream_t> capture_streams;
+  const int n = 1 << depth;
+  unsigned bx = 1;
+  while (bx * 256 < (unsigned)std::max(1, sc.max_group)) bx <<= 1;
+  int dev = 0;
+  HIP_CHECK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> guard(mu);
+  const auto key = std::make_tuple((int)sizeof(T), depth, iters, bx, dev, s);
+  auto it = cache.find(key);
+  if (it == cache.end()) {
+    RelaxGraph g;
+    HIP_CHECK(hipMalloc(&g.desc, sizeof(RelaxDesc<T>)));
+    hipStream_t& cs = capture_streams[dev];
+    if (!cs) HIP_CHECK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    HIP_CHECK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+    for (int q = 0; q <= n + 2 * (iters - 1); ++q) {
+      const int first = std::max(0, floordiv(q - n + 1, 2));
+      const int last = std::min(iters - 1, q / 2);
+      if (last < first) continue;
+      const int z0 = n - q + 2 * first;
+      const unsigned count = (unsigned)(last - first + 1);
+      for (int color = 0; color < 4; ++color) {
+        hipLaunchKernelGGL(k_relax_wave_graph<T>, dim3(bx, count), dim3(256), 0, cs, (const RelaxDesc<T>*)g.desc, z0, color);
+        ++g.launches;
+      }
+    }
+    hipGraph_t graph = nullptr;
+    HIP_CHECK(hipStreamEndCapture(cs, &graph));
+    HIP_CHECK(hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0));
+    HIP_CHECK(hipGraphDestroy(graph));
+    it = cache.emplace(key, g).first;
+  }
+  RelaxDesc<T> d{op.update_order, sc.offsets.data(), op.cols.data(), op.vals.data(), op.b.data(), op.inv_diag.data(),
+                 op.has_diag.data(), x, op.R};
+  HIP_CHECK(hipMemcpyAsync(it->second.desc, &d, sizeof(d), hipMemcpyHostToDevice, s));
+  HIP_CHECK(hipGraphLaunch(it->second.exec, s));
+  // the descriptor is rewritten by the next call on this stream only after
+  // this launch (stream order); keep the host copy until the copy has run
+  HIP_CHECK(hipStreamSynchronize(s));
+  return it->second.launches;
+}
+
+// Task list of k_relax_persistent (see the kernel's comment).
+struct RelaxTasks {
+  std::vector<int4> tasks, deps;
+  std::vector<int> chunks;
+};
+
+static RelaxTasks build_relax_tasks(const std::vector<int>& goff, int depth, int iters) {
+  const int n = 1 << depth;
+  RelaxTasks rt;
+  std::vector<int> index((size_t)(n + 1) * iters * 4, -1);
+  auto id = [&](int z, int t, int c) -> int& { return index[((size_t)z * iters + t) * 4 + c]; };
+  int phases = 0;
+  for (int front = n; front > -iters; --front)
+    for (int z = std::max(front, 0); z < std::min(front + iters, n + 1); ++z)
+      for (int c = 0; c < 4; ++c) id(z, z - front, c) = phases++;
+  rt.deps.resize(phases);
+  rt.chunks.resize(phases);
+  for (int front = n; front > -iters; --front)
+    for (int z = std::max(front, 0); z < std::min(front + iters, n + 1); ++z)
+      for (int c = 0; c < 4; ++c) {
+        const int t = z - front;
+        const int p = id(z, t, c);
+        int4 d = make_int4(-1, -1, -1, 0);
+        if (c > 0) {
+          d.x = id(z, t, c - 1);
+        } else {
+          if (t > 0) d.x = id(z, t - 1, 3);
+          if (z + 1 <= n) d.y = id(z + 1, t, 3);
+          if (z > 0 && t > 0) d.z = id(z - 1, t - 1, 3);
+        }
+        if (d.x >= p || d.y >= p || d.z >= p) throw PsrError(1, "relaxation task order violates a dependency");
+        rt.deps[p] = d;
+        const int g = 4 * z + c;
+        const int rows = goff[g + 1] - goff[g];
+        const int chunks = std::max(1, (rows + RELAX_CHUNK - 1) / RELAX_CHUNK);
+        rt.chunks[p] = chunks;
+        for (int k = 0; k < chunks; ++k) {
+          const int j0 = goff[g] + k * RELAX_CHUNK;
+          rt.tasks.push_back(make_int4(p, j0, std::min(goff[g + 1], j0 + RELAX_CHUNK), 0));
+        }
+      }
+  return rt;
+}
+
+template <class T> static int cooperative_grid() {
+  static int grid = 0;
+  if (!grid) {
+    int dev = 0, cus = 0, per_cu = 0;
+    HIP_CHECK(hipGetDevice(&dev));
+    HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_relax_persistent<T>, RELAX_CHUNK, 0));
+    grid = std::max(1, std::min(per_cu, 2) * cus);
+  }
+  return grid;
+}
+
+template <class T>
+int relax_sliced_persistent(const GsOperator<T>& op, const Schedule<T>& sc, T* x, int depth, int iters, hipStream_t s) {
+  RelaxTasks rt = build_relax_tasks(sc.host_offsets, depth, iters);
+  DArray<int4> tasks, deps;
+  DArray<int> chunks, done(rt.chunks.size(), s), error(1, s);
+  tasks.from_host(rt.tasks.data(), rt.tasks.size(), s);
+  deps.from_host(rt.deps.data(), rt.deps.size(), s);
+  chunks.from_host(rt.chunks.data(), rt.chunks.size(), s);
+  done.zero();
+  error.zero();
+  const int ntasks = (int)rt.tasks.size();
+  const int grid = std::min(cooperative_grid<T>(), ntasks);
+  const int* order = op.update_order;
+  const int* cols = op.cols.data();
+  const T* vals = op.vals.data();
+  const T* b = op.b.data();
+  const T* inv = op.inv_diag.data();
+  const unsigned char* hd = op.has_diag.data();
+  int R = op.R;
+  const int4* tp = tasks.data();
+  const int4* dp = deps.data();
+  const int* cp = chunks.data();
+  int* donep = done.data();
+  int* errp = error.data();
+  int spin_limit = 1 << 24;
+  void* args[] = {(void*)&order, (void*)&cols, (void*)&vals, (void*)&b, (void*)&inv, (void*)&hd, (void*)&x, (void*)&R,
+                  (void*)&tp, (void*)&ntasks, (void*)&dp, (void*)&cp, (void*)&donep, (void*)&errp, (void*)&spin_limit};
+  HIP_CHECK(hipLaunchCooperativeKernel((const void*)k_relax_persistent<T>, dim3(grid), dim3(RELAX_CHUNK), args, 0, s));
+  if (read_int(error.data(), s)) throw PsrError(1, "sliced relaxation dependency wait exceeded its spin limit");
+  return 1;
+}
+
+// PSR_RELAX=wave|persistent selects the direct-launch or the dependency-flag
+// relaxation (both give the same bits); the default replays a captured graph.
+static int relax_mode() {
+  static int mode = -1;
+  if (mode < 0) {
+    const char* e = std::getenv("PSR_RELAX");
+    const std::string v = e ? e : "";
+    mode = v == "wave" ? 1 : (v == "persistent" ? 2 : 0);
+  }
+  return mode;
+}
+
+template <class T>
+int relax_sliced_any(const GsOperator<T>& op, const Schedule<T>& sc, T* x, int depth, int iters, hipStream_t s) {
+  switch (relax_mode()) {
+    case 1: return relax_sliced(op, sc, x, depth, iters, s);
+    case 2: return relax_sliced_persistent(op, sc, x, depth, iters, s);
+    default: return relax_sliced_graph(op, sc, x, depth, iters, s);
+  }
+}
+
 template <class T>
 void relax_colored(const GsOperator<T>& op, const Schedule<T>& sc, T* x, int iters, bool forward, hipStream_t s) {
   if (!op.R) return;
-  hipLaunchKernelGGL(k_relax_block<T>, dim3(1), dim3(1024), 0, s, (const int*)sc.order.data(), (const int*)sc.offsets.data(),
+  hipLaunchKernelGGL(k_relax_block<T>, dim3(1), dim3(1024), 0, s, op.update_order, (const int*)sc.offsets.data(),
                      (const int*)op.cols.data(), (const T*)op.vals.data(), (const T*)op.b.data(), (const T*)op.inv_diag.data(),
                      (const unsigned char*)op.has_diag.data(), x, op.R, iters, forward ? 1 : 0);
   KERNEL_CHECK();
 }
 
+// row_order (optional, device): output position j holds row row_order[j]
 template <class T> static Stencil<double> assemble_stiffness(const LevelState<T>& rowsL, const LevelState<T>& colsL, int depth,
-                                                             int trial_depth, hipStream_t s) {
+                                                             int trial_depth, hipStream_t s, const int* row_order = nullptr) {
   Stencil<double> m;
   const int R = rowsL.basis_count;
   m.rows = R;
@@ -773,7 +1152,7 @@ template <class T> static Stencil<double> assemble_stiffness(const LevelState<T>
   KERNEL_CHECK();
   if (R) {
     hipLaunchKernelGGL(k_stiffness, dim3(blocks_for(R, 256)), dim3(256), 0, s, (const int64_t*)rowsL.basis_keys.data(), R,
-                       (const int64_t*)colsL.basis_keys.data(), (int64_t)colsL.basis_count, depth, trial_depth,
+                       row_order, (const int64_t*)colsL.basis_keys.data(), (int64_t)colsL.basis_count, depth, trial_depth,
                        (const D2*)table.data(), m.cols.data(), m.vals.data());
     KERNEL_CHECK();
   }
@@ -809,10 +1188,10 @@ template <class T> static PointSet<T> point_constraints(Result<T>& r, int d, hip
 
 template <class T>
 static void add_screening(Stencil<double>& m, const DArray<int64_t>& rows, const DArray<int64_t>& cols, const PointSet<T>& ps,
-                          int depth, int trial_depth, double* load, hipStream_t s) {
+                          int depth, int trial_depth, double* load, hipStream_t s, const int* row_order = nullptr) {
   if (!m.rows) return;
   hipLaunchKernelGGL(k_screening<T>, dim3(blocks_for(m.rows, 128)), dim3(128), 0, s, (const int64_t*)rows.data(), m.rows,
-                     (const int64_t*)cols.data(), depth, trial_depth, (const uint64_t*)ps.bins.data(),
+                     row_order, (const int64_t*)cols.data(), depth, trial_depth, (const uint64_t*)ps.bins.data(),
                      (const int*)ps.ids.data(), ps.P, (const T*)ps.pos.data(), (const T*)ps.w.data(),
                      (const int*)m.cols.data(), m.vals.data(), load);
   KERNEL_CHECK();
@@ -1040,39 +1419,72 @@ template <class T> void stage_solve(Result<T>& r) {
     LevelState<T>& L = r.levels[d];
     const int R = L.basis_count;
     if (!R) break;
-    Stencil<double> A = assemble_stiffness(L, L, d, d, s);
     PointSet<T> pts;
     L.point_load.reset(R, s);
     L.point_load.zero();
-    if (screened) {
-      pts = point_constraints(r, d, s);
-      add_screening(A, L.basis_keys, L.basis_keys, pts, d, d, L.point_load.data(), s);
-    }
     L.rhs.reset(R, s);
-    hipLaunchKernelGGL(k_rhs<T>, dim3(blocks_for(R, 256)), dim3(256), 0, s, (const double*)L.normal_load.data(),
-                       (const double*)L.point_load.data(), R, L.rhs.data());
-    KERNEL_CHECK();
-    L.A = cast_stencil<T>(std::move(A), s);
     if (d == base) {
+      Stencil<double> A = assemble_stiffness(L, L, d, d, s);
+      if (screened) {
+        pts = point_constraints(r, d, s);
+        add_screening(A, L.basis_keys, L.basis_keys, pts, d, d, L.point_load.data(), s);
+      }
+      hipLaunchKernelGGL(k_rhs<T>, dim3(blocks_for(R, 256)), dim3(256), 0, s, (const double*)L.normal_load.data(),
+                         (const double*)L.point_load.data(), R, L.rhs.data());
+      KERNEL_CHECK();
+      L.A = cast_stencil<T>(std::move(A), s);
       regular_multigrid(r, restr, s);
       cumulative.reset(R, s);
       HIP_CHECK(hipMemcpyAsync(cumulative.data(), L.correction.data(), sizeof(T) * R, hipMemcpyDeviceToDevice, s));
     } else {
-      Stencil<double> cross = assemble_stiffness(L, r.levels[d - 1], d, d - 1, s);
-      if (screened) add_screening(cross, L.basis_keys, r.levels[d - 1].basis_keys, pts, d, d - 1, (double*)nullptr, s);
+      // finer level: operator, cross operator and right-hand side are
+      // assembled directly in sliced schedule order (row j = order[j])
+      Schedule<T> sc = make_schedule<T>(L.basis_keys, R, d, true, s);
+      const int* ord = sc.order.data();
+      Stencil<double> A = assemble_stiffness(L, L, d, d, s, ord);
+      if (screened) {
+        pts = point_constraints(r, d, s);
+        add_screening(A, L.basis_keys, L.basis_keys, pts, d, d, L.point_load.data(), s, ord);
+      }
+      hipLaunchKernelGGL(k_rhs_sched<T>, dim3(blocks_for(R, 256)), dim3(256), 0, s, (const double*)L.normal_load.data(),
+                         (const double*)L.point_load.data(), ord, R, L.rhs.data());
+      KERNEL_CHECK();
+      L.A = cast_stencil<T>(std::move(A), s);
+      Stencil<double> cross = assemble_stiffness(L, r.levels[d - 1], d, d - 1, s, ord);
+      if (screened) add_screening(cross, L.basis_keys, r.levels[d - 1].basis_keys, pts, d, d - 1, (double*)nullptr, s, ord);
       L.cross = cast_stencil<T>(std::move(cross), s);
       hipLaunchKernelGGL(k_spmv<T>, dim3(blocks_for(R, 256)), dim3(256), 0, s, (const int*)L.cross.cols.data(),
                          (const T*)L.cross.vals.data(), R, 27, (const T*)cumulative.data(), L.rhs.data(), T(-1), T(1));
       KERNEL_CHECK();
-      Schedule<T> sc = make_schedule<T>(L.basis_keys, R, d, true, s);
-      GsOperator<T> op = gather_operator<T>(sc, L.A.cols, L.A.vals, L.rhs.data(), R, s);
+      GsOperator<T> op;
+      op.R = R;
+      op.vals = std::move(L.A.vals);
+      op.b = std::move(L.rhs);
+      op.inv_diag.reset(R, s);
+      op.has_diag.reset(R, s);
+      hipLaunchKernelGGL(k_diag_sched<T>, dim3(blocks_for(R, 256)), dim3(256), 0, s, ord, R, (const int*)L.A.cols.data(),
+                         (const T*)op.vals.data(), op.inv_diag.data(), op.has_diag.data());
+      KERNEL_CHECK();
+      // x is relaxed in schedule order: columns become schedule positions, so
+      // a phase writes contiguous ranges and gathers from nearby ranges
+      DArray<T> xs(R, s);
+      xs.zero();  // the level's correction starts at zero (solver.py:343)
+      {
+        DArray<int> inv(R, s);
+        hipLaunchKernelGGL(k_inverse_permutation, dim3(blocks_for(R, 256)), dim3(256), 0, s, ord, R, inv.data());
+        KERNEL_CHECK();
+        op.cols.reset(27 * (size_t)R, s);
+        hipLaunchKernelGGL(k_remap_columns, dim3(blocks_for(27 * (int64_t)R, 256)), dim3(256), 0, s,
+                           (const int*)L.A.cols.data(), (const int*)inv.data(), 27 * (int64_t)R, op.cols.data());
+        KERNEL_CHECK();
+      }
       typename Result<T>::RelaxStat st{d, R, 0, 0, nullptr, nullptr, 0.f};
       {
         DArray<unsigned long long> cnt(1, s);
         cnt.zero();
         const int64_t slots = 27 * (int64_t)R;
         const unsigned grid = (unsigned)std::min<int64_t>(1024, (slots + 255) / 256);
-        hipLaunchKernelGGL(k_count_present, dim3(grid), dim3(256), 0, s, (const int*)L.A.cols.data(), slots, cnt.data());
+        hipLaunchKernelGGL(k_count_present, dim3(grid), dim3(256), 0, s, (const int*)op.cols.data(), slots, cnt.data());
         KERNEL_CHECK();
         unsigned long long nnz = 0;
         read_bytes(&nnz, cnt.data(), sizeof(nnz), s);
@@ -1081,9 +1493,17 @@ template <class T> void stage_solve(Result<T>& r) {
       HIP_CHECK(hipEventCreate(&st.start));
       HIP_CHECK(hipEventCreate(&st.end));
       HIP_CHECK(hipEventRecord(st.start, s));
-      st.launches = relax_sliced(op, sc, L.correction.data(), d, r.prm.iters, s);
+      st.launches = relax_sliced_any(op, sc, xs.data(), d, r.prm.iters, s);
       HIP_CHECK(hipEventRecord(st.end, s));
       r.relax_stats.push_back(st);
+      hipLaunchKernelGGL(k_scatter_rows<T>, dim3(blocks_for(R, 256)), dim3(256), 0, s, ord, R, (const T*)xs.data(),
+                         L.correction.data());
+      KERNEL_CHECK();
+      // keep the schedule-ordered operator (canonical columns) and right-hand side
+      L.A.vals = std::move(op.vals);
+      L.rhs = std::move(op.b);
+      L.order = std::move(sc.order);
+      L.scheduled = true;
       DArray<T> next(R, s);
       HIP_CHECK(hipMemcpyAsync(next.data(), L.correction.data(), sizeof(T) * R, hipMemcpyDeviceToDevice, s));
       hipLaunchKernelGGL(k_spmv<T>, dim3(blocks_for(R, 256)), dim3(256), 0, s, (const int*)L.P.cols.data(),
@@ -1125,7 +1545,7 @@ void relax_sliced_csr(const int64_t* keys, int R, const int* offsets, const int*
   if (read_int(flag.data(), s)) throw PsrError(2, "relaxation rows hold at most 27 entries");
   Schedule<T> sc = make_schedule<T>(k, R, depth, true, s);
   GsOperator<T> op = gather_operator<T>(sc, cols, vals, b, R, s);
-  (void)relax_sliced(op, sc, x, depth, iters, s);
+  (void)relax_sliced_any(op, sc, x, depth, iters, s);
 }
 
 template void relax_sliced_csr<float>(const int64_t*, int, const int*, const int*, const float*, const float*, float*, int, int,
