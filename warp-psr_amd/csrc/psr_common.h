@@ -90,6 +90,43 @@ PSR_HD void find_run(const int64_t* __restrict__ keys, int64_t n, int d, int x, 
   }
 }
 
+// (x, y) column index of one level's sorted keys: keys (d, x, y, *) occupy
+// [start[x*dim + y], end[x*dim + y]).  Built for levels whose dim^2 table is
+// small; a null table falls back to the full-array search.  Same results.
+struct ColumnTable {
+  const int* start = nullptr;
+  const int* end = nullptr;
+  int dim = 0;
+};
+
+template <int MAXLEN>
+PSR_HD void find_run_table(const int64_t* __restrict__ keys, int64_t n, const ColumnTable& t, int d, int x, int y, int z0,
+                           int len, int64_t* out) {
+  if (!t.start) {
+    find_run<MAXLEN>(keys, n, d, x, y, z0, len, out);
+    return;
+  }
+#pragma unroll
+  for (int k = 0; k < MAXLEN; ++k) out[k] = -1;
+  if (x < 0 || y < 0 || x >= t.dim || y >= t.dim) return;
+  const int c = x * t.dim + y;
+  int64_t lo = t.start[c], hi = t.end[c];
+  const int64_t end = hi;
+  const int64_t key0 = basis_key(d, x, y, z0);
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (keys[mid] < key0) lo = mid + 1; else hi = mid;
+  }
+#pragma unroll
+  for (int k = 0; k < MAXLEN; ++k) {
+    if (k >= len) break;
+    if (lo < end && keys[lo] == basis_key(d, x, y, z0 + k)) {
+      out[k] = lo;
+      ++lo;
+    }
+  }
+}
+
 PSR_HD int imin(int a, int b) { return a < b ? a : b; }
 PSR_HD int imax(int a, int b) { return a > b ? a : b; }
 template <class T> PSR_HD T tmin(T a, T b) { return b < a ? b : a; }

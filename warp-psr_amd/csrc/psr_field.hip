@@ -9,35 +9,64 @@ namespace psr {
 
 namespace {
 
-// keys of level d occupy [off[d], off[d+1]) of the concatenated sorted keys
-struct LevelKeys {
+// keys of level d occupy [off[d], off[d+1]) of the concatenated sorted keys;
+// start/end (optional) are the column runs of every level
+struct FieldIndex {
   const int64_t* keys;
   const int* off;  // device, depth+2 entries
+  const int* start;
+  const int* end;
 };
 
-__device__ __forceinline__ int64_t find_level(const int64_t* keys, const int* off, int d, int64_t key) {
-  const int64_t lo = off[d];
-  const int64_t idx = find_key(keys + lo, (int64_t)off[d + 1] - lo, key);
-  return idx >= 0 ? idx + lo : -1;
+// first cell of level d's column table: sum over e < d of (2^e + 1)^2
+PSR_HD int64_t column_base(int d) { return (((int64_t)1 << (2 * d)) - 1) / 3 + 2 * (((int64_t)1 << d) - 1) + d; }
+constexpr int FIELD_TABLE_MAX_DEPTH = 11;
+
+// Absolute indices of keys (d, x, y, z0 + k), k < len (-1 when absent).
+template <int MAXLEN>
+__device__ __forceinline__ void level_run(const FieldIndex& f, int d, int x, int y, int z0, int len, int64_t* out) {
+  if (f.start) {
+    ColumnTable t;
+    t.start = f.start + column_base(d);
+    t.end = f.end + column_base(d);
+    t.dim = (1 << d) + 1;
+    find_run_table<MAXLEN>(f.keys, 0, t, d, x, y, z0, len, out);
+    return;
+  }
+  const int64_t lo = f.off[d];
+  find_run<MAXLEN>(f.keys + lo, (int64_t)f.off[d + 1] - lo, d, x, y, z0, len, out);
+#pragma unroll
+  for (int k = 0; k < MAXLEN; ++k)
+    if (k < len && out[k] >= 0) out[k] += lo;
+}
+
+__global__ void k_field_columns(const int64_t* __restrict__ keys, int64_t n, int* __restrict__ start, int* __restrict__ end) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  auto cell = [&](int64_t k) {
+    const int d = key_d(k);
+    return column_base(d) + (int64_t)key_x(k) * ((1 << d) + 1) + key_y(k);
+  };
+  const int64_t c = cell(keys[i]);
+  if (i == 0 || cell(keys[i - 1]) != c) start[c] = (int)i;
+  if (i == n - 1 || cell(keys[i + 1]) != c) end[c] = (int)(i + 1);
 }
 
 template <class T>
-__global__ void k_cumulative(const int64_t* __restrict__ keys, const int* __restrict__ off, int d, const T* __restrict__ coef,
-                             T* __restrict__ cum) {
-  const int64_t i = off[d] + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= off[d + 1]) return;
-  const int64_t key = keys[i];
+__global__ void k_cumulative(FieldIndex f, int d, const T* __restrict__ coef, T* __restrict__ cum) {
+  const int64_t i = f.off[d] + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= f.off[d + 1]) return;
+  const int64_t key = f.keys[i];
   const int x = key_x(key), y = key_y(key), z = key_z(key);
-  const int64_t lo = off[d - 1], cnt = (int64_t)off[d] - lo;
   T value = T(0);
   for (int a = 0; a < 2; ++a)
     for (int b = 0; b < 2; ++b) {  // corners a*4 + b*2 + c in order; one lookup per (x, y) column
       int64_t found[2];
-      find_run<2>(keys + lo, cnt, d - 1, x / 2 + a, y / 2 + b, z / 2, 2, found);
+      level_run<2>(f, d - 1, x / 2 + a, y / 2 + b, z / 2, 2, found);
       for (int c = 0; c < 2; ++c) {
         const T h = (T(1) - fabs(T(x) * T(.5) - T(x / 2 + a))) * (T(1) - fabs(T(y) * T(.5) - T(y / 2 + b))) *
                     (T(1) - fabs(T(z) * T(.5) - T(z / 2 + c)));
-        if (h > T(0) && found[c] >= 0) value = value + cum[lo + found[c]] * h;
+        if (h > T(0) && found[c] >= 0) value = value + cum[found[c]] * h;
       }
     }
   cum[i] = value + coef[i];
@@ -47,31 +76,29 @@ template <class T> PSR_HD int clampi(int v, int lo, int hi) { return v < lo ? lo
 
 // field.py:371-393 (add_level)
 template <class T>
-__device__ __forceinline__ void add_level(const int64_t* keys, const int* off, const T* coef, T p0, T p1, T p2, int d, T* res) {
+__device__ __forceinline__ void add_level(const FieldIndex& f, const T* coef, T p0, T p1, T p2, int d, T* res) {
   const int n = 1 << d;
   const T t[3] = {p0 * T(n), p1 * T(n), p2 * T(n)};
   int cell[3], base[3];
-  T f[3];
+  T fr[3];
   for (int a = 0; a < 3; ++a) {
     cell[a] = clampi<T>((int)ceil(t[a]) - 1, 0, n - 1);
     base[a] = (int)floor(t[a]);
-    f[a] = t[a] - T(base[a]);
+    fr[a] = t[a] - T(base[a]);
   }
-  const int64_t lo = off[d], cnt = (int64_t)off[d + 1] - lo;
   int64_t found[2][2][2];
   for (int a = 0; a < 2; ++a)
-    for (int b = 0; b < 2; ++b) find_run<2>(keys + lo, cnt, d, cell[0] + a, cell[1] + b, cell[2], 2, found[a][b]);
+    for (int b = 0; b < 2; ++b) level_run<2>(f, d, cell[0] + a, cell[1] + b, cell[2], 2, found[a][b]);
   for (int corner = 0; corner < 8; ++corner) {
     const int node[3] = {cell[0] + corner / 4, cell[1] + (corner / 2) % 2, cell[2] + corner % 2};
     if (imin(imin(node[0] - base[0], node[1] - base[1]), node[2] - base[2]) < 0) continue;
-    const int64_t fidx = found[corner / 4][(corner / 2) % 2][corner % 2];
-    const int64_t idx = fidx >= 0 ? fidx + lo : -1;
+    const int64_t idx = found[corner / 4][(corner / 2) % 2][corner % 2];
     if (idx < 0) continue;
     T h[3], sg[3];
     for (int a = 0; a < 3; ++a) {
-      h[a] = T(1) - f[a];
+      h[a] = T(1) - fr[a];
       sg[a] = T(-1);
-      if (node[a] > base[a]) { h[a] = f[a]; sg[a] = T(1); }
+      if (node[a] > base[a]) { h[a] = fr[a]; sg[a] = T(1); }
     }
     const T c = coef[idx];
     res[0] = res[0] + c * h[0] * h[1] * h[2];
@@ -85,8 +112,7 @@ template <class T> struct Affine { T L[9]; T t[3]; };
 
 // field.py:395-420 (reference evaluation kernel)
 template <class T>
-__global__ void k_evaluate(const int64_t* __restrict__ keys, const int* __restrict__ off, const T* __restrict__ coef,
-                           const T* __restrict__ cum, int depth, const T* __restrict__ queries, int64_t m, Affine<T> A,
+__global__ void k_evaluate(FieldIndex f, const T* __restrict__ coef, const T* __restrict__ cum, int depth, const T* __restrict__ queries, int64_t m, Affine<T> A,
                            int point_div_mode, const T* __restrict__ divisors, T* __restrict__ values, T* __restrict__ grads,
                            int* __restrict__ valid) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -112,22 +138,24 @@ __global__ void k_evaluate(const int64_t* __restrict__ keys, const int* __restri
     const int c0 = clampi<T>((int)ceil(p[0] * T(n)) - 1, 0, n - 1);
     const int c1 = clampi<T>((int)ceil(p[1] * T(n)) - 1, 0, n - 1);
     const int c2 = clampi<T>((int)ceil(p[2] * T(n)) - 1, 0, n - 1);
-    if (find_level(keys, off, d, basis_key(d, c0, c1, c2)) >= 0) { leaf = d; break; }
+    int64_t idx;
+    level_run<1>(f, d, c0, c1, c2, 1, &idx);
+    if (idx >= 0) { leaf = d; break; }
   }
   for (int a = 0; a < 3; ++a) {
     if (p[a] == T(0)) p[a] = T(1e-6);
     else if (p[a] == T(1)) p[a] = T(1.0 - 1e-6);
   }
   T res[4] = {T(0), T(0), T(0), T(0)};
-  add_level(keys, off, coef, p[0], p[1], p[2], leaf, res);
+  add_level(f, coef, p[0], p[1], p[2], leaf, res);
   if (leaf > 0) {
     T r2[4] = {T(0), T(0), T(0), T(0)};
-    add_level(keys, off, cum, p[0], p[1], p[2], leaf - 1, r2);
+    add_level(f, cum, p[0], p[1], p[2], leaf - 1, r2);
     for (int a = 0; a < 4; ++a) res[a] = res[a] + r2[a];
   }
   if (leaf < depth) {
     T r2[4] = {T(0), T(0), T(0), T(0)};
-    add_level(keys, off, coef, p[0], p[1], p[2], leaf + 1, r2);
+    add_level(f, coef, p[0], p[1], p[2], leaf + 1, r2);
     for (int a = 0; a < 4; ++a) res[a] = res[a] + r2[a];
   }
   values[i] = res[0];
@@ -225,17 +253,24 @@ template <class T> void reference_inverse(const T* m, T* out) {
     }
 }
 
+template <class T> static FieldIndex field_index(const Result<T>& r) {
+  FieldIndex f;
+  f.keys = r.keys.data();
+  f.off = r.offsets_dev.data();
+  f.start = r.column_start.empty() ? nullptr : r.column_start.data();
+  f.end = r.column_start.empty() ? nullptr : r.column_end.data();
+  return f;
+}
+
 template <class T>
 void evaluate_field(const Result<T>& r, const T* queries, int64_t m, const T* linear, const T* translation, T* values, T* grads,
                     int* valid, hipStream_t s) {
   if (m <= 0) return;
-  DArray<int> off(r.level_offsets.size(), s);
-  HIP_CHECK(hipMemcpyAsync(off.data(), r.level_offsets.data(), sizeof(int) * r.level_offsets.size(), hipMemcpyHostToDevice, s));
   Affine<T> A;
   for (int k = 0; k < 9; ++k) A.L[k] = linear[k];
   for (int k = 0; k < 3; ++k) A.t[k] = translation[k];
-  hipLaunchKernelGGL(k_evaluate<T>, dim3(blocks_for(m, 256)), dim3(256), 0, s, (const int64_t*)r.keys.data(),
-                     (const int*)off.data(), (const T*)r.coefficients.data(), (const T*)r.cumulative.data(), r.prm.depth,
+  hipLaunchKernelGGL(k_evaluate<T>, dim3(blocks_for(m, 256)), dim3(256), 0, s, field_index(r),
+                     (const T*)r.coefficients.data(), (const T*)r.cumulative.data(), r.prm.depth,
                      queries, m, A, 0, (const T*)nullptr, values, grads, valid);
   KERNEL_CHECK();
 }
@@ -261,15 +296,29 @@ template <class T> void stage_field(Result<T>& r) {
     else
       HIP_CHECK(hipMemsetAsync(r.coefficients.data() + o, 0, sizeof(T) * L.basis_count, s));
   }
-  DArray<int> off(D + 2, s);
-  HIP_CHECK(hipMemcpyAsync(off.data(), r.level_offsets.data(), sizeof(int) * (D + 2), hipMemcpyHostToDevice, s));
+  r.offsets_dev.reset(D + 2, s);
+  HIP_CHECK(hipMemcpyAsync(r.offsets_dev.data(), r.level_offsets.data(), sizeof(int) * (D + 2), hipMemcpyHostToDevice, s));
+  if (D <= FIELD_TABLE_MAX_DEPTH && K) {
+    const size_t cells = (size_t)column_base(D + 1);
+    r.column_start.reset(cells, s);
+    r.column_end.reset(cells, s);
+    r.column_start.zero();
+    r.column_end.zero();
+    hipLaunchKernelGGL(k_field_columns, dim3(blocks_for(K, 256)), dim3(256), 0, s, (const int64_t*)r.keys.data(), K,
+                       r.column_start.data(), r.column_end.data());
+    KERNEL_CHECK();
+  } else {
+    r.column_start.free();
+    r.column_end.free();
+  }
+  const FieldIndex f = field_index(r);
   r.cumulative.reset(K, s);
   if (K) HIP_CHECK(hipMemcpyAsync(r.cumulative.data(), r.coefficients.data(), sizeof(T) * K, hipMemcpyDeviceToDevice, s));
   for (int d = 1; d <= D; ++d) {
     const int cnt = r.levels[d].basis_count;
     if (!cnt) continue;
-    hipLaunchKernelGGL(k_cumulative<T>, dim3(blocks_for(cnt, 256)), dim3(256), 0, s, (const int64_t*)r.keys.data(),
-                       (const int*)off.data(), d, (const T*)r.coefficients.data(), r.cumulative.data());
+    hipLaunchKernelGGL(k_cumulative<T>, dim3(blocks_for(cnt, 256)), dim3(256), 0, s, f, d, (const T*)r.coefficients.data(),
+                       r.cumulative.data());
     KERNEL_CHECK();
   }
   // centroid values with the cube-space field (reconstruction.py:295-303)
@@ -278,8 +327,8 @@ template <class T> void stage_field(Result<T>& r) {
   Affine<T> I;
   for (int k = 0; k < 9; ++k) I.L[k] = (k % 4 == 0) ? T(1) : T(0);
   for (int k = 0; k < 3; ++k) I.t[k] = T(0);
-  hipLaunchKernelGGL(k_evaluate<T>, dim3(blocks_for(G, 256)), dim3(256), 0, s, (const int64_t*)r.keys.data(),
-                     (const int*)off.data(), (const T*)r.coefficients.data(), (const T*)r.cumulative.data(), D,
+  hipLaunchKernelGGL(k_evaluate<T>, dim3(blocks_for(G, 256)), dim3(256), 0, s, f, (const T*)r.coefficients.data(),
+                     (const T*)r.cumulative.data(), D,
                      (const T*)r.position_sums.data(), (int64_t)G, I, 1, (const T*)r.weights.data(), r.centroid_values.data(),
                      (T*)nullptr, (int*)nullptr);
   KERNEL_CHECK();

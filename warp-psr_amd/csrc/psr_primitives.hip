@@ -271,6 +271,37 @@ struct PinnedScratch {
 thread_local PinnedScratch g_pinned;
 }  // namespace
 
+namespace {
+__device__ __forceinline__ int column_of(int64_t k, int dim) { return key_x(k) * dim + key_y(k); }
+
+__global__ void k_column_index(const int64_t* __restrict__ keys, int64_t n, int dim, int* __restrict__ start,
+                               int* __restrict__ end) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int c = column_of(keys[i], dim);
+  if (i == 0 || column_of(keys[i - 1], dim) != c) start[c] = (int)i;
+  if (i == n - 1 || column_of(keys[i + 1], dim) != c) end[c] = (int)(i + 1);
+}
+}  // namespace
+
+void build_column_index(const int64_t* keys, int64_t n, int dim, ColumnIndex& out, hipStream_t s) {
+  out.dim = 0;
+  if (dim <= 0 || dim > COLUMN_TABLE_MAX_DIM || n <= 0) {
+    out.start.free();
+    out.end.free();
+    return;
+  }
+  const size_t cells = (size_t)dim * dim;
+  out.start.reset(cells, s);
+  out.end.reset(cells, s);
+  out.start.zero();
+  out.end.zero();
+  hipLaunchKernelGGL(k_column_index, dim3(blocks_for(n, 256)), dim3(256), 0, s, keys, n, dim, out.start.data(),
+                     out.end.data());
+  KERNEL_CHECK();
+  out.dim = dim;
+}
+
 void read_bytes(void* host, const void* dev, size_t bytes, hipStream_t s) {
   if (!bytes) return;
   if (bytes <= (1 << 16) && g_pinned.p) {

@@ -41,6 +41,7 @@ template <class T> struct RelaxOperator {
 template <class T> struct LevelState {
   DArray<int64_t> tree_keys, basis_keys;
   int basis_count = 0;
+  ColumnIndex columns;  // (x, y) runs of basis_keys
   // aggregates (density.py:228-240): keys and (sum x, sum y, sum z, count)
   DArray<int64_t> agg_keys;
   DArray<T> agg_vals;  // 4 per record
@@ -100,6 +101,10 @@ template <class T> struct Result {
   DArray<int64_t> keys;
   DArray<T> coefficients, cumulative, centroid_values;
   std::vector<int> level_offsets;  // concatenated key offsets per depth
+  DArray<int> offsets_dev;         // the same on the device
+  // (x, y) column runs of every level of keys (absolute indices), laid out
+  // level after level with (2^d + 1)^2 cells each; empty above depth 11
+  DArray<int> column_start, column_end;
   T iso;
   T cube_to_model[16], model_to_cube[16];
   // timings (seconds) per stage, from HIP events

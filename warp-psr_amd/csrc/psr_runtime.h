@@ -164,6 +164,21 @@ void read_bytes(void* host, const void* dev, size_t bytes, hipStream_t s);
 void fill_iota(int* out, int64_t n, hipStream_t s);
 // After heads + exclusive scan: compacted positions of heads (head_pos[g] = i).
 void compact_heads(const int* heads, const int* prefix, int64_t n, int* head_pos, hipStream_t s);
+// (x, y) column index over one level's sorted, same-depth keys whose x and y
+// lie in [0, dim): start/end of each column's run.  Levels whose dim exceeds
+// max_dim get no table (view() is then null and lookups search the array).
+struct ColumnIndex {
+  DArray<int> start, end;
+  int dim = 0;
+  ColumnTable view() const {
+    ColumnTable t;
+    if (dim) { t.start = start.data(); t.end = end.data(); t.dim = dim; }
+    return t;
+  }
+};
+void build_column_index(const int64_t* keys, int64_t n, int dim, ColumnIndex& out, hipStream_t s);
+constexpr int COLUMN_TABLE_MAX_DIM = 4097;  // depth 12 basis coordinates
+
 inline int bits_for(uint64_t maxval) {
   int b = 0;
   while (b < 64 && (maxval >> b) != 0) ++b;

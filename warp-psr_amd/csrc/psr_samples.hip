@@ -330,7 +330,7 @@ __global__ void k_density_splat_reduce(const uint64_t* __restrict__ keys, const 
 
 // density.py:108-150 (density_value at each aggregate centroid)
 template <class T>
-__global__ void k_density_eval(const int64_t* __restrict__ lkeys, const T* __restrict__ lvals, int64_t lcount,
+__global__ void k_density_eval(const int64_t* __restrict__ lkeys, const T* __restrict__ lvals, int64_t lcount, ColumnTable ct,
                                const T* __restrict__ sums, const T* __restrict__ weights, int G, int d, int stride,
                                T* __restrict__ out) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -350,7 +350,7 @@ __global__ void k_density_eval(const int64_t* __restrict__ lkeys, const T* __res
       const int tx = cx + x - 1, ty = cy + y - 1;
       if (tx < 0 || ty < 0 || tx >= n || ty >= n || zlo > zhi) continue;
       int64_t found[3];
-      find_run<3>(lkeys, lcount, d, tx, ty, zlo, zhi - zlo + 1, found);
+      find_run_table<3>(lkeys, lcount, ct, d, tx, ty, zlo, zhi - zlo + 1, found);
       for (int z = 0; z < 3; ++z) {
         const int tz = cz + z - 1;
         if (tz < zlo || tz > zhi) continue;
@@ -469,8 +469,10 @@ template <class T> void stage_density(Result<T>& r) {
                        total_records, d, (const int64_t*)L.agg_keys.data(), (const T*)L.agg_vals.data(),
                        (const int*)order.data(), r.density_scale, L.dens_keys.data(), L.dens_vals.data());
     KERNEL_CHECK();
+    ColumnIndex columns;
+    build_column_index((const int64_t*)L.dens_keys.data(), count, (1 << d) + 1, columns, s);
     hipLaunchKernelGGL(k_density_eval<T>, dim3(blocks_for(G, 256)), dim3(256), 0, s, (const int64_t*)L.dens_keys.data(),
-                       (const T*)L.dens_vals.data(), (int64_t)count, (const T*)r.position_sums.data(),
+                       (const T*)L.dens_vals.data(), (int64_t)count, columns.view(), (const T*)r.position_sums.data(),
                        (const T*)r.weights.data(), G, d, kd + 1, r.densities.data());
     KERNEL_CHECK();
   }

@@ -117,7 +117,7 @@ __global__ void k_normal_table(int test_depth, int normal_depth, int first, int 
 
 // ---------------------------------------------------------------- stiffness (stiffness.py:38-109)
 __global__ void k_stiffness(const int64_t* __restrict__ rows, int R, const int* __restrict__ row_order,
-                            const int64_t* __restrict__ cols, int64_t C, int depth, int trial_depth,
+                            const int64_t* __restrict__ cols, int64_t C, ColumnTable ct, int depth, int trial_depth,
                             const D2* __restrict__ table, int* __restrict__ out_cols, double* __restrict__ out_vals) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;  // output position
   if (j >= R) return;
@@ -134,7 +134,7 @@ __global__ void k_stiffness(const int64_t* __restrict__ rows, int R, const int* 
       const int qy = py + ky - 1;
       int64_t found[3] = {-1, -1, -1};
       if (qx >= 0 && qx <= lim && qy >= 0 && qy <= lim && zlo <= zhi)
-        find_run<3>(cols, C, trial_depth, qx, qy, zlo, zhi - zlo + 1, found);
+        find_run_table<3>(cols, C, ct, trial_depth, qx, qy, zlo, zhi - zlo + 1, found);
       for (int kz = 0; kz < 3; ++kz) {
         const int qz = pz + kz - 1;
         const int k = kx * 9 + ky * 3 + kz;
@@ -186,13 +186,30 @@ template <class T> PSR_HD T hat3(T p0, T p1, T p2, int cx, int cy, int cz, int d
   return h0 * h1 * h2;
 }
 
-// screening_stencils.py:34-70 (add): per row, points binned in the 8 cells at its vertex
+// Column runs of the sorted point bins (bx << 2d | by << d | bz): points of
+// column (bx, by) occupy [start[bx*n + by], end[bx*n + by]).
+__global__ void k_bin_columns(const uint64_t* __restrict__ bins, int P, int depth, int* __restrict__ start,
+                              int* __restrict__ end) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P) return;
+  const uint64_t c = bins[i] >> depth;
+  if (i == 0 || (bins[i - 1] >> depth) != c) start[c] = i;
+  if (i == P - 1 || (bins[i + 1] >> depth) != c) end[c] = i + 1;
+}
+
+// screening_stencils.py:34-70 (add): per row, points binned in the 8 cells at
+// its vertex.  One pass over the points with the 27 slot sums (and the point
+// load) held in registers; every sum receives its terms in the reference's
+// (cell, point) order.  Present slot k holds the trial function
+// (p + (kx-1, ky-1, kz-1)) with p the row's coordinates at trial_depth (the
+// stiffness slot layout), so its hat factors are shared per axis.
 template <class T>
-__global__ void k_screening(const int64_t* __restrict__ rows, int R, const int* __restrict__ row_order,
-                            const int64_t* __restrict__ cols_keys, int depth, int trial_depth,
-                            const uint64_t* __restrict__ bin_keys, const int* __restrict__ bin_ids, int P,
-                            const T* __restrict__ pos, const T* __restrict__ w, const int* __restrict__ scols,
-                            double* __restrict__ svals, double* __restrict__ load) {
+__global__ void __launch_bounds__(128) k_screening(const int64_t* __restrict__ rows, int R, const int* __restrict__ row_order,
+                                                   int depth, int trial_depth, const uint64_t* __restrict__ bin_keys,
+                                                   const int* __restrict__ bin_ids, int P, ColumnTable bin_cols,
+                                                   const T* __restrict__ pos, const T* __restrict__ w,
+                                                   const int* __restrict__ scols, double* __restrict__ svals,
+                                                   double* __restrict__ load) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;  // output position
   if (j >= R) return;
   const int i = row_order ? row_order[j] : j;
@@ -203,7 +220,9 @@ __global__ void k_screening(const int64_t* __restrict__ rows, int R, const int* 
   // k = 0..7 -> cell (cx - k/4, cy - k/2%2, cz - k%2); bins are sorted, so the
   // bins (cz-1) and (cz) of one column are adjacent and found with one search
   int rb[8], re[8];
+#pragma unroll
   for (int a = 0; a < 2; ++a)
+#pragma unroll
     for (int bb = 0; bb < 2; ++bb) {
       const int k0 = a * 4 + bb * 2;
       rb[k0] = re[k0] = rb[k0 + 1] = re[k0 + 1] = 0;
@@ -212,6 +231,11 @@ __global__ void k_screening(const int64_t* __restrict__ rows, int R, const int* 
       const uint64_t col = ((uint64_t)bx << (2 * depth)) | ((uint64_t)by << depth);
       const int zfirst = cz >= 1 ? cz - 1 : cz;
       int lo = 0, hi = P;
+      if (bin_cols.start) {
+        lo = bin_cols.start[bx * n + by];
+        hi = bin_cols.end[bx * n + by];
+      }
+      const int stop = hi;
       const uint64_t first_key = col | (uint64_t)zfirst;
       while (lo < hi) {
         const int mid = (lo + hi) >> 1;
@@ -219,47 +243,51 @@ __global__ void k_screening(const int64_t* __restrict__ rows, int R, const int* 
       }
       if (cz >= 1) {
         rb[k0 + 1] = lo;
-        while (lo < P && bin_keys[lo] == (col | (uint64_t)(cz - 1))) ++lo;
+        while (lo < stop && bin_keys[lo] == (col | (uint64_t)(cz - 1))) ++lo;
         re[k0 + 1] = lo;
       }
       if (cz < n) {
         rb[k0] = lo;
-        while (lo < P && bin_keys[lo] == (col | (uint64_t)cz)) ++lo;
+        while (lo < stop && bin_keys[lo] == (col | (uint64_t)cz)) ++lo;
         re[k0] = lo;
       }
     }
+  const int shift = depth - trial_depth;
+  const int px = (cx >> shift) - 1, py = (cy >> shift) - 1, pz = (cz >> shift) - 1;
+  const T st = T(1 << trial_depth);
+  double acc[27];
+#pragma unroll
+  for (int k = 0; k < 27; ++k) acc[k] = 0.0;
   double rhs = 0.0;
   for (int kk = 0; kk < 8; ++kk)
     for (int jj = rb[kk]; jj < re[kk]; ++jj) {
       const int index = bin_ids[jj];
       const T weight = w[index];
       if (weight != T(0)) {
-        const T phi = hat3(pos[3 * index], pos[3 * index + 1], pos[3 * index + 2], cx, cy, cz, depth);
+        const T p0 = pos[3 * index], p1 = pos[3 * index + 1], p2 = pos[3 * index + 2];
+        const T phi = hat3(p0, p1, p2, cx, cy, cz, depth);
         rhs += double(phi * weight);
+        T hx[3], hy[3], hz[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          const T h0 = T(1) - fabs(p0 * st - T(px + a));
+          const T h1 = T(1) - fabs(p1 * st - T(py + a));
+          const T h2 = T(1) - fabs(p2 * st - T(pz + a));
+          hx[a] = h0 > T(0) ? h0 : T(0);
+          hy[a] = h1 > T(0) ? h1 : T(0);
+          hz[a] = h2 > T(0) ? h2 : T(0);
+        }
+#pragma unroll
+        for (int k = 0; k < 27; ++k) {
+          const T trial = hx[k / 9] * hy[(k / 3) % 3] * hz[k % 3];
+          acc[k] += double(phi * trial * weight);
+        }
       }
     }
   if (load) load[j] = rhs;
-  // one accumulator at a time (slot outer): per slot the points arrive in the
-  // same (cell, point) order as the reference's point-outer loop
-  for (int k = 0; k < 27; ++k) {
-    const int c = scols[(int64_t)k * R + j];
-    if (c < 0) continue;
-    const int64_t ck = cols_keys[c];
-    const int qx = key_x(ck), qy = key_y(ck), qz = key_z(ck);
-    double acc = 0.0;
-    for (int kk = 0; kk < 8; ++kk)
-      for (int jj = rb[kk]; jj < re[kk]; ++jj) {
-        const int index = bin_ids[jj];
-        const T weight = w[index];
-        if (weight != T(0)) {
-          const T p0 = pos[3 * index], p1 = pos[3 * index + 1], p2 = pos[3 * index + 2];
-          const T phi = hat3(p0, p1, p2, cx, cy, cz, depth);
-          const T trial = hat3(p0, p1, p2, qx, qy, qz, trial_depth);
-          acc += double(phi * trial * weight);
-        }
-      }
-    svals[(int64_t)k * R + j] += acc;
-  }
+#pragma unroll
+  for (int k = 0; k < 27; ++k)
+    if (scols[(int64_t)k * R + j] >= 0) svals[(int64_t)k * R + j] += acc[k];
 }
 
 template <class T>
@@ -277,7 +305,7 @@ __global__ void k_rhs(const double* __restrict__ load, const double* __restrict_
 
 // ---------------------------------------------------------------- prolongation (transfer_stencils.py:15-66)
 template <class T>
-__global__ void k_prolongation(const int64_t* __restrict__ fine, int R, const int64_t* __restrict__ coarse, int64_t C, int depth,
+__global__ void k_prolongation(const int64_t* __restrict__ fine, int R, const int64_t* __restrict__ coarse, int64_t C, ColumnTable ct, int depth,
                                int* __restrict__ cols, T* __restrict__ vals) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= R) return;
@@ -290,7 +318,7 @@ __global__ void k_prolongation(const int64_t* __restrict__ fine, int R, const in
     for (int b = 0; b < 2; ++b) {
       const T wz0 = h[0][a] * h[1][b] * h[2][0], wz1 = h[0][a] * h[1][b] * h[2][1];
       int64_t found[2] = {-1, -1};
-      if (wz0 > T(0) || wz1 > T(0)) find_run<2>(coarse, C, depth - 1, f[0] / 2 + a, f[1] / 2 + b, f[2] / 2, 2, found);
+      if (wz0 > T(0) || wz1 > T(0)) find_run_table<2>(coarse, C, ct, depth - 1, f[0] / 2 + a, f[1] / 2 + b, f[2] / 2, 2, found);
       for (int c = 0; c < 2; ++c) {
         const int k = a * 4 + b * 2 + c;
         const T weight = c ? wz1 : wz0;
@@ -305,7 +333,7 @@ __global__ void k_prolongation(const int64_t* __restrict__ fine, int R, const in
 // o in {-1,0,1}^3 (ascending fine index = lexicographic o), weight 1/2 per
 // nonzero axis offset; exactly the rows of the transposed prolongation.
 __global__ void k_restriction(const int64_t* __restrict__ coarse, int C, const int64_t* __restrict__ fine, int64_t F,
-                              int fine_depth, int* __restrict__ cols) {
+                              ColumnTable ct, int fine_depth, int* __restrict__ cols) {
   int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= C) return;
   const int64_t key = coarse[r];
@@ -317,7 +345,7 @@ __global__ void k_restriction(const int64_t* __restrict__ coarse, int C, const i
       const int fx = x + kx - 1, fy = y + ky - 1;
       int64_t found[3] = {-1, -1, -1};
       if (fx >= 0 && fx <= lim && fy >= 0 && fy <= lim && zlo <= zhi)
-        find_run<3>(fine, F, fine_depth, fx, fy, zlo, zhi - zlo + 1, found);
+        find_run_table<3>(fine, F, ct, fine_depth, fx, fy, zlo, zhi - zlo + 1, found);
       for (int kz = 0; kz < 3; ++kz) {
         const int fz = z + kz - 1;
         cols[(int64_t)(kx * 9 + ky * 3 + kz) * C + r] = (fz >= zlo && fz <= zhi) ? (int)found[fz - zlo] : -1;
@@ -365,7 +393,7 @@ __global__ void k_spmv(const int* __restrict__ cols, const T* __restrict__ vals,
 
 // ---------------------------------------------------------------- normal loads (normal_stencils.py, normal_loads.py)
 template <class V>
-__global__ void k_normal_load(const int64_t* __restrict__ keys, int R, const int64_t* __restrict__ nkeys, int64_t ncount,
+__global__ void k_normal_load(const int64_t* __restrict__ keys, int R, const int64_t* __restrict__ nkeys, int64_t ncount, ColumnTable ct,
                               const V* __restrict__ nvals, int nstride, int test_depth, int normal_depth, int first, int width,
                               const D2* __restrict__ table, double* __restrict__ out) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -384,7 +412,7 @@ __global__ void k_normal_load(const int64_t* __restrict__ keys, int R, const int
       const int ny = e1 + y + first;
       if (ny < 0 || ny >= lim || zlo > zhi) continue;
       int64_t found[6];
-      find_run<6>(nkeys, ncount, normal_depth, nx, ny, zlo, zhi - zlo + 1, found);
+      find_run_table<6>(nkeys, ncount, ct, normal_depth, nx, ny, zlo, zhi - zlo + 1, found);
       for (int z = 0; z < width; ++z) {
         const int nz = e2 + z + first;
         if (nz < zlo || nz > zhi) continue;
@@ -404,19 +432,20 @@ __global__ void k_normal_load(const int64_t* __restrict__ keys, int R, const int
 // normal_loads.py:27-63 (prolong degree-2 coefficients with degree-1 weights, issue #35)
 template <class T>
 __global__ void k_normal_prolong(const int64_t* __restrict__ keys, int R, const int64_t* __restrict__ own_keys, int64_t own_count,
-                                 const T* __restrict__ own_vals, const int64_t* __restrict__ prev_keys, int64_t prev_count,
+                                 ColumnTable own_ct, const T* __restrict__ own_vals, const int64_t* __restrict__ prev_keys, int64_t prev_count, ColumnTable ct,
                                  const T* __restrict__ prev_vals, int depth, T* __restrict__ out) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= R) return;
   const int64_t key = keys[i];
   const int c[3] = {key_x(key), key_y(key), key_z(key)};
   T v0 = T(0), v1 = T(0), v2 = T(0);
-  const int64_t own = find_key(own_keys, own_count, key);
+  int64_t own;
+  find_run_table<1>(own_keys, own_count, own_ct, depth, c[0], c[1], c[2], 1, &own);
   if (own >= 0) { v0 = own_vals[3 * own]; v1 = own_vals[3 * own + 1]; v2 = own_vals[3 * own + 2]; }
   if (depth > 0) {
     int64_t idx[2][2][2];  // [x bit][y bit][z bit]
     for (int a = 0; a < 2; ++a)
-      for (int b = 0; b < 2; ++b) find_run<2>(prev_keys, prev_count, depth - 1, c[0] / 2 + a, c[1] / 2 + b, c[2] / 2, 2, idx[a][b]);
+      for (int b = 0; b < 2; ++b) find_run_table<2>(prev_keys, prev_count, ct, depth - 1, c[0] / 2 + a, c[1] / 2 + b, c[2] / 2, 2, idx[a][b]);
     for (int corner = 0; corner < 8; ++corner) {  // x is bit 0 here (normal_loads.py:285-297)
       int p[3] = {c[0] / 2, c[1] / 2, c[2] / 2};
       T weight = T(1);
@@ -1152,7 +1181,8 @@ template <class T> static Stencil<double> assemble_stiffness(const LevelState<T>
   KERNEL_CHECK();
   if (R) {
     hipLaunchKernelGGL(k_stiffness, dim3(blocks_for(R, 256)), dim3(256), 0, s, (const int64_t*)rowsL.basis_keys.data(), R,
-                       row_order, (const int64_t*)colsL.basis_keys.data(), (int64_t)colsL.basis_count, depth, trial_depth,
+                       row_order, (const int64_t*)colsL.basis_keys.data(), (int64_t)colsL.basis_count, colsL.columns.view(),
+                       depth, trial_depth,
                        (const D2*)table.data(), m.cols.data(), m.vals.data());
     KERNEL_CHECK();
   }
@@ -1164,6 +1194,7 @@ template <class T> struct PointSet {
   DArray<T> pos, w;
   DArray<uint64_t> bins;
   DArray<int> ids;
+  ColumnIndex columns;  // (bx, by) runs of the sorted bins
 };
 
 template <class T> static PointSet<T> point_constraints(Result<T>& r, int d, hipStream_t s) {
@@ -1182,17 +1213,28 @@ template <class T> static PointSet<T> point_constraints(Result<T>& r, int d, hip
                        ps.bins.data(), ps.ids.data());
     KERNEL_CHECK();
     sort_pairs_u64(ps.bins.data(), ps.ids.data(), ps.P, 3 * d, s);
+    const int n = 1 << d;
+    if (n <= COLUMN_TABLE_MAX_DIM) {
+      ps.columns.start.reset((size_t)n * n, s);
+      ps.columns.end.reset((size_t)n * n, s);
+      ps.columns.start.zero();
+      ps.columns.end.zero();
+      hipLaunchKernelGGL(k_bin_columns, dim3(blocks_for(ps.P, 256)), dim3(256), 0, s, (const uint64_t*)ps.bins.data(), ps.P, d,
+                         ps.columns.start.data(), ps.columns.end.data());
+      KERNEL_CHECK();
+      ps.columns.dim = n;
+    }
   }
   return ps;
 }
 
 template <class T>
-static void add_screening(Stencil<double>& m, const DArray<int64_t>& rows, const DArray<int64_t>& cols, const PointSet<T>& ps,
-                          int depth, int trial_depth, double* load, hipStream_t s, const int* row_order = nullptr) {
+static void add_screening(Stencil<double>& m, const DArray<int64_t>& rows, const PointSet<T>& ps, int depth, int trial_depth,
+                          double* load, hipStream_t s, const int* row_order = nullptr) {
   if (!m.rows) return;
   hipLaunchKernelGGL(k_screening<T>, dim3(blocks_for(m.rows, 128)), dim3(128), 0, s, (const int64_t*)rows.data(), m.rows,
-                     row_order, (const int64_t*)cols.data(), depth, trial_depth, (const uint64_t*)ps.bins.data(),
-                     (const int*)ps.ids.data(), ps.P, (const T*)ps.pos.data(), (const T*)ps.w.data(),
+                     row_order, depth, trial_depth, (const uint64_t*)ps.bins.data(), (const int*)ps.ids.data(), ps.P,
+                     ps.columns.view(), (const T*)ps.pos.data(), (const T*)ps.w.data(),
                      (const int*)m.cols.data(), m.vals.data(), load);
   KERNEL_CHECK();
 }
@@ -1212,8 +1254,8 @@ template <class T> static Stencil<T> cast_stencil(Stencil<double>&& m, hipStream
 }
 
 template <class T>
-static void normal_stencil_load(const LevelState<T>& testL, int test_depth, const int64_t* nkeys, int64_t ncount, const T* nvals,
-                                int normal_depth, double* out, hipStream_t s) {
+static void normal_stencil_load(const LevelState<T>& testL, int test_depth, const int64_t* nkeys, int64_t ncount, ColumnTable ct,
+                                const T* nvals, int normal_depth, double* out, hipStream_t s) {
   const int R = testL.basis_count;
   if (!R) return;
   if (!ncount) {
@@ -1228,7 +1270,7 @@ static void normal_stencil_load(const LevelState<T>& testL, int test_depth, cons
                      table.data());
   KERNEL_CHECK();
   hipLaunchKernelGGL(k_normal_load<T>, dim3(blocks_for(R, 128)), dim3(128), 0, s, (const int64_t*)testL.basis_keys.data(), R,
-                     nkeys, ncount, nvals, 3, test_depth, normal_depth, first, width, (const D2*)table.data(), out);
+                     nkeys, ncount, ct, nvals, 3, test_depth, normal_depth, first, width, (const D2*)table.data(), out);
   KERNEL_CHECK();
 }
 
@@ -1244,12 +1286,14 @@ template <class T> static void normal_loads(Result<T>& r, std::vector<DArray<int
     const T* nv = r.normal_vals.data() + 3 * r0;
     has_cumulative = has_cumulative || (r1 > r0);
     LevelState<T>& L = r.levels[d];
+    ColumnIndex records;
+    build_column_index(nk, r1 - r0, (1 << d) + 1, records, s);
     if (d >= base) {
       same[d].reset(L.basis_count, s);
-      normal_stencil_load(L, d, nk, r1 - r0, nv, d, same[d].data(), s);
+      normal_stencil_load(L, d, nk, r1 - r0, records.view(), nv, d, same[d].data(), s);
       if (d > base) {
         from_child[d - 1].reset(r.levels[d - 1].basis_count, s);
-        normal_stencil_load(r.levels[d - 1], d - 1, nk, r1 - r0, nv, d, from_child[d - 1].data(), s);
+        normal_stencil_load(r.levels[d - 1], d - 1, nk, r1 - r0, records.view(), nv, d, from_child[d - 1].data(), s);
       }
     }
     if (d < D) {
@@ -1257,15 +1301,16 @@ template <class T> static void normal_loads(Result<T>& r, std::vector<DArray<int
       if (L.basis_count) {
         const LevelState<T>* P = d > 0 ? &r.levels[d - 1] : nullptr;
         hipLaunchKernelGGL(k_normal_prolong<T>, dim3(blocks_for(L.basis_count, 256)), dim3(256), 0, s,
-                           (const int64_t*)L.basis_keys.data(), L.basis_count, nk, r1 - r0, nv,
+                           (const int64_t*)L.basis_keys.data(), L.basis_count, nk, r1 - r0, records.view(), nv,
                            P ? (const int64_t*)P->basis_keys.data() : (const int64_t*)nullptr,
-                           (int64_t)(P ? P->basis_count : 0), (const T*)previous.data(), d, cumulative.data());
+                           (int64_t)(P ? P->basis_count : 0), P ? P->columns.view() : ColumnTable{}, (const T*)previous.data(), d,
+                           cumulative.data());
         KERNEL_CHECK();
       }
       if (d >= base && has_cumulative) {
         from_parent[d + 1].reset(r.levels[d + 1].basis_count, s);
         normal_stencil_load(r.levels[d + 1], d + 1, (const int64_t*)L.basis_keys.data(), (int64_t)L.basis_count,
-                            (const T*)cumulative.data(), d, from_parent[d + 1].data(), s);
+                            L.columns.view(), (const T*)cumulative.data(), d, from_parent[d + 1].data(), s);
       }
       previous = std::move(cumulative);
     }
@@ -1384,6 +1429,10 @@ template <class T> static void regular_multigrid(Result<T>& r, std::vector<DArra
 template <class T> void stage_solve(Result<T>& r) {
   hipStream_t s = r.s;
   const int D = r.prm.depth, base = r.prm.base_depth;
+  for (int d = 0; d <= D; ++d) {
+    LevelState<T>& L = r.levels[d];
+    build_column_index((const int64_t*)L.basis_keys.data(), L.basis_count, (1 << d) + 1, L.columns, s);
+  }
   // prolongation P_d and restriction gathers (solver.py:340-342)
   std::vector<DArray<int>> restr(D + 1);
   for (int d = 1; d <= D; ++d) {
@@ -1396,14 +1445,14 @@ template <class T> void stage_solve(Result<T>& r) {
     if (F.basis_count) {
       hipLaunchKernelGGL(k_prolongation<T>, dim3(blocks_for(F.basis_count, 256)), dim3(256), 0, s,
                          (const int64_t*)F.basis_keys.data(), F.basis_count, (const int64_t*)C.basis_keys.data(),
-                         (int64_t)C.basis_count, d, F.P.cols.data(), F.P.vals.data());
+                         (int64_t)C.basis_count, C.columns.view(), d, F.P.cols.data(), F.P.vals.data());
       KERNEL_CHECK();
     }
     restr[d - 1].reset(27 * (size_t)C.basis_count, s);
     if (C.basis_count) {
       hipLaunchKernelGGL(k_restriction, dim3(blocks_for(C.basis_count, 256)), dim3(256), 0, s,
                          (const int64_t*)C.basis_keys.data(), C.basis_count, (const int64_t*)F.basis_keys.data(),
-                         (int64_t)F.basis_count, d, restr[d - 1].data());
+                         (int64_t)F.basis_count, F.columns.view(), d, restr[d - 1].data());
       KERNEL_CHECK();
     }
   }
@@ -1427,7 +1476,7 @@ template <class T> void stage_solve(Result<T>& r) {
       Stencil<double> A = assemble_stiffness(L, L, d, d, s);
       if (screened) {
         pts = point_constraints(r, d, s);
-        add_screening(A, L.basis_keys, L.basis_keys, pts, d, d, L.point_load.data(), s);
+        add_screening(A, L.basis_keys, pts, d, d, L.point_load.data(), s);
       }
       hipLaunchKernelGGL(k_rhs<T>, dim3(blocks_for(R, 256)), dim3(256), 0, s, (const double*)L.normal_load.data(),
                          (const double*)L.point_load.data(), R, L.rhs.data());
@@ -1444,14 +1493,14 @@ template <class T> void stage_solve(Result<T>& r) {
       Stencil<double> A = assemble_stiffness(L, L, d, d, s, ord);
       if (screened) {
         pts = point_constraints(r, d, s);
-        add_screening(A, L.basis_keys, L.basis_keys, pts, d, d, L.point_load.data(), s, ord);
+        add_screening(A, L.basis_keys, pts, d, d, L.point_load.data(), s, ord);
       }
       hipLaunchKernelGGL(k_rhs_sched<T>, dim3(blocks_for(R, 256)), dim3(256), 0, s, (const double*)L.normal_load.data(),
                          (const double*)L.point_load.data(), ord, R, L.rhs.data());
       KERNEL_CHECK();
       L.A = cast_stencil<T>(std::move(A), s);
       Stencil<double> cross = assemble_stiffness(L, r.levels[d - 1], d, d - 1, s, ord);
-      if (screened) add_screening(cross, L.basis_keys, r.levels[d - 1].basis_keys, pts, d, d - 1, (double*)nullptr, s, ord);
+      if (screened) add_screening(cross, L.basis_keys, pts, d, d - 1, (double*)nullptr, s, ord);
       L.cross = cast_stencil<T>(std::move(cross), s);
       hipLaunchKernelGGL(k_spmv<T>, dim3(blocks_for(R, 256)), dim3(256), 0, s, (const int*)L.cross.cols.data(),
                          (const T*)L.cross.vals.data(), R, 27, (const T*)cumulative.data(), L.rhs.data(), T(-1), T(1));
