@@ -5,9 +5,12 @@ Workload: 800,000 points on a smooth union of six seeded random ellipsoids
 (semi-axes 0.15-0.45), sampling density varying 10x, position noise 0.001,
 exact analytic normals; depth 9, full_depth 4, point_weight 2.0, iters 8.
 A step is one complete reconstruct() (inputs already resident in HBM) of the
-float32 path (float64 operator accumulation, float64 isovalue); the float64
-path is timed the same way and both are evaluated at 812,000 queries (every
-input plus 12,000 points straddling refinement interfaces) for the field error.
+mixed-precision path: float64 inputs, octree stages, vectors, base-level solve
+and isovalue, with the finer levels' Laplacian + screening operators stored and
+streamed as float32.  The full float64 path and the float32 path are timed the
+same way; mixed and float32 fields are compared with the float64 field at
+812,000 queries (every input plus 12,000 points straddling refinement
+interfaces).
 
 Prints one JSON line (metric/value per the benchmark contract) with a
 ``roofline`` object for the sliced Gauss-Seidel relaxation of the finest level
@@ -34,16 +37,17 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def relax_roofline(metadata, value_bytes):
+def relax_roofline(metadata, value_bytes, vector_bytes):
     """Algorithmic bytes of the finest sliced relaxation over its launch span.
 
-    Per sweep: nnz*(4 + s) (column index + value) + (R+1)*4 (row extent) +
-    R*s (b) + 2*R*s (x read and written), s = value bytes; times the sweeps.
+    Per sweep: nnz*(4 + v) (column index + operator value) + (R+1)*4 (row
+    extent) + R*s (b) + 2*R*s (x read and written), v = operator value bytes,
+    s = vector bytes; times the sweeps.
     """
     stats = metadata["relaxation"]
     fine = max(stats, key=lambda s: s["rows"])
-    s = value_bytes
-    per_sweep = fine["nnz"] * (4 + s) + (fine["rows"] + 1) * 4 + fine["rows"] * s + 2 * fine["rows"] * s
+    v, s = value_bytes, vector_bytes
+    per_sweep = fine["nnz"] * (4 + v) + (fine["rows"] + 1) * 4 + fine["rows"] * s + 2 * fine["rows"] * s
     total = per_sweep * fine["sweeps"]
     achieved = total / fine["seconds"] / 1e9
     return {
@@ -69,15 +73,15 @@ def cpu_baseline(points, normals, sample):
     p = points[:sample]
     n = normals[:sample]
     t = time.perf_counter()
-    oracle.reconstruct(p, n, **PARAMS)
+    oracle.reconstruct(p, n, mixed=True, **PARAMS)
     dt = time.perf_counter() - t
     return {
         "value": round(sample / dt / 1e6, 6),
         "unit": "Mpts/s",
         "cores": 1,
         "kind": "port",
-        "sample": f"first {sample:,} of the {len(points):,}-point cloud, same depth-9 parameters, "
-                  f"serial C++ oracle, {dt:.2f} s",
+        "sample": f"first {sample:,} of the {len(points):,}-point cloud, same depth-9 parameters and mixed "
+                  f"precision, serial C++ oracle, {dt:.2f} s",
     }
 
 
@@ -108,59 +112,68 @@ def main():
     tp64, tn64 = torch.from_numpy(p64).to(dev), torch.from_numpy(n64).to(dev)
     torch.cuda.synchronize()
 
-    def timed(tp, tn, steps, warmup, label):
+    def timed(tp, tn, steps, warmup, label, **extra):
         for _ in range(warmup):
-            r = reconstruct(tp, tn, **PARAMS)
+            r = reconstruct(tp, tn, **PARAMS, **extra)
         torch.cuda.synchronize()
         t = time.perf_counter()
         for _ in range(steps):
-            r = reconstruct(tp, tn, **PARAMS)
+            r = reconstruct(tp, tn, **PARAMS, **extra)
         torch.cuda.synchronize()
         dt = (time.perf_counter() - t) / steps
         log(f"{label}: {dt * 1e3:.1f} ms/step, {len(tp) / dt / 1e6:.3f} Mpts/s; stages {r.metadata['timings_seconds']}")
         return r, dt
 
-    r32, dt32 = timed(tp32, tn32, args.steps, args.warmup, "float32")
+    rmx, dtmx = timed(tp64, tn64, args.steps, args.warmup, "mixed", mixed=True)
     out = {
         "metric": METRIC,
-        "value": round(args.points / dt32 / 1e6, 4),
+        "value": round(args.points / dtmx / 1e6, 4),
         "unit": "Mpts/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(dt32 * 1e3, 3),
+        "ms_per_step": round(dtmx * 1e3, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "mixed",
         "data": "synthetic",
         "config": {"workload": "six-ellipsoid union, 800,000 points, 10x density variation, noise 0.001, "
                                "depth 9, full_depth 4, point_weight 2.0, iters 8 (SPSR)",
                    "points": args.points, "depth": 9, "full_depth": 4, "point_weight": 2.0,
-                   "basis_counts": r32.metadata["basis_counts"]},
+                   "precision": "mixed: float64 inputs, octree, vectors, base solve and isovalue; "
+                                "float32 finer-level operators",
+                   "basis_counts": rmx.metadata["basis_counts"]},
     }
-    out["roofline"] = relax_roofline(r32.metadata, 4)
-    out["stage_seconds"] = r32.metadata["timings_seconds"]
+    out["roofline"] = relax_roofline(rmx.metadata, 4, 8)
+    out["stage_seconds"] = rmx.metadata["timings_seconds"]
+    r32, dt32 = timed(tp32, tn32, max(1, args.steps // 2), 1, "float32")
+    out["float32"] = {"Mpts_s": round(args.points / dt32 / 1e6, 4), "ms_per_step": round(dt32 * 1e3, 3),
+                      "relax_roofline": relax_roofline(r32.metadata, 4, 4)}
     if not args.skip_fp64:
         r64, dt64 = timed(tp64, tn64, max(1, args.steps // 2), 1, "float64")
-        keys = r32.diagnostics.array("keys")
-        iq = synthetic.interface_queries(keys, 9, r32.diagnostics.array("model_to_cube"),
-                                         r32.diagnostics.array("cube_to_model"), p32)
-        q = np.concatenate([p32, iq])
-        v32, _, ok32 = r32.evaluate(torch.from_numpy(q).to(dev))
-        v64, _, ok64 = r64.evaluate(torch.from_numpy(q.astype(np.float64)).to(dev))
-        both = (ok32.bool() & ok64.bool()).cpu().numpy()
-        err = np.abs(v32.cpu().numpy().astype(np.float64) - v64.cpu().numpy())[both]
-        iso_err = abs(r32.iso_value - r64.iso_value)
-        out["fp64"] = {"Mpts_s": round(args.points / dt64 / 1e6, 4), "ms_per_step": round(dt64 * 1e3, 3),
-                       "speedup_f32_over_f64": round(dt64 / dt32, 3),
-                       "relax_roofline": relax_roofline(r64.metadata, 8)}
-        out["field_error_vs_fp64"] = {"queries": int(len(q)), "max_abs": float(err.max()),
-                                      "centered_max_abs": float(np.abs((v32.cpu().numpy().astype(np.float64) - r32.iso_value)
-                                                                       - (v64.cpu().numpy() - r64.iso_value))[both].max()),
-                                      "iso_abs": iso_err, "within_1e-6": bool(err.max() <= 1e-6)}
+        out["float64"] = {"Mpts_s": round(args.points / dt64 / 1e6, 4), "ms_per_step": round(dt64 * 1e3, 3),
+                          "relax_roofline": relax_roofline(r64.metadata, 8, 8)}
+        keys = r64.diagnostics.array("keys")
+        iq = synthetic.interface_queries(keys, 9, r64.diagnostics.array("model_to_cube"),
+                                         r64.diagnostics.array("cube_to_model"), p64)
+        q64 = np.concatenate([p64, iq])
+        v64, _, ok64 = r64.evaluate(torch.from_numpy(q64).to(dev))
+        v64, ok64 = v64.cpu().numpy(), ok64.cpu().numpy().astype(bool)
+
+        def error(r, q):
+            v, _, ok = r.evaluate(torch.from_numpy(q).to(dev))
+            v = v.cpu().numpy().astype(np.float64)
+            both = ok.cpu().numpy().astype(bool) & ok64
+            err = np.abs(v - v64)[both]
+            return {"queries": int(len(q)), "max_abs": float(err.max()), "iso_abs": abs(r.iso_value - r64.iso_value),
+                    "same_octree": bool(np.array_equal(r.diagnostics.array("keys"), keys)),
+                    "within_1e-6": bool(err.max() <= 1e-6)}
+
+        out["field_error_vs_fp64"] = error(rmx, q64)
+        out["float32"]["field_error_vs_fp64"] = error(r32, q64.astype(np.float32))
     if not args.skip_cpu:
-        out["cpu_baseline"] = cpu_baseline(p32, n32, args.cpu_sample)
+        out["cpu_baseline"] = cpu_baseline(p64, n64, args.cpu_sample)
     print(json.dumps(out), flush=True)
     return 0
 

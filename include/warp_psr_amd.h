@@ -46,6 +46,8 @@ typedef struct PsrParams {
   int base_v_cycles;       /* default 1 */
   double cg_accuracy;      /* default 1e-3 */
   int keep_intermediates;  /* keep per-stage arrays for psr_result_array (default 1) */
+  int float_operators;     /* 1: mixed precision -- a float64 run whose finer-level Laplacian +
+                              screening operators are stored as float32 (default 0; float64 only) */
 } PsrParams;
 
 typedef struct PsrResult PsrResult;

@@ -41,7 +41,7 @@ def _lib(openmp=False):
             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int,
             ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
             ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int,
-            ctypes.c_double, ctypes.POINTER(ctypes.c_void_p)]
+            ctypes.c_double, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
         lib.oracle_last_error.restype = ctypes.c_char_p
         lib.oracle_free.argtypes = [ctypes.c_void_p]
         lib.oracle_array.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int,
@@ -124,11 +124,16 @@ def resolve_params(depth=6, full_depth=3, base_depth=None, solve_depth=None, ker
                 base_v_cycles=base_v_cycles, cg_accuracy=cg_accuracy)
 
 
-def reconstruct(points, normals, *, openmp=False, **kwargs):
-    """Run the oracle on (n,3) float32/float64 numpy arrays."""
+def reconstruct(points, normals, *, openmp=False, mixed=False, **kwargs):
+    """Run the oracle on (n,3) float32/float64 numpy arrays.
+
+    mixed=True (float64 inputs only) rounds the finer levels' operator values
+    to float32, as the device's mixed-precision mode stores them."""
     p = resolve_params(**kwargs)
     points = np.ascontiguousarray(points)
     dtype = np.float64 if points.dtype == np.float64 else np.float32
+    if mixed and dtype != np.float64:
+        raise ValueError("mixed precision runs on float64 inputs")
     points = np.ascontiguousarray(points, dtype=dtype).reshape(-1, 3)
     normals = np.ascontiguousarray(normals, dtype=dtype).reshape(-1, 3)
     lib = _lib(openmp)
@@ -136,7 +141,7 @@ def reconstruct(points, normals, *, openmp=False, **kwargs):
     rc = lib.oracle_reconstruct(points.ctypes.data, normals.ctypes.data, len(points), 1 if dtype == np.float64 else 0,
                                 p["depth"], p["full_depth"], p["base_depth"], p["solve_depth"], p["kernel_depth"],
                                 p["samples_per_node"], p["scale"], p["point_weight"], int(p["exact_interpolation"]),
-                                p["iters"], p["base_v_cycles"], p["cg_accuracy"], ctypes.byref(out))
+                                p["iters"], p["base_v_cycles"], p["cg_accuracy"], int(bool(mixed)), ctypes.byref(out))
     if rc == 2:
         raise ValueError(lib.oracle_last_error().decode())
     if rc:

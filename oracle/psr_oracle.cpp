@@ -190,6 +190,7 @@ struct Params {
   int iters, base_v_cycles, exact;
   double samples_per_node, scale, point_weight, cg_accuracy;
   int threads;  // OpenMP threads for order-independent loops (1 = serial)
+  int float_operators = 0;  // finer-level operators rounded to float (float64 runs)
 };
 
 struct Error : std::runtime_error {
@@ -1266,6 +1267,12 @@ template <class T> std::vector<T> regular_multigrid(State<T>& s, const CSR<T>& A
   return x[base];
 }
 
+// Mixed precision: finer-level operator values stored in float and widened
+// back exactly (the device keeps them as float; vectors stay double).
+template <class T> void round_to_float(CSR<T>& a) {
+  for (auto& v : a.values) v = T(float(v));
+}
+
 template <class T> void solve_levels(State<T>& s) {
   const int finest = s.prm.depth, base = s.prm.base_depth;
   s.prolong.clear();
@@ -1293,6 +1300,7 @@ template <class T> void solve_levels(State<T>& s) {
     std::vector<T> b(s.basis[d].size());
     for (size_t i = 0; i < b.size(); ++i) b[i] = T(load[i]) + T(0.5) * T(point_load[i]);  // solver.py:119-126
     CSR<T> At = cast_csr<T>(A);
+    if (s.prm.float_operators && d != base) round_to_float(At);
     if (d == base) {
       s.corrections[d] = regular_multigrid(s, At, b, base);
       cumulative = s.corrections[d];
@@ -1300,6 +1308,7 @@ template <class T> void solve_levels(State<T>& s) {
       CSR<double> cross = assemble_stiffness(s.basis[d], s.basis[d - 1], d, d - 1);
       if (screened) add_screening(cross, s.basis[d], s.basis[d - 1], pts, d, d - 1);
       CSR<T> ct = cast_csr<T>(cross);
+      if (s.prm.float_operators) round_to_float(ct);
       spmv(ct, cumulative, b, T(-1), T(1));
       Schedule sched = row_schedule(s.basis[d], d, true);
       relax_sliced(At, b, s.corrections[d], sched, d, s.prm.iters);
@@ -1621,9 +1630,10 @@ const char* oracle_last_error() { return g_error.c_str(); }
 // (ValueError), 1 for any other failure.
 int oracle_reconstruct(const void* points, const void* normals, int64_t n, int dtype, int depth, int full_depth,
                        int base_depth, int solve_depth, int kernel_depth, double samples_per_node, double scale,
-                       double point_weight, int exact, int iters, int base_v_cycles, double cg_accuracy, void** out) {
+                       double point_weight, int exact, int iters, int base_v_cycles, double cg_accuracy,
+                       int float_operators, void** out) {
   Params p{depth, full_depth, base_depth, solve_depth, kernel_depth, iters, base_v_cycles, exact, samples_per_node,
-           scale, point_weight, cg_accuracy, 1};
+           scale, point_weight, cg_accuracy, 1, float_operators};
   try {
     Handle* h = new Handle();
     h->dtype = dtype;

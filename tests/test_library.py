@@ -46,6 +46,7 @@ def test_params_default():
     p = _params()
     assert (p.depth, p.full_depth, p.base_depth, p.solve_depth, p.kernel_depth) == (6, 3, -1, -1, -1)
     assert (p.samples_per_node, p.scale, p.point_weight, p.iters, p.base_v_cycles, p.cg_accuracy) == (1.5, 1.1, 2.0, 8, 1, 1e-3)
+    assert (p.keep_intermediates, p.float_operators) == (1, 0)
 
 
 @pytest.mark.parametrize("kw,message", [
@@ -75,6 +76,14 @@ def test_c_abi_rejects_empty_and_bad_dtype():
     assert "nonempty" in lib.psr_last_error().decode()
     assert lib.psr_reconstruct(None, None, 4, 7, ctypes.byref(_params()), None, ctypes.byref(out)) == 2
     assert lib.psr_relax_sliced(5, None, 1, None, None, None, None, None, 3, 8, None) == 3
+
+
+def test_c_abi_mixed_requires_float64():
+    lib = _lib.load()
+    out = ctypes.c_void_p()
+    rc = lib.psr_reconstruct(None, None, 4, 0, ctypes.byref(_params(float_operators=1)), None, ctypes.byref(out))
+    assert rc == 2
+    assert "float64" in lib.psr_last_error().decode()
 
 
 def test_python_validation_messages():

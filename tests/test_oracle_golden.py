@@ -229,3 +229,33 @@ def test_wavefront_order_equals_sequential():
                 for (z2, t2), pos2 in order_seq.items():
                     if abs(z - z2) <= 1 and (z, t) != (z2, t2) and pos < pos2:
                         assert wave[(z, t)] < wave[(z2, t2)], (depth, iters, z, t, z2, t2)
+
+
+def test_mixed_precision_error_bound():
+    """Float32 finer-level operators keep the float64 octree exactly and the
+    field within 1e-6 of the float64 run (inputs and interface queries)."""
+    p, n = synthetic.workload_ellipsoids(30000, dtype=np.float64)
+    kw = dict(depth=7, full_depth=4, point_weight=2.0)
+    full = oracle.reconstruct(p, n, **kw)
+    mixed = oracle.reconstruct(p, n, mixed=True, **kw)
+    keys = full.array("keys")
+    np.testing.assert_array_equal(mixed.array("keys"), keys)
+    # rounding is really applied: finer-level operators are float32 values
+    vals = mixed.array("matrix_values", 6)
+    assert np.array_equal(vals, vals.astype(np.float32).astype(np.float64))
+    assert not np.array_equal(vals, full.array("matrix_values", 6))
+    q = np.concatenate([p, synthetic.interface_queries(keys, 7, full.array("model_to_cube"),
+                                                       full.array("cube_to_model"), p, count=4000)])
+    vf, _, okf = full.evaluate(q)
+    vm, _, okm = mixed.evaluate(q)
+    both = okf.astype(bool) & okm.astype(bool)
+    err = np.abs(vm - vf)[both]
+    assert both.sum() > len(p) // 2
+    assert 0 < err.max() <= 1e-6
+    assert abs(mixed.iso_value - full.iso_value) <= 1e-6
+
+
+def test_mixed_requires_float64():
+    p, n = synthetic.fixture_input("sphere", 64, synthetic.SEED, "float")
+    with pytest.raises(ValueError):
+        oracle.reconstruct(p, n, mixed=True, depth=4, full_depth=2)

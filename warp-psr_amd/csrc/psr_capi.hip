@@ -53,6 +53,7 @@ Params resolve(const PsrParams& in) {
   q.kernel_depth = p.kernel_depth; q.samples_per_node = p.samples_per_node; q.scale = p.scale;
   q.point_weight = p.point_weight; q.exact_interpolation = p.exact_interpolation ? 1 : 0; q.iters = p.iters;
   q.base_v_cycles = p.base_v_cycles; q.cg_accuracy = p.cg_accuracy; q.keep_intermediates = p.keep_intermediates;
+  q.float_operators = p.float_operators ? 1 : 0;
   return q;
 }
 
@@ -119,18 +120,19 @@ std::vector<int> inverse_order(const DArray<int>& order, int R) {
 }
 
 // stencil -> compact CSR in canonical row order (present slots in slot order)
-template <class V>
+// (values exported as Out: float operators of mixed runs widen to double)
+template <class Out, class V>
 void stencil_csr(const Stencil<V>& m, int slots, const char* part, std::vector<char>& out, const std::vector<int>& inv) {
   std::vector<int> cols = m.cols.to_host();
   std::vector<V> vals = m.vals.to_host();
   const int R = m.rows;
   std::vector<int> offs(R + 1, 0), cc;
-  std::vector<V> vv;
+  std::vector<Out> vv;
   for (int i = 0; i < R; ++i) {
     const int j = inv.empty() ? i : inv[i];
     for (int k = 0; k < slots; ++k) {
       int c = cols[(size_t)k * R + j];
-      if (c >= 0) { cc.push_back(c); vv.push_back(vals[(size_t)k * R + j]); }
+      if (c >= 0) { cc.push_back(c); vv.push_back(Out(vals[(size_t)k * R + j])); }
     }
     offs[i + 1] = (int)cc.size();
   }
@@ -182,9 +184,17 @@ template <class T> bool export_array(Result<T>& r, const std::string& name, int 
   if (name == "corrections" && lvl) { put_dev(out, L().correction); return true; }
   const bool mat = lvl && (name.rfind("matrix_", 0) == 0 || name.rfind("cross_", 0) == 0);
   const std::vector<int> inv = mat && L().scheduled ? inverse_order(L().order, L().basis_count) : std::vector<int>();
-  if (lvl && name.rfind("matrix_", 0) == 0) { stencil_csr(L().A, 27, name.c_str() + 7, out, inv); return true; }
-  if (lvl && name.rfind("cross_", 0) == 0) { stencil_csr(L().cross, 27, name.c_str() + 6, out, inv); return true; }
-  if (lvl && name.rfind("prolong_", 0) == 0) { stencil_csr(L().P, 8, name.c_str() + 8, out, {}); return true; }
+  if (lvl && name.rfind("matrix_", 0) == 0) {
+    if (L().A32.rows) stencil_csr<T>(L().A32, 27, name.c_str() + 7, out, inv);
+    else stencil_csr<T>(L().A, 27, name.c_str() + 7, out, inv);
+    return true;
+  }
+  if (lvl && name.rfind("cross_", 0) == 0) {
+    if (L().cross32.rows) stencil_csr<T>(L().cross32, 27, name.c_str() + 6, out, inv);
+    else stencil_csr<T>(L().cross, 27, name.c_str() + 6, out, inv);
+    return true;
+  }
+  if (lvl && name.rfind("prolong_", 0) == 0) { stencil_csr<T>(L().P, 8, name.c_str() + 8, out, {}); return true; }
   if (name == "keys") { put_dev(out, r.keys); return true; }
   if (name == "coefficients") { put_dev(out, r.coefficients); return true; }
   if (name == "cumulative") { put_dev(out, r.cumulative); return true; }
@@ -199,7 +209,8 @@ template <class T> bool export_array(Result<T>& r, const std::string& name, int 
 template <class T> std::string metadata_json(const Result<T>& r) {
   std::ostringstream o;
   o.precision(17);
-  o << "{\"precision\": \"" << (sizeof(T) == 4 ? "float32" : "float64") << "\", \"basis_counts\": [";
+  o << "{\"precision\": \"" << (sizeof(T) == 4 ? "float32" : (r.prm.float_operators ? "mixed" : "float64"))
+    << "\", \"basis_counts\": [";
   for (size_t d = 0; d < r.levels.size(); ++d) o << (d ? ", " : "") << r.levels[d].basis_count;
   o << "], \"aggregates\": " << r.G << ", \"normal_records\": " << r.normal_keys.size() << ", \"timings_seconds\": {";
   for (size_t k = 0; k < r.timings.size(); ++k) o << (k ? ", " : "") << "\"" << r.timings[k].first << "\": " << r.timings[k].second;
@@ -250,6 +261,7 @@ void psr_params_default(PsrParams* p) {
   p->base_v_cycles = 1;
   p->cg_accuracy = 1e-3;
   p->keep_intermediates = 1;
+  p->float_operators = 0;
 }
 
 int psr_reconstruct(const void* points, const void* normals, int64_t n, int dtype, const PsrParams* params, void* stream,
@@ -260,6 +272,7 @@ int psr_reconstruct(const void* points, const void* normals, int64_t n, int dtyp
     if (dtype != 0 && dtype != 1)
       throw PsrError(2, "Reconstruction requires matching float32/float64 3D vectors on the same CUDA device");
     if (n <= 0) throw PsrError(2, "points and normals must have equal nonempty 1D shapes");
+    if (prm.float_operators && dtype != 1) throw PsrError(2, "mixed precision (float_operators) runs on float64 inputs");
     hipStream_t s = (hipStream_t)stream;
     auto res = std::make_unique<PsrResult>();
     res->dtype = dtype;

@@ -12,6 +12,7 @@ struct Params {
   int exact_interpolation = 0, iters = 8, base_v_cycles = 1;
   double cg_accuracy = 1e-3;
   int keep_intermediates = 1;  // keep per-stage arrays for diagnostics
+  int float_operators = 0;     // finer-level operators stored as float (double runs)
 };
 
 // 27-slot stencil matrix (slot k <-> neighbour offset (k/9-1, k/3%3-1, k%3-1)
@@ -21,21 +22,6 @@ template <class V> struct Stencil {
   int rows = 0, cols_count = 0;
   DArray<int> cols;
   DArray<V> vals;
-};
-
-// Gauss-Seidel operator in schedule order: position j holds row order[j];
-// values/columns are slot-major over positions; inv_diag = 1/A_ii (0 = skip).
-template <class T> struct RelaxOperator {
-  int rows = 0;
-  DArray<int> order;        // schedule position -> row
-  DArray<int> group_offsets;
-  std::vector<int> host_offsets;
-  int max_group = 0;
-  DArray<int> cols;
-  DArray<T> vals;
-  DArray<T> b;              // right-hand side in schedule order
-  DArray<T> inv_diag;
-  DArray<unsigned char> has_diag;
 };
 
 template <class T> struct LevelState {
@@ -50,6 +36,7 @@ template <class T> struct LevelState {
   DArray<T> dens_vals;
   // operators and solve
   Stencil<T> A, cross;      // cast to run precision
+  Stencil<float> A32, cross32;  // finer levels of mixed-precision runs (instead of A, cross)
   Stencil<T> P;             // prolongation rows (8 slots, x-major corner order)
   DArray<double> normal_load, point_load;
   DArray<T> rhs, correction;

@@ -376,15 +376,16 @@ __global__ void k_restrict_apply(const int* __restrict__ cols, int C, const V* _
 }
 
 // y = alpha*(A x) + beta*y with A a stencil (slots in column order)
-template <class T>
-__global__ void k_spmv(const int* __restrict__ cols, const T* __restrict__ vals, int R, int slots, const T* __restrict__ x,
+// (values of type V, widened exactly to T)
+template <class T, class V = T>
+__global__ void k_spmv(const int* __restrict__ cols, const V* __restrict__ vals, int R, int slots, const T* __restrict__ x,
                        T* __restrict__ y, T alpha, T beta) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= R) return;
   T v = T(0);
   for (int k = 0; k < slots; ++k) {
     const int c = cols[(int64_t)k * R + i];
-    if (c >= 0) v = v + vals[(int64_t)k * R + i] * x[c];
+    if (c >= 0) v = v + T(vals[(int64_t)k * R + i]) * x[c];
   }
   v = v * alpha;
   if (beta != T(0)) v = v + beta * y[i];
@@ -527,17 +528,20 @@ __global__ void k_galerkin(const int* __restrict__ rcols, int C, const int* __re
 }
 
 // ---------------------------------------------------------------- relaxation
-// relaxation.py:38-64 (_relax) on the schedule-ordered operator.
-template <class T>
+// relaxation.py:38-64 (_relax) on the schedule-ordered operator.  Operator
+// values have type V (T, or float operators under T = double), widened
+// exactly to T before use.
+template <class T, class V>
 __device__ __forceinline__ void gs_update(int j, int R, const int* __restrict__ order, const int* __restrict__ cols,
-                                          const T* __restrict__ vals, const T* __restrict__ b, const T* __restrict__ inv_diag,
+                                          const V* __restrict__ vals, const T* __restrict__ b, const T* __restrict__ inv_diag,
                                           const unsigned char* __restrict__ has_diag, T* __restrict__ x) {
   // All column loads, then all value/x loads in flight, then the subtraction
   // chain in slot order; absent slots are skipped by a select (no rounding).
   int c[27];
 #pragma unroll
   for (int k = 0; k < 27; ++k) c[k] = cols[(int64_t)k * R + j];
-  T v[27], xv[27];
+  V v[27];
+  T xv[27];
 #pragma unroll
   for (int k = 0; k < 27; ++k) {
     v[k] = vals[(int64_t)k * R + j];
@@ -546,7 +550,7 @@ __device__ __forceinline__ void gs_update(int j, int R, const int* __restrict__ 
   T r = b[j];
 #pragma unroll
   for (int k = 0; k < 27; ++k) {
-    const T t = r - xv[k] * v[k];
+    const T t = r - xv[k] * T(v[k]);
     r = c[k] >= 0 ? t : r;
   }
   if (has_diag[j]) {
@@ -559,121 +563,42 @@ __device__ __forceinline__ void gs_update(int j, int R, const int* __restrict__ 
 // apart, same colour.
 // waves_per_eu(1,4): the row update keeps all 81 loads of a row in flight
 // (needs ~100 VGPRs); occupancy is not what bounds these small launches.
-template <class T>
+template <class T, class V>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4)))
 k_relax_wave(const int* __restrict__ order, const int* __restrict__ goff, const int* __restrict__ cols,
-             const T* __restrict__ vals, const T* __restrict__ b, const T* __restrict__ inv_diag,
+             const V* __restrict__ vals, const T* __restrict__ b, const T* __restrict__ inv_diag,
              const unsigned char* __restrict__ has_diag, T* __restrict__ x, int R, int first_z, int color) {
   const int group = 4 * (first_z + 2 * (int)blockIdx.y) + color;
   const int j = goff[group] + blockIdx.x * blockDim.x + threadIdx.x;
   if (j < goff[group + 1]) gs_update(j, R, order, cols, vals, b, inv_diag, has_diag, x);
 }
 
-// Same launch with its arguments read from a device descriptor, so one
+// Same launch with its arguments read from a device descriptor block, so one
 // captured graph of the whole launch sequence serves every call of a shape.
-template <class T> struct RelaxDesc {
+// The block holds the buffer pointers and, at RELAX_GOFF_OFFSET, a copy of the
+// group offsets: both addresses follow from the kernel argument alone, so the
+// two loads are issued together.
+template <class T, class V> struct RelaxDesc {
   const int* order;
-  const int* goff;
   const int* cols;
-  const T* vals;
+  const V* vals;
   const T* b;
   const T* inv_diag;
   const unsigned char* has_diag;
   T* x;
   int R;
 };
+constexpr size_t RELAX_GOFF_OFFSET = 256;
 
-template <class T>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4)))
-k_relax_wave_graph(const RelaxDesc<T>* __restrict__ desc, int first_z, int color) {
-  const RelaxDesc<T> d = *desc;
+template <class T, class V, int BS>
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(1, 4)))
+k_relax_wave_graph(const char* __restrict__ block, int first_z, int color) {
+  const int* goff = (const int*)(block + RELAX_GOFF_OFFSET);
   const int group = 4 * (first_z + 2 * (int)blockIdx.y) + color;
-  const int j = d.goff[group] + blockIdx.x * blockDim.x + threadIdx.x;
-  if (j < d.goff[group + 1]) gs_update(j, d.R, d.order, d.cols, d.vals, d.b, d.inv_diag, d.has_diag, d.x);
-}
-
-// Dependency-driven sliced relaxation in one cooperative launch.
-//
-// Phase (z, t, c) = colour c of the t-th visit of slice z (relaxation.py:121-125).
-// Its rows are split into chunks of RELAX_CHUNK; chunk tasks are listed in the
-// reference's sequential phase order and assigned round-robin to co-resident
-// workgroups, which take their tasks in list order.  A phase waits for
-//   c > 0 : (z, t, c-1)
-//   c = 0 : (z, t-1, 3), (z+1, t, 3), (z-1, t-1, 3)
-// -- exactly the predecessors of the sequential order that touch the rows it
-// reads or writes -- so every row update sees the values the sequential order
-// gives it (same bits).  Every dependency precedes the waiting task in the list
-// and every owner takes its tasks in order, so the smallest unfinished task can
-// always run: no deadlock.  A bounded spin raises an error flag instead of
-// waiting forever.  Matrix rows are loaded before the wait (they do not depend
-// on x); x is gathered after an agent-scope acquire and published with an
-// agent-scope release before the phase counter is incremented.
-constexpr int RELAX_CHUNK = 256;
-
-template <class T>
-__global__ void __launch_bounds__(RELAX_CHUNK) k_relax_persistent(
-    const int* __restrict__ order, const int* __restrict__ cols, const T* __restrict__ vals, const T* __restrict__ b,
-    const T* __restrict__ inv_diag, const unsigned char* __restrict__ has_diag, T* __restrict__ x, int R,
-    const int4* __restrict__ tasks, int ntasks, const int4* __restrict__ deps, const int* __restrict__ chunks,
-    int* __restrict__ done, int* __restrict__ error, int spin_limit) {
-  for (int task = blockIdx.x; task < ntasks; task += gridDim.x) {
-    const int4 td = tasks[task];  // phase, first row, end row
-    const int j = td.y + (int)threadIdx.x;
-    const bool active = j < td.z;
-    int c[27];
-    T v[27];
-    T bj = T(0), inv = T(0);
-    bool hd = false;
-    if (active) {
-#pragma unroll
-      for (int k = 0; k < 27; ++k) {
-        c[k] = cols[(int64_t)k * R + j];
-        v[k] = vals[(int64_t)k * R + j];
-      }
-      bj = b[j];
-      inv = inv_diag[j];
-      hd = has_diag[j] != 0;
-    }
-    if (threadIdx.x == 0) {
-      const int4 dp = deps[td.x];
-      const int dl[3] = {dp.x, dp.y, dp.z};
-      for (int q = 0; q < 3; ++q) {
-        const int d = dl[q];
-        if (d < 0) continue;
-        const int need = chunks[d];
-        int spins = 0;
-        // once any wait has timed out, no task waits again: the grid drains
-        while (__hip_atomic_load(&done[d], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < need) {
-          if (++spins > spin_limit || ((spins & 255) == 0 && __hip_atomic_load(error, __ATOMIC_RELAXED,
-                                                                               __HIP_MEMORY_SCOPE_AGENT))) {
-            __hip_atomic_store(error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            break;
-          }
-          __builtin_amdgcn_s_sleep(1);
-        }
-      }
-    }
-    __syncthreads();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    if (active) {
-      T xv[27];
-#pragma unroll
-      for (int k = 0; k < 27; ++k) xv[k] = x[c[k] >= 0 ? c[k] : 0];
-      T r = bj;
-#pragma unroll
-      for (int k = 0; k < 27; ++k) {
-        const T t = r - xv[k] * v[k];
-        r = c[k] >= 0 ? t : r;
-      }
-      if (hd) {
-        const int i = order ? order[j] : j;
-        x[i] = x[i] + r * inv;
-      }
-    }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    __syncthreads();
-    if (threadIdx.x == 0) __hip_atomic_fetch_add(&done[td.x], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-  }
+  const int start = goff[group], end = goff[group + 1];
+  const RelaxDesc<T, V> d = *(const RelaxDesc<T, V>*)block;
+  const int j = start + (int)blockIdx.x * BS + (int)threadIdx.x;
+  if (j < end) gs_update(j, d.R, d.order, d.cols, d.vals, d.b, d.inv_diag, d.has_diag, d.x);
 }
 
 // relaxation.py:261-271 (8-colour, all sweeps) inside one workgroup: the
@@ -864,15 +789,15 @@ __global__ void k_scatter_rows(const int* __restrict__ order, int R, const T* __
 }
 
 // 1/A_ii of schedule-ordered rows (the entry whose column is the row itself)
-template <class T>
-__global__ void k_diag_sched(const int* __restrict__ order, int R, const int* __restrict__ cols, const T* __restrict__ vals,
+template <class T, class V = T>
+__global__ void k_diag_sched(const int* __restrict__ order, int R, const int* __restrict__ cols, const V* __restrict__ vals,
                              T* __restrict__ inv_diag, unsigned char* __restrict__ has_diag) {
   int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= R) return;
   const int i = order[j];
   T diagonal = T(0);
   for (int k = 0; k < 27; ++k)
-    if (cols[(int64_t)k * R + j] == i) diagonal = vals[(int64_t)k * R + j];
+    if (cols[(int64_t)k * R + j] == i) diagonal = T(vals[(int64_t)k * R + j]);
   has_diag[j] = diagonal != T(0);
   inv_diag[j] = diagonal != T(0) ? T(1) / diagonal : T(0);
 }
@@ -909,10 +834,11 @@ template <class T> static Schedule<T> make_schedule(const DArray<int64_t>& keys,
   return sc;
 }
 
-template <class T> struct GsOperator {
+template <class T, class V = T> struct GsOperator {
   int R = 0;
   DArray<int> cols;
-  DArray<T> vals, b, inv_diag;
+  DArray<V> vals;
+  DArray<T> b, inv_diag;
   DArray<unsigned char> has_diag;
   // row of x updated by schedule position j: update_order[j], or j itself
   // when x is kept in schedule order (nullptr)
@@ -968,8 +894,8 @@ __global__ void __launch_bounds__(256) k_count_present(const int* __restrict__ c
 }
 
 // relaxation.py:128-143 (wavefront_groups) + :189-210; returns the launch count
-template <class T>
-int relax_sliced(const GsOperator<T>& op, const Schedule<T>& sc, T* x, int depth, int iters, hipStream_t s) {
+template <class T, class V>
+int relax_sliced(const GsOperator<T, V>& op, const Schedule<T>& sc, T* x, int depth, int iters, hipStream_t s) {
   const int n = 1 << depth;
   const unsigned bx = (unsigned)std::max(1, (sc.max_group + 255) / 256);
   int launches = 0;
@@ -980,8 +906,8 @@ int relax_sliced(const GsOperator<T>& op, const Schedule<T>& sc, T* x, int depth
     const int z0 = n - q + 2 * first;
     const unsigned count = (unsigned)(last - first + 1);
     for (int color = 0; color < 4; ++color) {
-      hipLaunchKernelGGL(k_relax_wave<T>, dim3(bx, count), dim3(256), 0, s, op.update_order,
-                         (const int*)sc.offsets.data(), (const int*)op.cols.data(), (const T*)op.vals.data(),
+      hipLaunchKernelGGL((k_relax_wave<T, V>), dim3(bx, count), dim3(256), 0, s, op.update_order,
+                         (const int*)sc.offsets.data(), (const int*)op.cols.data(), (const V*)op.vals.data(),
                          (const T*)op.b.data(), (const T*)op.inv_diag.data(), (const unsigned char*)op.has_diag.data(), x,
                          op.R, z0, color);
       ++launches;
@@ -1002,45 +928,57 @@ struct RelaxGraph {
   int launches = 0;
 };
 
-template <class T>
-int relax_sliced_graph(const GsOperator<T>& op, const Schedule<T>& sc, T* x, int depth, int iters, hipStream_t s) {
+template <class T, class V, int BS> static void capture_relax_launches(hipStream_t cs, const char* desc, int n, int iters, unsigned bx,
+                                                             int& launches) {
+  for (int q = 0; q <= n + 2 * (iters - 1); ++q) {
+    const int first = std::max(0, floordiv(q - n + 1, 2));
+    const int last = std::min(iters - 1, q / 2);
+    if (last < first) continue;
+    const int z0 = n - q + 2 * first;
+    const unsigned count = (unsigned)(last - first + 1);
+    for (int color = 0; color < 4; ++color) {
+      hipLaunchKernelGGL((k_relax_wave_graph<T, V, BS>), dim3(bx, count), dim3(BS), 0, cs, desc, z0, color);
+      ++launches;
+    }
+  }
+}
+
+// 64-row workgroups spread one launch's rows over the most CUs (a launch holds
+// at most a few thousand rows per slice; occupancy does not bound it).
+constexpr int RELAX_GRAPH_BS = 64;
+
+template <class T, class V>
+int relax_sliced_graph(const GsOperator<T, V>& op, const Schedule<T>& sc, T* x, int depth, int iters, hipStream_t s) {
   static std::mutex mu;
   static std::map<std::tuple<int, int, int, unsigned, int, hipStream_t>, RelaxGraph> cache;
   static std::map<int, hipStream_t> capture_streams;
   const int n = 1 << depth;
   unsigned bx = 1;
-  while (bx * 256 < (unsigned)std::max(1, sc.max_group)) bx <<= 1;
+  while (bx * (unsigned)RELAX_GRAPH_BS < (unsigned)std::max(1, sc.max_group)) bx <<= 1;
+  const size_t groups = 4 * ((size_t)n + 1);
   int dev = 0;
   HIP_CHECK(hipGetDevice(&dev));
   std::lock_guard<std::mutex> guard(mu);
-  const auto key = std::make_tuple((int)sizeof(T), depth, iters, bx, dev, s);
+  const auto key = std::make_tuple((int)sizeof(V), depth, iters, bx, dev, s);
   auto it = cache.find(key);
   if (it == cache.end()) {
     RelaxGraph g;
-    HIP_CHECK(hipMalloc(&g.desc, sizeof(RelaxDesc<T>)));
+    HIP_CHECK(hipMalloc(&g.desc, RELAX_GOFF_OFFSET + sizeof(int) * (groups + 1)));
     hipStream_t& cs = capture_streams[dev];
     if (!cs) HIP_CHECK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
     HIP_CHECK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-    for (int q = 0; q <= n + 2 * (iters - 1); ++q) {
-      const int first = std::max(0, floordiv(q - n + 1, 2));
-      const int last = std::min(iters - 1, q / 2);
-      if (last < first) continue;
-      const int z0 = n - q + 2 * first;
-      const unsigned count = (unsigned)(last - first + 1);
-      for (int color = 0; color < 4; ++color) {
-        hipLaunchKernelGGL(k_relax_wave_graph<T>, dim3(bx, count), dim3(256), 0, cs, (const RelaxDesc<T>*)g.desc, z0, color);
-        ++g.launches;
-      }
-    }
+    capture_relax_launches<T, V, RELAX_GRAPH_BS>(cs, (const char*)g.desc, n, iters, bx, g.launches);
     hipGraph_t graph = nullptr;
     HIP_CHECK(hipStreamEndCapture(cs, &graph));
     HIP_CHECK(hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0));
     HIP_CHECK(hipGraphDestroy(graph));
     it = cache.emplace(key, g).first;
   }
-  RelaxDesc<T> d{op.update_order, sc.offsets.data(), op.cols.data(), op.vals.data(), op.b.data(), op.inv_diag.data(),
-                 op.has_diag.data(), x, op.R};
-  HIP_CHECK(hipMemcpyAsync(it->second.desc, &d, sizeof(d), hipMemcpyHostToDevice, s));
+  static_assert(sizeof(RelaxDesc<T, V>) <= RELAX_GOFF_OFFSET, "descriptor block layout");
+  RelaxDesc<T, V> d{op.update_order, op.cols.data(), op.vals.data(), op.b.data(), op.inv_diag.data(), op.has_diag.data(), x, op.R};
+  char* desc = (char*)it->second.desc;
+  HIP_CHECK(hipMemcpyAsync(desc, &d, sizeof(d), hipMemcpyHostToDevice, s));
+  HIP_CHECK(hipMemcpyAsync(desc + RELAX_GOFF_OFFSET, sc.offsets.data(), sizeof(int) * (groups + 1), hipMemcpyDeviceToDevice, s));
   HIP_CHECK(hipGraphLaunch(it->second.exec, s));
   // the descriptor is rewritten by the next call on this stream only after
   // this launch (stream order); keep the host copy until the copy has run
@@ -1048,111 +986,21 @@ int relax_sliced_graph(const GsOperator<T>& op, const Schedule<T>& sc, T* x, int
   return it->second.launches;
 }
 
-// Task list of k_relax_persistent (see the kernel's comment).
-struct RelaxTasks {
-  std::vector<int4> tasks, deps;
-  std::vector<int> chunks;
-};
-
-static RelaxTasks build_relax_tasks(const std::vector<int>& goff, int depth, int iters) {
-  const int n = 1 << depth;
-  RelaxTasks rt;
-  std::vector<int> index((size_t)(n + 1) * iters * 4, -1);
-  auto id = [&](int z, int t, int c) -> int& { return index[((size_t)z * iters + t) * 4 + c]; };
-  int phases = 0;
-  for (int front = n; front > -iters; --front)
-    for (int z = std::max(front, 0); z < std::min(front + iters, n + 1); ++z)
-      for (int c = 0; c < 4; ++c) id(z, z - front, c) = phases++;
-  rt.deps.resize(phases);
-  rt.chunks.resize(phases);
-  for (int front = n; front > -iters; --front)
-    for (int z = std::max(front, 0); z < std::min(front + iters, n + 1); ++z)
-      for (int c = 0; c < 4; ++c) {
-        const int t = z - front;
-        const int p = id(z, t, c);
-        int4 d = make_int4(-1, -1, -1, 0);
-        if (c > 0) {
-          d.x = id(z, t, c - 1);
-        } else {
-          if (t > 0) d.x = id(z, t - 1, 3);
-          if (z + 1 <= n) d.y = id(z + 1, t, 3);
-          if (z > 0 && t > 0) d.z = id(z - 1, t - 1, 3);
-        }
-        if (d.x >= p || d.y >= p || d.z >= p) throw PsrError(1, "relaxation task order violates a dependency");
-        rt.deps[p] = d;
-        const int g = 4 * z + c;
-        const int rows = goff[g + 1] - goff[g];
-        const int chunks = std::max(1, (rows + RELAX_CHUNK - 1) / RELAX_CHUNK);
-        rt.chunks[p] = chunks;
-        for (int k = 0; k < chunks; ++k) {
-          const int j0 = goff[g] + k * RELAX_CHUNK;
-          rt.tasks.push_back(make_int4(p, j0, std::min(goff[g + 1], j0 + RELAX_CHUNK), 0));
-        }
-      }
-  return rt;
-}
-
-template <class T> static int cooperative_grid() {
-  static int grid = 0;
-  if (!grid) {
-    int dev = 0, cus = 0, per_cu = 0;
-    HIP_CHECK(hipGetDevice(&dev));
-    HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_relax_persistent<T>, RELAX_CHUNK, 0));
-    grid = std::max(1, std::min(per_cu, 2) * cus);
-  }
-  return grid;
-}
-
-template <class T>
-int relax_sliced_persistent(const GsOperator<T>& op, const Schedule<T>& sc, T* x, int depth, int iters, hipStream_t s) {
-  RelaxTasks rt = build_relax_tasks(sc.host_offsets, depth, iters);
-  DArray<int4> tasks, deps;
-  DArray<int> chunks, done(rt.chunks.size(), s), error(1, s);
-  tasks.from_host(rt.tasks.data(), rt.tasks.size(), s);
-  deps.from_host(rt.deps.data(), rt.deps.size(), s);
-  chunks.from_host(rt.chunks.data(), rt.chunks.size(), s);
-  done.zero();
-  error.zero();
-  const int ntasks = (int)rt.tasks.size();
-  const int grid = std::min(cooperative_grid<T>(), ntasks);
-  const int* order = op.update_order;
-  const int* cols = op.cols.data();
-  const T* vals = op.vals.data();
-  const T* b = op.b.data();
-  const T* inv = op.inv_diag.data();
-  const unsigned char* hd = op.has_diag.data();
-  int R = op.R;
-  const int4* tp = tasks.data();
-  const int4* dp = deps.data();
-  const int* cp = chunks.data();
-  int* donep = done.data();
-  int* errp = error.data();
-  int spin_limit = 1 << 24;
-  void* args[] = {(void*)&order, (void*)&cols, (void*)&vals, (void*)&b, (void*)&inv, (void*)&hd, (void*)&x, (void*)&R,
-                  (void*)&tp, (void*)&ntasks, (void*)&dp, (void*)&cp, (void*)&donep, (void*)&errp, (void*)&spin_limit};
-  HIP_CHECK(hipLaunchCooperativeKernel((const void*)k_relax_persistent<T>, dim3(grid), dim3(RELAX_CHUNK), args, 0, s));
-  if (read_int(error.data(), s)) throw PsrError(1, "sliced relaxation dependency wait exceeded its spin limit");
-  return 1;
-}
-
-// PSR_RELAX=wave|persistent selects the direct-launch or the dependency-flag
-// relaxation (both give the same bits); the default replays a captured graph.
+// PSR_RELAX=wave selects direct launches (same bits); the default replays a
+// captured graph of the launch sequence.
 static int relax_mode() {
   static int mode = -1;
   if (mode < 0) {
     const char* e = std::getenv("PSR_RELAX");
-    const std::string v = e ? e : "";
-    mode = v == "wave" ? 1 : (v == "persistent" ? 2 : 0);
+    mode = (e && std::string(e) == "wave") ? 1 : 0;
   }
   return mode;
 }
 
-template <class T>
-int relax_sliced_any(const GsOperator<T>& op, const Schedule<T>& sc, T* x, int depth, int iters, hipStream_t s) {
+template <class T, class V>
+int relax_sliced_any(const GsOperator<T, V>& op, const Schedule<T>& sc, T* x, int depth, int iters, hipStream_t s) {
   switch (relax_mode()) {
     case 1: return relax_sliced(op, sc, x, depth, iters, s);
-    case 2: return relax_sliced_persistent(op, sc, x, depth, iters, s);
     default: return relax_sliced_graph(op, sc, x, depth, iters, s);
   }
 }
@@ -1426,6 +1274,90 @@ template <class T> static void regular_multigrid(Result<T>& r, std::vector<DArra
   B.correction = std::move(x[base]);
 }
 
+// One finer level (solver.py:343-360): operator, cross operator and
+// right-hand side assembled directly in sliced schedule order (row j =
+// order[j]), operators stored with value type V (T, or float in mixed runs),
+// b -= cross * cumulative, sliced Gauss-Seidel from zero, then the cumulative
+// solution is prolonged and accumulated.
+template <class T, class V>
+static void solve_finer(Result<T>& r, int d, Stencil<V>& Aop, Stencil<V>& Cop, bool screened, DArray<T>& cumulative,
+                        hipStream_t s) {
+  LevelState<T>& L = r.levels[d];
+  const int R = L.basis_count;
+  PointSet<T> pts;
+  Schedule<T> sc = make_schedule<T>(L.basis_keys, R, d, true, s);
+  const int* ord = sc.order.data();
+  Stencil<double> A = assemble_stiffness(L, L, d, d, s, ord);
+  if (screened) {
+    pts = point_constraints(r, d, s);
+    add_screening(A, L.basis_keys, pts, d, d, L.point_load.data(), s, ord);
+  }
+  hipLaunchKernelGGL(k_rhs_sched<T>, dim3(blocks_for(R, 256)), dim3(256), 0, s, (const double*)L.normal_load.data(),
+                     (const double*)L.point_load.data(), ord, R, L.rhs.data());
+  KERNEL_CHECK();
+  Aop = cast_stencil<V>(std::move(A), s);
+  Stencil<double> cross = assemble_stiffness(L, r.levels[d - 1], d, d - 1, s, ord);
+  if (screened) add_screening(cross, L.basis_keys, pts, d, d - 1, (double*)nullptr, s, ord);
+  Cop = cast_stencil<V>(std::move(cross), s);
+  hipLaunchKernelGGL((k_spmv<T, V>), dim3(blocks_for(R, 256)), dim3(256), 0, s, (const int*)Cop.cols.data(),
+                     (const V*)Cop.vals.data(), R, 27, (const T*)cumulative.data(), L.rhs.data(), T(-1), T(1));
+  KERNEL_CHECK();
+  GsOperator<T, V> op;
+  op.R = R;
+  op.vals = std::move(Aop.vals);
+  op.b = std::move(L.rhs);
+  op.inv_diag.reset(R, s);
+  op.has_diag.reset(R, s);
+  hipLaunchKernelGGL((k_diag_sched<T, V>), dim3(blocks_for(R, 256)), dim3(256), 0, s, ord, R, (const int*)Aop.cols.data(),
+                     (const V*)op.vals.data(), op.inv_diag.data(), op.has_diag.data());
+  KERNEL_CHECK();
+  // x is relaxed in schedule order: columns become schedule positions, so
+  // a phase writes contiguous ranges and gathers from nearby ranges
+  DArray<T> xs(R, s);
+  xs.zero();  // the level's correction starts at zero (solver.py:343)
+  {
+    DArray<int> inv(R, s);
+    hipLaunchKernelGGL(k_inverse_permutation, dim3(blocks_for(R, 256)), dim3(256), 0, s, ord, R, inv.data());
+    KERNEL_CHECK();
+    op.cols.reset(27 * (size_t)R, s);
+    hipLaunchKernelGGL(k_remap_columns, dim3(blocks_for(27 * (int64_t)R, 256)), dim3(256), 0, s, (const int*)Aop.cols.data(),
+                       (const int*)inv.data(), 27 * (int64_t)R, op.cols.data());
+    KERNEL_CHECK();
+  }
+  typename Result<T>::RelaxStat st{d, R, 0, 0, nullptr, nullptr, 0.f};
+  {
+    DArray<unsigned long long> cnt(1, s);
+    cnt.zero();
+    const int64_t slots = 27 * (int64_t)R;
+    const unsigned grid = (unsigned)std::min<int64_t>(1024, (slots + 255) / 256);
+    hipLaunchKernelGGL(k_count_present, dim3(grid), dim3(256), 0, s, (const int*)op.cols.data(), slots, cnt.data());
+    KERNEL_CHECK();
+    unsigned long long nnz = 0;
+    read_bytes(&nnz, cnt.data(), sizeof(nnz), s);
+    st.nnz = (int64_t)nnz;
+  }
+  HIP_CHECK(hipEventCreate(&st.start));
+  HIP_CHECK(hipEventCreate(&st.end));
+  HIP_CHECK(hipEventRecord(st.start, s));
+  st.launches = relax_sliced_any(op, sc, xs.data(), d, r.prm.iters, s);
+  HIP_CHECK(hipEventRecord(st.end, s));
+  r.relax_stats.push_back(st);
+  hipLaunchKernelGGL(k_scatter_rows<T>, dim3(blocks_for(R, 256)), dim3(256), 0, s, ord, R, (const T*)xs.data(),
+                     L.correction.data());
+  KERNEL_CHECK();
+  // keep the schedule-ordered operator (canonical columns) and right-hand side
+  Aop.vals = std::move(op.vals);
+  L.rhs = std::move(op.b);
+  L.order = std::move(sc.order);
+  L.scheduled = true;
+  DArray<T> next(R, s);
+  HIP_CHECK(hipMemcpyAsync(next.data(), L.correction.data(), sizeof(T) * R, hipMemcpyDeviceToDevice, s));
+  hipLaunchKernelGGL(k_spmv<T>, dim3(blocks_for(R, 256)), dim3(256), 0, s, (const int*)L.P.cols.data(),
+                     (const T*)L.P.vals.data(), R, 8, (const T*)cumulative.data(), next.data(), T(1), T(1));
+  KERNEL_CHECK();
+  cumulative = std::move(next);
+}
+
 template <class T> void stage_solve(Result<T>& r) {
   hipStream_t s = r.s;
   const int D = r.prm.depth, base = r.prm.base_depth;
@@ -1468,14 +1400,13 @@ template <class T> void stage_solve(Result<T>& r) {
     LevelState<T>& L = r.levels[d];
     const int R = L.basis_count;
     if (!R) break;
-    PointSet<T> pts;
     L.point_load.reset(R, s);
     L.point_load.zero();
     L.rhs.reset(R, s);
     if (d == base) {
       Stencil<double> A = assemble_stiffness(L, L, d, d, s);
       if (screened) {
-        pts = point_constraints(r, d, s);
+        PointSet<T> pts = point_constraints(r, d, s);
         add_screening(A, L.basis_keys, pts, d, d, L.point_load.data(), s);
       }
       hipLaunchKernelGGL(k_rhs<T>, dim3(blocks_for(R, 256)), dim3(256), 0, s, (const double*)L.normal_load.data(),
@@ -1485,80 +1416,10 @@ template <class T> void stage_solve(Result<T>& r) {
       regular_multigrid(r, restr, s);
       cumulative.reset(R, s);
       HIP_CHECK(hipMemcpyAsync(cumulative.data(), L.correction.data(), sizeof(T) * R, hipMemcpyDeviceToDevice, s));
+    } else if (r.prm.float_operators) {
+      solve_finer<T, float>(r, d, L.A32, L.cross32, screened, cumulative, s);
     } else {
-      // finer level: operator, cross operator and right-hand side are
-      // assembled directly in sliced schedule order (row j = order[j])
-      Schedule<T> sc = make_schedule<T>(L.basis_keys, R, d, true, s);
-      const int* ord = sc.order.data();
-      Stencil<double> A = assemble_stiffness(L, L, d, d, s, ord);
-      if (screened) {
-        pts = point_constraints(r, d, s);
-        add_screening(A, L.basis_keys, pts, d, d, L.point_load.data(), s, ord);
-      }
-      hipLaunchKernelGGL(k_rhs_sched<T>, dim3(blocks_for(R, 256)), dim3(256), 0, s, (const double*)L.normal_load.data(),
-                         (const double*)L.point_load.data(), ord, R, L.rhs.data());
-      KERNEL_CHECK();
-      L.A = cast_stencil<T>(std::move(A), s);
-      Stencil<double> cross = assemble_stiffness(L, r.levels[d - 1], d, d - 1, s, ord);
-      if (screened) add_screening(cross, L.basis_keys, pts, d, d - 1, (double*)nullptr, s, ord);
-      L.cross = cast_stencil<T>(std::move(cross), s);
-      hipLaunchKernelGGL(k_spmv<T>, dim3(blocks_for(R, 256)), dim3(256), 0, s, (const int*)L.cross.cols.data(),
-                         (const T*)L.cross.vals.data(), R, 27, (const T*)cumulative.data(), L.rhs.data(), T(-1), T(1));
-      KERNEL_CHECK();
-      GsOperator<T> op;
-      op.R = R;
-      op.vals = std::move(L.A.vals);
-      op.b = std::move(L.rhs);
-      op.inv_diag.reset(R, s);
-      op.has_diag.reset(R, s);
-      hipLaunchKernelGGL(k_diag_sched<T>, dim3(blocks_for(R, 256)), dim3(256), 0, s, ord, R, (const int*)L.A.cols.data(),
-                         (const T*)op.vals.data(), op.inv_diag.data(), op.has_diag.data());
-      KERNEL_CHECK();
-      // x is relaxed in schedule order: columns become schedule positions, so
-      // a phase writes contiguous ranges and gathers from nearby ranges
-      DArray<T> xs(R, s);
-      xs.zero();  // the level's correction starts at zero (solver.py:343)
-      {
-        DArray<int> inv(R, s);
-        hipLaunchKernelGGL(k_inverse_permutation, dim3(blocks_for(R, 256)), dim3(256), 0, s, ord, R, inv.data());
-        KERNEL_CHECK();
-        op.cols.reset(27 * (size_t)R, s);
-        hipLaunchKernelGGL(k_remap_columns, dim3(blocks_for(27 * (int64_t)R, 256)), dim3(256), 0, s,
-                           (const int*)L.A.cols.data(), (const int*)inv.data(), 27 * (int64_t)R, op.cols.data());
-        KERNEL_CHECK();
-      }
-      typename Result<T>::RelaxStat st{d, R, 0, 0, nullptr, nullptr, 0.f};
-      {
-        DArray<unsigned long long> cnt(1, s);
-        cnt.zero();
-        const int64_t slots = 27 * (int64_t)R;
-        const unsigned grid = (unsigned)std::min<int64_t>(1024, (slots + 255) / 256);
-        hipLaunchKernelGGL(k_count_present, dim3(grid), dim3(256), 0, s, (const int*)op.cols.data(), slots, cnt.data());
-        KERNEL_CHECK();
-        unsigned long long nnz = 0;
-        read_bytes(&nnz, cnt.data(), sizeof(nnz), s);
-        st.nnz = (int64_t)nnz;
-      }
-      HIP_CHECK(hipEventCreate(&st.start));
-      HIP_CHECK(hipEventCreate(&st.end));
-      HIP_CHECK(hipEventRecord(st.start, s));
-      st.launches = relax_sliced_any(op, sc, xs.data(), d, r.prm.iters, s);
-      HIP_CHECK(hipEventRecord(st.end, s));
-      r.relax_stats.push_back(st);
-      hipLaunchKernelGGL(k_scatter_rows<T>, dim3(blocks_for(R, 256)), dim3(256), 0, s, ord, R, (const T*)xs.data(),
-                         L.correction.data());
-      KERNEL_CHECK();
-      // keep the schedule-ordered operator (canonical columns) and right-hand side
-      L.A.vals = std::move(op.vals);
-      L.rhs = std::move(op.b);
-      L.order = std::move(sc.order);
-      L.scheduled = true;
-      DArray<T> next(R, s);
-      HIP_CHECK(hipMemcpyAsync(next.data(), L.correction.data(), sizeof(T) * R, hipMemcpyDeviceToDevice, s));
-      hipLaunchKernelGGL(k_spmv<T>, dim3(blocks_for(R, 256)), dim3(256), 0, s, (const int*)L.P.cols.data(),
-                         (const T*)L.P.vals.data(), R, 8, (const T*)cumulative.data(), next.data(), T(1), T(1));
-      KERNEL_CHECK();
-      cumulative = std::move(next);
+      solve_finer<T, T>(r, d, L.A, L.cross, screened, cumulative, s);
     }
     L.solved = true;
   }

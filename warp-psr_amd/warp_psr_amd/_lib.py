@@ -21,7 +21,7 @@ class PsrParams(ctypes.Structure):
         ("solve_depth", ctypes.c_int), ("kernel_depth", ctypes.c_int), ("samples_per_node", ctypes.c_double),
         ("scale", ctypes.c_double), ("point_weight", ctypes.c_double), ("exact_interpolation", ctypes.c_int),
         ("iters", ctypes.c_int), ("base_v_cycles", ctypes.c_int), ("cg_accuracy", ctypes.c_double),
-        ("keep_intermediates", ctypes.c_int),
+        ("keep_intermediates", ctypes.c_int), ("float_operators", ctypes.c_int),
     ]
 
 

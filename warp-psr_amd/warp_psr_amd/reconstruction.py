@@ -154,11 +154,14 @@ class _Diagnostics:
 
 def reconstruct(positions, normals, *, depth=6, full_depth=3, base_depth=None, solve_depth=None, kernel_depth=None,
                 samples_per_node=1.5, scale=1.1, point_weight=2.0, exact_interpolation=False, iters=8, base_v_cycles=1,
-                cg_accuracy=1e-3):
+                cg_accuracy=1e-3, mixed=False):
     """Reconstruct oriented (n, 3) float32/float64 ROCm tensors; point_weight=0 selects PSR.
 
     Degree one, Neumann boundaries, confidence disabled, library alignDir=0.
     The surface is ``result.evaluate(q)[0] == result.iso_value``.
+    ``mixed=True`` (float64 tensors only) stores the finer levels' Laplacian +
+    screening operators as float32 while every vector, the octree stages and
+    the base-level solve stay float64.
     """
     torch = _torch()
     if base_depth is None:
@@ -188,6 +191,8 @@ def reconstruct(positions, normals, *, depth=6, full_depth=3, base_depth=None, s
         raise ValueError("Reconstruction requires matching float32/float64 3D vectors on the same CUDA device")
     if not positions.shape[0]:
         raise ValueError("points and normals must have equal nonempty 1D shapes")
+    if mixed and positions.dtype != torch.float64:
+        raise ValueError("mixed precision (float_operators) runs on float64 inputs")
     p = positions.contiguous()
     n = normals.contiguous()
     params = _lib.PsrParams()
@@ -198,6 +203,7 @@ def reconstruct(positions, normals, *, depth=6, full_depth=3, base_depth=None, s
     params.samples_per_node, params.scale, params.point_weight = samples_per_node, scale, point_weight
     params.exact_interpolation = int(bool(exact_interpolation))
     params.iters, params.base_v_cycles, params.cg_accuracy = iters, base_v_cycles, cg_accuracy
+    params.float_operators = int(bool(mixed))
     handle = ctypes.c_void_p()
     dtype_code = 1 if p.dtype == torch.float64 else 0
     with torch.cuda.device(p.device):
@@ -206,7 +212,7 @@ def reconstruct(positions, normals, *, depth=6, full_depth=3, base_depth=None, s
     resolved = dict(depth=depth, full_depth=full_depth, base_depth=base_depth, solve_depth=solve_depth,
                     kernel_depth=kernel_depth, samples_per_node=samples_per_node, scale=scale,
                     point_weight=point_weight, exact_interpolation=bool(exact_interpolation), iters=iters,
-                    base_v_cycles=base_v_cycles, cg_accuracy=cg_accuracy)
+                    base_v_cycles=base_v_cycles, cg_accuracy=cg_accuracy, mixed=bool(mixed))
     return Reconstruction(handle.value, p.dtype, p.device, resolved)
 
 
