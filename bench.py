@@ -38,33 +38,37 @@ def log(msg):
 
 
 def relax_roofline(metadata, value_bytes, vector_bytes):
-    """Algorithmic bytes of the finest sliced relaxation over its launch span.
+    """Sliced Gauss-Seidel relaxation of every finer level (one kernel, one
+    launch per colour and pipeline step): algorithmic bytes over the HIP-event
+    span of its launches on the reconstruction stream.
 
-    Per sweep: nnz*(4 + v) (column index + operator value) + (R+1)*4 (row
-    extent) + R*s (b) + 2*R*s (x read and written), v = operator value bytes,
-    s = vector bytes; times the sweeps.
+    Per level and sweep: nnz*(4 + v) (column index + operator value) +
+    (R+1)*4 (row extent) + R*s (b) + 2*R*s (x read and written), v = operator
+    value bytes, s = vector bytes; times the sweeps, summed over levels.
     """
     stats = metadata["relaxation"]
-    fine = max(stats, key=lambda s: s["rows"])
     v, s = value_bytes, vector_bytes
-    per_sweep = fine["nnz"] * (4 + v) + (fine["rows"] + 1) * 4 + fine["rows"] * s + 2 * fine["rows"] * s
-    total = per_sweep * fine["sweeps"]
-    achieved = total / fine["seconds"] / 1e9
+    total = sum((st["nnz"] * (4 + v) + (st["rows"] + 1) * 4 + 3 * st["rows"] * s) * st["sweeps"] for st in stats)
+    seconds = sum(st["seconds"] for st in stats)
+    launches = sum(st["launches"] for st in stats)
+    achieved = total / seconds / 1e9
+    fine = max(stats, key=lambda st: st["rows"])
     return {
-        "kernel": "k_relax_wave (sliced Gauss-Seidel, finest level)",
+        "kernel": "k_relax_wave_graph (sliced Gauss-Seidel, all finer levels)",
         "bound": "hbm",
         "achieved": round(achieved, 2),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 5),
         "traffic": None,
-        "depth": fine["depth"],
-        "rows": fine["rows"],
-        "nnz": fine["nnz"],
-        "launches": fine["launches"],
+        "levels": [st["depth"] for st in stats],
+        "launches": launches,
         "algorithmic_bytes": total,
-        "seconds": fine["seconds"],
-        "avg_launch_us": round(fine["seconds"] / fine["launches"] * 1e6, 3),
+        "bytes_per_launch": round(total / launches),
+        "seconds": seconds,
+        "avg_launch_us": round(seconds / launches * 1e6, 3),
+        "finest": {"depth": fine["depth"], "rows": fine["rows"], "nnz": fine["nnz"], "launches": fine["launches"],
+                   "seconds": fine["seconds"]},
     }
 
 
@@ -96,8 +100,8 @@ def main():
     ap.add_argument("--skip-fp64", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the build targets single-GPU "
-              "reconstruction; see DESIGN for how the path would shard)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: a reconstruction runs on one "
+              "MI355X", file=sys.stderr)
         return 2
 
     import torch

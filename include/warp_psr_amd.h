@@ -9,18 +9,21 @@
  * from one stream at a time; independent results may run on distinct streams.
  *
  * Reference interfaces replaced (alecjacobson/warp-psr, Python over Warp):
- *   psr_params_default  -> keyword defaults of reconstruct()      src/warp_psr/reconstruction.py:159-187
- *   psr_reconstruct     -> reconstruct(positions, normals, ...)   src/warp_psr/reconstruction.py:159-371
+ *   psr_params_default  -> keyword defaults of reconstruct()      src/warp_psr/reconstruction.py:137-165
+ *   psr_reconstruct     -> reconstruct(positions, normals, ...)   src/warp_psr/reconstruction.py:137-349
  *   psr_evaluate        -> Reconstruction.evaluate / LevelField.evaluate
- *                                                                 src/warp_psr/reconstruction.py:151-152, field.py:150-184
- *   psr_result_iso      -> Reconstruction.iso_value               src/warp_psr/reconstruction.py:147-149
- *   psr_result_field    -> Reconstruction.device_field()          src/warp_psr/reconstruction.py:154-156, device_field.py:472-499
- *   psr_result_metadata_json -> Reconstruction.metadata          src/warp_psr/reconstruction.py:325-366
+ *                                                                 src/warp_psr/reconstruction.py:129-130, field.py:150-184
+ *   psr_result_iso      -> Reconstruction.iso_value               src/warp_psr/reconstruction.py:124-127
+ *   psr_result_field    -> Reconstruction.device_field()          src/warp_psr/reconstruction.py:132-134,
+ *                                                                 field.py:140-148, device_field.py:115-142
+ *   psr_result_metadata_json -> Reconstruction.metadata          src/warp_psr/reconstruction.py:303-346
  *   psr_result_array    -> Reconstruction.diagnostics (stage arrays)
- *                                                                 src/warp_psr/reconstruction.py:367-371
+ *                                                                 src/warp_psr/reconstruction.py:303-349
  *   psr_relax_sliced    -> relax_sliced(A, b, x, schedule, depth, iters)
- *                                                                 src/warp_psr/relaxation.py:189-210
+ *                                                                 src/warp_psr/relaxation.py:189-238
  *   psr_result_free     -> (Python garbage collection of Reconstruction)
+ *   PsrParams.float_operators -> the mixed precision mode (no reference counterpart;
+ *                                float64 run with float32 finer-level operators)
  */
 #ifndef WARP_PSR_AMD_H
 #define WARP_PSR_AMD_H
@@ -81,7 +84,10 @@ int psr_result_iso(const PsrResult* result, double* iso);
 int psr_result_field(const PsrResult* result, PsrField* field);
 /* Writes NUL-terminated JSON into buf (size bytes); *needed receives the full length + 1. */
 int psr_result_metadata_json(const PsrResult* result, char* buf, size_t size, size_t* needed);
-/* Host copy (owned by the result) of a named stage array; see DESIGN for names. */
+/* Host copy (owned by the result) of a named stage array, e.g. "sample_keys", "densities",
+ * "normal_keys", "tree_keys"/"basis_keys" (per level), "matrix_values"/"cross_values"/"rhs"/
+ * "corrections" (per solved level), "keys", "coefficients", "cumulative"; level -1 for
+ * whole-run arrays.  Integer arrays are int32/int64, values in the run precision. */
 int psr_result_array(PsrResult* result, const char* name, int level, const void** data, int64_t* nbytes);
 void psr_result_free(PsrResult* result);
 
@@ -90,6 +96,13 @@ void psr_result_free(PsrResult* result);
  * columns/values (<= 27 entries per row), b[rows], x[rows] updated in place. */
 int psr_relax_sliced(int dtype, const int64_t* keys, int64_t rows, const int32_t* offsets, const int32_t* columns,
                      const void* values, const void* b, void* x, int depth, int iters, void* stream);
+
+/* Evaluates any PsrField view (psr_result_field) at m device queries with the
+ * header-only psr_evaluate_field of warp_psr_amd_field.h -- the kernel-side
+ * evaluate_field(field, point) of src/warp_psr/device_field.py:91-112 --
+ * giving the same bits as psr_evaluate.  grads may be NULL. */
+int psr_field_evaluate(const PsrField* field, const void* queries, int64_t m, void* values, void* grads, int32_t* valid,
+                       void* stream);
 
 const char* psr_last_error(void);
 /* Frees the library's cached device blocks. */

@@ -62,3 +62,20 @@ def test_repeatable(cuda):
     for name in ("keys", "coefficients", "cumulative"):
         assert same(a.diagnostics.array(name), b.diagnostics.array(name))
     assert a.iso_value == b.iso_value
+
+
+@pytest.mark.parametrize("name,shape,count,precision,params", [FIXTURES[0], FIXTURES[5], FIXTURES[-2]],
+                         ids=[FIXTURES[0][0], FIXTURES[5][0], FIXTURES[-2][0]])
+def test_device_field_header_matches_evaluate(cuda, name, shape, count, precision, params):
+    """The header-only psr_evaluate_field (via psr_field_evaluate on the
+    zero-copy PsrField view) gives the bits of psr_evaluate and the oracle."""
+    import torch
+    p, gpu, ref = _run(cuda, shape, count, precision, params)
+    q = torch.from_numpy(fixture_queries(p)).to(cuda)
+    values, grads, valid = gpu.evaluate(q)
+    hv, hg, hvalid = gpu.device_field().evaluate(q)
+    np.testing.assert_array_equal(hvalid.cpu().numpy(), valid.cpu().numpy())
+    assert same(hv.cpu().numpy(), values.cpu().numpy())
+    assert same(hg.cpu().numpy(), grads.cpu().numpy())
+    rv, _, _ = ref.evaluate(q.cpu().numpy())
+    assert same(hv.cpu().numpy(), rv)

@@ -2,7 +2,7 @@
 
 * libwarp_psr_amd.so loads and exports every function include/*.h declares.
 * Parameter validation happens before any device work and reproduces the
-  reference's ValueError messages (src/warp_psr/reconstruction.py:181-234,
+  reference's ValueError messages (src/warp_psr/reconstruction.py:160-212,
   tests/test_reconstruction.py:36-39).
 """
 import ctypes

@@ -26,7 +26,7 @@ from warp_psr_amd import synthetic
 REPORTS = json.loads((Path(__file__).parent / "golden" / "reference_reports.json").read_text())
 FIXTURES = [f for f in REPORTS["fixtures"] if f["params"]["shape"] != "external" and not f["params"].get("variant")]
 # base_depth 0 float32 uses the reference's ordered serial normal loads
-# (ordered_normal_loads.py), which this build does not restate; its tree is
+# (ordered_normal_loads.py), which the oracle does not restate; its tree is
 # still checked, its isovalue against PoissonRecon's with the field tolerance.
 ORDERED_ROOT = {f["name"] for f in FIXTURES if f["params"]["base_depth"] == 0 and f["params"]["precision"] == "float"}
 

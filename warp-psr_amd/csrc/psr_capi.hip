@@ -3,6 +3,7 @@
 // HIP-event stage timings (reconstruction.py:241-250), result ownership and
 // the named stage-array export used by parity tests.
 #include "../../include/warp_psr_amd.h"
+#include "../../include/warp_psr_amd_field.h"
 #include "psr_result.h"
 
 #include <cmath>
@@ -245,6 +246,21 @@ template <class T> std::string metadata_json(const Result<T>& r) {
 
 }  // namespace
 
+namespace {
+template <class T>
+__global__ void k_field_evaluate(PsrField f, const T* __restrict__ queries, int64_t m, T* __restrict__ values,
+                                 T* __restrict__ grads, int32_t* __restrict__ valid) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const T q[3] = {queries[3 * i], queries[3 * i + 1], queries[3 * i + 2]};
+  T v, g[3];
+  const int ok = psr_evaluate_field<T>(f, q, &v, grads ? g : nullptr);
+  values[i] = v;
+  if (grads) { grads[3 * i] = g[0]; grads[3 * i + 1] = g[1]; grads[3 * i + 2] = g[2]; }
+  if (valid) valid[i] = ok;
+}
+}  // namespace
+
 extern "C" {
 
 void psr_params_default(PsrParams* p) {
@@ -389,6 +405,28 @@ void psr_result_free(PsrResult* result) {
   if (result->dtype == 0) delete (Result<float>*)result->impl;
   else delete (Result<double>*)result->impl;
   delete result;
+}
+
+int psr_field_evaluate(const PsrField* field, const void* queries, int64_t m, void* values, void* grads, int32_t* valid,
+                       void* stream) {
+  try {
+    if (!field || !field->keys || !field->level_offsets) throw PsrError(2, "field must be a psr_result_field view");
+    if (field->dtype != 0 && field->dtype != 1) throw PsrError(3, "field dtype must be 0 (float32) or 1 (float64)");
+    if (m < 0) throw PsrError(2, "query count must be nonnegative");
+    if (m == 0) return 0;
+    if (!queries || !values) throw PsrError(2, "queries and values must be non-null");
+    hipStream_t s = (hipStream_t)stream;
+    if (field->dtype == 0)
+      hipLaunchKernelGGL(k_field_evaluate<float>, dim3(blocks_for(m, 256)), dim3(256), 0, s, *field, (const float*)queries, m,
+                         (float*)values, (float*)grads, valid);
+    else
+      hipLaunchKernelGGL(k_field_evaluate<double>, dim3(blocks_for(m, 256)), dim3(256), 0, s, *field,
+                         (const double*)queries, m, (double*)values, (double*)grads, valid);
+    KERNEL_CHECK();
+    return 0;
+  } catch (const PsrError& e) {
+    return fail(e);
+  }
 }
 
 int psr_relax_sliced(int dtype, const int64_t* keys, int64_t rows, const int32_t* offsets, const int32_t* columns,

@@ -36,14 +36,17 @@ def _stream_handle(device):
 
 @dataclass
 class DeviceField:
-    """Zero-copy device view of a field (reference device_field.py:375-386).
+    """Zero-copy device view of a field (reference device_field.py:18-29, 115-142).
 
-    ``handle`` is the C ``PsrField`` struct; HIP kernels can evaluate it with
-    ``psr_evaluate_field`` from include/warp_psr_amd_field.h.  Keep the owning
-    Reconstruction alive and unmodified while kernels use it.
+    ``handle`` is the C ``PsrField`` struct; HIP kernels evaluate it with the
+    header-only ``psr_evaluate_field`` of include/warp_psr_amd_field.h (the
+    reference's ``evaluate_field(field, point)``, device_field.py:91-112).
+    Keep the owning Reconstruction alive and unmodified while kernels use it.
     """
     handle: _lib.PsrField
     owner: object = dc_field(repr=False)
+    dtype: object = None
+    device: object = None
 
     @property
     def depth(self):
@@ -52,6 +55,23 @@ class DeviceField:
     @property
     def count(self):
         return self.handle.count
+
+    def evaluate(self, queries):
+        """psr_field_evaluate: the header-only device evaluation over queries."""
+        torch = _torch()
+        if (not isinstance(queries, torch.Tensor) or queries.dtype != self.dtype or queries.device != self.device
+                or queries.ndim != 2 or queries.shape[1] != 3):
+            raise ValueError("Query precision/device must match coefficients")
+        q = queries.contiguous()
+        m = q.shape[0]
+        values = torch.empty(m, dtype=self.dtype, device=self.device)
+        gradients = torch.empty((m, 3), dtype=self.dtype, device=self.device)
+        valid = torch.empty(m, dtype=torch.int32, device=self.device)
+        if m:
+            _lib.check(_lib.load().psr_field_evaluate(ctypes.byref(self.handle), q.data_ptr(), m, values.data_ptr(),
+                                                      gradients.data_ptr(), valid.data_ptr(),
+                                                      _stream_handle(self.device)))
+        return values, gradients, valid
 
 
 class LevelField:
@@ -88,7 +108,7 @@ class LevelField:
         return values, gradients, valid
 
     def device_field(self):
-        return DeviceField(self._field, self._result)
+        return DeviceField(self._field, self._result, self.dtype, self.device)
 
 
 class _ResultHandle:
