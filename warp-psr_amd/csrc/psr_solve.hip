@@ -619,6 +619,49 @@ __global__ void __launch_bounds__(1024) k_relax_block(const int* __restrict__ or
   }
 }
 
+// The same with x resident in LDS for the whole call (levels whose x fits):
+// the matrix rows are streamed from L2 while every x gather and update stays
+// on-chip; the updates and their order are unchanged.
+template <class T>
+__global__ void __launch_bounds__(1024) k_relax_block_lds(const int* __restrict__ order, const int* __restrict__ goff,
+                                                          const int* __restrict__ cols, const T* __restrict__ vals,
+                                                          const T* __restrict__ b, const T* __restrict__ inv_diag,
+                                                          const unsigned char* __restrict__ has_diag, T* __restrict__ x,
+                                                          int R, int iters, int forward) {
+  extern __shared__ unsigned char lds_raw[];
+  T* xs = (T*)lds_raw;
+  for (int i = threadIdx.x; i < R; i += blockDim.x) xs[i] = x[i];
+  __syncthreads();
+  for (int it = 0; it < iters; ++it) {
+    for (int c = 0; c < 8; ++c) {
+      const int g = forward ? c : 7 - c;
+      for (int j = goff[g] + threadIdx.x; j < goff[g + 1]; j += blockDim.x) {
+        int cc[27];
+#pragma unroll
+        for (int k = 0; k < 27; ++k) cc[k] = cols[(int64_t)k * R + j];
+        T v[27], xv[27];
+#pragma unroll
+        for (int k = 0; k < 27; ++k) {
+          v[k] = vals[(int64_t)k * R + j];
+          xv[k] = xs[cc[k] >= 0 ? cc[k] : 0];
+        }
+        T r = b[j];
+#pragma unroll
+        for (int k = 0; k < 27; ++k) {
+          const T t = r - xv[k] * v[k];
+          r = cc[k] >= 0 ? t : r;
+        }
+        if (has_diag[j]) {
+          const int i = order[j];
+          xs[i] = xs[i] + r * inv_diag[j];
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (int i = threadIdx.x; i < R; i += blockDim.x) x[i] = xs[i];
+}
+
 // schedule: group per row
 __global__ void k_groups_of(const int64_t* __restrict__ keys, int R, int sliced, uint64_t* __restrict__ g, int* __restrict__ ids) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1005,9 +1048,21 @@ int relax_sliced_any(const GsOperator<T, V>& op, const Schedule<T>& sc, T* x, in
   }
 }
 
+// x of up to 8,192 double (16,384 float) rows stays in LDS (64 KiB)
+constexpr size_t RELAX_BLOCK_LDS_BYTES = 65536;
+
 template <class T>
 void relax_colored(const GsOperator<T>& op, const Schedule<T>& sc, T* x, int iters, bool forward, hipStream_t s) {
   if (!op.R) return;
+  const size_t lds = sizeof(T) * (size_t)op.R;
+  if (op.update_order && lds <= RELAX_BLOCK_LDS_BYTES) {
+    hipLaunchKernelGGL(k_relax_block_lds<T>, dim3(1), dim3(1024), lds, s, op.update_order, (const int*)sc.offsets.data(),
+                       (const int*)op.cols.data(), (const T*)op.vals.data(), (const T*)op.b.data(),
+                       (const T*)op.inv_diag.data(), (const unsigned char*)op.has_diag.data(), x, op.R, iters,
+                       forward ? 1 : 0);
+    KERNEL_CHECK();
+    return;
+  }
   hipLaunchKernelGGL(k_relax_block<T>, dim3(1), dim3(1024), 0, s, op.update_order, (const int*)sc.offsets.data(),
                      (const int*)op.cols.data(), (const T*)op.vals.data(), (const T*)op.b.data(), (const T*)op.inv_diag.data(),
                      (const unsigned char*)op.has_diag.data(), x, op.R, iters, forward ? 1 : 0);
