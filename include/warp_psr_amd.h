@@ -22,6 +22,9 @@
  *   psr_relax_sliced    -> relax_sliced(A, b, x, schedule, depth, iters)
  *                                                                 src/warp_psr/relaxation.py:189-238
  *   psr_result_free     -> (Python garbage collection of Reconstruction)
+ *   psr_field_vjp       -> DifferentiableReconstruction.evaluate backward
+ *                                                                 src/warp_psr/field_adjoint.py:255-288, field.py:214-286
+ *   psr_backward        -> reconstruct_differentiable backward    src/warp_psr/reconstruction_adjoint.py:89-177
  *   PsrParams.float_operators -> the mixed precision mode (no reference counterpart;
  *                                float64 run with float32 finer-level operators)
  */
@@ -96,6 +99,26 @@ void psr_result_free(PsrResult* result);
  * columns/values (<= 27 entries per row), b[rows], x[rows] updated in place. */
 int psr_relax_sliced(int dtype, const int64_t* keys, int64_t rows, const int32_t* offsets, const int32_t* columns,
                      const void* values, const void* b, void* x, int depth, int iters, void* stream);
+
+/* Reverse mode (first order, octree / grouping / splat depths / solver branches
+ * fixed; src/warp_psr/reconstruction_adjoint.py:61-185).
+ *
+ * psr_field_vjp: value seeds (m) and model-gradient seeds (m x 3, either may be
+ * NULL) at model-space queries -> float64 gradients w.r.t. the K concatenated
+ * coefficients and cumulative coefficients (accumulated; K = PsrField.count)
+ * and w.r.t. the queries (m x 3 in the run precision, accumulated; may be NULL).
+ * Mirrors DifferentiableField.evaluate's backward (field_adjoint.py:62-104,
+ * 255-288, field.py:214-286).
+ *
+ * psr_backward: coefficient / cumulative gradients (either may be NULL) and an
+ * isovalue seed -> gradient w.r.t. the input normals (n x 3, accumulated);
+ * `normals` are the forward's input normals.  Gradients w.r.t. positions are
+ * not provided.  Returns 2 when the root conjugate-gradient solve ran (its
+ * reverse is not provided). */
+int psr_field_vjp(PsrResult* result, const void* queries, int64_t m, const void* value_seed, const void* grad_seed,
+                  double* coef_grad, double* cum_grad, void* query_grad, void* stream);
+int psr_backward(PsrResult* result, const double* coef_grad, const double* cum_grad, double iso_seed, const void* normals,
+                 int64_t n, void* grad_normals, void* stream);
 
 /* Evaluates any PsrField view (psr_result_field) at m device queries with the
  * header-only psr_evaluate_field of warp_psr_amd_field.h -- the kernel-side

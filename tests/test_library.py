@@ -108,3 +108,14 @@ def test_metadata_parser_shapes():
     text = (ROOT / "warp-psr_amd" / "csrc" / "psr_capi.hip").read_text()
     for key in ("basis_counts", "timings_seconds", "relaxation", "cube_to_model", "parameters", "iso_value"):
         assert f'\\"{key}\\"' in text
+
+
+def test_differentiable_rejects_tracked_positions():
+    import torch
+    from warp_psr_amd import reconstruct_differentiable
+    p = torch.zeros((4, 3), dtype=torch.float64, requires_grad=True)
+    n = torch.ones((4, 3), dtype=torch.float64, requires_grad=True)
+    with pytest.raises(ValueError, match="positions"):
+        reconstruct_differentiable(p, n)
+    with pytest.raises(ValueError, match="requires_grad"):
+        reconstruct_differentiable(p.detach(), n.detach())

@@ -429,6 +429,50 @@ int psr_field_evaluate(const PsrField* field, const void* queries, int64_t m, vo
   }
 }
 
+int psr_field_vjp(PsrResult* result, const void* queries, int64_t m, const void* value_seed, const void* grad_seed,
+                  double* coef_grad, double* cum_grad, void* query_grad, void* stream) {
+  try {
+    if (!result) throw PsrError(2, "null result");
+    if (m < 0) throw PsrError(2, "query count must be nonnegative");
+    if (m == 0 || (!value_seed && !grad_seed)) return 0;
+    if (!queries) throw PsrError(2, "queries must be non-null");
+    hipStream_t s = (hipStream_t)stream;
+    if (result->dtype == 0)
+      field_vjp(*(Result<float>*)result->impl, (const float*)queries, m, (const float*)value_seed, (const float*)grad_seed,
+                coef_grad, cum_grad, (float*)query_grad, s);
+    else
+      field_vjp(*(Result<double>*)result->impl, (const double*)queries, m, (const double*)value_seed, (const double*)grad_seed,
+                coef_grad, cum_grad, (double*)query_grad, s);
+    return 0;
+  } catch (const PsrError& e) {
+    return fail(e);
+  }
+}
+
+int psr_backward(PsrResult* result, const double* coef_grad, const double* cum_grad, double iso_seed, const void* normals,
+                 int64_t n, void* grad_normals, void* stream) {
+  try {
+    if (!result) throw PsrError(2, "null result");
+    if (!normals || !grad_normals) throw PsrError(2, "normals and grad_normals must be non-null");
+    hipStream_t s = (hipStream_t)stream;
+    if (result->dtype == 0) {
+      auto* r = (Result<float>*)result->impl;
+      if (n != r->n) throw PsrError(2, "normals must be the reconstruction's input normals");
+      backward_normals(*r, coef_grad, cum_grad, iso_seed, (float*)grad_normals, (const float*)normals, n, s);
+    } else {
+      auto* r = (Result<double>*)result->impl;
+      if (n != r->n) throw PsrError(2, "normals must be the reconstruction's input normals");
+      backward_normals(*r, coef_grad, cum_grad, iso_seed, (double*)grad_normals, (const double*)normals, n, s);
+    }
+    return 0;
+  } catch (const PsrError& e) {
+    return fail(e);
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return 1;
+  }
+}
+
 int psr_relax_sliced(int dtype, const int64_t* keys, int64_t rows, const int32_t* offsets, const int32_t* columns,
                      const void* values, const void* b, void* x, int depth, int iters, void* stream) {
   try {

@@ -1,44 +1,13 @@
 // Stage 6: cumulative coefficients (src/warp_psr/field.py:187-210), the
 // reference evaluation (field.py:364-421), the isovalue
 // (reconstruction.py:76-138) and the cube/model transforms (transforms.py).
-#include "psr_result.h"
+#include "psr_field_index.h"
 
 #include <cmath>
 
 namespace psr {
 
 namespace {
-
-// keys of level d occupy [off[d], off[d+1]) of the concatenated sorted keys;
-// start/end (optional) are the column runs of every level
-struct FieldIndex {
-  const int64_t* keys;
-  const int* off;  // device, depth+2 entries
-  const int* start;
-  const int* end;
-};
-
-// first cell of level d's column table: sum over e < d of (2^e + 1)^2
-PSR_HD int64_t column_base(int d) { return (((int64_t)1 << (2 * d)) - 1) / 3 + 2 * (((int64_t)1 << d) - 1) + d; }
-constexpr int FIELD_TABLE_MAX_DEPTH = 11;
-
-// Absolute indices of keys (d, x, y, z0 + k), k < len (-1 when absent).
-template <int MAXLEN>
-__device__ __forceinline__ void level_run(const FieldIndex& f, int d, int x, int y, int z0, int len, int64_t* out) {
-  if (f.start) {
-    ColumnTable t;
-    t.start = f.start + column_base(d);
-    t.end = f.end + column_base(d);
-    t.dim = (1 << d) + 1;
-    find_run_table<MAXLEN>(f.keys, 0, t, d, x, y, z0, len, out);
-    return;
-  }
-  const int64_t lo = f.off[d];
-  find_run<MAXLEN>(f.keys + lo, (int64_t)f.off[d + 1] - lo, d, x, y, z0, len, out);
-#pragma unroll
-  for (int k = 0; k < MAXLEN; ++k)
-    if (k < len && out[k] >= 0) out[k] += lo;
-}
 
 __global__ void k_field_columns(const int64_t* __restrict__ keys, int64_t n, int* __restrict__ start, int* __restrict__ end) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -107,8 +76,6 @@ __device__ __forceinline__ void add_level(const FieldIndex& f, const T* coef, T 
     res[3] = res[3] + c * T(n) * h[0] * h[1] * sg[2];
   }
 }
-
-template <class T> struct Affine { T L[9]; T t[3]; };
 
 // field.py:395-420 (reference evaluation kernel)
 template <class T>
@@ -253,14 +220,6 @@ template <class T> void reference_inverse(const T* m, T* out) {
     }
 }
 
-template <class T> static FieldIndex field_index(const Result<T>& r) {
-  FieldIndex f;
-  f.keys = r.keys.data();
-  f.off = r.offsets_dev.data();
-  f.start = r.column_start.empty() ? nullptr : r.column_start.data();
-  f.end = r.column_start.empty() ? nullptr : r.column_end.data();
-  return f;
-}
 
 template <class T>
 void evaluate_field(const Result<T>& r, const T* queries, int64_t m, const T* linear, const T* translation, T* values, T* grads,

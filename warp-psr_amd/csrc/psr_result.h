@@ -28,6 +28,8 @@ template <class T> struct LevelState {
   DArray<int64_t> tree_keys, basis_keys;
   int basis_count = 0;
   ColumnIndex columns;  // (x, y) runs of basis_keys
+  // restriction gathers to this level from level d+1: 27 fine indices per row
+  DArray<int> restriction;
   // aggregates (density.py:228-240): keys and (sum x, sum y, sum z, count)
   DArray<int64_t> agg_keys;
   DArray<T> agg_vals;  // 4 per record
@@ -127,6 +129,21 @@ template <class T> void stage_samples(Result<T>& r, const T* pts, const T* nrm);
 template <class T> void stage_density(Result<T>& r);
 template <class T> void stage_normals(Result<T>& r);
 template <class T> void stage_tree(Result<T>& r);
+// Field reverse mode (psr_adjoint.hip): value / model-gradient seeds at
+// model-space queries -> float64 coefficient and cumulative gradients
+// (atomically accumulated) and query gradients (accumulated, may be null).
+template <class T>
+void field_vjp(Result<T>& r, const T* queries, int64_t m, const T* gvalue, const T* ggrad, double* gcoef, double* gcum,
+               T* gquery, hipStream_t s);
+// Coefficient / cumulative / isovalue seeds -> input-normal gradients
+// (accumulated into gnormals, n x 3).
+template <class T>
+void backward_normals(Result<T>& r, const double* gcoef, const double* gcum, double iso_seed, T* gnormals, const T* normals,
+                      int64_t n, hipStream_t s);
+// Reverse of the cascadic solve and normal loads: gcoef (float64, one per
+// concatenated coefficient) gives float64 gradients of the normal records'
+// three values (accumulated into grecords).
+template <class T> void solve_backward(Result<T>& r, const double* gcoef, double* grecords, hipStream_t s);
 template <class T> void stage_solve(Result<T>& r);
 template <class T> void stage_field(Result<T>& r);
 template <class T>

@@ -1478,6 +1478,7 @@ template <class T> void stage_solve(Result<T>& r) {
     }
     L.solved = true;
   }
+  for (int d = 0; d < D; ++d) r.levels[d].restriction = std::move(restr[d]);
 }
 
 // CSR -> slot layout in entry order (GS only iterates present slots in order)
@@ -1513,6 +1514,423 @@ void relax_sliced_csr(const int64_t* keys, int R, const int* offsets, const int*
   (void)relax_sliced_any(op, sc, x, depth, iters, s);
 }
 
+// ================================================================ adjoint
+// Reverse of the cascadic solve and of the normal loads with respect to the
+// normal-record values, holding the octree, matrices, schedules and the root
+// solve's branch fixed (cascadic_adjoint.py:71-104, relaxation_adjoint.py,
+// multigrid_adjoint.py, normal_load_adjoint.py:127-161).  Gradients are
+// float64 throughout; scatters to shared neighbours use float64 atomics, as
+// the reference's reverse kernels do (relaxation_adjoint.py:47-66).
+
+// Reverse of one Gauss-Seidel launch (relaxation_adjoint.py:47-66 without the
+// matrix-value terms): rows of the launch's groups in reverse of the forward
+// update, x_i += (b_i - sum_k A_ik x_k) / A_ii.  x is canonical (row
+// order[j] at schedule position j); rows of one launch are uncoupled.
+template <class T, class V>
+__global__ void __launch_bounds__(256) k_gs_reverse(const int* __restrict__ order, const int* __restrict__ goff, int first_group,
+                                                    int group_stride, const int* __restrict__ cols, const V* __restrict__ vals,
+                                                    const T* __restrict__ inv_diag, const unsigned char* __restrict__ has_diag,
+                                                    int R, double* __restrict__ gx, double* __restrict__ gb) {
+  const int group = first_group + (int)blockIdx.y * group_stride;
+  const int j = goff[group] + (int)blockIdx.x * blockDim.x + (int)threadIdx.x;
+  if (j >= goff[group + 1] || !has_diag[j]) return;
+  const int i = order[j];
+  const double rbar = gx[i] * double(inv_diag[j]);
+  if (rbar == 0.0) return;
+  gb[i] += rbar;
+  for (int k = 0; k < 27; ++k) {
+    const int c = cols[(int64_t)k * R + j];
+    if (c >= 0) atomicAdd(&gx[c], -rbar * double(vals[(int64_t)k * R + j]));
+  }
+}
+
+// gx[col] += alpha * A[row, col] * gy[row] (A^T gy), rows at positions j of
+// canonical row order[j] (or j)
+template <class V>
+__global__ void k_stencil_scatter(const int* __restrict__ order, const int* __restrict__ cols, const V* __restrict__ vals, int R,
+                                  int slots, const double* __restrict__ gy, double alpha, double* __restrict__ gx) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= R) return;
+  const double g = gy[order ? order[j] : j] * alpha;
+  if (g == 0.0) return;
+  for (int k = 0; k < slots; ++k) {
+    const int c = cols[(int64_t)k * R + j];
+    if (c >= 0) atomicAdd(&gx[c], g * double(vals[(int64_t)k * R + j]));
+  }
+}
+
+// transpose of k_restrict_apply: gx[fine] += w_k * gy[coarse]
+__global__ void k_restrict_scatter(const int* __restrict__ cols, int C, const double* __restrict__ gy, double* __restrict__ gx) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= C) return;
+  const double g = gy[r];
+  if (g == 0.0) return;
+  for (int k = 0; k < 27; ++k) {
+    const int c = cols[(int64_t)k * C + r];
+    if (c >= 0) atomicAdd(&gx[c], restriction_weight(k) * g);
+  }
+}
+
+__global__ void k_axpy_d(double* __restrict__ y, const double* __restrict__ x, double a, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] += a * x[i];
+}
+
+// transpose of k_normal_load: record gradients from the load gradient
+__global__ void k_normal_load_vjp(const int64_t* __restrict__ keys, int R, const int64_t* __restrict__ nkeys, int64_t ncount,
+                                  ColumnTable ct, int test_depth, int normal_depth, int first, int width,
+                                  const D2* __restrict__ table, const double* __restrict__ gout, double* __restrict__ gvals) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= R) return;
+  const double g = gout[i];
+  if (g == 0.0) return;
+  const int64_t key = keys[i];
+  const int c0 = key_x(key), c1 = key_y(key), c2 = key_z(key);
+  const int e0 = normal_center(c0, test_depth, normal_depth), e1 = normal_center(c1, test_depth, normal_depth),
+            e2 = normal_center(c2, test_depth, normal_depth);
+  const int lim = 1 << normal_depth;
+  const int zlo = imax(e2 + first, 0), zhi = imin(e2 + first + width - 1, lim - 1);
+  for (int x = 0; x < width; ++x) {
+    const int nx = e0 + x + first;
+    if (nx < 0 || nx >= lim) continue;
+    for (int y = 0; y < width; ++y) {
+      const int ny = e1 + y + first;
+      if (ny < 0 || ny >= lim || zlo > zhi) continue;
+      int64_t found[6];
+      find_run_table<6>(nkeys, ncount, ct, normal_depth, nx, ny, zlo, zhi - zlo + 1, found);
+      for (int z = 0; z < width; ++z) {
+        const int nz = e2 + z + first;
+        if (nz < zlo || nz > zhi) continue;
+        const int64_t jn = found[nz - zlo];
+        if (jn < 0) continue;
+        const D2 ix = table[c0 * width + x], iy = table[c1 * width + y], iz = table[c2 * width + z];
+        atomicAdd(&gvals[3 * jn], g * ix.v1 * iy.v0 * iz.v0);
+        atomicAdd(&gvals[3 * jn + 1], g * ix.v0 * iy.v1 * iz.v0);
+        atomicAdd(&gvals[3 * jn + 2], g * ix.v0 * iy.v0 * iz.v1);
+      }
+    }
+  }
+}
+
+// transpose of k_normal_prolong: own records and the previous level's
+// cumulative normal coefficients from this level's
+template <class T>
+__global__ void k_normal_prolong_vjp(const int64_t* __restrict__ keys, int R, const int64_t* __restrict__ own_keys,
+                                     int64_t own_count, ColumnTable own_ct, const int64_t* __restrict__ prev_keys,
+                                     int64_t prev_count, ColumnTable ct, int depth, const double* __restrict__ gcum,
+                                     double* __restrict__ gown, double* __restrict__ gprev) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= R) return;
+  const double g0 = gcum[3 * i], g1 = gcum[3 * i + 1], g2 = gcum[3 * i + 2];
+  if (g0 == 0.0 && g1 == 0.0 && g2 == 0.0) return;
+  const int64_t key = keys[i];
+  const int c[3] = {key_x(key), key_y(key), key_z(key)};
+  int64_t own;
+  find_run_table<1>(own_keys, own_count, own_ct, depth, c[0], c[1], c[2], 1, &own);
+  if (own >= 0) {  // one row per record key: no other writer in this launch
+    gown[3 * own] += g0; gown[3 * own + 1] += g1; gown[3 * own + 2] += g2;
+  }
+  if (depth == 0) return;
+  int64_t idx[2][2][2];
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b) find_run_table<2>(prev_keys, prev_count, ct, depth - 1, c[0] / 2 + a, c[1] / 2 + b, c[2] / 2, 2, idx[a][b]);
+  for (int corner = 0; corner < 8; ++corner) {
+    int p[3] = {c[0] / 2, c[1] / 2, c[2] / 2};
+    T weight = T(1);
+    for (int axis = 0; axis < 3; ++axis) {
+      if ((corner >> axis) & 1) p[axis] += 1;
+      weight = weight * (T(1) - fabs(T(c[axis]) * T(0.5) - T(p[axis])));
+    }
+    if (weight > T(0)) {
+      const int64_t j = idx[corner & 1][(corner >> 1) & 1][(corner >> 2) & 1];
+      if (j >= 0) {
+        const double w = double(weight);
+        atomicAdd(&gprev[3 * j], w * g0);
+        atomicAdd(&gprev[3 * j + 1], w * g1);
+        atomicAdd(&gprev[3 * j + 2], w * g2);
+      }
+    }
+  }
+}
+
+namespace {
+
+DArray<double> zeros_d(size_t n, hipStream_t s) {
+  DArray<double> a(n, s);
+  a.zero();
+  return a;
+}
+
+// reverse of relax_sliced: the launch program backwards, colours 3..0
+template <class T, class V>
+void gs_sliced_reverse(const Schedule<T>& sc, const int* cols, const V* vals, const T* inv_diag, const unsigned char* has_diag,
+                       int R, int depth, int iters, double* gx, double* gb, hipStream_t s) {
+  const int n = 1 << depth;
+  const unsigned bx = (unsigned)std::max(1, (sc.max_group + 255) / 256);
+  std::vector<std::pair<int, int>> program;  // (first slice, slices)
+  for (int q = 0; q <= n + 2 * (iters - 1); ++q) {
+    const int first = std::max(0, floordiv(q - n + 1, 2));
+    const int last = std::min(iters - 1, q / 2);
+    if (last < first) continue;
+    program.emplace_back(n - q + 2 * first, last - first + 1);
+  }
+  for (auto it = program.rbegin(); it != program.rend(); ++it)
+    for (int color = 3; color >= 0; --color) {
+      hipLaunchKernelGGL((k_gs_reverse<T, V>), dim3(bx, (unsigned)it->second), dim3(256), 0, s, (const int*)sc.order.data(),
+                         (const int*)sc.offsets.data(), 4 * it->first + color, 8, cols, vals, inv_diag, has_diag, R, gx, gb);
+    }
+  KERNEL_CHECK();
+}
+
+// reverse of relax_colored (8-colour sweeps, forward 0..7 or reverse 7..0)
+template <class T>
+void gs_colored_reverse(const GsOperator<T>& op, const Schedule<T>& sc, int iters, bool forward, double* gx, double* gb,
+                        hipStream_t s) {
+  if (!op.R) return;
+  const unsigned bx = (unsigned)std::max(1, (sc.max_group + 255) / 256);
+  for (int it = 0; it < iters; ++it)
+    for (int c = 0; c < 8; ++c) {
+      const int g = forward ? 7 - c : c;
+      hipLaunchKernelGGL((k_gs_reverse<T, T>), dim3(bx, 1), dim3(256), 0, s, op.update_order, (const int*)sc.offsets.data(), g, 0,
+                         (const int*)op.cols.data(), (const T*)op.vals.data(), (const T*)op.inv_diag.data(),
+                         (const unsigned char*)op.has_diag.data(), op.R, gx, gb);
+    }
+  KERNEL_CHECK();
+}
+
+template <class V>
+void stencil_scatter(const int* order, const Stencil<V>& m, int slots, const double* gy, double alpha, double* gx, hipStream_t s) {
+  if (!m.rows) return;
+  hipLaunchKernelGGL(k_stencil_scatter<V>, dim3(blocks_for(m.rows, 256)), dim3(256), 0, s, order, (const int*)m.cols.data(),
+                     (const V*)m.vals.data(), m.rows, slots, gy, alpha, gx);
+  KERNEL_CHECK();
+}
+
+// One finer level backwards: gx (w.r.t. its correction, consumed) gives the
+// right-hand-side gradient grhs and adds to gcum_prev (w.r.t. the cumulative
+// solution of the level below).
+template <class T, class V>
+void finer_backward(Result<T>& r, int d, const Stencil<V>& Aop, const Stencil<V>& Cop, DArray<double>& gx, DArray<double>& grhs,
+                    DArray<double>& gcum_prev, hipStream_t s) {
+  LevelState<T>& L = r.levels[d];
+  const int R = L.basis_count;
+  Schedule<T> sc = make_schedule<T>(L.basis_keys, R, d, true, s);
+  DArray<T> inv_diag(R, s);
+  DArray<unsigned char> has_diag(R, s);
+  hipLaunchKernelGGL((k_diag_sched<T, V>), dim3(blocks_for(R, 256)), dim3(256), 0, s, (const int*)sc.order.data(), R,
+                     (const int*)Aop.cols.data(), (const V*)Aop.vals.data(), inv_diag.data(), has_diag.data());
+  KERNEL_CHECK();
+  grhs = zeros_d(R, s);
+  gs_sliced_reverse<T, V>(sc, (const int*)Aop.cols.data(), (const V*)Aop.vals.data(), (const T*)inv_diag.data(),
+                          (const unsigned char*)has_diag.data(), R, d, r.prm.iters, gx.data(), grhs.data(), s);
+  // b = rhs - cross * cumulative(d-1)
+  stencil_scatter<V>((const int*)sc.order.data(), Cop, 27, (const double*)grhs.data(), -1.0, gcum_prev.data(), s);
+}
+
+}  // namespace
+
+// transpose of normal_stencil_load
+template <class T>
+void normal_load_vjp(const LevelState<T>& testL, int test_depth, const int64_t* nkeys, int64_t ncount, ColumnTable ct,
+                     int normal_depth, const double* gout, double* gvals, hipStream_t s) {
+  const int R = testL.basis_count;
+  if (!R || !ncount) return;
+  const int width = test_depth < normal_depth ? 6 : 4;
+  const int first = test_depth < normal_depth ? -3 : -2;
+  const int rows = (1 << test_depth) + 1;
+  DArray<D2> table((size_t)rows * width, s);
+  hipLaunchKernelGGL(k_normal_table, dim3(blocks_for(rows * width, 256)), dim3(256), 0, s, test_depth, normal_depth, first, width,
+                     table.data());
+  KERNEL_CHECK();
+  hipLaunchKernelGGL(k_normal_load_vjp, dim3(blocks_for(R, 128)), dim3(128), 0, s, (const int64_t*)testL.basis_keys.data(), R,
+                     nkeys, ncount, ct, test_depth, normal_depth, first, width, (const D2*)table.data(), gout, gvals);
+  KERNEL_CHECK();
+}
+
+// Reverse of normal_loads (normal_loads.py:66-190, normal_load_adjoint.py:127-161):
+// gload[d] (w.r.t. the level's load, canonical; empty = zero) gives the
+// gradient w.r.t. every normal record's three values (grecords, accumulated).
+template <class T>
+void normal_loads_backward(Result<T>& r, std::vector<DArray<double>>& gload, int top, double* grecords, hipStream_t s) {
+  const int D = r.prm.depth, base = r.prm.base_depth;
+  auto has = [&](int d) { return d <= top && d < (int)gload.size() && !gload[d].empty(); };
+  // finer(d-1) = R finer(d) + from_child(d-1): G_d = gload_d + R^T G_{d-1}
+  std::vector<DArray<double>> G(D + 1);
+  for (int d = base; d <= D; ++d) {
+    const int R = r.levels[d].basis_count;
+    G[d] = zeros_d(R, s);
+    if (has(d) && R) {
+      hipLaunchKernelGGL(k_axpy_d, dim3(blocks_for(R, 256)), dim3(256), 0, s, G[d].data(), (const double*)gload[d].data(), 1.0,
+                         (int64_t)R);
+      KERNEL_CHECK();
+    }
+    const int C = d > base ? r.levels[d - 1].basis_count : 0;
+    if (C && R) {
+      hipLaunchKernelGGL(k_restrict_scatter, dim3(blocks_for(C, 256)), dim3(256), 0, s,
+                         (const int*)r.levels[d - 1].restriction.data(), C, (const double*)G[d - 1].data(), G[d].data());
+      KERNEL_CHECK();
+    }
+  }
+  std::vector<DArray<double>> gcumn(D + 1);
+  bool has_cumulative = false;
+  for (int d = 0; d <= D; ++d) {
+    const int64_t r0 = r.normal_offsets[d], r1 = r.normal_offsets[d + 1];
+    const int64_t* nk = r.normal_keys.data() + r0;
+    double* gv = grecords + 3 * r0;
+    has_cumulative = has_cumulative || (r1 > r0);
+    LevelState<T>& L = r.levels[d];
+    ColumnIndex records;
+    build_column_index(nk, r1 - r0, (1 << d) + 1, records, s);
+    if (d >= base) {
+      if (has(d)) normal_load_vjp(L, d, nk, r1 - r0, records.view(), d, (const double*)gload[d].data(), gv, s);
+      if (d > base) normal_load_vjp(r.levels[d - 1], d - 1, nk, r1 - r0, records.view(), d, (const double*)G[d - 1].data(), gv, s);
+    }
+    if (d < D) {
+      gcumn[d] = zeros_d(3 * (size_t)L.basis_count, s);
+      if (d >= base && has_cumulative && has(d + 1))
+        normal_load_vjp(r.levels[d + 1], d + 1, (const int64_t*)L.basis_keys.data(), (int64_t)L.basis_count, L.columns.view(), d,
+                        (const double*)gload[d + 1].data(), gcumn[d].data(), s);
+    }
+  }
+  // cumulative normal coefficients, finest first (normal_loads.py:27-63)
+  for (int d = D - 1; d >= 0; --d) {
+    LevelState<T>& L = r.levels[d];
+    if (!L.basis_count) continue;
+    const int64_t r0 = r.normal_offsets[d], r1 = r.normal_offsets[d + 1];
+    ColumnIndex records;
+    build_column_index(r.normal_keys.data() + r0, r1 - r0, (1 << d) + 1, records, s);
+    const LevelState<T>* P = d > 0 ? &r.levels[d - 1] : nullptr;
+    hipLaunchKernelGGL(k_normal_prolong_vjp<T>, dim3(blocks_for(L.basis_count, 256)), dim3(256), 0, s,
+                       (const int64_t*)L.basis_keys.data(), L.basis_count, (const int64_t*)(r.normal_keys.data() + r0), r1 - r0,
+                       records.view(), P ? (const int64_t*)P->basis_keys.data() : (const int64_t*)nullptr,
+                       (int64_t)(P ? P->basis_count : 0), P ? P->columns.view() : ColumnTable{}, d,
+                       (const double*)gcumn[d].data(), grecords + 3 * r0, d > 0 ? gcumn[d - 1].data() : (double*)nullptr);
+    KERNEL_CHECK();
+  }
+}
+
+template <class T> void solve_backward(Result<T>& r, const double* gcoef, double* grecords, hipStream_t s) {
+  const int D = r.prm.depth, base = r.prm.base_depth;
+  if (!r.cg_runs.empty())
+    throw PsrError(2, "backward through the root conjugate-gradient solve is not supported (it ran in this reconstruction)");
+  int top = base - 1;
+  for (int d = base; d <= r.prm.solve_depth && d <= D; ++d)
+    if (r.levels[d].solved) top = d;
+  if (top < base) return;
+  std::vector<DArray<double>> grhs(D + 1);
+  // finer levels: cumulative(d) = P_d cumulative(d-1) + x_d
+  DArray<double> gcum;
+  for (int d = top; d > base; --d) {
+    LevelState<T>& L = r.levels[d];
+    const int R = L.basis_count, C = r.levels[d - 1].basis_count;
+    DArray<double> gx(R, s);
+    HIP_CHECK(hipMemcpyAsync(gx.data(), gcoef + r.level_offsets[d], sizeof(double) * R, hipMemcpyDeviceToDevice, s));
+    if (!gcum.empty()) {
+      hipLaunchKernelGGL(k_axpy_d, dim3(blocks_for(R, 256)), dim3(256), 0, s, gx.data(), (const double*)gcum.data(), 1.0,
+                         (int64_t)R);
+      KERNEL_CHECK();
+    }
+    DArray<double> gprev = zeros_d(C, s);
+    if (!gcum.empty()) stencil_scatter<T>(nullptr, L.P, 8, (const double*)gcum.data(), 1.0, gprev.data(), s);
+    if (r.prm.float_operators) finer_backward<T, float>(r, d, L.A32, L.cross32, gx, grhs[d], gprev, s);
+    else finer_backward<T, T>(r, d, L.A, L.cross, gx, grhs[d], gprev, s);
+    gcum = std::move(gprev);
+  }
+  // base level: regular multigrid (linear in its right-hand side while the
+  // root solve is skipped)
+  {
+    std::vector<int> Rn(base + 1);
+    for (int d = 0; d <= base; ++d) Rn[d] = r.levels[d].basis_count;
+    DArray<double> gxb(Rn[base], s);
+    HIP_CHECK(hipMemcpyAsync(gxb.data(), gcoef + r.level_offsets[base], sizeof(double) * Rn[base], hipMemcpyDeviceToDevice, s));
+    if (!gcum.empty() && Rn[base]) {
+      hipLaunchKernelGGL(k_axpy_d, dim3(blocks_for(Rn[base], 256)), dim3(256), 0, s, gxb.data(), (const double*)gcum.data(), 1.0,
+                         (int64_t)Rn[base]);
+      KERNEL_CHECK();
+    }
+    grhs[base] = zeros_d(Rn[base], s);
+    if (base > 0) {
+      // rebuild the Galerkin hierarchy, schedules and operators
+      std::vector<DArray<int>> mcols(base + 1);
+      std::vector<DArray<T>> mvals(base + 1);
+      LevelState<T>& B = r.levels[base];
+      mcols[base].reset(27 * (size_t)Rn[base], s);
+      mvals[base].reset(27 * (size_t)Rn[base], s);
+      HIP_CHECK(hipMemcpyAsync(mcols[base].data(), B.A.cols.data(), sizeof(int) * 27 * Rn[base], hipMemcpyDeviceToDevice, s));
+      HIP_CHECK(hipMemcpyAsync(mvals[base].data(), B.A.vals.data(), sizeof(T) * 27 * Rn[base], hipMemcpyDeviceToDevice, s));
+      for (int d = base; d > 0; --d) {
+        const int C = Rn[d - 1], F = Rn[d];
+        mcols[d - 1].reset(27 * (size_t)C, s);
+        mvals[d - 1].reset(27 * (size_t)C, s);
+        if (C) {
+          hipLaunchKernelGGL(k_galerkin<T>, dim3(blocks_for(C, 64)), dim3(64), 0, s, (const int*)r.levels[d - 1].restriction.data(),
+                             C, (const int*)mcols[d].data(), (const T*)mvals[d].data(), F, (const int*)r.levels[d].P.cols.data(),
+                             (const T*)r.levels[d].P.vals.data(), mcols[d - 1].data(), mvals[d - 1].data(),
+                             (const int64_t*)r.levels[d - 1].basis_keys.data(), (int64_t)C, d - 1);
+          KERNEL_CHECK();
+        }
+      }
+      DArray<T> zero_b(Rn[base], s);
+      zero_b.zero();
+      std::vector<Schedule<T>> sched;
+      std::vector<GsOperator<T>> ops;
+      for (int d = 0; d <= base; ++d) {
+        sched.push_back(make_schedule<T>(r.levels[d].basis_keys, Rn[d], d, false, s));
+        ops.push_back(gather_operator<T>(sched[d], mcols[d], mvals[d], zero_b.data(), Rn[d], s));
+      }
+      std::vector<DArray<double>> gxl(base + 1), grl(base + 1);
+      for (int d = 0; d < base; ++d) {
+        gxl[d].reset(Rn[d], s);
+        grl[d].reset(Rn[d], s);
+      }
+      gxl[base] = std::move(gxb);
+      grl[base].reset(Rn[base], s);
+      const int iters = r.prm.iters;
+      for (int cycle = r.prm.base_v_cycles - 1; cycle >= 0; --cycle) {
+        for (int d = 0; d < base; ++d) {
+          gxl[d].zero();
+          grl[d].zero();
+        }
+        grl[base].zero();
+        // up: x_d += P_d x_{d-1}, then reverse-colour sweeps
+        for (int d = base; d >= 1; --d) {
+          gs_colored_reverse<T>(ops[d], sched[d], iters, false, gxl[d].data(), grl[d].data(), s);
+          stencil_scatter<T>(nullptr, r.levels[d].P, 8, (const double*)gxl[d].data(), 1.0, gxl[d - 1].data(), s);
+        }
+        // root: skipped, so x_0 does not depend on its right-hand side
+        // down: forward sweeps, residual, restriction
+        for (int d = 1; d <= base; ++d) {
+          DArray<double> gres = zeros_d(Rn[d], s);
+          if (Rn[d - 1]) {
+            hipLaunchKernelGGL(k_restrict_scatter, dim3(blocks_for(Rn[d - 1], 256)), dim3(256), 0, s,
+                               (const int*)r.levels[d - 1].restriction.data(), Rn[d - 1], (const double*)grl[d - 1].data(),
+                               gres.data());
+            KERNEL_CHECK();
+          }
+          if (Rn[d]) {
+            hipLaunchKernelGGL(k_axpy_d, dim3(blocks_for(Rn[d], 256)), dim3(256), 0, s, grl[d].data(), (const double*)gres.data(),
+                               1.0, (int64_t)Rn[d]);
+            KERNEL_CHECK();
+            // A_d^T over the level's canonical stencil
+            hipLaunchKernelGGL(k_stencil_scatter<T>, dim3(blocks_for(Rn[d], 256)), dim3(256), 0, s, (const int*)nullptr,
+                               (const int*)mcols[d].data(), (const T*)mvals[d].data(), Rn[d], 27, (const double*)gres.data(), -1.0,
+                               gxl[d].data());
+            KERNEL_CHECK();
+          }
+          gs_colored_reverse<T>(ops[d], sched[d], iters, true, gxl[d].data(), grl[d].data(), s);
+        }
+        if (Rn[base]) {
+          hipLaunchKernelGGL(k_axpy_d, dim3(blocks_for(Rn[base], 256)), dim3(256), 0, s, grhs[base].data(),
+                             (const double*)grl[base].data(), 1.0, (int64_t)Rn[base]);
+          KERNEL_CHECK();
+        }
+      }
+    }
+  }
+  // right-hand side = fp(load) + fp(0.5) * fp(point_load) (solver.py:119-126)
+  normal_loads_backward(r, grhs, top, grecords, s);
+}
+
 template void relax_sliced_csr<float>(const int64_t*, int, const int*, const int*, const float*, const float*, float*, int, int,
                                       hipStream_t);
 template void relax_sliced_csr<double>(const int64_t*, int, const int*, const int*, const double*, const double*, double*, int,
@@ -1520,5 +1938,7 @@ template void relax_sliced_csr<double>(const int64_t*, int, const int*, const in
 
 template void stage_solve<float>(Result<float>&);
 template void stage_solve<double>(Result<double>&);
+template void solve_backward<float>(Result<float>&, const double*, double*, hipStream_t);
+template void solve_backward<double>(Result<double>&, const double*, double*, hipStream_t);
 
 }  // namespace psr
