@@ -151,6 +151,7 @@ def main():
     }
     out["roofline"] = relax_roofline(rmx.metadata, 4, 8)
     out["stage_seconds"] = rmx.metadata["timings_seconds"]
+    out["normal_average_wait_seconds"] = rmx.metadata["normal_average_wait_seconds"]
     r32, dt32 = timed(tp32, tn32, max(1, args.steps // 2), 1, "float32")
     out["float32"] = {"Mpts_s": round(args.points / dt32 / 1e6, 4), "ms_per_step": round(dt32 * 1e3, 3),
                       "relax_roofline": relax_roofline(r32.metadata, 4, 4)}

@@ -5,7 +5,8 @@ centered field loss (values and model-space gradients at queries, random
 seeds) is differentiated with reconstruct_differentiable and compared along a
 random direction with central differences of the bit-exact forward, after
 checking that the perturbed runs keep every discrete assignment (octree keys,
-normal-record keys, sample groups, root-solve branch).  float64, eps 1e-6,
+normal-record keys, sample groups, root-solve branch).  The cycles and root
+cases run the root conjugate gradient, whose iterations are reversed too.  float64, eps 1e-6,
 tolerances as the reference's float64 case (rtol 3e-5, atol 2e-7).
 """
 import numpy as np
@@ -21,6 +22,8 @@ CASES = [
     ("cycles", "sphere", 512, dict(depth=4, full_depth=2, point_weight=2.0, base_v_cycles=2, iters=3)),
     ("torus", "torus", 8192, dict(depth=6, full_depth=3, point_weight=2.0)),
     ("mixed", "torus", 8192, dict(depth=6, full_depth=3, point_weight=2.0, mixed=True)),
+    ("root-padded-psr", "sphere", 512, dict(depth=4, full_depth=2, base_depth=0, point_weight=0.0)),
+    ("root-screened", "sphere", 512, dict(depth=4, full_depth=0, base_depth=0, point_weight=2.0)),
 ]
 
 
@@ -57,10 +60,8 @@ def test_normal_gradient_matches_central_differences(cuda, name, shape, count, p
     seed_v = torch.from_numpy(rng.normal(size=len(q))).to(cuda)
     seed_g = torch.from_numpy(rng.normal(size=(len(q), 3)) * 1e-2).to(cuda)
     loss = (values * seed_v).sum() + (grads * seed_g).sum()
-    if rec.metadata["root_cg_cycles"]:
-        with pytest.raises(ValueError, match="conjugate-gradient"):
-            loss.backward()
-        return
+    if name.startswith("root") or name == "cycles":
+        assert rec.metadata["root_cg_cycles"], "this case exercises the reversed root conjugate gradient"
     loss.backward()
     gradient = tn.grad.cpu().numpy()
     assert np.isfinite(gradient).all() and np.abs(gradient).max() > 0

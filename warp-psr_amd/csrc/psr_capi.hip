@@ -230,7 +230,9 @@ template <class T> std::string metadata_json(const Result<T>& r) {
     << ", \"base_v_cycles\": " << p.base_v_cycles << ", \"cg_accuracy\": " << p.cg_accuracy << "}";
   o << ", \"root_cg_cycles\": [";
   for (size_t k = 0; k < r.cg_runs.size(); ++k) o << (k ? ", " : "") << r.cg_runs[k];
-  o << "], \"relaxation\": [";
+  float wait_ms = 0.f;
+  if (r.join_begin && hipEventElapsedTime(&wait_ms, r.join_begin, r.join_end) != hipSuccess) wait_ms = 0.f;
+  o << "], \"normal_average_wait_seconds\": " << wait_ms * 1e-3 << ", \"relaxation\": [";
   for (size_t k = 0; k < r.relax_stats.size(); ++k) {
     const auto& st = r.relax_stats[k];
     o << (k ? ", " : "") << "{\"depth\": " << st.depth << ", \"rows\": " << st.rows << ", \"nnz\": " << st.nnz

@@ -78,14 +78,25 @@ template <class T> struct Result {
   hipEvent_t average_done = nullptr;
   bool average_pending = false;
   DArray<T> average_contrib;
+  // HIP-event span of the main stream's wait for the side stream (metadata)
+  hipEvent_t join_begin = nullptr, join_end = nullptr;
   void join_average() {
     if (!average_pending) return;
+    if (!join_begin) {
+      HIP_CHECK(hipEventCreate(&join_begin));
+      HIP_CHECK(hipEventCreate(&join_end));
+    }
+    HIP_CHECK(hipEventRecord(join_begin, s));
     HIP_CHECK(hipStreamWaitEvent(s, average_done, 0));
+    HIP_CHECK(hipEventRecord(join_end, s));
     average_pending = false;
   }
   // tree + solve per level
   std::vector<LevelState<T>> levels;
   std::vector<int> cg_runs;
+  // root solve inputs per base V-cycle (right-hand side, initial x), kept for
+  // the reverse pass
+  std::vector<DArray<T>> root_b, root_x;
   // field
   DArray<int64_t> keys;
   DArray<T> coefficients, cumulative, centroid_values;
@@ -109,6 +120,11 @@ template <class T> struct Result {
   };
   std::vector<RelaxStat> relax_stats;
   ~Result() {
+    if (join_begin) {
+      (void)hipEventSynchronize(join_end);
+      (void)hipEventDestroy(join_begin);
+      (void)hipEventDestroy(join_end);
+    }
     if (average_done) {
       (void)hipEventSynchronize(average_done);
       (void)hipEventDestroy(average_done);

@@ -703,6 +703,53 @@ __global__ void k_gather_operator(const int* __restrict__ order, int R, const in
   inv_diag[j] = diagonal != T(0) ? T(1) / diagonal : T(0);
 }
 
+// Root level (<= 64 rows): padded row of node i when the mean term is on
+// (solver.py:30-33), and the operator applied by the root CG.
+__device__ __forceinline__ int root_row(const int64_t* __restrict__ keys, int i) {
+  const int64_t key = keys[i];
+  const int cx = key_x(key) + 2, cy = key_y(key) + 2, cz = key_z(key) + 2;
+  return cz * 16 + (cx & 1) + 2 * (cy & 1) + 2 * (cx & 2) + 4 * (cy & 2);
+}
+
+template <class T>
+__device__ void root_apply(const int* __restrict__ cols, const T* __restrict__ vals, int R, const int64_t* __restrict__ keys, int dc,
+                           const T* v, T* out) {
+  if (!dc) {
+    for (int i = 0; i < R; ++i) {
+      T a = T(0);
+      for (int k = 0; k < 27; ++k) {
+        const int c = cols[k * R + i];
+        if (c >= 0) a = a + vals[k * R + i] * v[c];
+      }
+      out[i] = a;
+    }
+    return;
+  }
+  // padded rows: entries sorted by padded column
+  for (int i = 0; i < 64; ++i) out[i] = T(0);
+  for (int i = 0; i < R; ++i) {
+    int pc[27];
+    T pv[27];
+    int cnt = 0;
+    for (int k = 0; k < 27; ++k) {
+      const int c = cols[k * R + i];
+      if (c < 0) continue;
+      int q = root_row(keys, c);
+      int pos = cnt++;
+      while (pos > 0 && pc[pos - 1] > q) { pc[pos] = pc[pos - 1]; pv[pos] = pv[pos - 1]; --pos; }
+      pc[pos] = q;
+      pv[pos] = vals[k * R + i];
+    }
+    T a = T(0);
+    for (int e = 0; e < cnt; ++e) a = a + pv[e] * v[pc[e]];
+    out[root_row(keys, i)] = a;
+  }
+  T mean = T(0);
+  for (int i = 0; i < 64; ++i) mean = mean + v[i];
+  mean = mean / T(64);
+  for (int i = 0; i < 64; ++i) out[i] = out[i] + T(1) * mean;
+}
+
 // Root solve on one lane: skip when |r|^2 <= eps^2, else unpreconditioned CG
 // (solver.py:276-297; Warp's cg restated, bitwise parity unpinned).
 template <class T>
@@ -716,49 +763,9 @@ __global__ void k_root_solve(const int* __restrict__ cols, const T* __restrict__
   T* p = work + 128;
   T* ap = work + 192;
   T* xx = dc ? pad_x : x;
-  auto row_of = [&](int i) {
-    const int64_t key = keys[i];
-    const int cx = key_x(key) + 2, cy = key_y(key) + 2, cz = key_z(key) + 2;
-    return cz * 16 + (cx & 1) + 2 * (cy & 1) + 2 * (cx & 2) + 4 * (cy & 2);  // solver.py:30-33
-  };
   for (int i = 0; i < m; ++i) b[i] = T(0);
-  for (int i = 0; i < R; ++i) b[dc ? row_of(i) : i] = rhs[i];
-  auto apply = [&](const T* v, T* out) {
-    if (!dc) {
-      for (int i = 0; i < R; ++i) {
-        T a = T(0);
-        for (int k = 0; k < 27; ++k) {
-          const int c = cols[k * R + i];
-          if (c >= 0) a = a + vals[k * R + i] * v[c];
-        }
-        out[i] = a;
-      }
-      return;
-    }
-    // padded rows: entries sorted by padded column
-    for (int i = 0; i < 64; ++i) out[i] = T(0);
-    for (int i = 0; i < R; ++i) {
-      int pc[27];
-      T pv[27];
-      int cnt = 0;
-      for (int k = 0; k < 27; ++k) {
-        const int c = cols[k * R + i];
-        if (c < 0) continue;
-        int q = row_of(c);
-        int pos = cnt++;
-        while (pos > 0 && pc[pos - 1] > q) { pc[pos] = pc[pos - 1]; pv[pos] = pv[pos - 1]; --pos; }
-        pc[pos] = q;
-        pv[pos] = vals[k * R + i];
-      }
-      T a = T(0);
-      for (int e = 0; e < cnt; ++e) a = a + pv[e] * v[pc[e]];
-      out[row_of(i)] = a;
-    }
-    T mean = T(0);
-    for (int i = 0; i < 64; ++i) mean = mean + v[i];
-    mean = mean / T(64);
-    for (int i = 0; i < 64; ++i) out[i] = out[i] + T(1) * mean;
-  };
+  for (int i = 0; i < R; ++i) b[dc ? root_row(keys, i) : i] = rhs[i];
+  auto apply = [&](const T* v, T* out) { root_apply<T>(cols, vals, R, keys, dc, v, out); };
   for (int i = 0; i < m; ++i) r[i] = b[i];
   if (base == 0 && cycle > 0) {
     apply(xx, ap);
@@ -793,6 +800,134 @@ __global__ void k_root_solve(const int* __restrict__ cols, const T* __restrict__
     rz = rz_new;
     for (int i = 0; i < m; ++i) p[i] = r[i] + beta * p[i];
   }
+}
+
+// Reverse of k_root_solve when its conjugate gradient ran (solver_adjoint.py
+// finite_root_cg, with the iteration count and the skip branch fixed): the
+// forward iterates are recomputed with the forward's arithmetic and stored,
+// then the iteration map is reversed in float64.  gx (layout of the solve:
+// padded 64 rows with the mean term, else R) enters as the gradient w.r.t.
+// the solution and leaves as the gradient w.r.t. the initial x; grhs
+// (canonical) accumulates the right-hand-side gradient.
+template <class T>
+__global__ void k_root_cg_vjp(const int* __restrict__ cols, const T* __restrict__ vals, int R, const int64_t* __restrict__ keys,
+                              const T* __restrict__ rhs, const T* __restrict__ x_init, int dc, int base, int cycle, double acc,
+                              int maxiter, double* __restrict__ store, double* __restrict__ gx, double* __restrict__ grhs) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const int m = dc ? 64 : R;
+  T b[64], xx[64], r[64], p[64], ap[64];
+  for (int i = 0; i < m; ++i) b[i] = T(0);
+  for (int i = 0; i < R; ++i) b[dc ? root_row(keys, i) : i] = rhs[i];
+  for (int i = 0; i < m; ++i) xx[i] = x_init[i];
+  auto apply = [&](const T* v, T* out) { root_apply<T>(cols, vals, R, keys, dc, v, out); };
+  for (int i = 0; i < m; ++i) r[i] = b[i];
+  if (base == 0 && cycle > 0) {
+    apply(xx, ap);
+    for (int i = 0; i < m; ++i) r[i] = b[i] - ap[i];
+  }
+  double norm2 = 0.0;
+  for (int i = 0; i < m; ++i) norm2 += double(r[i]) * double(r[i]);
+  if (!(norm2 > acc * acc)) return;  // skipped: x unchanged
+  const double tol = base == 0 ? 0.0 : acc;
+  const double atol = base == 0 ? acc * sqrt(norm2) : 0.0;
+  apply(xx, ap);
+  for (int i = 0; i < m; ++i) r[i] = b[i] - ap[i];
+  double bnorm = 0.0;
+  for (int i = 0; i < m; ++i) bnorm += double(b[i]) * double(b[i]);
+  const double threshold = fmax(tol * sqrt(bnorm), atol);
+  double rz = 0.0;
+  for (int i = 0; i < m; ++i) rz += double(r[i]) * double(r[i]);
+  for (int i = 0; i < m; ++i) p[i] = r[i];
+  // store: per iteration k, r_k | p_k | ap_k (64 each) and rz_k, pap_k, alpha_k, beta_k
+  auto vec = [&](int k, int which) { return store + ((size_t)k * 3 + which) * 64; };
+  double* scal = store + (size_t)(maxiter + 1) * 3 * 64;
+  for (int i = 0; i < m; ++i) { vec(0, 0)[i] = double(r[i]); vec(0, 1)[i] = double(p[i]); }
+  scal[0] = rz;
+  int K = 0;
+  for (int it = 0; it < maxiter; ++it) {
+    if (sqrt(rz) <= threshold) break;
+    apply(p, ap);
+    double pap = 0.0;
+    for (int i = 0; i < m; ++i) pap += double(p[i]) * double(ap[i]);
+    if (pap == 0.0) break;
+    const T alpha = T(rz / pap);
+    for (int i = 0; i < m; ++i) { xx[i] = xx[i] + alpha * p[i]; r[i] = r[i] - alpha * ap[i]; }
+    double rz_new = 0.0;
+    for (int i = 0; i < m; ++i) rz_new += double(r[i]) * double(r[i]);
+    const T beta = T(rz_new / rz);
+    for (int i = 0; i < m; ++i) vec(it, 2)[i] = double(ap[i]);
+    scal[4 * it + 1] = pap;
+    scal[4 * it + 2] = double(alpha);
+    scal[4 * it + 3] = double(beta);
+    rz = rz_new;
+    for (int i = 0; i < m; ++i) p[i] = r[i] + beta * p[i];
+    for (int i = 0; i < m; ++i) { vec(it + 1, 0)[i] = double(r[i]); vec(it + 1, 1)[i] = double(p[i]); }
+    scal[4 * (it + 1)] = rz;
+    K = it + 1;
+  }
+  // A^T v in float64 (the padded operator's mean term is symmetric)
+  auto apply_t = [&](const double* v, double* out) {
+    for (int i = 0; i < m; ++i) out[i] = 0.0;
+    for (int i = 0; i < R; ++i)
+      for (int k = 0; k < 27; ++k) {
+        const int c = cols[k * R + i];
+        if (c < 0) continue;
+        if (dc) out[root_row(keys, c)] += double(vals[k * R + i]) * v[root_row(keys, i)];
+        else out[c] += double(vals[k * R + i]) * v[i];
+      }
+    if (dc) {
+      double mean = 0.0;
+      for (int i = 0; i < 64; ++i) mean += v[i];
+      mean /= 64.0;
+      for (int i = 0; i < 64; ++i) out[i] += mean;
+    }
+  };
+  double xb[64], rb[64], pb[64], pk[64], apb[64], tmp[64];
+  for (int i = 0; i < m; ++i) { xb[i] = gx[i]; rb[i] = 0.0; pb[i] = 0.0; }
+  double rzb = 0.0;
+  for (int k = K - 1; k >= 0; --k) {
+    const double* rk1 = vec(k + 1, 0);
+    const double* pk_ = vec(k, 1);
+    const double* apk = vec(k, 2);
+    const double rzk = scal[4 * k], rzk1 = scal[4 * (k + 1)], papk = scal[4 * k + 1], alpha = scal[4 * k + 2],
+                 beta = scal[4 * k + 3];
+    // p_{k+1} = r_{k+1} + beta_k p_k
+    double betab = 0.0;
+    for (int i = 0; i < m; ++i) { rb[i] += pb[i]; betab += pb[i] * pk_[i]; pk[i] = beta * pb[i]; }
+    // beta_k = rz_{k+1} / rz_k
+    rzb += betab / rzk;
+    double rzbk = -betab * rzk1 / (rzk * rzk);
+    // rz_{k+1} = r_{k+1} . r_{k+1}
+    for (int i = 0; i < m; ++i) rb[i] += 2.0 * rzb * rk1[i];
+    // r_{k+1} = r_k - alpha ap_k ; x_{k+1} = x_k + alpha p_k
+    double alphab = 0.0;
+    for (int i = 0; i < m; ++i) {
+      alphab += -rb[i] * apk[i] + xb[i] * pk_[i];
+      apb[i] = -alpha * rb[i];
+      pk[i] += alpha * xb[i];
+    }
+    // alpha_k = rz_k / pap_k ; pap_k = p_k . ap_k
+    rzbk += alphab / papk;
+    const double papb = -alphab * rzk / (papk * papk);
+    for (int i = 0; i < m; ++i) { pk[i] += papb * apk[i]; apb[i] += papb * pk_[i]; }
+    // ap_k = A p_k
+    apply_t(apb, tmp);
+    for (int i = 0; i < m; ++i) pb[i] = pk[i] + tmp[i];
+    rzb = rzbk;
+  }
+  // p_0 = r_0, rz_0 = r_0 . r_0, r_0 = b - A x_init
+  const double* r0 = vec(0, 0);
+  for (int i = 0; i < m; ++i) rb[i] += pb[i] + 2.0 * rzb * r0[i];
+  apply_t(rb, tmp);
+  for (int i = 0; i < m; ++i) gx[i] = xb[i] - tmp[i];
+  for (int i = 0; i < R; ++i) grhs[i] += rb[dc ? root_row(keys, i) : i];
+}
+
+// final solution of a padded root solve: gradient w.r.t. pad_x
+__global__ void k_root_pad_grad(const int64_t* __restrict__ keys, int R, const double* __restrict__ gx, double* __restrict__ gpad) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  for (int i = 0; i < 64; ++i) gpad[i] = 0.0;
+  for (int i = 0; i < R; ++i) gpad[root_row(keys, i)] += gx[i];
 }
 
 template <class T>
@@ -1308,6 +1443,13 @@ template <class T> static void regular_multigrid(Result<T>& r, std::vector<DArra
       }
       x[d - 1].zero();
     }
+    {  // the root solve's inputs, for the reverse pass
+      DArray<T> rb(std::max(Rn[0], 1), s), rx(64, s);
+      if (Rn[0]) HIP_CHECK(hipMemcpyAsync(rb.data(), rhs[0].data(), sizeof(T) * Rn[0], hipMemcpyDeviceToDevice, s));
+      HIP_CHECK(hipMemcpyAsync(rx.data(), dc ? pad_x.data() : x[0].data(), sizeof(T) * (dc ? 64 : Rn[0]), hipMemcpyDeviceToDevice, s));
+      r.root_b.push_back(std::move(rb));
+      r.root_x.push_back(std::move(rx));
+    }
     hipLaunchKernelGGL(k_root_solve<T>, dim3(1), dim3(64), 0, s, (const int*)mcols[0].data(), (const T*)mvals[0].data(), Rn[0],
                        (const int64_t*)r.levels[0].basis_keys.data(), (const T*)rhs[0].data(), x[0].data(), pad_x.data(),
                        work.data(), dc, base, cycle, r.prm.cg_accuracy, Rn[0], ran.data());
@@ -1811,8 +1953,6 @@ void normal_loads_backward(Result<T>& r, std::vector<DArray<double>>& gload, int
 
 template <class T> void solve_backward(Result<T>& r, const double* gcoef, double* grecords, hipStream_t s) {
   const int D = r.prm.depth, base = r.prm.base_depth;
-  if (!r.cg_runs.empty())
-    throw PsrError(2, "backward through the root conjugate-gradient solve is not supported (it ran in this reconstruction)");
   int top = base - 1;
   for (int d = base; d <= r.prm.solve_depth && d <= D; ++d)
     if (r.levels[d].solved) top = d;
@@ -1849,6 +1989,30 @@ template <class T> void solve_backward(Result<T>& r, const double* gcoef, double
       KERNEL_CHECK();
     }
     grhs[base] = zeros_d(Rn[base], s);
+    auto ran = [&](int cycle) { return std::find(r.cg_runs.begin(), r.cg_runs.end(), cycle) != r.cg_runs.end(); };
+    const int maxiter = Rn[0];
+    DArray<double> cg_store((size_t)(maxiter + 1) * 3 * 64 + 4 * (size_t)(maxiter + 1) + 64, s);
+    auto root_reverse = [&](int cycle, const int* cols0, const T* vals0, double* gx0, double* grhs0, int dc) {
+      hipLaunchKernelGGL(k_root_cg_vjp<T>, dim3(1), dim3(64), 0, s, cols0, vals0, Rn[0], (const int64_t*)r.levels[0].basis_keys.data(),
+                         (const T*)r.root_b[cycle].data(), (const T*)r.root_x[cycle].data(), dc, base, cycle, r.prm.cg_accuracy,
+                         maxiter, cg_store.data(), gx0, grhs0);
+      KERNEL_CHECK();
+    };
+    if (base == 0 && Rn[0] && !r.cg_runs.empty()) {
+      // the whole base solve is the root solve, once per cycle, x carried
+      const int dc = r.prm.point_weight == 0.0 ? 1 : 0;
+      DArray<double> gpad(64, s);
+      double* g = gxb.data();
+      if (dc) {
+        hipLaunchKernelGGL(k_root_pad_grad, dim3(1), dim3(64), 0, s, (const int64_t*)r.levels[0].basis_keys.data(), Rn[0],
+                           (const double*)gxb.data(), gpad.data());
+        KERNEL_CHECK();
+        g = gpad.data();
+      }
+      for (int cycle = r.prm.base_v_cycles - 1; cycle >= 0; --cycle)
+        if (ran(cycle)) root_reverse(cycle, (const int*)r.levels[0].A.cols.data(), (const T*)r.levels[0].A.vals.data(), g,
+                                     grhs[base].data(), dc);
+    }
     if (base > 0) {
       // rebuild the Galerkin hierarchy, schedules and operators
       std::vector<DArray<int>> mcols(base + 1);
@@ -1897,7 +2061,9 @@ template <class T> void solve_backward(Result<T>& r, const double* gcoef, double
           gs_colored_reverse<T>(ops[d], sched[d], iters, false, gxl[d].data(), grl[d].data(), s);
           stencil_scatter<T>(nullptr, r.levels[d].P, 8, (const double*)gxl[d].data(), 1.0, gxl[d - 1].data(), s);
         }
-        // root: skipped, so x_0 does not depend on its right-hand side
+        // root: when skipped x_0 stays zero; when its CG ran, reverse it
+        if (ran(cycle) && Rn[0]) root_reverse(cycle, (const int*)mcols[0].data(), (const T*)mvals[0].data(), gxl[0].data(),
+                                              grl[0].data(), 0);
         // down: forward sweeps, residual, restriction
         for (int d = 1; d <= base; ++d) {
           DArray<double> gres = zeros_d(Rn[d], s);
