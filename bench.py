@@ -98,6 +98,9 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=50000)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-fp64", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device copy (read 1 GiB, write 1 GiB) three times, for scaling "
+                         "FETCH_SIZE/WRITE_SIZE counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: a reconstruction runs on one "
@@ -106,6 +109,16 @@ def main():
 
     import torch
     from warp_psr_amd import reconstruct, synthetic
+
+    if args.copy_calibration:
+        n = 1 << 28  # 1 GiB of float32
+        src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.mul(src, 1.0, out=dst)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "dispatches": 3}), flush=True)
+        return 0
 
     t0 = time.perf_counter()
     p32, n32 = synthetic.workload_ellipsoids(args.points, dtype=np.float32)

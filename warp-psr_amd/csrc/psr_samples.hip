@@ -542,8 +542,10 @@ __global__ void k_average_contributions(const int* __restrict__ order, int G, co
 // One component chain per wave (three waves on three SIMDs).  A skipped
 // contribution is exactly (0,0,0); the running sums start at +0 and never
 // become -0, so adding it is the identity and the reference's `> 0` test
-// (normals.py:160-163) needs no branch.  Each wave loads 64 contributions
-// and lane 0's chain reads them with readlane, in order.
+// (normals.py:160-163) needs no branch.  Each wave holds AVG_ROUND x 64
+// contributions in registers (lane j holds elements base + u*64 + j), loads
+// the next round while the chain reads the current one with readlane, in
+// element order.
 template <class T> __device__ __forceinline__ T lane_value(T v, int lane);
 template <> __device__ __forceinline__ float lane_value<float>(float v, int lane) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
@@ -555,15 +557,32 @@ template <> __device__ __forceinline__ double lane_value<double>(double v, int l
   return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
+constexpr int AVG_ROUND = 8;
+
 template <class T>
 __global__ void __launch_bounds__(192) k_average_serial(const T* __restrict__ contrib, int G, T* __restrict__ out) {
   __shared__ T totals[3];
   const int comp = threadIdx.x >> 6, lane = threadIdx.x & 63;
   T acc = T(0);
-  for (int base = 0; base < G; base += 64) {
-    const T v = base + lane < G ? contrib[3 * (int64_t)(base + lane) + comp] : T(0);
+  T cur[AVG_ROUND], nxt[AVG_ROUND];
 #pragma unroll
-    for (int j = 0; j < 64; ++j) acc = acc + lane_value(v, j);
+  for (int u = 0; u < AVG_ROUND; ++u) {
+    const int e = u * 64 + lane;
+    cur[u] = e < G ? contrib[3 * (int64_t)e + comp] : T(0);
+  }
+  for (int base = 0; base < G; base += 64 * AVG_ROUND) {
+    const int next = base + 64 * AVG_ROUND;
+#pragma unroll
+    for (int u = 0; u < AVG_ROUND; ++u) {
+      const int e = next + u * 64 + lane;
+      nxt[u] = e < G ? contrib[3 * (int64_t)e + comp] : T(0);
+    }
+#pragma unroll
+    for (int u = 0; u < AVG_ROUND; ++u)
+#pragma unroll
+      for (int j = 0; j < 64; ++j) acc = acc + lane_value(cur[u], j);
+#pragma unroll
+    for (int u = 0; u < AVG_ROUND; ++u) cur[u] = nxt[u];
   }
   if (lane == 0) totals[comp] = acc;
   __syncthreads();
