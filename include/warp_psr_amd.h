@@ -79,6 +79,19 @@ void psr_params_default(PsrParams* p);
 int psr_reconstruct(const void* points, const void* normals, int64_t n, int dtype, const PsrParams* params,
                     void* stream, PsrResult** out);
 
+/* Reconstructs `batch` independent clouds concurrently: each on one of up to
+ * max_concurrency (<= 0: 8) library worker streams and host threads, after
+ * the caller's `stream` (which produced the inputs).  points[i]/normals[i]
+ * are device arrays of counts[i] x 3 (all of `dtype`); params holds one
+ * PsrParams for all clouds (params_count 1) or one per cloud.  Returns when
+ * every result is complete; out[i] receives cloud i's result.  Invalid
+ * parameters fail before any device work; on any failure every result is
+ * freed, out[] is NULLed and the message names the cloud.  No reference
+ * counterpart (the reference reconstructs one cloud per call,
+ * src/warp_psr/reconstruction.py:137-349); for callers with many small clouds. */
+int psr_reconstruct_batch(const void* const* points, const void* const* normals, const int64_t* counts, int batch, int dtype,
+                          const PsrParams* params, int params_count, int max_concurrency, void* stream, PsrResult** out);
+
 /* queries: device m x 3 in model space; values m, grads m x 3 (may be NULL), valid m int32. */
 int psr_evaluate(const PsrResult* result, const void* queries, int64_t m, void* values, void* grads, int32_t* valid,
                  void* stream);

@@ -79,3 +79,26 @@ def test_device_field_header_matches_evaluate(cuda, name, shape, count, precisio
     assert same(hg.cpu().numpy(), grads.cpu().numpy())
     rv, _, _ = ref.evaluate(q.cpu().numpy())
     assert same(hv.cpu().numpy(), rv)
+
+
+def test_batch_matches_single_reconstructions(cuda):
+    """psr_reconstruct_batch: every cloud's result equals its own reconstruct() bit for bit."""
+    import torch
+    from warp_psr_amd import reconstruct, reconstruct_batch
+    clouds, opts = [], []
+    for i, (shape, count, params) in enumerate([("sphere", 512, dict(depth=4, full_depth=2)),
+                                                ("torus", 8192, dict(depth=6, full_depth=3)),
+                                                ("cluster", 1024, dict(depth=5, full_depth=2, point_weight=0.0)),
+                                                ("plane", 1024, dict(depth=5, full_depth=2)),
+                                                ("sphere", 512, dict(depth=5, full_depth=1, iters=3)),
+                                                ("disconnected", 1024, dict(depth=5, full_depth=2))]):
+        p, n = fixture_arrays(shape, count, "double")
+        clouds.append((torch.from_numpy(p).to(cuda), torch.from_numpy(n).to(cuda)))
+        opts.append(params)
+    batch = reconstruct_batch(clouds, per_cloud=opts, max_concurrency=4)
+    assert len(batch) == len(clouds)
+    for (p, n), params, b in zip(clouds, opts, batch):
+        single = reconstruct(p, n, **params)
+        for name in ("keys", "coefficients", "cumulative"):
+            assert same(b.diagnostics.array(name), single.diagnostics.array(name)), name
+        assert b.iso_value == single.iso_value

@@ -119,3 +119,21 @@ def test_differentiable_rejects_tracked_positions():
         reconstruct_differentiable(p, n)
     with pytest.raises(ValueError, match="requires_grad"):
         reconstruct_differentiable(p.detach(), n.detach())
+
+
+def test_c_abi_batch_validates_before_device_work():
+    lib = _lib.load()
+    count = 2
+    vp = ctypes.c_void_p
+    pts = (vp * count)(None, None)
+    counts = (ctypes.c_int64 * count)(4, 0)
+    params = (_lib.PsrParams * 1)(_params())
+    out = (vp * count)()
+    rc = lib.psr_reconstruct_batch(pts, pts, counts, count, 0, params, 1, 4, None, out)
+    assert rc == 2 and "cloud 1" in lib.psr_last_error().decode()
+    bad = (_lib.PsrParams * 1)(_params(depth=16))
+    counts = (ctypes.c_int64 * count)(4, 4)
+    assert lib.psr_reconstruct_batch(pts, pts, counts, count, 0, bad, 1, 4, None, out) == 2
+    assert "Require 0 <= base_depth" in lib.psr_last_error().decode()
+    assert lib.psr_reconstruct_batch(pts, pts, counts, count, 0, params, 3, 4, None, out) == 2
+    assert lib.psr_reconstruct_batch(pts, pts, counts, 0, 0, params, 1, 4, None, out) == 0

@@ -10,6 +10,8 @@
 #include <cstring>
 #include <map>
 #include <sstream>
+#include <atomic>
+#include <thread>
 
 using namespace psr;
 
@@ -19,6 +21,9 @@ struct PsrResult {
   std::map<std::string, std::vector<char>> cache;
   DArray<int> level_offsets_dev;
   std::string metadata;
+  // built on a library worker stream (batched): its blocks return to that
+  // stream's cache, so freeing waits for every stream that may still read it
+  bool sync_on_free = false;
 };
 
 namespace {
@@ -248,6 +253,46 @@ template <class T> std::string metadata_json(const Result<T>& r) {
 
 }  // namespace
 
+// ------------------------------------------------------------ one call
+void check_call(int64_t n, int dtype, const Params& prm) {
+  if (dtype != 0 && dtype != 1)
+    throw PsrError(2, "Reconstruction requires matching float32/float64 3D vectors on the same CUDA device");
+  if (n <= 0) throw PsrError(2, "points and normals must have equal nonempty 1D shapes");
+  if (prm.float_operators && dtype != 1) throw PsrError(2, "mixed precision (float_operators) runs on float64 inputs");
+}
+
+PsrResult* reconstruct_one(const void* points, const void* normals, int64_t n, int dtype, const Params& prm, hipStream_t s) {
+  auto res = std::make_unique<PsrResult>();
+  res->dtype = dtype;
+  if (dtype == 0) {
+    Result<float>* r = run<float>((const float*)points, (const float*)normals, n, prm, s);
+    res->impl = r;
+    res->metadata = metadata_json(*r);
+    res->level_offsets_dev.from_host(r->level_offsets.data(), r->level_offsets.size(), s);
+  } else {
+    Result<double>* r = run<double>((const double*)points, (const double*)normals, n, prm, s);
+    res->impl = r;
+    res->metadata = metadata_json(*r);
+    res->level_offsets_dev.from_host(r->level_offsets.data(), r->level_offsets.size(), s);
+  }
+  return res.release();
+}
+
+// Persistent worker streams per device for batched calls (default priority;
+// kept so the relaxation graphs cached per stream are reused across calls).
+std::vector<hipStream_t> worker_streams(int dev, int count) {
+  static std::mutex mu;
+  static std::map<int, std::vector<hipStream_t>> streams;
+  std::lock_guard<std::mutex> g(mu);
+  auto& v = streams[dev];
+  while ((int)v.size() < count) {
+    hipStream_t s;
+    HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    v.push_back(s);
+  }
+  return std::vector<hipStream_t>(v.begin(), v.begin() + count);
+}
+
 namespace {
 template <class T>
 __global__ void k_field_evaluate(PsrField f, const T* __restrict__ queries, int64_t m, T* __restrict__ values,
@@ -287,25 +332,75 @@ int psr_reconstruct(const void* points, const void* normals, int64_t n, int dtyp
   try {
     if (!params || !out) throw PsrError(2, "params and out must be non-null");
     Params prm = resolve(*params);
-    if (dtype != 0 && dtype != 1)
-      throw PsrError(2, "Reconstruction requires matching float32/float64 3D vectors on the same CUDA device");
-    if (n <= 0) throw PsrError(2, "points and normals must have equal nonempty 1D shapes");
-    if (prm.float_operators && dtype != 1) throw PsrError(2, "mixed precision (float_operators) runs on float64 inputs");
-    hipStream_t s = (hipStream_t)stream;
-    auto res = std::make_unique<PsrResult>();
-    res->dtype = dtype;
-    if (dtype == 0) {
-      Result<float>* r = run<float>((const float*)points, (const float*)normals, n, prm, s);
-      res->impl = r;
-      res->metadata = metadata_json(*r);
-      res->level_offsets_dev.from_host(r->level_offsets.data(), r->level_offsets.size(), s);
-    } else {
-      Result<double>* r = run<double>((const double*)points, (const double*)normals, n, prm, s);
-      res->impl = r;
-      res->metadata = metadata_json(*r);
-      res->level_offsets_dev.from_host(r->level_offsets.data(), r->level_offsets.size(), s);
+    check_call(n, dtype, prm);
+    *out = reconstruct_one(points, normals, n, dtype, prm, (hipStream_t)stream);
+    return 0;
+  } catch (const PsrError& e) {
+    return fail(e);
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return 1;
+  }
+}
+
+int psr_reconstruct_batch(const void* const* points, const void* const* normals, const int64_t* counts, int batch, int dtype,
+                          const PsrParams* params, int params_count, int max_concurrency, void* stream, PsrResult** out) {
+  try {
+    if (batch < 0) throw PsrError(2, "batch must be nonnegative");
+    if (batch == 0) return 0;
+    if (!points || !normals || !counts || !params || !out) throw PsrError(2, "batch arrays must be non-null");
+    if (params_count != 1 && params_count != batch) throw PsrError(2, "params_count must be 1 or the batch size");
+    std::vector<Params> prm(batch);
+    for (int i = 0; i < batch; ++i) {
+      try {
+        prm[i] = resolve(params[params_count == 1 ? 0 : i]);
+        check_call(counts[i], dtype, prm[i]);
+      } catch (const PsrError& e) {
+        throw PsrError(e.code, "cloud " + std::to_string(i) + ": " + e.what());
+      }
+      out[i] = nullptr;
     }
-    *out = res.release();
+    int dev = 0;
+    HIP_CHECK(hipGetDevice(&dev));
+    const int workers = std::max(1, std::min(max_concurrency > 0 ? max_concurrency : 8, batch));
+    std::vector<hipStream_t> streams = worker_streams(dev, workers);
+    // the caller's stream has produced the inputs: every worker waits for it
+    hipEvent_t ready;
+    HIP_CHECK(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
+    HIP_CHECK(hipEventRecord(ready, (hipStream_t)stream));
+    for (hipStream_t ws : streams) HIP_CHECK(hipStreamWaitEvent(ws, ready, 0));
+    std::atomic<int> next(0);
+    std::vector<int> codes(batch, 0);
+    std::vector<std::string> messages(batch);
+    auto work = [&](int w) {
+      if (hipSetDevice(dev) != hipSuccess) return;
+      for (int i = next++; i < batch; i = next++) {
+        try {
+          out[i] = reconstruct_one(points[i], normals[i], counts[i], dtype, prm[i], streams[w]);
+          out[i]->sync_on_free = true;
+          HIP_CHECK(hipStreamSynchronize(streams[w]));
+        } catch (const PsrError& e) {
+          codes[i] = e.code;
+          messages[i] = e.what();
+        } catch (const std::exception& e) {
+          codes[i] = 1;
+          messages[i] = e.what();
+        }
+      }
+    };
+    std::vector<std::thread> threads;
+    for (int w = 1; w < workers; ++w) threads.emplace_back(work, w);
+    work(0);
+    for (auto& t : threads) t.join();
+    (void)hipEventDestroy(ready);
+    for (int i = 0; i < batch; ++i)
+      if (codes[i]) {
+        for (int j = 0; j < batch; ++j) {
+          psr_result_free(out[j]);
+          out[j] = nullptr;
+        }
+        throw PsrError(codes[i], "cloud " + std::to_string(i) + ": " + messages[i]);
+      }
     return 0;
   } catch (const PsrError& e) {
     return fail(e);
@@ -404,6 +499,7 @@ int psr_result_array(PsrResult* result, const char* name, int level, const void*
 
 void psr_result_free(PsrResult* result) {
   if (!result) return;
+  if (result->sync_on_free) (void)hipDeviceSynchronize();
   if (result->dtype == 0) delete (Result<float>*)result->impl;
   else delete (Result<double>*)result->impl;
   delete result;

@@ -9,7 +9,7 @@ there is no CPU fallback.  ``reconstruct_differentiable`` differentiates field
 losses with respect to the input normals (reconstruction_adjoint.py:61-185).
 """
 from .differentiable import DifferentiableReconstruction, reconstruct_differentiable
-from .reconstruction import reconstruct, Reconstruction, LevelField, DeviceField
+from .reconstruction import reconstruct, reconstruct_batch, Reconstruction, LevelField, DeviceField
 
-__all__ = ["reconstruct", "reconstruct_differentiable", "Reconstruction", "DifferentiableReconstruction", "LevelField",
+__all__ = ["reconstruct", "reconstruct_batch", "reconstruct_differentiable", "Reconstruction", "DifferentiableReconstruction", "LevelField",
            "DeviceField"]

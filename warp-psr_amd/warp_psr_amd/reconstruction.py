@@ -172,17 +172,10 @@ class _Diagnostics:
         return np.frombuffer(buf, dtype=dt, count=count).copy()
 
 
-def reconstruct(positions, normals, *, depth=6, full_depth=3, base_depth=None, solve_depth=None, kernel_depth=None,
-                samples_per_node=1.5, scale=1.1, point_weight=2.0, exact_interpolation=False, iters=8, base_v_cycles=1,
-                cg_accuracy=1e-3, mixed=False):
-    """Reconstruct oriented (n, 3) float32/float64 ROCm tensors; point_weight=0 selects PSR.
-
-    Degree one, Neumann boundaries, confidence disabled, library alignDir=0.
-    The surface is ``result.evaluate(q)[0] == result.iso_value``.
-    ``mixed=True`` (float64 tensors only) stores the finer levels' Laplacian +
-    screening operators as float32 while every vector, the octree stages and
-    the base-level solve stay float64.
-    """
+def _prepare(positions, normals, depth=6, full_depth=3, base_depth=None, solve_depth=None, kernel_depth=None,
+             samples_per_node=1.5, scale=1.1, point_weight=2.0, exact_interpolation=False, iters=8, base_v_cycles=1,
+             cg_accuracy=1e-3, mixed=False):
+    """Reference validation (reconstruction.py:160-212), then the C parameters."""
     torch = _torch()
     if base_depth is None:
         base_depth = full_depth
@@ -213,27 +206,81 @@ def reconstruct(positions, normals, *, depth=6, full_depth=3, base_depth=None, s
         raise ValueError("points and normals must have equal nonempty 1D shapes")
     if mixed and positions.dtype != torch.float64:
         raise ValueError("mixed precision (float_operators) runs on float64 inputs")
-    p = positions.contiguous()
-    n = normals.contiguous()
     params = _lib.PsrParams()
-    lib = _lib.load()
-    lib.psr_params_default(ctypes.byref(params))
+    _lib.load().psr_params_default(ctypes.byref(params))
     params.depth, params.full_depth, params.base_depth = depth, full_depth, base_depth
     params.solve_depth, params.kernel_depth = solve_depth, kernel_depth
     params.samples_per_node, params.scale, params.point_weight = samples_per_node, scale, point_weight
     params.exact_interpolation = int(bool(exact_interpolation))
     params.iters, params.base_v_cycles, params.cg_accuracy = iters, base_v_cycles, cg_accuracy
     params.float_operators = int(bool(mixed))
-    handle = ctypes.c_void_p()
-    dtype_code = 1 if p.dtype == torch.float64 else 0
-    with torch.cuda.device(p.device):
-        _lib.check(lib.psr_reconstruct(p.data_ptr(), n.data_ptr(), p.shape[0], dtype_code, ctypes.byref(params),
-                                       _stream_handle(p.device), ctypes.byref(handle)))
     resolved = dict(depth=depth, full_depth=full_depth, base_depth=base_depth, solve_depth=solve_depth,
                     kernel_depth=kernel_depth, samples_per_node=samples_per_node, scale=scale,
                     point_weight=point_weight, exact_interpolation=bool(exact_interpolation), iters=iters,
                     base_v_cycles=base_v_cycles, cg_accuracy=cg_accuracy, mixed=bool(mixed))
+    return positions.contiguous(), normals.contiguous(), params, resolved
+
+
+def reconstruct(positions, normals, *, depth=6, full_depth=3, base_depth=None, solve_depth=None, kernel_depth=None,
+                samples_per_node=1.5, scale=1.1, point_weight=2.0, exact_interpolation=False, iters=8, base_v_cycles=1,
+                cg_accuracy=1e-3, mixed=False):
+    """Reconstruct oriented (n, 3) float32/float64 ROCm tensors; point_weight=0 selects PSR.
+
+    Degree one, Neumann boundaries, confidence disabled, library alignDir=0.
+    The surface is ``result.evaluate(q)[0] == result.iso_value``.
+    ``mixed=True`` (float64 tensors only) stores the finer levels' Laplacian +
+    screening operators as float32 while every vector, the octree stages and
+    the base-level solve stay float64.
+    """
+    torch = _torch()
+    p, n, params, resolved = _prepare(
+        positions, normals, depth=depth, full_depth=full_depth, base_depth=base_depth, solve_depth=solve_depth,
+        kernel_depth=kernel_depth, samples_per_node=samples_per_node, scale=scale, point_weight=point_weight,
+        exact_interpolation=exact_interpolation, iters=iters, base_v_cycles=base_v_cycles, cg_accuracy=cg_accuracy,
+        mixed=mixed)
+    handle = ctypes.c_void_p()
+    dtype_code = 1 if p.dtype == torch.float64 else 0
+    with torch.cuda.device(p.device):
+        _lib.check(_lib.load().psr_reconstruct(p.data_ptr(), n.data_ptr(), p.shape[0], dtype_code, ctypes.byref(params),
+                                               _stream_handle(p.device), ctypes.byref(handle)))
     return Reconstruction(handle.value, p.dtype, p.device, resolved)
+
+
+def reconstruct_batch(clouds, *, max_concurrency=8, **options):
+    """Reconstruct many independent clouds concurrently (psr_reconstruct_batch).
+
+    ``clouds`` is a sequence of (positions, normals) pairs of one dtype on one
+    device; ``options`` are reconstruct()'s keywords, applied to every cloud,
+    or a sequence of per-cloud option dicts under ``per_cloud=[...]``.  Each
+    result equals reconstruct() of that cloud bit for bit.  No reference
+    counterpart: the reference reconstructs one cloud per call.
+    """
+    torch = _torch()
+    per_cloud = options.pop("per_cloud", None)
+    clouds = list(clouds)
+    if per_cloud is not None and len(per_cloud) != len(clouds):
+        raise ValueError("per_cloud must hold one option dict per cloud")
+    if not clouds:
+        return []
+    prepared = []
+    for i, (positions, normals) in enumerate(clouds):
+        opts = dict(options)
+        if per_cloud is not None:
+            opts.update(per_cloud[i])
+        prepared.append(_prepare(positions, normals, **opts))
+    dtype, device = prepared[0][0].dtype, prepared[0][0].device
+    if any(p.dtype != dtype or p.device != device for p, _, _, _ in prepared):
+        raise ValueError("batched clouds must share one dtype and device")
+    count = len(prepared)
+    pts = (ctypes.c_void_p * count)(*[p.data_ptr() for p, _, _, _ in prepared])
+    nrm = (ctypes.c_void_p * count)(*[n.data_ptr() for _, n, _, _ in prepared])
+    counts = (ctypes.c_int64 * count)(*[p.shape[0] for p, _, _, _ in prepared])
+    params = (_lib.PsrParams * count)(*[prm for _, _, prm, _ in prepared])
+    out = (ctypes.c_void_p * count)()
+    with torch.cuda.device(device):
+        _lib.check(_lib.load().psr_reconstruct_batch(pts, nrm, counts, count, 1 if dtype == torch.float64 else 0, params,
+                                                     count, int(max_concurrency), _stream_handle(device), out))
+    return [Reconstruction(out[i], dtype, device, prepared[i][3]) for i in range(count)]
 
 
 def relax_sliced(keys, offsets, columns, values, b, x, depth, iters):
