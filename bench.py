@@ -72,6 +72,35 @@ def relax_roofline(metadata, value_bytes, vector_bytes):
     }
 
 
+def batch_throughput(torch, synthetic, dev, concurrency=8):
+    """64 independent float32 clouds (5k-200k points, depth 7/8): one batched
+    call on `concurrency` worker streams vs the same clouds one by one."""
+    from warp_psr_amd import reconstruct, reconstruct_batch
+    t0 = time.perf_counter()
+    work = synthetic.workload_batch()
+    log(f"batch workload generated in {time.perf_counter() - t0:.1f}s")
+    clouds = [(torch.from_numpy(p).to(dev), torch.from_numpy(n).to(dev)) for p, n, _ in work]
+    opts = [dict(depth=d, full_depth=4, point_weight=2.0) for _, _, d in work]
+    total = sum(len(p) for p, _, _ in work)
+    torch.cuda.synchronize()
+    reconstruct_batch(clouds, per_cloud=opts, max_concurrency=concurrency)  # warm-up (graph captures)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    results = reconstruct_batch(clouds, per_cloud=opts, max_concurrency=concurrency)
+    torch.cuda.synchronize()
+    batched = time.perf_counter() - t
+    del results
+    t = time.perf_counter()
+    for (p, n), o in zip(clouds, opts):
+        r = reconstruct(p, n, **o)
+    torch.cuda.synchronize()
+    serial = time.perf_counter() - t
+    log(f"batch: {len(work)} clouds, {total} points, {batched * 1e3:.1f} ms batched, {serial * 1e3:.1f} ms one by one")
+    return {"clouds": len(work), "points": total, "depths": "7/8", "full_depth": 4, "dtype": "f32",
+            "concurrency": concurrency, "seconds": round(batched, 4), "Mpts_s": round(total / batched / 1e6, 3),
+            "one_by_one_seconds": round(serial, 4), "one_by_one_Mpts_s": round(total / serial / 1e6, 3)}
+
+
 def cpu_baseline(points, normals, sample):
     import oracle  # the checker; timed here as the CPU reference path
     p = points[:sample]
@@ -98,6 +127,7 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=50000)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-fp64", action="store_true")
+    ap.add_argument("--skip-batch", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (read 1 GiB, write 1 GiB) three times, for scaling "
                          "FETCH_SIZE/WRITE_SIZE counters")
@@ -190,6 +220,8 @@ def main():
 
         out["field_error_vs_fp64"] = error(rmx, q64)
         out["float32"]["field_error_vs_fp64"] = error(r32, q64.astype(np.float32))
+    if not args.skip_batch:
+        out["batch"] = batch_throughput(torch, synthetic, dev)
     if not args.skip_cpu:
         out["cpu_baseline"] = cpu_baseline(p64, n64, args.cpu_sample)
     print(json.dumps(out), flush=True)

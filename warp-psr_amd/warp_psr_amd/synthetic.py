@@ -202,6 +202,26 @@ def workload_ellipsoids(count=800000, seed=SEED, dtype=np.float32, noise=0.001):
     return p.astype(dtype), n.astype(dtype)
 
 
+def workload_batch(clouds=64, seed=SEED, dtype=np.float32, smallest=5000, largest=200000):
+    """Many independent clouds: sizes log-uniform in [smallest, largest], shapes
+    cycling sphere / torus / six-ellipsoid union (own seeds), scale U[0.2, 2],
+    translation U[-5, 5]^3, depth 7 or 8.  Returns [(points, normals, depth)]."""
+    rng = np.random.default_rng(seed + 101)
+    out = []
+    for i in range(clouds):
+        count = int(np.exp(rng.uniform(np.log(smallest), np.log(largest))))
+        kind = i % 3
+        if kind == 0:
+            p, n, _ = workload_sphere(count, queries=0, seed=seed + 1000 + i, dtype=np.float64)
+        elif kind == 1:
+            p, n, _ = workload_torus(count, queries=0, seed=seed + 1000 + i, dtype=np.float64)
+        else:
+            p, n = workload_ellipsoids(count, seed=seed + 1000 + i, dtype=np.float64)
+        p = p * rng.uniform(0.2, 2.0) + rng.uniform(-5.0, 5.0, size=3)
+        out.append((p.astype(dtype), n.astype(dtype), int(rng.integers(7, 9))))
+    return out
+
+
 def workload_displaced_sphere(count=100000, seed=SEED, dtype=np.float64, amplitude=0.05):
     """Unit sphere displaced radially by amplitude*sin(6 theta)*cos(4 phi), analytic normals."""
     rng = np.random.default_rng(seed)
