@@ -28,6 +28,8 @@ template <class T> struct LevelState {
   DArray<int64_t> tree_keys, basis_keys;
   int basis_count = 0;
   ColumnIndex columns;  // (x, y) runs of basis_keys
+  DArray<int> dense;    // z-major dense index of basis_keys during the solve stage
+  int dense_dim = 0;
   // restriction gathers to this level from level d+1: 27 fine indices per row
   DArray<int> restriction;
   // aggregates (density.py:228-240): keys and (sum x, sum y, sum z, count)

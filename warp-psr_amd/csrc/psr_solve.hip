@@ -116,9 +116,20 @@ __global__ void k_normal_table(int test_depth, int normal_depth, int first, int 
 }
 
 // ---------------------------------------------------------------- stiffness (stiffness.py:38-109)
+// z-major dense index of one level's basis keys: dense[(z*dim + x)*dim + y] =
+// key index or -1.  Rows taken in sliced schedule order share a z slice, so
+// their neighbour reads fall in a few contiguous (x, y) lines.
+__global__ void k_dense_fill(const int64_t* __restrict__ keys, int n, int dim, int* __restrict__ dense) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t k = keys[i];
+  dense[((int64_t)key_z(k) * dim + key_x(k)) * dim + key_y(k)] = i;
+}
+
 __global__ void k_stiffness(const int64_t* __restrict__ rows, int R, const int* __restrict__ row_order,
-                            const int64_t* __restrict__ cols, int64_t C, ColumnTable ct, int depth, int trial_depth,
-                            const D2* __restrict__ table, int* __restrict__ out_cols, double* __restrict__ out_vals) {
+                            const int64_t* __restrict__ cols, int64_t C, ColumnTable ct, const int* __restrict__ dense,
+                            int dense_dim, int depth, int trial_depth, const D2* __restrict__ table, int* __restrict__ out_cols,
+                            double* __restrict__ out_vals) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;  // output position
   if (j >= R) return;
   const int i = row_order ? row_order[j] : j;
@@ -133,8 +144,13 @@ __global__ void k_stiffness(const int64_t* __restrict__ rows, int R, const int* 
     for (int ky = 0; ky < 3; ++ky) {
       const int qy = py + ky - 1;
       int64_t found[3] = {-1, -1, -1};
-      if (qx >= 0 && qx <= lim && qy >= 0 && qy <= lim && zlo <= zhi)
-        find_run_table<3>(cols, C, ct, trial_depth, qx, qy, zlo, zhi - zlo + 1, found);
+      if (qx >= 0 && qx <= lim && qy >= 0 && qy <= lim && zlo <= zhi) {
+        if (dense) {
+          for (int z = zlo; z <= zhi; ++z) found[z - zlo] = dense[((int64_t)z * dense_dim + qx) * dense_dim + qy];
+        } else {
+          find_run_table<3>(cols, C, ct, trial_depth, qx, qy, zlo, zhi - zlo + 1, found);
+        }
+      }
       for (int kz = 0; kz < 3; ++kz) {
         const int qz = pz + kz - 1;
         const int k = kx * 9 + ky * 3 + kz;
@@ -393,10 +409,24 @@ __global__ void k_spmv(const int* __restrict__ cols, const V* __restrict__ vals,
 }
 
 // ---------------------------------------------------------------- normal loads (normal_stencils.py, normal_loads.py)
+// Dense index of a key set: zmajor ? (z*dim + x)*dim + y : (x*dim + y)*dim + z
+struct DenseIndex {
+  const int* idx = nullptr;
+  int dim = 0;
+  int zmajor = 0;
+};
+
+__global__ void k_dense_fill_xyz(const int64_t* __restrict__ keys, int64_t n, int dim, int* __restrict__ dense) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t k = keys[i];
+  dense[((int64_t)key_x(k) * dim + key_y(k)) * dim + key_z(k)] = (int)i;
+}
+
 template <class V>
 __global__ void k_normal_load(const int64_t* __restrict__ keys, int R, const int64_t* __restrict__ nkeys, int64_t ncount, ColumnTable ct,
-                              const V* __restrict__ nvals, int nstride, int test_depth, int normal_depth, int first, int width,
-                              const D2* __restrict__ table, double* __restrict__ out) {
+                              DenseIndex dn, const V* __restrict__ nvals, int nstride, int test_depth, int normal_depth, int first,
+                              int width, const D2* __restrict__ table, double* __restrict__ out) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= R) return;
   const int64_t key = keys[i];
@@ -413,7 +443,12 @@ __global__ void k_normal_load(const int64_t* __restrict__ keys, int R, const int
       const int ny = e1 + y + first;
       if (ny < 0 || ny >= lim || zlo > zhi) continue;
       int64_t found[6];
-      find_run_table<6>(nkeys, ncount, ct, normal_depth, nx, ny, zlo, zhi - zlo + 1, found);
+      if (dn.idx) {
+        for (int z = zlo; z <= zhi; ++z)
+          found[z - zlo] = dn.zmajor ? dn.idx[((int64_t)z * dn.dim + nx) * dn.dim + ny] : dn.idx[((int64_t)nx * dn.dim + ny) * dn.dim + z];
+      } else {
+        find_run_table<6>(nkeys, ncount, ct, normal_depth, nx, ny, zlo, zhi - zlo + 1, found);
+      }
       for (int z = 0; z < width; ++z) {
         const int nz = e2 + z + first;
         if (nz < zlo || nz > zhi) continue;
@@ -476,54 +511,73 @@ __global__ void k_add_f64(double* __restrict__ y, const double* __restrict__ x, 
 // Row r of R*A*P with the reference spgemm order: contributions to an entry
 // arrive in ascending middle index (R row order, then (RA) row order).
 template <class T>
-__global__ void k_galerkin(const int* __restrict__ rcols, int C, const int* __restrict__ acols, const T* __restrict__ avals, int F,
-                           const int* __restrict__ pcols, const T* __restrict__ pvals, int* __restrict__ out_cols,
-                           T* __restrict__ out_vals, const int64_t* __restrict__ ckeys, int64_t Ccount, int coarse_depth) {
-  int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= C) return;
-  T ra[125];
-  bool rap[125];
-  int rak[125];
-  for (int t = 0; t < 125; ++t) { ra[t] = T(0); rap[t] = false; rak[t] = -1; }
-  for (int kr = 0; kr < 27; ++kr) {  // fine k = 2r + o_kr, ascending
-    const int k = rcols[(int64_t)kr * C + r];
-    if (k < 0) continue;
-    const T rw = T(restriction_weight(kr));
-    const int ox = kr / 9 - 1, oy = (kr / 3) % 3 - 1, oz = kr % 3 - 1;
-    for (int j = 0; j < 27; ++j) {
-      const int c = acols[(int64_t)j * F + k];
-      if (c < 0) continue;
-      const int tx = ox + j / 9 - 1 + 2, ty = oy + (j / 3) % 3 - 1 + 2, tz = oz + j % 3 - 1 + 2;
-      const int t = tx * 25 + ty * 5 + tz;
-      ra[t] = ra[t] + rw * avals[(int64_t)j * F + k];
-      rap[t] = true;
-      rak[t] = c;
-    }
-  }
-  T out[27];
-  bool outp[27];
-  for (int t = 0; t < 27; ++t) { out[t] = T(0); outp[t] = false; }
+__global__ void __launch_bounds__(128) k_galerkin(const int* __restrict__ rcols, int C, const int* __restrict__ acols,
+                                                  const T* __restrict__ avals, int F, const int* __restrict__ pcols,
+                                                  const T* __restrict__ pvals, int* __restrict__ out_cols, T* __restrict__ out_vals,
+                                                  const int64_t* __restrict__ ckeys, int64_t Ccount, int coarse_depth) {
+  // one workgroup per coarse row r.  (R A)_rt for the 125 fine offsets t:
+  // thread t sums its contributions in ascending R-row order (one A slot per
+  // R entry reaches t), then thread s sums row s of (R A) P over t ascending
+  // and P slot ascending -- the sequential order of both products.
+  __shared__ T ra[125];
+  __shared__ int rak[125];
+  __shared__ int cs[125][8];
+  __shared__ T cv[125][8];
+  const int r = blockIdx.x;
+  const int t = threadIdx.x;
   const int64_t rkey = ckeys[r];
   const int rx = key_x(rkey), ry = key_y(rkey), rz = key_z(rkey);
-  for (int t = 0; t < 125; ++t) {  // (RA) row entries in ascending fine column
-    if (!rap[t]) continue;
-    const int c = rak[t];
+  if (t < 125) {
+    const int tx = t / 25, ty = (t / 5) % 5, tz = t % 5;
+    T acc = T(0);
+    int col = -1;
+    for (int kr = 0; kr < 27; ++kr) {  // fine k = 2r + o_kr, ascending
+      const int k = rcols[(int64_t)kr * C + r];
+      if (k < 0) continue;
+      const int jx = tx - kr / 9, jy = ty - (kr / 3) % 3, jz = tz - kr % 3;
+      if ((unsigned)jx > 2u || (unsigned)jy > 2u || (unsigned)jz > 2u) continue;
+      const int j = jx * 9 + jy * 3 + jz;
+      const int c = acols[(int64_t)j * F + k];
+      if (c < 0) continue;
+      acc = acc + T(restriction_weight(kr)) * avals[(int64_t)j * F + k];
+      col = c;
+    }
+    ra[t] = acc;
+    rak[t] = col;
     for (int kp = 0; kp < 8; ++kp) {
-      const int pc = pcols[(int64_t)kp * F + c];
-      if (pc < 0) continue;
-      const int64_t pkey = ckeys[pc];
-      const int ox = key_x(pkey) - rx + 1, oy = key_y(pkey) - ry + 1, oz = key_z(pkey) - rz + 1;
-      if ((unsigned)ox > 2u || (unsigned)oy > 2u || (unsigned)oz > 2u) continue;  // never for trilinear P
-      const int s = ox * 9 + oy * 3 + oz;
-      out[s] = out[s] + ra[t] * pvals[(int64_t)kp * F + c];
-      outp[s] = true;
+      int sidx = -1;
+      T pv = T(0);
+      if (col >= 0) {
+        const int pc = pcols[(int64_t)kp * F + col];
+        if (pc >= 0) {
+          const int64_t pkey = ckeys[pc];
+          const int ox = key_x(pkey) - rx + 1, oy = key_y(pkey) - ry + 1, oz = key_z(pkey) - rz + 1;
+          if ((unsigned)ox <= 2u && (unsigned)oy <= 2u && (unsigned)oz <= 2u) {  // always for trilinear P
+            sidx = ox * 9 + oy * 3 + oz;
+            pv = pvals[(int64_t)kp * F + col];
+          }
+        }
+      }
+      cs[t][kp] = sidx;
+      cv[t][kp] = pv;
     }
   }
-  for (int s = 0; s < 27; ++s) {
+  __syncthreads();
+  if (t < 27) {
+    T out = T(0);
+    bool present = false;
+    for (int u = 0; u < 125; ++u) {
+      if (rak[u] < 0) continue;
+      for (int kp = 0; kp < 8; ++kp)
+        if (cs[u][kp] == t) {
+          out = out + ra[u] * cv[u][kp];
+          present = true;
+        }
+    }
     int col = -1;
-    if (outp[s]) col = (int)find_key(ckeys, Ccount, basis_key(coarse_depth, rx + s / 9 - 1, ry + (s / 3) % 3 - 1, rz + s % 3 - 1));
-    out_cols[(int64_t)s * C + r] = col;
-    out_vals[(int64_t)s * C + r] = out[s];
+    if (present) col = (int)find_key(ckeys, Ccount, basis_key(coarse_depth, rx + t / 9 - 1, ry + (t / 3) % 3 - 1, rz + t % 3 - 1));
+    out_cols[(int64_t)t * C + r] = col;
+    out_vals[(int64_t)t * C + r] = out;
   }
 }
 
@@ -590,15 +644,36 @@ template <class T, class V> struct RelaxDesc {
 };
 constexpr size_t RELAX_GOFF_OFFSET = 256;
 
+// Each launch also reads the operator rows that the same workgroup index of
+// the next launch (next_z, next_color, next_count; next_count 0 = none)
+// will update: they do not depend on x, and the read leaves them in the
+// XCD's L2 (workgroups go to XCDs by index) and the memory-side cache, so
+// the next launch's first dependent loads are cache hits.
 template <class T, class V, int BS>
 __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(1, 4)))
-k_relax_wave_graph(const char* __restrict__ block, int first_z, int color) {
+k_relax_wave_graph(const char* __restrict__ block, int first_z, int color, int next_z, int next_color, int next_count) {
   const int* goff = (const int*)(block + RELAX_GOFF_OFFSET);
   const int group = 4 * (first_z + 2 * (int)blockIdx.y) + color;
   const int start = goff[group], end = goff[group + 1];
   const RelaxDesc<T, V> d = *(const RelaxDesc<T, V>*)block;
   const int j = start + (int)blockIdx.x * BS + (int)threadIdx.x;
+  int jn = -1;
+  if ((int)blockIdx.y < next_count) {
+    const int g2 = 4 * (next_z + 2 * (int)blockIdx.y) + next_color;
+    const int c = goff[g2] + (int)blockIdx.x * BS + (int)threadIdx.x;
+    if (c < goff[g2 + 1]) jn = c;
+  }
   if (j < end) gs_update(j, d.R, d.order, d.cols, d.vals, d.b, d.inv_diag, d.has_diag, d.x);
+  if (jn >= 0) {
+    unsigned acc = 0;
+#pragma unroll
+    for (int k = 0; k < 27; ++k) {
+      acc ^= (unsigned)d.cols[(int64_t)k * d.R + jn];
+      acc ^= __builtin_bit_cast(unsigned, float(d.vals[(int64_t)k * d.R + jn]));
+    }
+    acc ^= __builtin_bit_cast(unsigned, float(d.b[jn])) ^ __builtin_bit_cast(unsigned, float(d.inv_diag[jn]));
+    if (d.R < 0) d.x[0] = T(acc);  // never taken (R >= 0): keeps the reads
+  }
 }
 
 // relaxation.py:261-271 (8-colour, all sweeps) inside one workgroup: the
@@ -985,6 +1060,9 @@ inline int floordiv(int a, int b) { return (a >= 0) ? a / b : -((-a + b - 1) / b
 }  // namespace
 
 // ---------------------------------------------------------------- host side
+// dense z-major index for levels up to depth 9 (513^3 ints = 541 MB)
+constexpr int64_t DENSE_INDEX_MAX_CELLS = (int64_t)1 << 28;
+
 template <class T> struct Schedule {
   DArray<int> order, offsets;
   std::vector<int> host_offsets;
@@ -1108,16 +1186,20 @@ struct RelaxGraph {
 
 template <class T, class V, int BS> static void capture_relax_launches(hipStream_t cs, const char* desc, int n, int iters, unsigned bx,
                                                              int& launches) {
+  struct Launch { int z0, color, count; };
+  std::vector<Launch> program;
   for (int q = 0; q <= n + 2 * (iters - 1); ++q) {
     const int first = std::max(0, floordiv(q - n + 1, 2));
     const int last = std::min(iters - 1, q / 2);
     if (last < first) continue;
-    const int z0 = n - q + 2 * first;
-    const unsigned count = (unsigned)(last - first + 1);
-    for (int color = 0; color < 4; ++color) {
-      hipLaunchKernelGGL((k_relax_wave_graph<T, V, BS>), dim3(bx, count), dim3(BS), 0, cs, desc, z0, color);
-      ++launches;
-    }
+    for (int color = 0; color < 4; ++color) program.push_back({n - q + 2 * first, color, last - first + 1});
+  }
+  for (size_t k = 0; k < program.size(); ++k) {
+    const Launch& l = program[k];
+    const Launch nx = k + 1 < program.size() ? program[k + 1] : Launch{0, 0, 0};
+    hipLaunchKernelGGL((k_relax_wave_graph<T, V, BS>), dim3(bx, (unsigned)l.count), dim3(BS), 0, cs, desc, l.z0, l.color, nx.z0,
+                       nx.color, nx.count);
+    ++launches;
   }
 }
 
@@ -1220,7 +1302,7 @@ template <class T> static Stencil<double> assemble_stiffness(const LevelState<T>
   if (R) {
     hipLaunchKernelGGL(k_stiffness, dim3(blocks_for(R, 256)), dim3(256), 0, s, (const int64_t*)rowsL.basis_keys.data(), R,
                        row_order, (const int64_t*)colsL.basis_keys.data(), (int64_t)colsL.basis_count, colsL.columns.view(),
-                       depth, trial_depth,
+                       colsL.dense_dim ? (const int*)colsL.dense.data() : (const int*)nullptr, colsL.dense_dim, depth, trial_depth,
                        (const D2*)table.data(), m.cols.data(), m.vals.data());
     KERNEL_CHECK();
   }
@@ -1293,7 +1375,7 @@ template <class T> static Stencil<T> cast_stencil(Stencil<double>&& m, hipStream
 
 template <class T>
 static void normal_stencil_load(const LevelState<T>& testL, int test_depth, const int64_t* nkeys, int64_t ncount, ColumnTable ct,
-                                const T* nvals, int normal_depth, double* out, hipStream_t s) {
+                                DenseIndex dn, const T* nvals, int normal_depth, double* out, hipStream_t s) {
   const int R = testL.basis_count;
   if (!R) return;
   if (!ncount) {
@@ -1308,7 +1390,7 @@ static void normal_stencil_load(const LevelState<T>& testL, int test_depth, cons
                      table.data());
   KERNEL_CHECK();
   hipLaunchKernelGGL(k_normal_load<T>, dim3(blocks_for(R, 128)), dim3(128), 0, s, (const int64_t*)testL.basis_keys.data(), R,
-                     nkeys, ncount, ct, nvals, 3, test_depth, normal_depth, first, width, (const D2*)table.data(), out);
+                     nkeys, ncount, ct, dn, nvals, 3, test_depth, normal_depth, first, width, (const D2*)table.data(), out);
   KERNEL_CHECK();
 }
 
@@ -1326,12 +1408,25 @@ template <class T> static void normal_loads(Result<T>& r, std::vector<DArray<int
     LevelState<T>& L = r.levels[d];
     ColumnIndex records;
     build_column_index(nk, r1 - r0, (1 << d) + 1, records, s);
+    // dense (x, y, z)-major index of the level's records for the tests taken
+    // in canonical order (consecutive z)
+    DArray<int> rdense;
+    DenseIndex rdn;
+    const int64_t rdim = (1 << d) + 1;
+    if (r1 > r0 && d >= 5 && rdim * rdim * rdim <= DENSE_INDEX_MAX_CELLS) {
+      rdense.reset((size_t)(rdim * rdim * rdim), s);
+      HIP_CHECK(hipMemsetAsync(rdense.data(), 0xff, sizeof(int) * (size_t)(rdim * rdim * rdim), s));
+      hipLaunchKernelGGL(k_dense_fill_xyz, dim3(blocks_for(r1 - r0, 256)), dim3(256), 0, s, nk, r1 - r0, (int)rdim, rdense.data());
+      KERNEL_CHECK();
+      rdn.idx = rdense.data();
+      rdn.dim = (int)rdim;
+    }
     if (d >= base) {
       same[d].reset(L.basis_count, s);
-      normal_stencil_load(L, d, nk, r1 - r0, records.view(), nv, d, same[d].data(), s);
+      normal_stencil_load(L, d, nk, r1 - r0, records.view(), rdn, nv, d, same[d].data(), s);
       if (d > base) {
         from_child[d - 1].reset(r.levels[d - 1].basis_count, s);
-        normal_stencil_load(r.levels[d - 1], d - 1, nk, r1 - r0, records.view(), nv, d, from_child[d - 1].data(), s);
+        normal_stencil_load(r.levels[d - 1], d - 1, nk, r1 - r0, records.view(), rdn, nv, d, from_child[d - 1].data(), s);
       }
     }
     if (d < D) {
@@ -1347,8 +1442,14 @@ template <class T> static void normal_loads(Result<T>& r, std::vector<DArray<int
       }
       if (d >= base && has_cumulative) {
         from_parent[d + 1].reset(r.levels[d + 1].basis_count, s);
+        DenseIndex bdn;
+        if (L.dense_dim) {
+          bdn.idx = L.dense.data();
+          bdn.dim = L.dense_dim;
+          bdn.zmajor = 1;
+        }
         normal_stencil_load(r.levels[d + 1], d + 1, (const int64_t*)L.basis_keys.data(), (int64_t)L.basis_count,
-                            L.columns.view(), (const T*)cumulative.data(), d, from_parent[d + 1].data(), s);
+                            L.columns.view(), bdn, (const T*)cumulative.data(), d, from_parent[d + 1].data(), s);
       }
       previous = std::move(cumulative);
     }
@@ -1401,7 +1502,7 @@ template <class T> static void regular_multigrid(Result<T>& r, std::vector<DArra
     mcols[d - 1].reset(27 * (size_t)C, s);
     mvals[d - 1].reset(27 * (size_t)C, s);
     if (C) {
-      hipLaunchKernelGGL(k_galerkin<T>, dim3(blocks_for(C, 64)), dim3(64), 0, s, (const int*)restr[d - 1].data(), C,
+      hipLaunchKernelGGL(k_galerkin<T>, dim3(C), dim3(128), 0, s, (const int*)restr[d - 1].data(), C,
                          (const int*)mcols[d].data(), (const T*)mvals[d].data(), F, (const int*)r.levels[d].P.cols.data(),
                          (const T*)r.levels[d].P.vals.data(), mcols[d - 1].data(), mvals[d - 1].data(),
                          (const int64_t*)r.levels[d - 1].basis_keys.data(), (int64_t)C, d - 1);
@@ -1561,6 +1662,15 @@ template <class T> void stage_solve(Result<T>& r) {
   for (int d = 0; d <= D; ++d) {
     LevelState<T>& L = r.levels[d];
     build_column_index((const int64_t*)L.basis_keys.data(), L.basis_count, (1 << d) + 1, L.columns, s);
+    const int64_t dim = (1 << d) + 1;
+    if (L.basis_count && d >= 5 && dim * dim * dim <= DENSE_INDEX_MAX_CELLS) {
+      L.dense.reset((size_t)(dim * dim * dim), s);
+      HIP_CHECK(hipMemsetAsync(L.dense.data(), 0xff, sizeof(int) * (size_t)(dim * dim * dim), s));  // -1
+      hipLaunchKernelGGL(k_dense_fill, dim3(blocks_for(L.basis_count, 256)), dim3(256), 0, s, (const int64_t*)L.basis_keys.data(),
+                         L.basis_count, (int)dim, L.dense.data());
+      KERNEL_CHECK();
+      L.dense_dim = (int)dim;
+    }
   }
   // prolongation P_d and restriction gathers (solver.py:340-342)
   std::vector<DArray<int>> restr(D + 1);
@@ -1621,6 +1731,10 @@ template <class T> void stage_solve(Result<T>& r) {
     L.solved = true;
   }
   for (int d = 0; d < D; ++d) r.levels[d].restriction = std::move(restr[d]);
+  for (int d = 0; d <= D; ++d) {
+    r.levels[d].dense.free();
+    r.levels[d].dense_dim = 0;
+  }
 }
 
 // CSR -> slot layout in entry order (GS only iterates present slots in order)
@@ -2027,7 +2141,7 @@ template <class T> void solve_backward(Result<T>& r, const double* gcoef, double
         mcols[d - 1].reset(27 * (size_t)C, s);
         mvals[d - 1].reset(27 * (size_t)C, s);
         if (C) {
-          hipLaunchKernelGGL(k_galerkin<T>, dim3(blocks_for(C, 64)), dim3(64), 0, s, (const int*)r.levels[d - 1].restriction.data(),
+          hipLaunchKernelGGL(k_galerkin<T>, dim3(C), dim3(128), 0, s, (const int*)r.levels[d - 1].restriction.data(),
                              C, (const int*)mcols[d].data(), (const T*)mvals[d].data(), F, (const int*)r.levels[d].P.cols.data(),
                              (const T*)r.levels[d].P.vals.data(), mcols[d - 1].data(), mvals[d - 1].data(),
                              (const int64_t*)r.levels[d - 1].basis_keys.data(), (int64_t)C, d - 1);
