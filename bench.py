@@ -37,7 +37,19 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def relax_roofline(metadata, value_bytes, vector_bytes):
+TRAFFIC = ROOT / "profiles" / "relax_traffic_ellipsoid800k_d9.json"
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed FETCH_SIZE/WRITE_SIZE
+    counter runs (scaled by the copy calibration), or None."""
+    if not TRAFFIC.exists():
+        return None
+    doc = json.loads(TRAFFIC.read_text())
+    return doc["bytes_per_launch"] if doc.get("kernel") == kernel else None
+
+
+def relax_roofline(metadata, value_bytes, vector_bytes, kernel=None):
     """Sliced Gauss-Seidel relaxation of every finer level (one kernel, one
     launch per colour and pipeline step): algorithmic bytes over the HIP-event
     span of its launches on the reconstruction stream.
@@ -60,7 +72,8 @@ def relax_roofline(metadata, value_bytes, vector_bytes):
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 5),
-        "traffic": None,
+        "traffic": measured_traffic(kernel) if kernel else None,
+        "traffic_unit": "bytes per launch (FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
         "levels": [st["depth"] for st in stats],
         "launches": launches,
         "algorithmic_bytes": total,
@@ -192,7 +205,7 @@ def main():
                                 "float32 finer-level operators",
                    "basis_counts": rmx.metadata["basis_counts"]},
     }
-    out["roofline"] = relax_roofline(rmx.metadata, 4, 8)
+    out["roofline"] = relax_roofline(rmx.metadata, 4, 8, "k_relax_wave_graph<double, float, 64>")
     out["stage_seconds"] = rmx.metadata["timings_seconds"]
     out["normal_average_wait_seconds"] = rmx.metadata["normal_average_wait_seconds"]
     r32, dt32 = timed(tp32, tn32, max(1, args.steps // 2), 1, "float32")

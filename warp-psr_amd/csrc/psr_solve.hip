@@ -644,36 +644,15 @@ template <class T, class V> struct RelaxDesc {
 };
 constexpr size_t RELAX_GOFF_OFFSET = 256;
 
-// Each launch also reads the operator rows that the same workgroup index of
-// the next launch (next_z, next_color, next_count; next_count 0 = none)
-// will update: they do not depend on x, and the read leaves them in the
-// XCD's L2 (workgroups go to XCDs by index) and the memory-side cache, so
-// the next launch's first dependent loads are cache hits.
 template <class T, class V, int BS>
 __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(1, 4)))
-k_relax_wave_graph(const char* __restrict__ block, int first_z, int color, int next_z, int next_color, int next_count) {
+k_relax_wave_graph(const char* __restrict__ block, int first_z, int color) {
   const int* goff = (const int*)(block + RELAX_GOFF_OFFSET);
   const int group = 4 * (first_z + 2 * (int)blockIdx.y) + color;
   const int start = goff[group], end = goff[group + 1];
   const RelaxDesc<T, V> d = *(const RelaxDesc<T, V>*)block;
   const int j = start + (int)blockIdx.x * BS + (int)threadIdx.x;
-  int jn = -1;
-  if ((int)blockIdx.y < next_count) {
-    const int g2 = 4 * (next_z + 2 * (int)blockIdx.y) + next_color;
-    const int c = goff[g2] + (int)blockIdx.x * BS + (int)threadIdx.x;
-    if (c < goff[g2 + 1]) jn = c;
-  }
   if (j < end) gs_update(j, d.R, d.order, d.cols, d.vals, d.b, d.inv_diag, d.has_diag, d.x);
-  if (jn >= 0) {
-    unsigned acc = 0;
-#pragma unroll
-    for (int k = 0; k < 27; ++k) {
-      acc ^= (unsigned)d.cols[(int64_t)k * d.R + jn];
-      acc ^= __builtin_bit_cast(unsigned, float(d.vals[(int64_t)k * d.R + jn]));
-    }
-    acc ^= __builtin_bit_cast(unsigned, float(d.b[jn])) ^ __builtin_bit_cast(unsigned, float(d.inv_diag[jn]));
-    if (d.R < 0) d.x[0] = T(acc);  // never taken (R >= 0): keeps the reads
-  }
 }
 
 // relaxation.py:261-271 (8-colour, all sweeps) inside one workgroup: the
@@ -1186,20 +1165,16 @@ struct RelaxGraph {
 
 template <class T, class V, int BS> static void capture_relax_launches(hipStream_t cs, const char* desc, int n, int iters, unsigned bx,
                                                              int& launches) {
-  struct Launch { int z0, color, count; };
-  std::vector<Launch> program;
   for (int q = 0; q <= n + 2 * (iters - 1); ++q) {
     const int first = std::max(0, floordiv(q - n + 1, 2));
     const int last = std::min(iters - 1, q / 2);
     if (last < first) continue;
-    for (int color = 0; color < 4; ++color) program.push_back({n - q + 2 * first, color, last - first + 1});
-  }
-  for (size_t k = 0; k < program.size(); ++k) {
-    const Launch& l = program[k];
-    const Launch nx = k + 1 < program.size() ? program[k + 1] : Launch{0, 0, 0};
-    hipLaunchKernelGGL((k_relax_wave_graph<T, V, BS>), dim3(bx, (unsigned)l.count), dim3(BS), 0, cs, desc, l.z0, l.color, nx.z0,
-                       nx.color, nx.count);
-    ++launches;
+    const int z0 = n - q + 2 * first;
+    const unsigned count = (unsigned)(last - first + 1);
+    for (int color = 0; color < 4; ++color) {
+      hipLaunchKernelGGL((k_relax_wave_graph<T, V, BS>), dim3(bx, count), dim3(BS), 0, cs, desc, z0, color);
+      ++launches;
+    }
   }
 }
 
