@@ -91,10 +91,14 @@ template <class T> Result<T>* run(const T* pts, const T* nrm, int64_t n, const P
       HIP_CHECK(hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
       r->timings.push_back({names[k], ms / 1000.f});
     }
-    for (auto& st : r->relax_stats) {
+    for (size_t k = 0; k < r->relax_stats.size(); ++k) {
+      auto& st = r->relax_stats[k];
       float ms = 0.f;
       HIP_CHECK(hipEventElapsedTime(&ms, st.start, st.end));
       st.seconds = ms / 1000.f;
+      unsigned long long nnz = 0;
+      if (k < r->relax_nnz.size()) read_bytes(&nnz, r->relax_nnz[k].data(), sizeof(nnz), s);
+      st.nnz = (int64_t)nnz;
     }
   } catch (...) {
     for (auto& e : ev) (void)hipEventDestroy(e);

@@ -121,6 +121,8 @@ template <class T> struct Result {
     float seconds;
   };
   std::vector<RelaxStat> relax_stats;
+  // device nonzero counters of the relaxed levels, read once the run is done
+  std::vector<DArray<unsigned long long>> relax_nnz;
   ~Result() {
     if (join_begin) {
       (void)hipEventSynchronize(join_end);

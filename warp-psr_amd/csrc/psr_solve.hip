@@ -213,6 +213,21 @@ __global__ void k_bin_columns(const uint64_t* __restrict__ bins, int P, int dept
   if (i == P - 1 || (bins[i + 1] >> depth) != c) end[c] = i + 1;
 }
 
+// Dense z-major ranges of the sorted point bins: points of cell (bx, by, bz)
+// occupy [start[c], end[c]), c = (bz*n + bx)*n + by (empty cells: 0, 0).
+__global__ void k_bin_cells(const uint64_t* __restrict__ bins, int P, int depth, int* __restrict__ start,
+                            int* __restrict__ end) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P) return;
+  const uint64_t b = bins[i];
+  const int64_t n = (int64_t)1 << depth;
+  const int64_t bx = (int64_t)(b >> (2 * depth)), by = (int64_t)((b >> depth) & (uint64_t)(n - 1)),
+                bz = (int64_t)(b & (uint64_t)(n - 1));
+  const int64_t c = (bz * n + bx) * n + by;
+  if (i == 0 || bins[i - 1] != b) start[c] = i;
+  if (i == P - 1 || bins[i + 1] != b) end[c] = i + 1;
+}
+
 // screening_stencils.py:34-70 (add): per row, points binned in the 8 cells at
 // its vertex.  One pass over the points with the 27 slot sums (and the point
 // load) held in registers; every sum receives its terms in the reference's
@@ -223,6 +238,7 @@ template <class T>
 __global__ void __launch_bounds__(128) k_screening(const int64_t* __restrict__ rows, int R, const int* __restrict__ row_order,
                                                    int depth, int trial_depth, const uint64_t* __restrict__ bin_keys,
                                                    const int* __restrict__ bin_ids, int P, ColumnTable bin_cols,
+                                                   const int* __restrict__ cell_start, const int* __restrict__ cell_end,
                                                    const T* __restrict__ pos, const T* __restrict__ w,
                                                    const int* __restrict__ scols, double* __restrict__ svals,
                                                    double* __restrict__ load) {
@@ -244,6 +260,19 @@ __global__ void __launch_bounds__(128) k_screening(const int64_t* __restrict__ r
       rb[k0] = re[k0] = rb[k0 + 1] = re[k0 + 1] = 0;
       const int bx = cx - a, by = cy - bb;
       if (bx < 0 || by < 0 || bx >= n || by >= n) continue;
+      if (cell_start) {  // dense z-major cell ranges
+        if (cz >= 1) {
+          const int64_t c = ((int64_t)(cz - 1) * n + bx) * n + by;
+          rb[k0 + 1] = cell_start[c];
+          re[k0 + 1] = cell_end[c];
+        }
+        if (cz < n) {
+          const int64_t c = ((int64_t)cz * n + bx) * n + by;
+          rb[k0] = cell_start[c];
+          re[k0] = cell_end[c];
+        }
+        continue;
+      }
       const uint64_t col = ((uint64_t)bx << (2 * depth)) | ((uint64_t)by << depth);
       const int zfirst = cz >= 1 ? cz - 1 : cz;
       int lo = 0, hi = P;
@@ -643,6 +672,15 @@ template <class T, class V> struct RelaxDesc {
   int R;
 };
 constexpr size_t RELAX_GOFF_OFFSET = 256;
+
+// Writes a descriptor block on the stream: the descriptor arrives as a kernel
+// argument, so no host buffer has to outlive the call.
+template <class T, class V>
+__global__ void k_write_relax_desc(char* __restrict__ block, RelaxDesc<T, V> d, const int* __restrict__ goff, int count) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *(RelaxDesc<T, V>*)block = d;
+  int* dst = (int*)(block + RELAX_GOFF_OFFSET);
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) dst[i] = goff[i];
+}
 
 template <class T, class V, int BS>
 __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(1, 4)))
@@ -1212,12 +1250,12 @@ int relax_sliced_graph(const GsOperator<T, V>& op, const Schedule<T>& sc, T* x, 
   static_assert(sizeof(RelaxDesc<T, V>) <= RELAX_GOFF_OFFSET, "descriptor block layout");
   RelaxDesc<T, V> d{op.update_order, op.cols.data(), op.vals.data(), op.b.data(), op.inv_diag.data(), op.has_diag.data(), x, op.R};
   char* desc = (char*)it->second.desc;
-  HIP_CHECK(hipMemcpyAsync(desc, &d, sizeof(d), hipMemcpyHostToDevice, s));
-  HIP_CHECK(hipMemcpyAsync(desc + RELAX_GOFF_OFFSET, sc.offsets.data(), sizeof(int) * (groups + 1), hipMemcpyDeviceToDevice, s));
+  // the next call on this stream rewrites the block only after this launch
+  // (stream order)
+  hipLaunchKernelGGL((k_write_relax_desc<T, V>), dim3(blocks_for(groups + 1, 256)), dim3(256), 0, s, desc, d,
+                     (const int*)sc.offsets.data(), (int)(groups + 1));
+  KERNEL_CHECK();
   HIP_CHECK(hipGraphLaunch(it->second.exec, s));
-  // the descriptor is rewritten by the next call on this stream only after
-  // this launch (stream order); keep the host copy until the copy has run
-  HIP_CHECK(hipStreamSynchronize(s));
   return it->second.launches;
 }
 
@@ -1290,6 +1328,7 @@ template <class T> struct PointSet {
   DArray<uint64_t> bins;
   DArray<int> ids;
   ColumnIndex columns;  // (bx, by) runs of the sorted bins
+  DArray<int> cell_start, cell_end;  // dense z-major cell ranges (depth <= 9)
 };
 
 template <class T> static PointSet<T> point_constraints(Result<T>& r, int d, hipStream_t s) {
@@ -1309,7 +1348,16 @@ template <class T> static PointSet<T> point_constraints(Result<T>& r, int d, hip
     KERNEL_CHECK();
     sort_pairs_u64(ps.bins.data(), ps.ids.data(), ps.P, 3 * d, s);
     const int n = 1 << d;
-    if (n <= COLUMN_TABLE_MAX_DIM) {
+    const int64_t cells = (int64_t)n * n * n;
+    if (d >= 5 && cells <= DENSE_INDEX_MAX_CELLS) {
+      ps.cell_start.reset((size_t)cells, s);
+      ps.cell_end.reset((size_t)cells, s);
+      ps.cell_start.zero();
+      ps.cell_end.zero();
+      hipLaunchKernelGGL(k_bin_cells, dim3(blocks_for(ps.P, 256)), dim3(256), 0, s, (const uint64_t*)ps.bins.data(), ps.P, d,
+                         ps.cell_start.data(), ps.cell_end.data());
+      KERNEL_CHECK();
+    } else if (n <= COLUMN_TABLE_MAX_DIM) {
       ps.columns.start.reset((size_t)n * n, s);
       ps.columns.end.reset((size_t)n * n, s);
       ps.columns.start.zero();
@@ -1329,7 +1377,9 @@ static void add_screening(Stencil<double>& m, const DArray<int64_t>& rows, const
   if (!m.rows) return;
   hipLaunchKernelGGL(k_screening<T>, dim3(blocks_for(m.rows, 128)), dim3(128), 0, s, (const int64_t*)rows.data(), m.rows,
                      row_order, depth, trial_depth, (const uint64_t*)ps.bins.data(), (const int*)ps.ids.data(), ps.P,
-                     ps.columns.view(), (const T*)ps.pos.data(), (const T*)ps.w.data(),
+                     ps.columns.view(), ps.cell_start.empty() ? (const int*)nullptr : (const int*)ps.cell_start.data(),
+                     ps.cell_end.empty() ? (const int*)nullptr : (const int*)ps.cell_end.data(), (const T*)ps.pos.data(),
+                     (const T*)ps.w.data(),
                      (const int*)m.cols.data(), m.vals.data(), load);
   KERNEL_CHECK();
 }
@@ -1598,16 +1648,14 @@ static void solve_finer(Result<T>& r, int d, Stencil<V>& Aop, Stencil<V>& Cop, b
     KERNEL_CHECK();
   }
   typename Result<T>::RelaxStat st{d, R, 0, 0, nullptr, nullptr, 0.f};
-  {
+  {  // stored nonzeros, read after the run (no synchronisation here)
     DArray<unsigned long long> cnt(1, s);
     cnt.zero();
     const int64_t slots = 27 * (int64_t)R;
     const unsigned grid = (unsigned)std::min<int64_t>(1024, (slots + 255) / 256);
     hipLaunchKernelGGL(k_count_present, dim3(grid), dim3(256), 0, s, (const int*)op.cols.data(), slots, cnt.data());
     KERNEL_CHECK();
-    unsigned long long nnz = 0;
-    read_bytes(&nnz, cnt.data(), sizeof(nnz), s);
-    st.nnz = (int64_t)nnz;
+    r.relax_nnz.push_back(std::move(cnt));
   }
   HIP_CHECK(hipEventCreate(&st.start));
   HIP_CHECK(hipEventCreate(&st.end));
