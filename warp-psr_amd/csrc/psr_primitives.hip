@@ -111,72 +111,192 @@ __global__ void __launch_bounds__(RS_THREADS) k_radix_count(const K* __restrict_
   counts[threadIdx.x * nblocks + blockIdx.x] = hist[threadIdx.x];
 }
 
+constexpr int RS_CHUNK = RS_TILE / RS_WAVES;  // items one wave ranks: RS_ITEMS rounds of 64
+
+// Lanes of the wave holding the same 8-bit digit (and the same validity).
+__device__ __forceinline__ uint64_t digit_peers(int digit, bool valid) {
+  uint64_t peers = __ballot(valid);
+  if (!valid) peers = ~peers;
+#pragma unroll
+  for (int b = 0; b < 8; ++b) {
+    const bool bit = (digit >> b) & 1;
+    const uint64_t m = __ballot(bit);
+    peers &= bit ? m : ~m;
+  }
+  return peers;
+}
+
+// Stable rank of RS_ITEMS keys per lane inside the wave's chunk.  Each wave
+// owns one row of hist: the lanes of a wave execute each LDS instruction
+// together, so every lane reads the running count before the group's first
+// lane advances it, and no barrier is needed inside the chunk.
 template <class K>
-__global__ void __launch_bounds__(RS_THREADS) k_radix_scatter(const K* __restrict__ keys, const int* __restrict__ vals,
-                                                               K* __restrict__ keys_out, int* __restrict__ vals_out,
-                                                               int64_t n, int shift, const int* __restrict__ offsets,
-                                                               int nblocks) {
-  __shared__ int base_off[256];
-  __shared__ int running[256];
-  __shared__ int wave_cnt[RS_WAVES][256];
-  base_off[threadIdx.x] = offsets[threadIdx.x * nblocks + blockIdx.x];
-  running[threadIdx.x] = 0;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+__device__ __forceinline__ void wave_rank(const K (&key)[RS_ITEMS], int64_t first, int64_t n, int shift, int* hist_row,
+                                          int (&pos)[RS_ITEMS]) {
+  const int lane = threadIdx.x & 63;
   const uint64_t lt_mask = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
-  const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+#pragma unroll
   for (int r = 0; r < RS_ITEMS; ++r) {
-#pragma unroll
-    for (int w = 0; w < RS_WAVES; ++w) wave_cnt[w][threadIdx.x] = 0;
-    __syncthreads();
-    const int64_t i = base + r * RS_THREADS + threadIdx.x;
-    const bool valid = i < n;
-    K key = valid ? keys[i] : K(0);
-    int val = valid ? vals[i] : 0;
-    const int digit = (int)((key >> shift) & 255);
-    uint64_t peers = __ballot(valid);
-    if (!valid) peers = ~peers;
-#pragma unroll
-    for (int b = 0; b < 8; ++b) {
-      const bool bit = (digit >> b) & 1;
-      uint64_t m = __ballot(bit);
-      peers &= bit ? m : ~m;
-    }
+    const bool valid = first + r * 64 < n;
+    const int digit = (int)((key[r] >> shift) & 255);
+    const uint64_t peers = digit_peers(digit, valid);
     const int rank = __popcll(peers & lt_mask);
-    if (valid && rank == 0) wave_cnt[wave][digit] = __popcll(peers);
-    __syncthreads();
-    if (valid) {
-      int pos = base_off[digit] + running[digit] + rank;
-      for (int w = 0; w < wave; ++w) pos += wave_cnt[w][digit];
-      keys_out[pos] = key;
-      vals_out[pos] = val;
-    }
-    __syncthreads();
-    int add = 0;
-#pragma unroll
-    for (int w = 0; w < RS_WAVES; ++w) add += wave_cnt[w][threadIdx.x];
-    running[threadIdx.x] += add;
-    // the next round's clear of wave_cnt happens after this barrier
-    __syncthreads();
+    const int before = hist_row[digit];
+    pos[r] = before + rank;
+    if (valid && rank == 0) hist_row[digit] = before + __popcll(peers);
   }
 }
 
-template <class K> void sort_pairs(K* keys, int* vals, int64_t n, int bits, hipStream_t s) {
+// One pass over a tile of RS_TILE items: wave w ranks the contiguous chunk
+// [w*RS_CHUNK, (w+1)*RS_CHUNK) of the tile, the per-wave digit counts become
+// the tile's offsets (global digit offset + earlier waves), and each item is
+// written to offset + rank.  Items keep input order within a digit.  KO may
+// be narrower or wider than KI: the first and last passes of a sort of
+// 64-bit keys whose bits fit 32 convert on the way through.
+template <class KI, class KO>
+__global__ void __launch_bounds__(RS_THREADS) k_radix_scatter(const KI* __restrict__ keys, const int* __restrict__ vals,
+                                                               KO* __restrict__ keys_out, int* __restrict__ vals_out,
+                                                               int64_t n, int shift, const int* __restrict__ offsets,
+                                                               int nblocks) {
+  __shared__ int hist[RS_WAVES][256];
+#pragma unroll
+  for (int w = 0; w < RS_WAVES; ++w) hist[w][threadIdx.x] = 0;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t first = (int64_t)blockIdx.x * RS_TILE + wave * RS_CHUNK + lane;
+  KI key[RS_ITEMS];
+  int val[RS_ITEMS], pos[RS_ITEMS];
+#pragma unroll
+  for (int r = 0; r < RS_ITEMS; ++r) {
+    const int64_t i = first + r * 64;
+    key[r] = i < n ? keys[i] : KI(0);
+    val[r] = i < n ? vals[i] : 0;
+  }
+  __syncthreads();
+  wave_rank(key, first, n, shift, hist[wave], pos);
+  __syncthreads();
+  {
+    int run = offsets[threadIdx.x * nblocks + blockIdx.x];
+#pragma unroll
+    for (int w = 0; w < RS_WAVES; ++w) {
+      const int c = hist[w][threadIdx.x];
+      hist[w][threadIdx.x] = run;
+      run += c;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < RS_ITEMS; ++r) {
+    if (first + r * 64 < n) {
+      const int p = hist[wave][(int)((key[r] >> shift) & 255)] + pos[r];
+      keys_out[p] = (KO)key[r];
+      vals_out[p] = val[r];
+    }
+  }
+}
+
+// Whole sort of n <= RS_TILE pairs in one workgroup: every pass ranks and
+// scatters inside LDS, with the same per-digit order as the tiled passes.
+template <class K>
+__global__ void __launch_bounds__(RS_THREADS) k_sort_tile(K* __restrict__ keys, int* __restrict__ vals, int n, int bits) {
+  __shared__ K skey[RS_TILE];
+  __shared__ int sval[RS_TILE];
+  __shared__ int hist[RS_WAVES][256];
+  for (int i = threadIdx.x; i < n; i += RS_THREADS) {
+    skey[i] = keys[i];
+    sval[i] = vals[i];
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int first = wave * RS_CHUNK + lane;
+  for (int shift = 0; shift < bits; shift += 8) {
+#pragma unroll
+    for (int w = 0; w < RS_WAVES; ++w) hist[w][threadIdx.x] = 0;
+    __syncthreads();
+    K key[RS_ITEMS];
+    int val[RS_ITEMS], pos[RS_ITEMS];
+#pragma unroll
+    for (int r = 0; r < RS_ITEMS; ++r) {
+      const int i = first + r * 64;
+      key[r] = i < n ? skey[i] : K(0);
+      val[r] = i < n ? sval[i] : 0;
+    }
+    wave_rank(key, first, n, shift, hist[wave], pos);
+    __syncthreads();
+    int c[RS_WAVES], count = 0;
+#pragma unroll
+    for (int w = 0; w < RS_WAVES; ++w) {
+      c[w] = hist[w][threadIdx.x];
+      count += c[w];
+    }
+    int total;
+    int run = block_exclusive_scan(count, &total);
+#pragma unroll
+    for (int w = 0; w < RS_WAVES; ++w) {
+      hist[w][threadIdx.x] = run;
+      run += c[w];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < RS_ITEMS; ++r) {
+      if (first + r * 64 < n) {
+        const int p = hist[wave][(int)((key[r] >> shift) & 255)] + pos[r];
+        skey[p] = key[r];
+        sval[p] = val[r];
+      }
+    }
+    __syncthreads();
+  }
+  for (int i = threadIdx.x; i < n; i += RS_THREADS) {
+    keys[i] = skey[i];
+    vals[i] = sval[i];
+  }
+}
+
+// One counting + scan + scatter pass from (kin, vin) to (kout, vout).
+template <class KI, class KO>
+void radix_pass(const KI* kin, const int* vin, KO* kout, int* vout, int64_t n, int shift, int* counts, int nblocks,
+                hipStream_t s) {
+  hipLaunchKernelGGL(k_radix_count<KI>, dim3(nblocks), dim3(RS_THREADS), 0, s, kin, n, shift, counts, nblocks);
+  KERNEL_CHECK();
+  scan_exclusive_i32(counts, counts, (int64_t)256 * nblocks, nullptr, s);
+  hipLaunchKernelGGL((k_radix_scatter<KI, KO>), dim3(nblocks), dim3(RS_THREADS), 0, s, kin, vin, kout, vout, n, shift,
+                     (const int*)counts, nblocks);
+  KERNEL_CHECK();
+}
+
+template <class K> void radix_sort_pairs(K* keys, int* vals, int64_t n, int bits, hipStream_t s) {
   if (n <= 1 || bits <= 0) return;
+  if (n <= RS_TILE) {
+    hipLaunchKernelGGL(k_sort_tile<K>, dim3(1), dim3(RS_THREADS), 0, s, keys, vals, (int)n, bits);
+    KERNEL_CHECK();
+    return;
+  }
   const int nblocks = (int)((n + RS_TILE - 1) / RS_TILE);
-  DArray<K> kalt(n, s);
+  const int passes = (bits + 7) / 8;
   DArray<int> valt(n, s);
   DArray<int> counts((size_t)256 * nblocks, s);
-  K* kin = keys;
   int* vin = vals;
-  K* kout = kalt.data();
   int* vout = valt.data();
-  for (int shift = 0; shift < bits; shift += 8) {
-    hipLaunchKernelGGL(k_radix_count<K>, dim3(nblocks), dim3(RS_THREADS), 0, s, kin, n, shift, counts.data(), nblocks);
-    KERNEL_CHECK();
-    scan_exclusive_i32(counts.data(), counts.data(), (int64_t)256 * nblocks, nullptr, s);
-    hipLaunchKernelGGL(k_radix_scatter<K>, dim3(nblocks), dim3(RS_THREADS), 0, s, kin, vin, kout, vout, n, shift,
-                       counts.data(), nblocks);
-    KERNEL_CHECK();
+  if (sizeof(K) == 8 && passes >= 2 && bits <= 32) {
+    // 64-bit keys whose bits fit 32: the middle passes move 32-bit keys
+    DArray<uint32_t> ka(n, s), kb(n, s);
+    radix_pass(keys, vin, ka.data(), vout, n, 0, counts.data(), nblocks, s);
+    std::swap(vin, vout);
+    uint32_t* kin = ka.data();
+    uint32_t* kout = kb.data();
+    for (int p = 1; p + 1 < passes; ++p) {
+      radix_pass(kin, vin, kout, vout, n, 8 * p, counts.data(), nblocks, s);
+      std::swap(kin, kout);
+      std::swap(vin, vout);
+    }
+    radix_pass(kin, vin, keys, vout, n, 8 * (passes - 1), counts.data(), nblocks, s);
+    if (vout != vals) HIP_CHECK(hipMemcpyAsync(vals, vout, n * sizeof(int), hipMemcpyDeviceToDevice, s));
+    return;
+  }
+  DArray<K> kalt(n, s);
+  K* kin = keys;
+  K* kout = kalt.data();
+  for (int p = 0; p < passes; ++p) {
+    radix_pass(kin, vin, kout, vout, n, 8 * p, counts.data(), nblocks, s);
     std::swap(kin, kout);
     std::swap(vin, vout);
   }
@@ -236,8 +356,8 @@ void scan_exclusive_i32(const int* in, int* out, int64_t n, int* total_dev, hipS
   }
 }
 
-void sort_pairs_u32(uint32_t* keys, int* vals, int64_t n, int bits, hipStream_t s) { sort_pairs(keys, vals, n, bits, s); }
-void sort_pairs_u64(uint64_t* keys, int* vals, int64_t n, int bits, hipStream_t s) { sort_pairs(keys, vals, n, bits, s); }
+void sort_pairs_u32(uint32_t* keys, int* vals, int64_t n, int bits, hipStream_t s) { radix_sort_pairs(keys, vals, n, bits, s); }
+void sort_pairs_u64(uint64_t* keys, int* vals, int64_t n, int bits, hipStream_t s) { radix_sort_pairs(keys, vals, n, bits, s); }
 
 void mark_heads_u32(const uint32_t* keys, int64_t n, uint32_t sentinel, int* heads, hipStream_t s) {
   if (n <= 0) return;

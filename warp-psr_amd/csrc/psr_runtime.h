@@ -157,6 +157,14 @@ void sort_pairs_u64(uint64_t* keys, int* vals, int64_t n, int bits, hipStream_t 
 // heads[i] = 1 where sorted keys start a new run and are not the sentinel.
 void mark_heads_u32(const uint32_t* keys, int64_t n, uint32_t sentinel, int* heads, hipStream_t s);
 void mark_heads_u64(const uint64_t* keys, int64_t n, uint64_t sentinel, int* heads, hipStream_t s);
+inline void sort_pairs(uint32_t* keys, int* vals, int64_t n, int bits, hipStream_t s) { sort_pairs_u32(keys, vals, n, bits, s); }
+inline void sort_pairs(uint64_t* keys, int* vals, int64_t n, int bits, hipStream_t s) { sort_pairs_u64(keys, vals, n, bits, s); }
+inline void mark_heads(const uint32_t* keys, int64_t n, uint32_t sentinel, int* heads, hipStream_t s) {
+  mark_heads_u32(keys, n, sentinel, heads, s);
+}
+inline void mark_heads(const uint64_t* keys, int64_t n, uint64_t sentinel, int* heads, hipStream_t s) {
+  mark_heads_u64(keys, n, sentinel, heads, s);
+}
 // Reads one int from device memory (synchronises the stream).
 int read_int(const int* dev, hipStream_t s);
 void read_bytes(void* host, const void* dev, size_t bytes, hipStream_t s);

@@ -482,23 +482,56 @@ __global__ void k_first_keys(const int* __restrict__ first, int G, uint64_t* __r
 }
 
 template <class T>
-__global__ void k_normal_splat_keys(const int* __restrict__ order, int G, const T* __restrict__ sums,
-                                    const T* __restrict__ nsums, const T* __restrict__ weights, const T* __restrict__ dw,
-                                    int minimum, int maximum, uint64_t* __restrict__ out, int* __restrict__ ids) {
-  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= 54 * (int64_t)G) return;
-  const int i = (int)(idx / 54), k = (int)(idx % 54);
-  const int D = maximum;
-  NormalRecord<T> rec = normal_record<T>(order[i], k, sums, nsums, weights, dw, minimum, maximum, false);
-  uint64_t packed = (uint64_t)(D + 1) << (3 * D);
-  if (rec.valid)
-    packed = ((uint64_t)rec.top << (3 * D)) | ((uint64_t)rec.x << (2 * D)) | ((uint64_t)rec.y << D) | (uint64_t)rec.z;
-  out[idx] = packed;
-  ids[idx] = (int)idx;
+__device__ __forceinline__ bool normal_record_valid(int64_t idx, int G, const int* __restrict__ order, const T* __restrict__ sums,
+                                                    const T* __restrict__ nsums, const T* __restrict__ weights,
+                                                    const T* __restrict__ dw, int minimum, int maximum,
+                                                    NormalRecord<T>* rec) {
+  if (idx >= 54 * (int64_t)G) return false;
+  *rec = normal_record<T>(order[idx / 54], (int)(idx % 54), sums, nsums, weights, dw, minimum, maximum, false);
+  return rec->valid;
 }
 
+// Valid splat records per 256-record block (normals.py:46-104 skips records
+// off the grid and the second depth when the fraction is 1).
 template <class T>
-__global__ void k_normal_splat_reduce(const uint64_t* __restrict__ keys, const int* __restrict__ ids,
+__global__ void __launch_bounds__(256) k_normal_splat_count(const int* __restrict__ order, int G, const T* __restrict__ sums,
+                                                            const T* __restrict__ nsums, const T* __restrict__ weights,
+                                                            const T* __restrict__ dw, int minimum, int maximum,
+                                                            int* __restrict__ block_counts) {
+  const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  NormalRecord<T> rec;
+  const bool valid = normal_record_valid(idx, G, order, sums, nsums, weights, dw, minimum, maximum, &rec);
+  const int count = __syncthreads_count(valid);
+  if (threadIdx.x == 0) block_counts[blockIdx.x] = count;
+}
+
+// The valid records, compacted in record order: key top<<3D | x<<2D | y<<D | z
+// and the record id (54 * group + slot).
+template <class T, class K>
+__global__ void __launch_bounds__(256) k_normal_splat_keys(const int* __restrict__ order, int G, const T* __restrict__ sums,
+                                                           const T* __restrict__ nsums, const T* __restrict__ weights,
+                                                           const T* __restrict__ dw, int minimum, int maximum,
+                                                           const int* __restrict__ block_offsets, int64_t capacity,
+                                                           K* __restrict__ out, int* __restrict__ ids) {
+  __shared__ int wave_count[4];
+  const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  NormalRecord<T> rec;
+  const bool valid = normal_record_valid(idx, G, order, sums, nsums, weights, dw, minimum, maximum, &rec);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint64_t ballot = __ballot(valid);
+  if (lane == 0) wave_count[wave] = __popcll(ballot);
+  __syncthreads();
+  if (!valid) return;
+  int pos = block_offsets[blockIdx.x] + __popcll(ballot & ((lane == 0) ? 0ull : ((~0ull) >> (64 - lane))));
+  for (int w = 0; w < wave; ++w) pos += wave_count[w];
+  if (pos >= capacity) return;  // the counting pass saw the same records
+  const int D = maximum;
+  out[pos] = ((K)rec.top << (3 * D)) | ((K)rec.x << (2 * D)) | ((K)rec.y << D) | (K)rec.z;
+  ids[pos] = (int)idx;
+}
+
+template <class T, class K>
+__global__ void k_normal_splat_reduce(const K* __restrict__ keys, const int* __restrict__ ids,
                                       const int* __restrict__ head_pos, int count, int64_t total,
                                       const int* __restrict__ order, const T* __restrict__ sums,
                                       const T* __restrict__ nsums, const T* __restrict__ weights, const T* __restrict__ dw,
@@ -506,7 +539,7 @@ __global__ void k_normal_splat_reduce(const uint64_t* __restrict__ keys, const i
   int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= count) return;
   int64_t j = head_pos[g];
-  const uint64_t key = keys[j];
+  const K key = keys[j];
   T a = T(0), b = T(0), c = T(0);
   while (j < total && keys[j] == key) {
     const int id = ids[j];
@@ -515,7 +548,7 @@ __global__ void k_normal_splat_reduce(const uint64_t* __restrict__ keys, const i
     ++j;
   }
   const int D = maximum;
-  const uint64_t mask = ((uint64_t)1 << D) - 1;
+  const K mask = ((K)1 << D) - 1;
   out_keys[g] = basis_key((int)(key >> (3 * D)), (int)((key >> (2 * D)) & mask), (int)((key >> D) & mask), (int)(key & mask));
   // normals.py:106-109 (_negative)
   out_vals[3 * g] = -a; out_vals[3 * g + 1] = -b; out_vals[3 * g + 2] = -c;
@@ -603,6 +636,45 @@ __global__ void k_depth_offsets(const int64_t* __restrict__ keys, int64_t n, int
   out[d] = (int)lower_bound(keys, n, (int64_t)d << 48);
 }
 
+// normals.py:46-109: valid splat records compacted, sorted by key (ties in
+// record order), reduced per key in that order.
+template <class T, class K> void normal_splats(Result<T>& r, int minimum, int maximum, K sentinel) {
+  hipStream_t s = r.s;
+  const int G = r.G;
+  const int64_t records = 54 * (int64_t)G;
+  const int blocks = (int)blocks_for(records, 256);
+  DArray<int> block_counts(blocks, s), block_offsets(blocks, s), total(1, s);
+  hipLaunchKernelGGL(k_normal_splat_count<T>, dim3(blocks), dim3(256), 0, s, (const int*)r.normal_order.data(), G,
+                     (const T*)r.position_sums.data(), (const T*)r.normal_sums.data(), (const T*)r.weights.data(),
+                     (const T*)r.dw.data(), minimum, maximum, block_counts.data());
+  KERNEL_CHECK();
+  scan_exclusive_i32(block_counts.data(), block_offsets.data(), blocks, total.data(), s);
+  const int64_t valid = read_int(total.data(), s);
+  DArray<K> rkeys(valid, s);
+  DArray<int> rids(valid, s), heads(valid, s), prefix(valid, s);
+  hipLaunchKernelGGL((k_normal_splat_keys<T, K>), dim3(blocks), dim3(256), 0, s, (const int*)r.normal_order.data(), G,
+                     (const T*)r.position_sums.data(), (const T*)r.normal_sums.data(), (const T*)r.weights.data(),
+                     (const T*)r.dw.data(), minimum, maximum, (const int*)block_offsets.data(), valid, rkeys.data(),
+                     rids.data());
+  KERNEL_CHECK();
+  sort_pairs(rkeys.data(), rids.data(), valid, bits_for((uint64_t)sentinel), s);
+  mark_heads(rkeys.data(), valid, sentinel, heads.data(), s);
+  scan_exclusive_i32(heads.data(), prefix.data(), valid, total.data(), s);
+  const int count = valid ? read_int(total.data(), s) : 0;
+  DArray<int> head_pos(count, s);
+  compact_heads(heads.data(), prefix.data(), valid, head_pos.data(), s);
+  r.normal_keys.reset(count, s);
+  r.normal_vals.reset(3 * (size_t)count, s);
+  if (count) {
+    hipLaunchKernelGGL((k_normal_splat_reduce<T, K>), dim3(blocks_for(count, 256)), dim3(256), 0, s,
+                       (const K*)rkeys.data(), (const int*)rids.data(), (const int*)head_pos.data(), count, valid,
+                       (const int*)r.normal_order.data(), (const T*)r.position_sums.data(),
+                       (const T*)r.normal_sums.data(), (const T*)r.weights.data(), (const T*)r.dw.data(), minimum,
+                       maximum, r.normal_keys.data(), r.normal_vals.data());
+    KERNEL_CHECK();
+  }
+}
+
 template <class T> void stage_normals(Result<T>& r) {
   hipStream_t s = r.s;
   const int G = r.G;
@@ -616,30 +688,11 @@ template <class T> void stage_normals(Result<T>& r) {
     KERNEL_CHECK();
     sort_pairs_u64(keys.data(), r.normal_order.data(), G, bits_for((uint64_t)r.n), s);
   }
-  const int64_t total_records = 54 * (int64_t)G;
-  DArray<uint64_t> rkeys(total_records, s);
-  DArray<int> rids(total_records, s), heads(total_records, s), prefix(total_records, s), total(1, s);
-  hipLaunchKernelGGL(k_normal_splat_keys<T>, dim3(blocks_for(total_records, 256)), dim3(256), 0, s,
-                     (const int*)r.normal_order.data(), G, (const T*)r.position_sums.data(), (const T*)r.normal_sums.data(),
-                     (const T*)r.weights.data(), (const T*)r.dw.data(), minimum, maximum, rkeys.data(), rids.data());
-  KERNEL_CHECK();
   const uint64_t sentinel = (uint64_t)(maximum + 1) << (3 * maximum);
-  sort_pairs_u64(rkeys.data(), rids.data(), total_records, bits_for(sentinel), s);
-  mark_heads_u64(rkeys.data(), total_records, sentinel, heads.data(), s);
-  scan_exclusive_i32(heads.data(), prefix.data(), total_records, total.data(), s);
-  const int count = read_int(total.data(), s);
-  DArray<int> head_pos(count, s);
-  compact_heads(heads.data(), prefix.data(), total_records, head_pos.data(), s);
-  r.normal_keys.reset(count, s);
-  r.normal_vals.reset(3 * (size_t)count, s);
-  if (count) {
-    hipLaunchKernelGGL(k_normal_splat_reduce<T>, dim3(blocks_for(count, 256)), dim3(256), 0, s,
-                       (const uint64_t*)rkeys.data(), (const int*)rids.data(), (const int*)head_pos.data(), count,
-                       total_records, (const int*)r.normal_order.data(), (const T*)r.position_sums.data(),
-                       (const T*)r.normal_sums.data(), (const T*)r.weights.data(), (const T*)r.dw.data(), minimum,
-                       maximum, r.normal_keys.data(), r.normal_vals.data());
-    KERNEL_CHECK();
-  }
+  if (bits_for(sentinel) <= 32)
+    normal_splats<T, uint32_t>(r, minimum, maximum, (uint32_t)sentinel);
+  else
+    normal_splats<T, uint64_t>(r, minimum, maximum, sentinel);
   // average (normals.py:221-227): contributions here, the serial chain on a
   // side stream (joined before the first use of the average)
   r.average_contrib.reset(3 * (size_t)G, s);
@@ -664,8 +717,8 @@ template <class T> void stage_normals(Result<T>& r) {
   r.average_pending = true;
   // per-depth record offsets (normals.py:17-29, 228-234)
   DArray<int> offsets(maximum + 2, s);
-  hipLaunchKernelGGL(k_depth_offsets, dim3(1), dim3(64), 0, s, (const int64_t*)r.normal_keys.data(), (int64_t)count,
-                     maximum + 2, offsets.data());
+  hipLaunchKernelGGL(k_depth_offsets, dim3(1), dim3(64), 0, s, (const int64_t*)r.normal_keys.data(),
+                     (int64_t)r.normal_keys.size(), maximum + 2, offsets.data());
   KERNEL_CHECK();
   r.normal_offsets.resize(maximum + 2);
   read_bytes(r.normal_offsets.data(), offsets.data(), sizeof(int) * (maximum + 2), s);
