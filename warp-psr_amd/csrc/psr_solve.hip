@@ -12,11 +12,15 @@
 #include "psr_result.h"
 
 #include <algorithm>
+#include <condition_variable>
+#include <exception>
+#include <thread>
 #include <cmath>
 #include <cstdlib>
 #include <map>
 #include <mutex>
 #include <tuple>
+#include <type_traits>
 
 namespace psr {
 
@@ -126,21 +130,23 @@ __global__ void k_dense_fill(const int64_t* __restrict__ keys, int n, int dim, i
   dense[((int64_t)key_z(k) * dim + key_x(k)) * dim + key_y(k)] = i;
 }
 
-__global__ void k_stiffness(const int64_t* __restrict__ rows, int R, const int* __restrict__ row_order,
-                            const int64_t* __restrict__ cols, int64_t C, ColumnTable ct, const int* __restrict__ dense,
-                            int dense_dim, int depth, int trial_depth, const D2* __restrict__ table, int* __restrict__ out_cols,
-                            double* __restrict__ out_vals) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;  // output position
-  if (j >= R) return;
-  const int i = row_order ? row_order[j] : j;
-  const int64_t key = rows[i];
+// One row's 27 stiffness slots: slot k = kx*9 + ky*3 + kz holds the trial
+// function (p + (kx-1, ky-1, kz-1)), p the row's coordinates at trial_depth;
+// emit(k, index, value) receives the trial function's index (-1 if absent)
+// and the entry (0 if absent), slots in order.
+template <class Emit>
+__device__ __forceinline__ void stiffness_row(int64_t key, const int64_t* __restrict__ cols, int64_t C, ColumnTable ct,
+                                              const int* __restrict__ dense, int dense_dim, int depth, int trial_depth,
+                                              const D2* __restrict__ table, Emit&& emit) {
   const int cx = key_x(key), cy = key_y(key), cz = key_z(key);
   const int shift = depth - trial_depth;
   const int px = cx >> shift, py = cy >> shift, pz = cz >> shift;
   const int lim = 1 << trial_depth;
   const int zlo = imax(pz - 1, 0), zhi = imin(pz + 1, lim);
+#pragma unroll
   for (int kx = 0; kx < 3; ++kx) {
     const int qx = px + kx - 1;
+#pragma unroll
     for (int ky = 0; ky < 3; ++ky) {
       const int qy = py + ky - 1;
       int64_t found[3] = {-1, -1, -1};
@@ -151,6 +157,7 @@ __global__ void k_stiffness(const int64_t* __restrict__ rows, int R, const int* 
           find_run_table<3>(cols, C, ct, trial_depth, qx, qy, zlo, zhi - zlo + 1, found);
         }
       }
+#pragma unroll
       for (int kz = 0; kz < 3; ++kz) {
         const int qz = pz + kz - 1;
         const int k = kx * 9 + ky * 3 + kz;
@@ -160,11 +167,23 @@ __global__ void k_stiffness(const int64_t* __restrict__ rows, int R, const int* 
           const D2 x = table[cx * 3 + kx], y = table[cy * 3 + ky], z = table[cz * 3 + kz];
           v = x.v1 * y.v0 * z.v0 + x.v0 * y.v1 * z.v0 + x.v0 * y.v0 * z.v1;
         }
-        out_cols[(int64_t)k * R + j] = (int)index;
-        out_vals[(int64_t)k * R + j] = v;
+        emit(k, index, v);
       }
     }
   }
+}
+
+__global__ void k_stiffness(const int64_t* __restrict__ rows, int R, const int* __restrict__ row_order,
+                            const int64_t* __restrict__ cols, int64_t C, ColumnTable ct, const int* __restrict__ dense,
+                            int dense_dim, int depth, int trial_depth, const D2* __restrict__ table, int* __restrict__ out_cols,
+                            double* __restrict__ out_vals) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;  // output position
+  if (j >= R) return;
+  const int i = row_order ? row_order[j] : j;
+  stiffness_row(rows[i], cols, C, ct, dense, dense_dim, depth, trial_depth, table, [&](int k, int64_t index, double v) {
+    out_cols[(int64_t)k * R + j] = (int)index;
+    out_vals[(int64_t)k * R + j] = v;
+  });
 }
 
 // ---------------------------------------------------------------- constraints (constraints.py:20-84)
@@ -234,18 +253,26 @@ __global__ void k_bin_cells(const uint64_t* __restrict__ bins, int P, int depth,
 // (cell, point) order.  Present slot k holds the trial function
 // (p + (kx-1, ky-1, kz-1)) with p the row's coordinates at trial_depth (the
 // stiffness slot layout), so its hat factors are shared per axis.
+// Sorted point bins of one level: the points of cell (bx, by, bz) are found
+// through the dense z-major cell ranges (depth <= 9), else through the (bx, by)
+// column runs and a search of the sorted bin keys.
+template <class T> struct PointBins {
+  const uint64_t* bin_keys;
+  const int* bin_ids;
+  int P;
+  ColumnTable bin_cols;
+  const int* cell_start;
+  const int* cell_end;
+  const T* pos;
+  const T* w;
+};
+
+// One row's screening sums (27 slots in the stiffness slot layout) and point
+// load, from the points binned in the 8 cells at the row's vertex.  Every
+// sum receives its terms in the reference's (cell, point) order.
 template <class T>
-__global__ void __launch_bounds__(128) k_screening(const int64_t* __restrict__ rows, int R, const int* __restrict__ row_order,
-                                                   int depth, int trial_depth, const uint64_t* __restrict__ bin_keys,
-                                                   const int* __restrict__ bin_ids, int P, ColumnTable bin_cols,
-                                                   const int* __restrict__ cell_start, const int* __restrict__ cell_end,
-                                                   const T* __restrict__ pos, const T* __restrict__ w,
-                                                   const int* __restrict__ scols, double* __restrict__ svals,
-                                                   double* __restrict__ load) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;  // output position
-  if (j >= R) return;
-  const int i = row_order ? row_order[j] : j;
-  const int64_t key = rows[i];
+__device__ __forceinline__ void screening_row(int64_t key, int depth, int trial_depth, const PointBins<T>& pb,
+                                              double (&acc)[27], double& rhs) {
   const int cx = key_x(key), cy = key_y(key), cz = key_z(key);
   const int n = 1 << depth;
   // point ranges of the 8 cells at the vertex, in the reference's cell order
@@ -260,56 +287,55 @@ __global__ void __launch_bounds__(128) k_screening(const int64_t* __restrict__ r
       rb[k0] = re[k0] = rb[k0 + 1] = re[k0 + 1] = 0;
       const int bx = cx - a, by = cy - bb;
       if (bx < 0 || by < 0 || bx >= n || by >= n) continue;
-      if (cell_start) {  // dense z-major cell ranges
+      if (pb.cell_start) {  // dense z-major cell ranges
         if (cz >= 1) {
           const int64_t c = ((int64_t)(cz - 1) * n + bx) * n + by;
-          rb[k0 + 1] = cell_start[c];
-          re[k0 + 1] = cell_end[c];
+          rb[k0 + 1] = pb.cell_start[c];
+          re[k0 + 1] = pb.cell_end[c];
         }
         if (cz < n) {
           const int64_t c = ((int64_t)cz * n + bx) * n + by;
-          rb[k0] = cell_start[c];
-          re[k0] = cell_end[c];
+          rb[k0] = pb.cell_start[c];
+          re[k0] = pb.cell_end[c];
         }
         continue;
       }
       const uint64_t col = ((uint64_t)bx << (2 * depth)) | ((uint64_t)by << depth);
       const int zfirst = cz >= 1 ? cz - 1 : cz;
-      int lo = 0, hi = P;
-      if (bin_cols.start) {
-        lo = bin_cols.start[bx * n + by];
-        hi = bin_cols.end[bx * n + by];
+      int lo = 0, hi = pb.P;
+      if (pb.bin_cols.start) {
+        lo = pb.bin_cols.start[bx * n + by];
+        hi = pb.bin_cols.end[bx * n + by];
       }
       const int stop = hi;
       const uint64_t first_key = col | (uint64_t)zfirst;
       while (lo < hi) {
         const int mid = (lo + hi) >> 1;
-        if (bin_keys[mid] < first_key) lo = mid + 1; else hi = mid;
+        if (pb.bin_keys[mid] < first_key) lo = mid + 1; else hi = mid;
       }
       if (cz >= 1) {
         rb[k0 + 1] = lo;
-        while (lo < stop && bin_keys[lo] == (col | (uint64_t)(cz - 1))) ++lo;
+        while (lo < stop && pb.bin_keys[lo] == (col | (uint64_t)(cz - 1))) ++lo;
         re[k0 + 1] = lo;
       }
       if (cz < n) {
         rb[k0] = lo;
-        while (lo < stop && bin_keys[lo] == (col | (uint64_t)cz)) ++lo;
+        while (lo < stop && pb.bin_keys[lo] == (col | (uint64_t)cz)) ++lo;
         re[k0] = lo;
       }
     }
   const int shift = depth - trial_depth;
   const int px = (cx >> shift) - 1, py = (cy >> shift) - 1, pz = (cz >> shift) - 1;
   const T st = T(1 << trial_depth);
-  double acc[27];
 #pragma unroll
   for (int k = 0; k < 27; ++k) acc[k] = 0.0;
-  double rhs = 0.0;
+  rhs = 0.0;
   for (int kk = 0; kk < 8; ++kk)
     for (int jj = rb[kk]; jj < re[kk]; ++jj) {
-      const int index = bin_ids[jj];
-      const T weight = w[index];
+      const int index = pb.bin_ids[jj];
+      const T weight = pb.w[index];
       if (weight != T(0)) {
-        const T p0 = pos[3 * index], p1 = pos[3 * index + 1], p2 = pos[3 * index + 2];
+        const T p0 = pb.pos[3 * index], p1 = pb.pos[3 * index + 1], p2 = pb.pos[3 * index + 2];
         const T phi = hat3(p0, p1, p2, cx, cy, cz, depth);
         rhs += double(phi * weight);
         T hx[3], hy[3], hz[3];
@@ -329,10 +355,77 @@ __global__ void __launch_bounds__(128) k_screening(const int64_t* __restrict__ r
         }
       }
     }
+}
+
+// screening_stencils.py:34-70 (add): per row, the screening sums added to
+// the present slots of an assembled stencil, and the point load.
+template <class T>
+__global__ void __launch_bounds__(128) k_screening(const int64_t* __restrict__ rows, int R, const int* __restrict__ row_order,
+                                                   int depth, int trial_depth, PointBins<T> pb,
+                                                   const int* __restrict__ scols, double* __restrict__ svals,
+                                                   double* __restrict__ load) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;  // output position
+  if (j >= R) return;
+  const int i = row_order ? row_order[j] : j;
+  double acc[27], rhs;
+  screening_row(rows[i], depth, trial_depth, pb, acc, rhs);
   if (load) load[j] = rhs;
 #pragma unroll
   for (int k = 0; k < 27; ++k)
     if (scols[(int64_t)k * R + j] >= 0) svals[(int64_t)k * R + j] += acc[k];
+}
+
+// stiffness.py:38-109 with screening_stencils.py:34-70 for the sliced finer
+// levels, in one pass per schedule position j: the 27 stiffness entries,
+// plus (SCREEN) the row's screening sums, rounded once to the stored type V
+// -- the same value the separate stiffness, screening and cast passes give.
+// For the level's own operator (inv != nullptr) the columns are also written
+// as schedule positions, with the diagonal factors 1/A_ii and the count of
+// present slots taken from the same registers.
+template <class T, class V, bool SCREEN>
+__global__ void __launch_bounds__(128) k_level_operator(const int64_t* __restrict__ rows, int R, const int* __restrict__ row_order,
+                                                        const int64_t* __restrict__ cols, int64_t C, ColumnTable ct,
+                                                        const int* __restrict__ dense, int dense_dim, int depth, int trial_depth,
+                                                        const D2* __restrict__ table, PointBins<T> pb, double* __restrict__ load,
+                                                        int* __restrict__ out_cols, V* __restrict__ out_vals,
+                                                        const int* __restrict__ inv, int* __restrict__ sched_cols,
+                                                        T* __restrict__ inv_diag, unsigned char* __restrict__ has_diag,
+                                                        unsigned long long* __restrict__ present) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;  // output position
+  int count = 0;
+  if (j < R) {
+    const int i = row_order[j];
+    const int64_t key = rows[i];
+    double acc[27], rhs = 0.0;
+    if (SCREEN) {
+      screening_row(key, depth, trial_depth, pb, acc, rhs);
+      if (load) load[j] = rhs;
+    }
+    T diagonal = T(0);
+    stiffness_row(key, cols, C, ct, dense, dense_dim, depth, trial_depth, table, [&](int k, int64_t index, double v) {
+      const bool found = index >= 0;
+      const V stored = V(found && SCREEN ? v + acc[k] : v);
+      out_cols[(int64_t)k * R + j] = (int)index;
+      out_vals[(int64_t)k * R + j] = stored;
+      if (inv) {
+        sched_cols[(int64_t)k * R + j] = found ? inv[index] : -1;
+        if (index == i) diagonal = T(stored);
+        count += found ? 1 : 0;
+      }
+    });
+    if (inv) {
+      has_diag[j] = diagonal != T(0);
+      inv_diag[j] = diagonal != T(0) ? T(1) / diagonal : T(0);
+    }
+  }
+  if (present) {
+    __shared__ unsigned long long block_count;
+    if (threadIdx.x == 0) block_count = 0;
+    __syncthreads();
+    if (count) atomicAdd(&block_count, (unsigned long long)count);
+    __syncthreads();
+    if (threadIdx.x == 0 && block_count) atomicAdd(present, block_count);
+  }
 }
 
 template <class T>
@@ -1331,9 +1424,11 @@ template <class T> struct PointSet {
   DArray<int> cell_start, cell_end;  // dense z-major cell ranges (depth <= 9)
 };
 
-template <class T> static PointSet<T> point_constraints(Result<T>& r, int d, hipStream_t s) {
+// join: order s after the serial normal average (the weights use it); a
+// caller on another stream orders that stream itself.
+template <class T> static PointSet<T> point_constraints(Result<T>& r, int d, hipStream_t s, bool join = true) {
   PointSet<T> ps;
-  r.join_average();  // the weights use the serial normal average
+  if (join) r.join_average();
   LevelState<T>& L = r.levels[d];
   ps.P = (int)L.agg_keys.size();
   ps.pos.reset(3 * (size_t)ps.P, s);
@@ -1371,16 +1466,19 @@ template <class T> static PointSet<T> point_constraints(Result<T>& r, int d, hip
   return ps;
 }
 
+template <class T> static PointBins<T> point_bins(const PointSet<T>& ps) {
+  return PointBins<T>{(const uint64_t*)ps.bins.data(), (const int*)ps.ids.data(), ps.P, ps.columns.view(),
+                      ps.cell_start.empty() ? (const int*)nullptr : (const int*)ps.cell_start.data(),
+                      ps.cell_end.empty() ? (const int*)nullptr : (const int*)ps.cell_end.data(), (const T*)ps.pos.data(),
+                      (const T*)ps.w.data()};
+}
+
 template <class T>
 static void add_screening(Stencil<double>& m, const DArray<int64_t>& rows, const PointSet<T>& ps, int depth, int trial_depth,
                           double* load, hipStream_t s, const int* row_order = nullptr) {
   if (!m.rows) return;
   hipLaunchKernelGGL(k_screening<T>, dim3(blocks_for(m.rows, 128)), dim3(128), 0, s, (const int64_t*)rows.data(), m.rows,
-                     row_order, depth, trial_depth, (const uint64_t*)ps.bins.data(), (const int*)ps.ids.data(), ps.P,
-                     ps.columns.view(), ps.cell_start.empty() ? (const int*)nullptr : (const int*)ps.cell_start.data(),
-                     ps.cell_end.empty() ? (const int*)nullptr : (const int*)ps.cell_end.data(), (const T*)ps.pos.data(),
-                     (const T*)ps.w.data(),
-                     (const int*)m.cols.data(), m.vals.data(), load);
+                     row_order, depth, trial_depth, point_bins(ps), (const int*)m.cols.data(), m.vals.data(), load);
   KERNEL_CHECK();
 }
 
@@ -1396,6 +1494,48 @@ template <class T> static Stencil<T> cast_stencil(Stencil<double>&& m, hipStream
     KERNEL_CHECK();
   }
   return out;
+}
+
+// A sliced finer level's operator (colsL = rowsL, trial_depth = depth; op and
+// present given) or cross operator (colsL one level coarser) in one pass of
+// k_level_operator, rows in schedule order ord (inv = its inverse).
+template <class T, class V>
+static void level_operator(const LevelState<T>& rowsL, const LevelState<T>& colsL, int depth, int trial_depth,
+                           const PointSet<T>* pts, double* load, const int* ord, const int* inv, Stencil<V>& out,
+                           GsOperator<T, V>* op, unsigned long long* present, hipStream_t s) {
+  const int R = rowsL.basis_count;
+  out.rows = R;
+  out.cols_count = colsL.basis_count;
+  out.cols.reset(27 * (size_t)R, s);
+  out.vals.reset(27 * (size_t)R, s);
+  if (op) {
+    op->cols.reset(27 * (size_t)R, s);
+    op->inv_diag.reset(R, s);
+    op->has_diag.reset(R, s);
+  }
+  if (!R) return;
+  const int res = 1 << depth;
+  DArray<D2> table((res + 1) * 3, s);
+  hipLaunchKernelGGL(k_hat_table, dim3(blocks_for((res + 1) * 3, 256)), dim3(256), 0, s, depth, trial_depth, table.data());
+  KERNEL_CHECK();
+  const PointBins<T> pb = pts ? point_bins(*pts) : PointBins<T>{};
+  const int* dense = colsL.dense_dim ? (const int*)colsL.dense.data() : (const int*)nullptr;
+  int* sched = op ? op->cols.data() : nullptr;
+  T* inv_diag = op ? op->inv_diag.data() : nullptr;
+  unsigned char* has_diag = op ? op->has_diag.data() : nullptr;
+  if (pts)
+    hipLaunchKernelGGL((k_level_operator<T, V, true>), dim3(blocks_for(R, 128)), dim3(128), 0, s,
+                       (const int64_t*)rowsL.basis_keys.data(), R, ord, (const int64_t*)colsL.basis_keys.data(),
+                       (int64_t)colsL.basis_count, colsL.columns.view(), dense, colsL.dense_dim, depth, trial_depth,
+                       (const D2*)table.data(), pb, load, out.cols.data(), out.vals.data(), inv, sched, inv_diag, has_diag,
+                       present);
+  else
+    hipLaunchKernelGGL((k_level_operator<T, V, false>), dim3(blocks_for(R, 128)), dim3(128), 0, s,
+                       (const int64_t*)rowsL.basis_keys.data(), R, ord, (const int64_t*)colsL.basis_keys.data(),
+                       (int64_t)colsL.basis_count, colsL.columns.view(), dense, colsL.dense_dim, depth, trial_depth,
+                       (const D2*)table.data(), pb, load, out.cols.data(), out.vals.data(), inv, sched, inv_diag, has_diag,
+                       present);
+  KERNEL_CHECK();
 }
 
 template <class T>
@@ -1597,70 +1737,69 @@ template <class T> static void regular_multigrid(Result<T>& r, std::vector<DArra
   B.correction = std::move(x[base]);
 }
 
-// One finer level (solver.py:343-360): operator, cross operator and
-// right-hand side assembled directly in sliced schedule order (row j =
-// order[j]), operators stored with value type V (T, or float in mixed runs),
-// b -= cross * cumulative, sliced Gauss-Seidel from zero, then the cumulative
-// solution is prolonged and accumulated.
+// Everything of one finer level that does not depend on the coarser levels'
+// solution: the sliced schedule, the operator and the cross operator with
+// their screening terms, stored with value type V (T, or float in mixed
+// runs) in schedule order (row j = order[j]), the point load, the diagonal
+// factors and the operator's columns as schedule positions.
+template <class T, class V> struct FinerLevel {
+  Schedule<T> sc;
+  GsOperator<T, V> op;  // cols (schedule positions), vals, inv_diag, has_diag; b is set by the solve
+  DArray<unsigned long long> nnz;  // stored nonzeros, read after the run
+};
+
 template <class T, class V>
-static void solve_finer(Result<T>& r, int d, Stencil<V>& Aop, Stencil<V>& Cop, bool screened, DArray<T>& cumulative,
+static void prepare_finer(Result<T>& r, int d, Stencil<V>& Aop, Stencil<V>& Cop, bool screened, FinerLevel<T, V>& f,
+                          hipStream_t s) {
+  LevelState<T>& L = r.levels[d];
+  const int R = L.basis_count;
+  f.sc = make_schedule<T>(L.basis_keys, R, d, true, s);
+  const int* ord = f.sc.order.data();
+  L.point_load.reset(R, s);
+  if (!screened) L.point_load.zero();
+  PointSet<T> pts;
+  if (screened) pts = point_constraints(r, d, s, false);
+  // x is relaxed in schedule order: columns become schedule positions, so
+  // a phase writes contiguous ranges and gathers from nearby ranges
+  DArray<int> inv(R, s);
+  hipLaunchKernelGGL(k_inverse_permutation, dim3(blocks_for(R, 256)), dim3(256), 0, s, ord, R, inv.data());
+  KERNEL_CHECK();
+  f.nnz.reset(1, s);
+  f.nnz.zero();
+  GsOperator<T, V>& op = f.op;
+  op.R = R;
+  level_operator<T, V>(L, L, d, d, screened ? &pts : nullptr, L.point_load.data(), ord, inv.data(), Aop, &op, f.nnz.data(), s);
+  op.vals = std::move(Aop.vals);
+  level_operator<T, V>(L, r.levels[d - 1], d, d - 1, screened ? &pts : nullptr, (double*)nullptr, ord, (const int*)nullptr,
+                       Cop, (GsOperator<T, V>*)nullptr, (unsigned long long*)nullptr, s);
+}
+
+// One finer level's solve (solver.py:343-360): right-hand side, b -= cross *
+// cumulative, sliced Gauss-Seidel from zero, then the cumulative solution is
+// prolonged and accumulated.
+template <class T, class V>
+static void solve_finer(Result<T>& r, int d, Stencil<V>& Aop, Stencil<V>& Cop, FinerLevel<T, V>& f, DArray<T>& cumulative,
                         hipStream_t s) {
   LevelState<T>& L = r.levels[d];
   const int R = L.basis_count;
-  PointSet<T> pts;
-  Schedule<T> sc = make_schedule<T>(L.basis_keys, R, d, true, s);
-  const int* ord = sc.order.data();
-  Stencil<double> A = assemble_stiffness(L, L, d, d, s, ord);
-  if (screened) {
-    pts = point_constraints(r, d, s);
-    add_screening(A, L.basis_keys, pts, d, d, L.point_load.data(), s, ord);
-  }
+  const int* ord = f.sc.order.data();
+  GsOperator<T, V>& op = f.op;
+  L.rhs.reset(R, s);
   hipLaunchKernelGGL(k_rhs_sched<T>, dim3(blocks_for(R, 256)), dim3(256), 0, s, (const double*)L.normal_load.data(),
                      (const double*)L.point_load.data(), ord, R, L.rhs.data());
   KERNEL_CHECK();
-  Aop = cast_stencil<V>(std::move(A), s);
-  Stencil<double> cross = assemble_stiffness(L, r.levels[d - 1], d, d - 1, s, ord);
-  if (screened) add_screening(cross, L.basis_keys, pts, d, d - 1, (double*)nullptr, s, ord);
-  Cop = cast_stencil<V>(std::move(cross), s);
   hipLaunchKernelGGL((k_spmv<T, V>), dim3(blocks_for(R, 256)), dim3(256), 0, s, (const int*)Cop.cols.data(),
                      (const V*)Cop.vals.data(), R, 27, (const T*)cumulative.data(), L.rhs.data(), T(-1), T(1));
   KERNEL_CHECK();
-  GsOperator<T, V> op;
-  op.R = R;
-  op.vals = std::move(Aop.vals);
   op.b = std::move(L.rhs);
-  op.inv_diag.reset(R, s);
-  op.has_diag.reset(R, s);
-  hipLaunchKernelGGL((k_diag_sched<T, V>), dim3(blocks_for(R, 256)), dim3(256), 0, s, ord, R, (const int*)Aop.cols.data(),
-                     (const V*)op.vals.data(), op.inv_diag.data(), op.has_diag.data());
-  KERNEL_CHECK();
-  // x is relaxed in schedule order: columns become schedule positions, so
-  // a phase writes contiguous ranges and gathers from nearby ranges
   DArray<T> xs(R, s);
   xs.zero();  // the level's correction starts at zero (solver.py:343)
-  {
-    DArray<int> inv(R, s);
-    hipLaunchKernelGGL(k_inverse_permutation, dim3(blocks_for(R, 256)), dim3(256), 0, s, ord, R, inv.data());
-    KERNEL_CHECK();
-    op.cols.reset(27 * (size_t)R, s);
-    hipLaunchKernelGGL(k_remap_columns, dim3(blocks_for(27 * (int64_t)R, 256)), dim3(256), 0, s, (const int*)Aop.cols.data(),
-                       (const int*)inv.data(), 27 * (int64_t)R, op.cols.data());
-    KERNEL_CHECK();
-  }
+  r.relax_nnz.push_back(std::move(f.nnz));
   typename Result<T>::RelaxStat st{d, R, 0, 0, nullptr, nullptr, 0.f};
-  {  // stored nonzeros, read after the run (no synchronisation here)
-    DArray<unsigned long long> cnt(1, s);
-    cnt.zero();
-    const int64_t slots = 27 * (int64_t)R;
-    const unsigned grid = (unsigned)std::min<int64_t>(1024, (slots + 255) / 256);
-    hipLaunchKernelGGL(k_count_present, dim3(grid), dim3(256), 0, s, (const int*)op.cols.data(), slots, cnt.data());
-    KERNEL_CHECK();
-    r.relax_nnz.push_back(std::move(cnt));
-  }
   HIP_CHECK(hipEventCreate(&st.start));
   HIP_CHECK(hipEventCreate(&st.end));
   HIP_CHECK(hipEventRecord(st.start, s));
-  st.launches = relax_sliced_any(op, sc, xs.data(), d, r.prm.iters, s);
+  st.launches = relax_sliced_any(op, f.sc, xs.data(), d, r.prm.iters, s);
   HIP_CHECK(hipEventRecord(st.end, s));
   r.relax_stats.push_back(st);
   hipLaunchKernelGGL(k_scatter_rows<T>, dim3(blocks_for(R, 256)), dim3(256), 0, s, ord, R, (const T*)xs.data(),
@@ -1669,7 +1808,7 @@ static void solve_finer(Result<T>& r, int d, Stencil<V>& Aop, Stencil<V>& Cop, b
   // keep the schedule-ordered operator (canonical columns) and right-hand side
   Aop.vals = std::move(op.vals);
   L.rhs = std::move(op.b);
-  L.order = std::move(sc.order);
+  L.order = std::move(f.sc.order);
   L.scheduled = true;
   DArray<T> next(R, s);
   HIP_CHECK(hipMemcpyAsync(next.data(), L.correction.data(), sizeof(T) * R, hipMemcpyDeviceToDevice, s));
@@ -1679,9 +1818,182 @@ static void solve_finer(Result<T>& r, int d, Stencil<V>& Aop, Stencil<V>& Cop, b
   cumulative = std::move(next);
 }
 
+// The finer levels' operators: A and cross, or their float32 copies in
+// mixed runs.
+template <class V, class T> static Stencil<V>& finer_operator(LevelState<T>& L, bool cross) {
+  if constexpr (std::is_same<V, T>::value)
+    return cross ? L.cross : L.A;
+  else
+    return cross ? L.cross32 : L.A32;
+}
+
+// One assembly stream per (device, solve stream), kept for the process: its
+// pool blocks and the host thread's launches are reused by every call.
+static hipStream_t assembly_stream(int dev, hipStream_t s) {
+  static std::mutex mu;
+  static std::map<std::pair<int, hipStream_t>, hipStream_t> streams;
+  std::lock_guard<std::mutex> g(mu);
+  hipStream_t& as = streams[{dev, s}];
+  if (!as) HIP_CHECK(hipStreamCreateWithFlags(&as, hipStreamNonBlocking));
+  return as;
+}
+
+// Builds the finer levels' operators on the assembly stream from a host
+// thread while the solve stream runs the base solve and the coarser levels'
+// relaxation (which leave most of the device idle).  Level d's arrays are
+// allocated on the assembly stream, used by the solve stream after it waits
+// on level d's event, and returned to the assembly stream's pool only after
+// the thread has finished (the destructor), so the thread never reuses a
+// block the solve stream still reads.
+template <class T, class V> class FinerAssembly {
+ public:
+  FinerAssembly(Result<T>& r, int first, int last, bool screened, hipStream_t s)
+      : r_(r), first_(first), last_(last), s_(s), levels_(std::max(0, last - first + 1)), events_(levels_.size()) {
+    if (levels_.empty()) return;
+    int dev = 0;
+    HIP_CHECK(hipGetDevice(&dev));
+    as_ = assembly_stream(dev, s);
+    for (auto& e : events_) HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    hipEvent_t fork;
+    HIP_CHECK(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+    HIP_CHECK(hipEventRecord(fork, s));
+    HIP_CHECK(hipStreamWaitEvent(as_, fork, 0));
+    HIP_CHECK(hipEventDestroy(fork));
+    // the point constraints' weights use the serial normal average
+    if (r.average_pending) HIP_CHECK(hipStreamWaitEvent(as_, r.average_done, 0));
+    HIP_CHECK(hipEventCreateWithFlags(&gate_, hipEventDisableTiming));
+    thread_ = std::thread([this, dev, screened] { run(dev, screened); });
+  }
+  FinerAssembly(const FinerAssembly&) = delete;
+  FinerAssembly& operator=(const FinerAssembly&) = delete;
+  ~FinerAssembly() {
+    if (!thread_.joinable()) return;
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      cancel_ = true;
+      cv_.notify_all();
+    }
+    thread_.join();
+    hipEvent_t end;
+    if (hipEventCreateWithFlags(&end, hipEventDisableTiming) == hipSuccess) {
+      (void)hipEventRecord(end, as_);
+      (void)hipStreamWaitEvent(s_, end, 0);
+      (void)hipEventDestroy(end);
+    }
+    for (auto& e : events_) (void)hipEventDestroy(e);
+    (void)hipEventDestroy(gate_);
+  }
+  // Levels after the first two are assembled only once the solve stream has
+  // passed this point (the base solve's single-workgroup kernels slow down
+  // several times when they share compute units with the assembly).
+  void open_gate() {
+    if (!thread_.joinable()) return;
+    HIP_CHECK(hipEventRecord(gate_, s_));
+    std::lock_guard<std::mutex> g(mu_);
+    gate_open_ = true;
+    cv_.notify_all();
+  }
+  // Level d's operators, once the solve stream is ordered after them.
+  FinerLevel<T, V>& wait(int d) {
+    const int k = d - first_;
+    std::unique_lock<std::mutex> g(mu_);
+    cv_.wait(g, [&] { return ready_ > k || error_ || done_; });
+    if (ready_ <= k) {
+      if (error_) std::rethrow_exception(error_);
+      throw PsrError(1, "finer level operators were not assembled");
+    }
+    g.unlock();
+    HIP_CHECK(hipStreamWaitEvent(s_, events_[k], 0));
+    return levels_[k];
+  }
+
+ private:
+  void run(int dev, bool screened) {
+    try {
+      HIP_CHECK(hipSetDevice(dev));
+      for (int d = first_; d <= last_; ++d) {
+        {
+          std::lock_guard<std::mutex> g(mu_);
+          if (cancel_) break;
+        }
+        if (d == first_ + 2) {
+          std::unique_lock<std::mutex> g(mu_);
+          cv_.wait(g, [&] { return gate_open_ || cancel_; });
+          if (cancel_) break;
+          g.unlock();
+          HIP_CHECK(hipStreamWaitEvent(as_, gate_, 0));
+        }
+        LevelState<T>& L = r_.levels[d];
+        if (!L.basis_count) break;
+        prepare_finer<T, V>(r_, d, finer_operator<V>(L, false), finer_operator<V>(L, true), screened,
+                            levels_[d - first_], as_);
+        HIP_CHECK(hipEventRecord(events_[d - first_], as_));
+        std::lock_guard<std::mutex> g(mu_);
+        ready_ = d - first_ + 1;
+        cv_.notify_all();
+      }
+    } catch (...) {
+      std::lock_guard<std::mutex> g(mu_);
+      error_ = std::current_exception();
+    }
+    std::lock_guard<std::mutex> g(mu_);
+    done_ = true;
+    cv_.notify_all();
+  }
+
+  Result<T>& r_;
+  int first_, last_;
+  hipStream_t s_, as_ = nullptr;
+  hipEvent_t gate_ = nullptr;
+  std::vector<FinerLevel<T, V>> levels_;
+  std::vector<hipEvent_t> events_;
+  std::thread thread_;
+  std::mutex mu_;
+  std::condition_variable cv_;
+  int ready_ = 0;
+  bool cancel_ = false, done_ = false, gate_open_ = false;
+  std::exception_ptr error_;
+};
+
+// Normal loads, then the cascadic solve from the base level up
+// (solver.py:332-360); the finer levels' operators are assembled alongside.
+template <class T, class V>
+static void solve_levels(Result<T>& r, std::vector<DArray<int>>& restr, bool screened, hipStream_t s) {
+  const int base = r.prm.base_depth;
+  FinerAssembly<T, V> finer(r, base + 1, r.prm.solve_depth, screened, s);
+  normal_loads(r, restr, s);
+  DArray<T> cumulative;
+  for (int d = base; d <= r.prm.solve_depth; ++d) {
+    LevelState<T>& L = r.levels[d];
+    const int R = L.basis_count;
+    if (!R) break;
+    if (d == base) {
+      L.point_load.reset(R, s);
+      L.point_load.zero();
+      L.rhs.reset(R, s);
+      Stencil<double> A = assemble_stiffness(L, L, d, d, s);
+      if (screened) {
+        PointSet<T> pts = point_constraints(r, d, s);
+        add_screening(A, L.basis_keys, pts, d, d, L.point_load.data(), s);
+      }
+      hipLaunchKernelGGL(k_rhs<T>, dim3(blocks_for(R, 256)), dim3(256), 0, s, (const double*)L.normal_load.data(),
+                         (const double*)L.point_load.data(), R, L.rhs.data());
+      KERNEL_CHECK();
+      L.A = cast_stencil<T>(std::move(A), s);
+      regular_multigrid(r, restr, s);
+      cumulative.reset(R, s);
+      HIP_CHECK(hipMemcpyAsync(cumulative.data(), L.correction.data(), sizeof(T) * R, hipMemcpyDeviceToDevice, s));
+      finer.open_gate();
+    } else {
+      solve_finer<T, V>(r, d, finer_operator<V>(L, false), finer_operator<V>(L, true), finer.wait(d), cumulative, s);
+    }
+    L.solved = true;
+  }
+}
+
 template <class T> void stage_solve(Result<T>& r) {
   hipStream_t s = r.s;
-  const int D = r.prm.depth, base = r.prm.base_depth;
+  const int D = r.prm.depth;
   for (int d = 0; d <= D; ++d) {
     LevelState<T>& L = r.levels[d];
     build_column_index((const int64_t*)L.basis_keys.data(), L.basis_count, (1 << d) + 1, L.columns, s);
@@ -1723,36 +2035,11 @@ template <class T> void stage_solve(Result<T>& r) {
     L.correction.reset(L.basis_count, s);
     L.correction.zero();
   }
-  normal_loads(r, restr, s);
   const bool screened = r.prm.point_weight > 0 && !r.prm.exact_interpolation;
-  DArray<T> cumulative;
-  for (int d = base; d <= r.prm.solve_depth; ++d) {
-    LevelState<T>& L = r.levels[d];
-    const int R = L.basis_count;
-    if (!R) break;
-    L.point_load.reset(R, s);
-    L.point_load.zero();
-    L.rhs.reset(R, s);
-    if (d == base) {
-      Stencil<double> A = assemble_stiffness(L, L, d, d, s);
-      if (screened) {
-        PointSet<T> pts = point_constraints(r, d, s);
-        add_screening(A, L.basis_keys, pts, d, d, L.point_load.data(), s);
-      }
-      hipLaunchKernelGGL(k_rhs<T>, dim3(blocks_for(R, 256)), dim3(256), 0, s, (const double*)L.normal_load.data(),
-                         (const double*)L.point_load.data(), R, L.rhs.data());
-      KERNEL_CHECK();
-      L.A = cast_stencil<T>(std::move(A), s);
-      regular_multigrid(r, restr, s);
-      cumulative.reset(R, s);
-      HIP_CHECK(hipMemcpyAsync(cumulative.data(), L.correction.data(), sizeof(T) * R, hipMemcpyDeviceToDevice, s));
-    } else if (r.prm.float_operators) {
-      solve_finer<T, float>(r, d, L.A32, L.cross32, screened, cumulative, s);
-    } else {
-      solve_finer<T, T>(r, d, L.A, L.cross, screened, cumulative, s);
-    }
-    L.solved = true;
-  }
+  if (r.prm.float_operators)
+    solve_levels<T, float>(r, restr, screened, s);
+  else
+    solve_levels<T, T>(r, restr, screened, s);
   for (int d = 0; d < D; ++d) r.levels[d].restriction = std::move(restr[d]);
   for (int d = 0; d <= D; ++d) {
     r.levels[d].dense.free();
