@@ -18,6 +18,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <tuple>
 #include <type_traits>
@@ -749,12 +750,8 @@ k_relax_wave(const int* __restrict__ order, const int* __restrict__ goff, const 
   if (j < goff[group + 1]) gs_update(j, R, order, cols, vals, b, inv_diag, has_diag, x);
 }
 
-// Same launch with its arguments read from a device descriptor block, so one
-// captured graph of the whole launch sequence serves every call of a shape.
-// The block holds the buffer pointers and, at RELAX_GOFF_OFFSET, a copy of the
-// group offsets: both addresses follow from the kernel argument alone, so the
-// two loads are issued together.
-template <class T, class V> struct RelaxDesc {
+// Buffers of one sliced relaxation, passed by value in every launch.
+template <class T, class V> struct RelaxArgs {
   const int* order;
   const int* cols;
   const V* vals;
@@ -764,26 +761,22 @@ template <class T, class V> struct RelaxDesc {
   T* x;
   int R;
 };
-constexpr size_t RELAX_GOFF_OFFSET = 256;
 
-// Writes a descriptor block on the stream: the descriptor arrives as a kernel
-// argument, so no host buffer has to outlive the call.
-template <class T, class V>
-__global__ void k_write_relax_desc(char* __restrict__ block, RelaxDesc<T, V> d, const int* __restrict__ goff, int count) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) *(RelaxDesc<T, V>*)block = d;
-  int* dst = (int*)(block + RELAX_GOFF_OFFSET);
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) dst[i] = goff[i];
-}
+// Row ranges of one launch's slices (a wavefront holds at most `iters`
+// slices, two apart), in schedule positions: slice t is [start[t], end[t]).
+constexpr int RELAX_MAX_SLICES = 16;
+struct SliceRanges {
+  int start[RELAX_MAX_SLICES], end[RELAX_MAX_SLICES];
+};
 
+// The same launch with everything it needs in its arguments: the first loads
+// of a row depend on nothing but the kernel argument segment.
 template <class T, class V, int BS>
 __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(1, 4)))
-k_relax_wave_graph(const char* __restrict__ block, int first_z, int color) {
-  const int* goff = (const int*)(block + RELAX_GOFF_OFFSET);
-  const int group = 4 * (first_z + 2 * (int)blockIdx.y) + color;
-  const int start = goff[group], end = goff[group + 1];
-  const RelaxDesc<T, V> d = *(const RelaxDesc<T, V>*)block;
-  const int j = start + (int)blockIdx.x * BS + (int)threadIdx.x;
-  if (j < end) gs_update(j, d.R, d.order, d.cols, d.vals, d.b, d.inv_diag, d.has_diag, d.x);
+k_relax_wave_args(RelaxArgs<T, V> d, SliceRanges rg) {
+  const int t = blockIdx.y;
+  const int j = rg.start[t] + (int)blockIdx.x * BS + (int)threadIdx.x;
+  if (j < rg.end[t]) gs_update(j, d.R, d.order, d.cols, d.vals, d.b, d.inv_diag, d.has_diag, d.x);
 }
 
 // relaxation.py:261-271 (8-colour, all sweeps) inside one workgroup: the
@@ -1283,73 +1276,99 @@ int relax_sliced(const GsOperator<T, V>& op, const Schedule<T>& sc, T* x, int de
   return launches;
 }
 
-// Captured launch sequence of relax_sliced, cached per (precision, depth,
-// iters, row capacity, device, caller stream): the graph reads its buffers
-// through a descriptor that each call rewrites (stream-ordered) before
-// launching it.  Capture happens on a private stream (the default stream
-// cannot be captured); the graph then runs on the caller's stream.
+// The launch sequence of relax_sliced as a graph of kernel nodes, cached per
+// (precision, depth, iters, row capacity, device, caller stream).  Each call
+// waits for the previous launch of the graph to finish (normally long done),
+// writes its buffers and slice ranges into every node's arguments
+// (hipGraphExecKernelNodeSetParams) and launches the graph on the caller's
+// stream.
 struct RelaxGraph {
+  hipGraph_t graph = nullptr;  // kept: the node handles below belong to it
   hipGraphExec_t exec = nullptr;
-  void* desc = nullptr;
-  int launches = 0;
+  std::vector<hipGraphNode_t> nodes;
+  std::vector<int> first_z, slices, color;  // per node
+  hipKernelNodeParams params{};
+  hipEvent_t done = nullptr;  // completion of the last launch
+  std::mutex mu;              // one call at a time writes the arguments and launches
 };
-
-template <class T, class V, int BS> static void capture_relax_launches(hipStream_t cs, const char* desc, int n, int iters, unsigned bx,
-                                                             int& launches) {
-  for (int q = 0; q <= n + 2 * (iters - 1); ++q) {
-    const int first = std::max(0, floordiv(q - n + 1, 2));
-    const int last = std::min(iters - 1, q / 2);
-    if (last < first) continue;
-    const int z0 = n - q + 2 * first;
-    const unsigned count = (unsigned)(last - first + 1);
-    for (int color = 0; color < 4; ++color) {
-      hipLaunchKernelGGL((k_relax_wave_graph<T, V, BS>), dim3(bx, count), dim3(BS), 0, cs, desc, z0, color);
-      ++launches;
-    }
-  }
-}
 
 // 64-row workgroups spread one launch's rows over the most CUs (a launch holds
 // at most a few thousand rows per slice; occupancy does not bound it).
 constexpr int RELAX_GRAPH_BS = 64;
 
 template <class T, class V>
+static std::unique_ptr<RelaxGraph> build_relax_graph(int n, int iters, unsigned bx) {
+  auto g = std::make_unique<RelaxGraph>();
+  HIP_CHECK(hipGraphCreate(&g->graph, 0));
+  hipGraph_t graph = g->graph;
+  RelaxArgs<T, V> d{};
+  SliceRanges rg{};
+  void* args[2] = {&d, &rg};
+  hipKernelNodeParams& kp = g->params;
+  kp.func = reinterpret_cast<void*>(&k_relax_wave_args<T, V, RELAX_GRAPH_BS>);
+  kp.blockDim = dim3(RELAX_GRAPH_BS);
+  kp.sharedMemBytes = 0;
+  kp.extra = nullptr;
+  hipGraphNode_t prev = nullptr;
+  for (int q = 0; q <= n + 2 * (iters - 1); ++q) {
+    const int first = std::max(0, floordiv(q - n + 1, 2));
+    const int last = std::min(iters - 1, q / 2);
+    if (last < first) continue;
+    const int z0 = n - q + 2 * first;
+    for (int color = 0; color < 4; ++color) {
+      kp.gridDim = dim3(bx, (unsigned)(last - first + 1));
+      kp.kernelParams = args;
+      hipGraphNode_t node = nullptr;
+      HIP_CHECK(hipGraphAddKernelNode(&node, graph, prev ? &prev : nullptr, prev ? 1 : 0, &kp));
+      g->nodes.push_back(node);
+      g->first_z.push_back(z0);
+      g->slices.push_back(last - first + 1);
+      g->color.push_back(color);
+      prev = node;
+    }
+  }
+  HIP_CHECK(hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0));
+  HIP_CHECK(hipEventCreateWithFlags(&g->done, hipEventDisableTiming));
+  return g;
+}
+
+template <class T, class V>
 int relax_sliced_graph(const GsOperator<T, V>& op, const Schedule<T>& sc, T* x, int depth, int iters, hipStream_t s) {
   static std::mutex mu;
-  static std::map<std::tuple<int, int, int, unsigned, int, hipStream_t>, RelaxGraph> cache;
-  static std::map<int, hipStream_t> capture_streams;
+  static std::map<std::tuple<int, int, int, unsigned, int, hipStream_t>, std::unique_ptr<RelaxGraph>> cache;
   const int n = 1 << depth;
   unsigned bx = 1;
   while (bx * (unsigned)RELAX_GRAPH_BS < (unsigned)std::max(1, sc.max_group)) bx <<= 1;
-  const size_t groups = 4 * ((size_t)n + 1);
   int dev = 0;
   HIP_CHECK(hipGetDevice(&dev));
-  std::lock_guard<std::mutex> guard(mu);
-  const auto key = std::make_tuple((int)sizeof(V), depth, iters, bx, dev, s);
-  auto it = cache.find(key);
-  if (it == cache.end()) {
-    RelaxGraph g;
-    HIP_CHECK(hipMalloc(&g.desc, RELAX_GOFF_OFFSET + sizeof(int) * (groups + 1)));
-    hipStream_t& cs = capture_streams[dev];
-    if (!cs) HIP_CHECK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
-    HIP_CHECK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-    capture_relax_launches<T, V, RELAX_GRAPH_BS>(cs, (const char*)g.desc, n, iters, bx, g.launches);
-    hipGraph_t graph = nullptr;
-    HIP_CHECK(hipStreamEndCapture(cs, &graph));
-    HIP_CHECK(hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0));
-    HIP_CHECK(hipGraphDestroy(graph));
-    it = cache.emplace(key, g).first;
+  RelaxGraph* g = nullptr;
+  {
+    std::lock_guard<std::mutex> guard(mu);
+    std::unique_ptr<RelaxGraph>& slot = cache[std::make_tuple((int)sizeof(V), depth, iters, bx, dev, s)];
+    if (!slot) slot = build_relax_graph<T, V>(n, iters, bx);
+    g = slot.get();
   }
-  static_assert(sizeof(RelaxDesc<T, V>) <= RELAX_GOFF_OFFSET, "descriptor block layout");
-  RelaxDesc<T, V> d{op.update_order, op.cols.data(), op.vals.data(), op.b.data(), op.inv_diag.data(), op.has_diag.data(), x, op.R};
-  char* desc = (char*)it->second.desc;
-  // the next call on this stream rewrites the block only after this launch
-  // (stream order)
-  hipLaunchKernelGGL((k_write_relax_desc<T, V>), dim3(blocks_for(groups + 1, 256)), dim3(256), 0, s, desc, d,
-                     (const int*)sc.offsets.data(), (int)(groups + 1));
-  KERNEL_CHECK();
-  HIP_CHECK(hipGraphLaunch(it->second.exec, s));
-  return it->second.launches;
+  std::lock_guard<std::mutex> guard(g->mu);
+  HIP_CHECK(hipEventSynchronize(g->done));
+  RelaxArgs<T, V> d{op.update_order, op.cols.data(), op.vals.data(), op.b.data(), op.inv_diag.data(), op.has_diag.data(), x, op.R};
+  SliceRanges rg;
+  void* args[2] = {&d, &rg};
+  hipKernelNodeParams kp = g->params;
+  kp.kernelParams = args;
+  const std::vector<int>& off = sc.host_offsets;
+  for (size_t i = 0; i < g->nodes.size(); ++i) {
+    for (int t = 0; t < RELAX_MAX_SLICES; ++t) rg.start[t] = rg.end[t] = 0;
+    for (int t = 0; t < g->slices[i]; ++t) {
+      const int group = 4 * (g->first_z[i] + 2 * t) + g->color[i];
+      rg.start[t] = off[group];
+      rg.end[t] = off[group + 1];
+    }
+    kp.gridDim = dim3(bx, (unsigned)g->slices[i]);
+    HIP_CHECK(hipGraphExecKernelNodeSetParams(g->exec, g->nodes[i], &kp));
+  }
+  HIP_CHECK(hipGraphLaunch(g->exec, s));
+  HIP_CHECK(hipEventRecord(g->done, s));
+  return (int)g->nodes.size();
 }
 
 // PSR_RELAX=wave selects direct launches (same bits); the default replays a
@@ -1367,7 +1386,9 @@ template <class T, class V>
 int relax_sliced_any(const GsOperator<T, V>& op, const Schedule<T>& sc, T* x, int depth, int iters, hipStream_t s) {
   switch (relax_mode()) {
     case 1: return relax_sliced(op, sc, x, depth, iters, s);
-    default: return relax_sliced_graph(op, sc, x, depth, iters, s);
+    default:
+      if (iters > RELAX_MAX_SLICES) return relax_sliced(op, sc, x, depth, iters, s);
+      return relax_sliced_graph(op, sc, x, depth, iters, s);
   }
 }
 
