@@ -66,7 +66,7 @@ def relax_roofline(metadata, value_bytes, vector_bytes, kernel=None):
     achieved = total / seconds / 1e9
     fine = max(stats, key=lambda st: st["rows"])
     return {
-        "kernel": "k_relax_wave_graph (sliced Gauss-Seidel, all finer levels)",
+        "kernel": "k_relax_wave_args (sliced Gauss-Seidel, all finer levels)",
         "bound": "hbm",
         "achieved": round(achieved, 2),
         "peak": HBM_PEAK_GBS,
@@ -205,7 +205,7 @@ def main():
                                 "float32 finer-level operators",
                    "basis_counts": rmx.metadata["basis_counts"]},
     }
-    out["roofline"] = relax_roofline(rmx.metadata, 4, 8, "k_relax_wave_graph<double, float, 64>")
+    out["roofline"] = relax_roofline(rmx.metadata, 4, 8, "k_relax_wave_args<double, float, 64>")
     out["stage_seconds"] = rmx.metadata["timings_seconds"]
     out["normal_average_wait_seconds"] = rmx.metadata["normal_average_wait_seconds"]
     r32, dt32 = timed(tp32, tn32, max(1, args.steps // 2), 1, "float32")
