@@ -25,6 +25,8 @@
  *   psr_field_vjp       -> DifferentiableReconstruction.evaluate backward
  *                                                                 src/warp_psr/field_adjoint.py:255-288, field.py:214-286
  *   psr_backward        -> reconstruct_differentiable backward    src/warp_psr/reconstruction_adjoint.py:89-177
+ *   psr_field_vjp_inputs, psr_backward_inputs -> the same, for positions and normals
+ *                                                                 src/warp_psr/reconstruction_adjoint.py:89-177
  *   PsrParams.float_operators -> the mixed precision mode (no reference counterpart;
  *                                float64 run with float32 finer-level operators)
  */
@@ -125,13 +127,31 @@ int psr_relax_sliced(int dtype, const int64_t* keys, int64_t rows, const int32_t
  *
  * psr_backward: coefficient / cumulative gradients (either may be NULL) and an
  * isovalue seed -> gradient w.r.t. the input normals (n x 3, accumulated);
- * `normals` are the forward's input normals.  Gradients w.r.t. positions are
- * not provided.  Returns 2 when the root conjugate-gradient solve ran (its
- * reverse is not provided). */
+ * `normals` are the forward's input normals.
+ *
+ * psr_field_vjp_inputs: psr_field_vjp plus transform_grad (4 float64 on the
+ * device, accumulated, may be NULL): the loss's gradient w.r.t. the sample
+ * normalisation (inv, t1, t2, t3) through the model-to-cube query mapping
+ * (transforms.py:56-79), which depends on the input positions' bounds.
+ *
+ * psr_backward_inputs: the same seeds (with transform_grad from
+ * psr_field_vjp_inputs, or NULL) -> gradients w.r.t. the input positions
+ * and/or normals (n x 3 each, accumulated; one output may be NULL).  `points`
+ * and `normals` are the forward's inputs.  Position gradients go through the
+ * isovalue centroids, the normal splats' basis, depth and weight, the density
+ * levels and aggregates, and the sample transform with its extrema owners
+ * (sample_adjoint.py:16-102, splat_adjoint.py:104-188); they need a run
+ * without screening (point_weight 0 or exact_interpolation), else the call
+ * returns 2. */
 int psr_field_vjp(PsrResult* result, const void* queries, int64_t m, const void* value_seed, const void* grad_seed,
                   double* coef_grad, double* cum_grad, void* query_grad, void* stream);
 int psr_backward(PsrResult* result, const double* coef_grad, const double* cum_grad, double iso_seed, const void* normals,
                  int64_t n, void* grad_normals, void* stream);
+int psr_field_vjp_inputs(PsrResult* result, const void* queries, int64_t m, const void* value_seed, const void* grad_seed,
+                         double* coef_grad, double* cum_grad, void* query_grad, double* transform_grad, void* stream);
+int psr_backward_inputs(PsrResult* result, const double* coef_grad, const double* cum_grad, double iso_seed,
+                        const double* transform_grad, const void* points, const void* normals, int64_t n, void* grad_points,
+                        void* grad_normals, void* stream);
 
 /* Evaluates any PsrField view (psr_result_field) at m device queries with the
  * header-only psr_evaluate_field of warp_psr_amd_field.h -- the kernel-side

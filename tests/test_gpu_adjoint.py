@@ -120,3 +120,75 @@ def test_gradient_accumulates_and_repeats(cuda):
     first = tn.grad.clone()
     values.sum().backward()
     np.testing.assert_allclose(tn.grad.cpu().numpy(), 2 * first.cpu().numpy(), rtol=1e-9, atol=1e-12)
+
+
+# Unscreened runs: positions enter through the sample transform (bounds and
+# their extrema owners), the group sums, the density levels and depth/weight,
+# the normal splats' B-spline weights and the isovalue centroids
+# (reconstruction_adjoint.py:89-177, sample_adjoint.py, splat_adjoint.py).
+POSITION_CASES = [
+    ("psr", "sphere", 512, "double", dict(depth=4, full_depth=2, point_weight=0.0)),
+    ("exact", "sphere", 512, "double", dict(depth=4, full_depth=2, point_weight=2.0, exact_interpolation=True)),
+    ("root-padded-psr", "sphere", 512, "double", dict(depth=4, full_depth=2, base_depth=0, point_weight=0.0)),
+    ("torus-psr", "torus", 8192, "double", dict(depth=6, full_depth=3, point_weight=0.0)),
+    ("psr-float", "sphere", 512, "float", dict(depth=4, full_depth=2, point_weight=0.0)),
+]
+
+
+def _position_assignments(r, p):
+    return _assignments(r) + [np.argmin(p, axis=0), np.argmax(p, axis=0)]
+
+
+@pytest.mark.parametrize("name,shape,count,precision,params", POSITION_CASES, ids=[c[0] for c in POSITION_CASES])
+def test_position_gradient_matches_central_differences(cuda, name, shape, count, precision, params):
+    import torch
+    from warp_psr_amd import reconstruct, reconstruct_differentiable
+    p, n = fixture_arrays(shape, count, precision)
+    rng = np.random.default_rng(283)
+    tp = torch.from_numpy(p).to(cuda).requires_grad_(True)
+    tn = torch.from_numpy(n).to(cuda).requires_grad_(True)
+    rec = reconstruct_differentiable(tp, tn, **params)
+    q = torch.from_numpy(_queries(p.astype(np.float64), rng).astype(p.dtype)).to(cuda)
+    values, grads, valid = rec.evaluate(q, centered=True)
+    assert bool(valid.bool().all())
+    seed_v = torch.from_numpy(rng.normal(size=len(q))).to(cuda)
+    seed_g = torch.from_numpy(rng.normal(size=(len(q), 3)) * 1e-2).to(cuda)
+    loss = (values.double() * seed_v).sum() + (grads.double() * seed_g).sum()
+    loss.backward()
+    eps = 1e-6 if precision == "double" else 1e-4
+    rtol, atol = (3e-5, 2e-7) if precision == "double" else (2e-2, 2e-3)
+    reference = reconstruct(tp.detach(), tn.detach(), **params)
+    fixed = _position_assignments(reference, p)
+    for tracked, array, gradient in ((0, p, tp.grad), (1, n, tn.grad)):
+        g = gradient.cpu().numpy().astype(np.float64)
+        assert np.isfinite(g).all() and np.abs(g).max() > 0
+        if tracked == 0:
+            # a few random points, plus (float64) the bounds' owners whose runner-up is well clear of
+            # the step, so that no cell or owner changes; in float32 an owner's step moves every cell
+            owners = []
+            if precision == "double":
+                for a in range(3):
+                    col = np.sort(p[:, a])
+                    if col[1] - col[0] > 1e3 * eps:
+                        owners.append(np.argmin(p[:, a]))
+                    if col[-1] - col[-2] > 1e3 * eps:
+                        owners.append(np.argmax(p[:, a]))
+            interior = np.setdiff1d(np.arange(len(p)), np.concatenate([np.argmin(p, axis=0), np.argmax(p, axis=0)]))
+            chosen = np.unique(np.concatenate([np.asarray(owners, dtype=np.int64),
+                                               rng.choice(interior, 24 if precision == "double" else 6, replace=False)]))
+            direction = np.zeros(p.shape)
+            direction[chosen] = rng.normal(size=(len(chosen), 3))
+        else:
+            direction = rng.normal(size=array.shape)
+        direction = direction.astype(array.dtype)
+        losses = []
+        for sign in (1, -1):
+            moved = (array + sign * eps * direction).astype(array.dtype)
+            pp, nn = (moved, n) if tracked == 0 else (p, moved)
+            r = reconstruct(torch.from_numpy(pp).to(cuda), torch.from_numpy(nn).to(cuda), **params)
+            for a, b in zip(fixed, _position_assignments(r, pp)):
+                np.testing.assert_array_equal(a, b)
+            loss_value, _ = _loss(r, q, seed_v, seed_g)
+            losses.append(loss_value)
+        fd = (losses[0] - losses[1]) / (2 * eps)
+        np.testing.assert_allclose(np.sum(g * direction.astype(np.float64)), fd, rtol=rtol, atol=atol)

@@ -110,15 +110,27 @@ def test_metadata_parser_shapes():
         assert f'\\"{key}\\"' in text
 
 
-def test_differentiable_rejects_tracked_positions():
+def test_differentiable_validates_tracked_inputs():
+    """Tracked positions need an unscreened run; something must be tracked
+    (reconstruction_adjoint.py:70-85); options are constants."""
     import torch
     from warp_psr_amd import reconstruct_differentiable
     p = torch.zeros((4, 3), dtype=torch.float64, requires_grad=True)
     n = torch.ones((4, 3), dtype=torch.float64, requires_grad=True)
-    with pytest.raises(ValueError, match="positions"):
+    with pytest.raises(ValueError, match="point_weight=0 or exact_interpolation"):
         reconstruct_differentiable(p, n)
+    with pytest.raises(ValueError, match="point_weight=0 or exact_interpolation"):
+        reconstruct_differentiable(p, n.detach(), point_weight=1.0)
     with pytest.raises(ValueError, match="requires_grad"):
         reconstruct_differentiable(p.detach(), n.detach())
+    with pytest.raises(ValueError, match="constant scalar"):
+        reconstruct_differentiable(p.detach(), n, point_weight=torch.tensor(1.0))
+
+
+def test_c_abi_backward_inputs_validates_arguments():
+    lib = _lib.load()
+    assert lib.psr_backward_inputs(None, None, None, ctypes.c_double(0.0), None, None, None, 0, None, None, None) == 2
+    assert lib.psr_field_vjp_inputs(None, None, 0, None, None, None, None, None, None, None) == 2
 
 
 def test_c_abi_batch_validates_before_device_work():

@@ -23,8 +23,9 @@ template <class T> PSR_HD int clamp_i(int v, int lo, int hi) { return v < lo ? l
 // gradient.  Lookup decisions are the forward's (same T arithmetic).
 template <class T>
 __device__ void query_vjp(const FieldIndex& f, int depth, const T pin[3], double gv, const double gg[3], const T* coef,
-                          const T* cum, double* gcoef, double* gcum, double qp[3]) {
+                          const T* cum, double* gcoef, double* gcum, double qp[3], double fg[3] = nullptr) {
   qp[0] = qp[1] = qp[2] = 0.0;
+  if (fg) fg[0] = fg[1] = fg[2] = 0.0;
   if (gv == 0.0 && gg[0] == 0.0 && gg[1] == 0.0 && gg[2] == 0.0) return;
   T p[3] = {pin[0], pin[1], pin[2]};
   const T lo = tmin(tmin(p[0], p[1]), p[2]), hi = tmax(tmax(p[0], p[1]), p[2]);
@@ -82,6 +83,8 @@ __device__ void query_vjp(const FieldIndex& f, int depth, const T pin[3], double
       qp[0] += gv * c * bg[0] + (gg[1] * xy + gg[2] * xz);
       qp[1] += gv * c * bg[1] + (gg[0] * xy + gg[2] * yz);
       qp[2] += gv * c * bg[2] + (gg[0] * xz + gg[1] * yz);
+      if (fg)
+        for (int a = 0; a < 3; ++a) fg[a] += c * bg[a];
     }
   }
   for (int a = 0; a < 3; ++a)
@@ -91,36 +94,61 @@ __device__ void query_vjp(const FieldIndex& f, int depth, const T pin[3], double
 // Model-space queries (cube = L q + t; the returned gradient is L^T of the
 // cube gradient): value/gradient seeds -> coefficient, cumulative and query
 // gradients (field_adjoint.py:62-104, 144-154).
+// With gtransform, the loss's dependence on the normalisation through the
+// query mapping is reduced into gtransform[4] = d/d(inv, t1, t2, t3): the
+// cube point is (inv q1 + t1, inv q2 + t2, inv q0 + t3) (the model-to-cube
+// map, transforms.py:66-79) and the model gradient is L^T grad f.
 template <class T>
-__global__ void k_query_vjp(FieldIndex f, int depth, const T* __restrict__ coef, const T* __restrict__ cum,
-                            const T* __restrict__ queries, int64_t m, Affine<T> A, const T* __restrict__ gvalue,
-                            const T* __restrict__ ggrad, double* __restrict__ gcoef, double* __restrict__ gcum,
-                            T* __restrict__ gquery) {
+__global__ void __launch_bounds__(256) k_query_vjp(FieldIndex f, int depth, const T* __restrict__ coef, const T* __restrict__ cum,
+                                                   const T* __restrict__ queries, int64_t m, Affine<T> A,
+                                                   const T* __restrict__ gvalue, const T* __restrict__ ggrad,
+                                                   double* __restrict__ gcoef, double* __restrict__ gcum,
+                                                   T* __restrict__ gquery, double* __restrict__ gtransform) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= m) return;
-  const T q0 = queries[3 * i], q1 = queries[3 * i + 1], q2 = queries[3 * i + 2];
-  T p[3];
-  for (int r = 0; r < 3; ++r) p[r] = A.L[3 * r] * q0 + A.L[3 * r + 1] * q1 + A.L[3 * r + 2] * q2 + A.t[r];
-  const double gv = gvalue ? double(gvalue[i]) : 0.0;
-  double gg[3] = {0.0, 0.0, 0.0};
-  if (ggrad) {  // model gradient = L^T cube gradient, so the cube seed is L * seed
-    const double s0 = double(ggrad[3 * i]), s1 = double(ggrad[3 * i + 1]), s2 = double(ggrad[3 * i + 2]);
-    for (int r = 0; r < 3; ++r) gg[r] = double(A.L[3 * r]) * s0 + double(A.L[3 * r + 1]) * s1 + double(A.L[3 * r + 2]) * s2;
+  double part[4] = {0.0, 0.0, 0.0, 0.0};
+  if (i < m) {
+    const T q0 = queries[3 * i], q1 = queries[3 * i + 1], q2 = queries[3 * i + 2];
+    T p[3];
+    for (int r = 0; r < 3; ++r) p[r] = A.L[3 * r] * q0 + A.L[3 * r + 1] * q1 + A.L[3 * r + 2] * q2 + A.t[r];
+    const double gv = gvalue ? double(gvalue[i]) : 0.0;
+    double gg[3] = {0.0, 0.0, 0.0}, sm[3] = {0.0, 0.0, 0.0};
+    if (ggrad) {  // model gradient = L^T cube gradient, so the cube seed is L * seed
+      for (int c = 0; c < 3; ++c) sm[c] = double(ggrad[3 * i + c]);
+      for (int r = 0; r < 3; ++r) gg[r] = double(A.L[3 * r]) * sm[0] + double(A.L[3 * r + 1]) * sm[1] + double(A.L[3 * r + 2]) * sm[2];
+    }
+    double qp[3], fg[3];
+    query_vjp<T>(f, depth, p, gv, gg, coef, cum, gcoef, gcum, qp, gtransform ? fg : nullptr);
+    if (gquery)
+      for (int c = 0; c < 3; ++c)
+        gquery[3 * i + c] += T(double(A.L[c]) * qp[0] + double(A.L[3 + c]) * qp[1] + double(A.L[6 + c]) * qp[2]);
+    if (gtransform) {
+      part[0] = qp[0] * double(q1) + qp[1] * double(q2) + qp[2] * double(q0) + sm[1] * fg[0] + sm[2] * fg[1] + sm[0] * fg[2];
+      part[1] = qp[0];
+      part[2] = qp[1];
+      part[3] = qp[2];
+    }
   }
-  double qp[3];
-  query_vjp<T>(f, depth, p, gv, gg, coef, cum, gcoef, gcum, qp);
-  if (gquery)
-    for (int c = 0; c < 3; ++c)
-      gquery[3 * i + c] += T(double(A.L[c]) * qp[0] + double(A.L[3 + c]) * qp[1] + double(A.L[6 + c]) * qp[2]);
+  if (!gtransform) return;
+  __shared__ double red[4][256];
+  for (int c = 0; c < 4; ++c) red[c][threadIdx.x] = part[c];
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st)
+      for (int c = 0; c < 4; ++c) red[c][threadIdx.x] += red[c][threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x < 4) atomicAdd(&gtransform[threadIdx.x], red[threadIdx.x][0]);
 }
 
 // Isovalue = sum w v(centroid) / sum w (reconstruction.py:101-116): each
 // centroid value receives seed * w / mass (field_adjoint.py:134-142); the
 // centroids are sums / w in cube space.
+// With gsums, the centroid's own gradient (seed * w / mass * grad f) reaches
+// the position sums through p = sums / w.
 template <class T>
 __global__ void k_iso_vjp(FieldIndex f, int depth, const T* __restrict__ coef, const T* __restrict__ cum,
                           const T* __restrict__ sums, const T* __restrict__ weights, int G, double scale,
-                          double* __restrict__ gcoef, double* __restrict__ gcum) {
+                          double* __restrict__ gcoef, double* __restrict__ gcum, double* __restrict__ gsums) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= G) return;
   const T w = weights[i];
@@ -128,6 +156,8 @@ __global__ void k_iso_vjp(FieldIndex f, int depth, const T* __restrict__ coef, c
   const double gg[3] = {0.0, 0.0, 0.0};
   double qp[3];
   query_vjp<T>(f, depth, p, scale * double(w), gg, coef, cum, gcoef, gcum, qp);
+  if (gsums)
+    for (int a = 0; a < 3; ++a) gsums[3 * i + a] += qp[a] / double(w);
 }
 
 __global__ void k_sum_weights(const void* __restrict__ weights, int is_double, int G, double* __restrict__ out) {
@@ -171,33 +201,46 @@ __global__ void k_add_d(double* __restrict__ y, const double* __restrict__ x, in
   if (i < n) y[i] += x[i];
 }
 
-// normals.py:46-104 reversed for the normal sums: every valid splat of
-// aggregate src adds scale_k * (-record gradient) (records are negated sums)
-// to d(value)/d(n'), n' = nsums/w / |nsums/w| * w, then through the
-// normalisation to nsums.
+// normals.py:46-104 reversed: every valid splat of aggregate src adds
+// scale_k * (-record gradient) (records are negated sums) to d(value)/d(n'),
+// n' = nsums/w / |nsums/w| * w, then through the normalisation to nsums.
+// With gsums / gdw, the record scale's derivatives (centroid, splat depth,
+// weight) reach the position sums and the density's (depth, weight).
 template <class T>
 __global__ void k_normal_splat_vjp(int G, const T* __restrict__ sums, const T* __restrict__ nsums, const T* __restrict__ weights,
                                    const T* __restrict__ dw, int minimum, int maximum, const int64_t* __restrict__ nkeys,
                                    const int64_t* __restrict__ noff, const double* __restrict__ grec,
-                                   double* __restrict__ gnsums) {
+                                   double* __restrict__ gnsums, double* __restrict__ gsums, double* __restrict__ gdw) {
   const int src = blockIdx.x * blockDim.x + threadIdx.x;
   if (src >= G) return;
-  double gn[3] = {0.0, 0.0, 0.0};
+  double gn[3] = {0.0, 0.0, 0.0}, gp[3] = {0.0, 0.0, 0.0}, gdepth = 0.0, gwgt = 0.0;
   bool any = false;
   for (int k = 0; k < 54; ++k) {
-    const NormalRecord<T> rec = normal_record<T>(src, k, sums, nsums, weights, dw, minimum, maximum, true);
-    if (!rec.valid) continue;
-    const int64_t lo = noff[rec.top], hi = noff[rec.top + 1];
-    const int64_t idx = find_key(nkeys + lo, hi - lo, basis_key(rec.top, rec.x, rec.y, rec.z));
+    const RecordJacobian J = normal_record_jacobian<T>(src, k, sums, nsums, weights, dw, minimum, maximum);
+    if (!J.valid) continue;
+    const int64_t lo = noff[J.top], hi = noff[J.top + 1];
+    const int64_t idx = find_key(nkeys + lo, hi - lo, basis_key(J.top, J.x, J.y, J.z));
     if (idx < 0) continue;
     const double* g = grec + 3 * (lo + idx);
-    gn[0] -= rec.scale * g[0];
-    gn[1] -= rec.scale * g[1];
-    gn[2] -= rec.scale * g[2];
+    const double gv[3] = {-g[0], -g[1], -g[2]};  // gradient of the record's un-negated value
+    gn[0] += J.s * gv[0];
+    gn[1] += J.s * gv[1];
+    gn[2] += J.s * gv[2];
+    const double dot = J.n[0] * gv[0] + J.n[1] * gv[1] + J.n[2] * gv[2];
+    for (int a = 0; a < 3; ++a) gp[a] += dot * J.ds_dp[a];
+    gdepth += dot * J.ds_ddepth;
+    gwgt += dot * J.ds_dwgt;
     any = true;
   }
   if (!any) return;
   const double w = double(weights[src]);
+  if (gsums)
+    for (int a = 0; a < 3; ++a) gsums[3 * src + a] += gp[a] / w;
+  if (gdw) {
+    gdw[2 * src] += gdepth;
+    gdw[2 * src + 1] += gwgt;
+  }
+  if (!gnsums) return;
   const double m0 = double(nsums[3 * src]) / w, m1 = double(nsums[3 * src + 1]) / w, m2 = double(nsums[3 * src + 2]) / w;
   const double len = sqrt(m0 * m0 + m1 * m1 + m2 * m2);
   if (!(len > 0.0)) return;
@@ -207,6 +250,190 @@ __global__ void k_normal_splat_vjp(int G, const T* __restrict__ sums, const T* _
   gnsums[3 * src] += gn[0] / len - m0 * dot / l3;
   gnsums[3 * src + 1] += gn[1] / len - m1 * dot / l3;
   gnsums[3 * src + 2] += gn[2] / len - m2 * dot / l3;
+}
+
+// density.py:152-182 (_depths) reversed: the selected level d is the
+// forward's (same T comparisons); raw depth and weight h^2 / density feed
+// back to density[d] and, below the target, density[d + 1].
+template <class T>
+__global__ void k_depths_vjp(const T* __restrict__ dens, int G, int kd, T target, const double* __restrict__ gdw,
+                             double* __restrict__ gdens) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= G) return;
+  const int stride = kd + 1;
+  const double graw = gdw[2 * i], gw = gdw[2 * i + 1];
+  if (graw == 0.0 && gw == 0.0) return;
+  int d = kd;
+  T density = dens[(int64_t)i * stride + d];
+  if (density >= target) {
+    const double D = double(density), h = 1.0 / double(1 << d);
+    gdens[(int64_t)i * stride + d] += graw / (D * LN2_D * 2.0) - gw * h * h / (D * D);
+    return;
+  }
+  T fine = density;
+  while (density < target && d > 0) {
+    fine = density;
+    d -= 1;
+    density = dens[(int64_t)i * stride + d];
+  }
+  const double D = double(density), F = double(fine), h = 1.0 / double(1 << d);
+  const double L1 = log(D / double(target)), L2 = log(D / F);
+  double gD = -gw * h * h / (D * D);
+  if (L2 != 0.0 && d + 1 <= kd) {
+    gD += graw * (L2 - L1) / (D * L2 * L2);
+    gdens[(int64_t)i * stride + d + 1] += graw * L1 / (L2 * L2 * F);
+  }
+  gdens[(int64_t)i * stride + d] += gD;
+}
+
+// density.py:108-150 reversed at one level d: the centroid's B-spline
+// derivatives (to the position sums) and each record's weight in the value
+// (to the density splat records, float64 atomics).
+template <class T>
+__global__ void k_density_eval_vjp(const int64_t* __restrict__ lkeys, const T* __restrict__ lvals, int64_t lcount,
+                                   const T* __restrict__ sums, const T* __restrict__ weights, int G, int d, int stride,
+                                   const double* __restrict__ gdens, double* __restrict__ gsums, double* __restrict__ glvals) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= G) return;
+  const double g = gdens[(int64_t)i * stride + d];
+  if (g == 0.0) return;
+  const T w = weights[i];
+  const T p[3] = {sums[3 * i] / w, sums[3 * i + 1] / w, sums[3 * i + 2] / w};
+  const int n = 1 << d;
+  const T h = T(1) / T(n);
+  int c[3];
+  double bv[3][3], db[3][3];
+  for (int a = 0; a < 3; ++a) {
+    c[a] = cell_index(p[a], n);
+    const double t = double((p[a] - T(c[a]) * h) / h);
+    const Quad q = quadratic(t);
+    for (int o = 0; o < 3; ++o) {
+      bv[a][o] = quad_at(q, o);
+      db[a][o] = quad_deriv(t, o) / double(h);
+    }
+  }
+  double gp[3] = {0.0, 0.0, 0.0};
+  for (int k = 0; k < 27; ++k) {
+    const int x = k / 9, y = k / 3 % 3, z = k % 3;
+    const int tx = c[0] + x - 1, ty = c[1] + y - 1, tz = c[2] + z - 1;
+    if (imin(imin(tx, ty), tz) < 0 || imax(imax(tx, ty), tz) >= n) continue;
+    const int64_t idx = find_key(lkeys, lcount, basis_key(d, tx, ty, tz));
+    if (idx < 0) continue;
+    const double v = double(lvals[idx]);
+    gp[0] += db[0][x] * bv[1][y] * bv[2][z] * v;
+    gp[1] += bv[0][x] * db[1][y] * bv[2][z] * v;
+    gp[2] += bv[0][x] * bv[1][y] * db[2][z] * v;
+    atomicAdd(&glvals[idx], g * bv[0][x] * bv[1][y] * bv[2][z]);
+  }
+  for (int a = 0; a < 3; ++a) gsums[3 * i + a] += g * gp[a] / double(w);
+}
+
+// density.py:74-106 (_splat) reversed at one level d: aggregate a's 27
+// records weigh v[3] * scale * basis(p - node), p = v[0:3] / v[3]; the record
+// gradients reach the aggregate's position sums.
+template <class T>
+__global__ void k_density_splat_vjp(const int64_t* __restrict__ akeys, const T* __restrict__ avals, int64_t m, int d, T scale,
+                                    const int64_t* __restrict__ lkeys, int64_t lcount, const double* __restrict__ glvals,
+                                    double* __restrict__ gagg) {
+  const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= m) return;
+  const int64_t key = akeys[a];
+  const int c[3] = {key_x(key), key_y(key), key_z(key)};
+  const T* v = avals + 4 * a;
+  const T p[3] = {v[0] / v[3], v[1] / v[3], v[2] / v[3]};
+  const int n = 1 << d;
+  const T h = T(1) / T(n);
+  double bv[3][3], db[3][3];
+  for (int ax = 0; ax < 3; ++ax) {
+    const double t = double((p[ax] - T(c[ax]) * h) / h);
+    const Quad q = quadratic(t);
+    for (int o = 0; o < 3; ++o) {
+      bv[ax][o] = quad_at(q, o);
+      db[ax][o] = quad_deriv(t, o) / double(h);
+    }
+  }
+  const double weight = double(v[3] * scale);
+  double gp[3] = {0.0, 0.0, 0.0};
+  for (int k = 0; k < 27; ++k) {
+    const int x = k / 9, y = k / 3 % 3, z = k % 3;
+    const int tx = c[0] + x - 1, ty = c[1] + y - 1, tz = c[2] + z - 1;
+    if (imin(imin(tx, ty), tz) < 0 || imax(imax(tx, ty), tz) >= n) continue;
+    const int64_t idx = find_key(lkeys, lcount, basis_key(d, tx, ty, tz));
+    if (idx < 0) continue;
+    const double g = glvals[idx] * weight;
+    gp[0] += g * db[0][x] * bv[1][y] * bv[2][z];
+    gp[1] += g * bv[0][x] * db[1][y] * bv[2][z];
+    gp[2] += g * bv[0][x] * bv[1][y] * db[2][z];
+  }
+  const double inv = 1.0 / double(v[3]);
+  for (int ax = 0; ax < 3; ++ax) gagg[3 * a + ax] = gp[ax] * inv;
+}
+
+// Aggregates are sums over their leaf groups (density.py:228-240): group g's
+// position sums receive the gradient of its ancestor at level d.
+__global__ void k_aggregate_to_groups(const int64_t* __restrict__ gkeys, int G, int depth, const int64_t* __restrict__ akeys,
+                                      int64_t m, int d, const double* __restrict__ gagg, double* __restrict__ gsums) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= G) return;
+  const int64_t key = gkeys[g];
+  const int s = depth - d;
+  const int64_t idx = find_key(akeys, m, basis_key(d, key_x(key) >> s, key_y(key) >> s, key_z(key) >> s));
+  if (idx < 0) return;
+  for (int a = 0; a < 3; ++a) gsums[3 * g + a] += gagg[3 * idx + a];
+}
+
+// samples.py:90-134 reversed for the positions: q = (P1, P2, P0) * inv +
+// (t1, t2, t3) summed per group.  The transform's own gradients (inv, t1..t3)
+// are reduced into acc[4] for the bounds' extrema owners.
+template <class T>
+__global__ void __launch_bounds__(256) k_sample_position_vjp(const T* __restrict__ pts, int64_t n, double inv,
+                                                             const int* __restrict__ group, const int* __restrict__ status,
+                                                             const double* __restrict__ gsums, T* __restrict__ gpoints,
+                                                             double* __restrict__ acc) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double part[4] = {0.0, 0.0, 0.0, 0.0};
+  if (i < n) {
+    const int g = group[i];
+    if (g >= 0 && status[i] == 0) {
+      const double gq[3] = {gsums[3 * g], gsums[3 * g + 1], gsums[3 * g + 2]};
+      gpoints[3 * i] += T(gq[2] * inv);
+      gpoints[3 * i + 1] += T(gq[0] * inv);
+      gpoints[3 * i + 2] += T(gq[1] * inv);
+      part[0] = gq[0] * double(pts[3 * i + 1]) + gq[1] * double(pts[3 * i + 2]) + gq[2] * double(pts[3 * i]);
+      part[1] = gq[0];
+      part[2] = gq[1];
+      part[3] = gq[2];
+    }
+  }
+  __shared__ double red[4][256];
+  for (int c = 0; c < 4; ++c) red[c][threadIdx.x] = part[c];
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st)
+      for (int c = 0; c < 4; ++c) red[c][threadIdx.x] += red[c][threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x < 4) atomicAdd(&acc[threadIdx.x], red[threadIdx.x][0]);
+}
+
+// samples.py bounds owners (sample_adjoint.py:16-31): the lowest index whose
+// coordinate equals each of the six bounds.
+template <class T>
+__global__ void k_extrema_owners(const T* __restrict__ pts, int64_t n, const T* __restrict__ bounds, int* __restrict__ owners) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int a = 0; a < 3; ++a) {
+    const T v = pts[3 * i + a];
+    if (v == bounds[a]) atomicMin(&owners[a], (int)i);
+    if (v == bounds[a + 3]) atomicMin(&owners[a + 3], (int)i);
+  }
+}
+
+template <class T>
+__global__ void k_add_bound_grads(const int* __restrict__ owners, const double* __restrict__ gb, T* __restrict__ gpoints) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  for (int b = 0; b < 6; ++b)
+    if (owners[b] >= 0 && owners[b] < 0x7fffffff) gpoints[3 * (int64_t)owners[b] + b % 3] += T(gb[b]);
 }
 
 // samples.py:90-134 reversed for the normals: nsums[g] = sum of unit normals
@@ -232,11 +459,15 @@ __global__ void k_unit_normal_vjp(const T* __restrict__ nrm, int64_t n, T scale,
   gnormals[3 * i] += T(gv[2] * s);
 }
 
+__global__ void k_fill_owners(int* __restrict__ owners) {
+  if (threadIdx.x < 6 && blockIdx.x == 0) owners[threadIdx.x] = 0x7fffffff;
+}
+
 }  // namespace
 
 template <class T>
 void field_vjp(Result<T>& r, const T* queries, int64_t m, const T* gvalue, const T* ggrad, double* gcoef, double* gcum,
-               T* gquery, hipStream_t s) {
+               T* gquery, double* gtransform, hipStream_t s) {
   if (m <= 0) return;
   Affine<T> A;
   for (int i = 0; i < 3; ++i) {
@@ -245,31 +476,72 @@ void field_vjp(Result<T>& r, const T* queries, int64_t m, const T* gvalue, const
   }
   hipLaunchKernelGGL(k_query_vjp<T>, dim3(blocks_for(m, 256)), dim3(256), 0, s, field_index(r), r.prm.depth,
                      (const T*)r.coefficients.data(), (const T*)r.cumulative.data(), queries, m, A, gvalue, ggrad, gcoef, gcum,
-                     gquery);
+                     gquery, gtransform);
   KERNEL_CHECK();
 }
 
+// samples.py:54-88 (_transform) reversed on the host: gradients of (inv, t1,
+// t2, t3) -> the six bounds, with Warp's max adjoint (the first operand when
+// it is strictly larger).  The normal scale t4 does not reach the unit
+// normals, which are invariant to it.
+template <class T> static void transform_vjp(const T* b, double scale, const double acc[4], double gb[6]) {
+  for (int k = 0; k < 6; ++k) gb[k] = 0.0;
+  const double e[3] = {double(b[3] - b[0]), double(b[4] - b[1]), double(b[5] - b[2])};
+  const T e1 = b[4] - b[1], e2 = b[5] - b[2], e0 = b[3] - b[0];
+  const T inner = wmax(e1, e2);
+  const T width_t = wmax(e0, inner) * T(scale);
+  const double width = double(width_t), inv = 1.0 / width;
+  double ginv = acc[0], gwidth = 0.0;
+  for (int a = 0; a < 3; ++a) {
+    const int axis = (a + 1) % 3;
+    const double origin = (double(b[axis + 3]) + double(b[axis])) / 2.0 - width / 2.0;
+    const double gt = acc[1 + a];
+    const double gorigin = -gt * inv;
+    ginv += -gt * origin;
+    gb[axis + 3] += gorigin / 2.0;
+    gb[axis] += gorigin / 2.0;
+    gwidth += -gorigin / 2.0;
+  }
+  gwidth += -ginv * inv * inv;
+  const double gext = gwidth * scale;
+  int arg = 0;
+  if (!(e0 > inner)) arg = (e1 > e2) ? 1 : 2;
+  (void)e;
+  gb[arg + 3] += gext;
+  gb[arg] -= gext;
+}
+
+// Gradients of a field loss with respect to the input normals and, for runs
+// without screening, the input positions (reconstruction_adjoint.py:89-177):
+// coefficients and cumulative coefficients (+ isovalue) -> cascadic solve
+// and normal loads -> normal records -> splats -> (normal sums, centroids,
+// splat depth and weight) -> density levels and aggregates -> samples.
 template <class T>
-void backward_normals(Result<T>& r, const double* gcoef_in, const double* gcum_in, double iso_seed, T* gnormals,
-                      const T* normals, int64_t n, hipStream_t s) {
+void backward_inputs(Result<T>& r, const double* gcoef_in, const double* gcum_in, double iso_seed, const double* gtransform,
+                     const T* points, const T* normals, int64_t n, T* gpoints, T* gnormals, hipStream_t s) {
   const int64_t K = r.level_offsets.back();
-  const int D = r.prm.depth;
+  const int D = r.prm.depth, G = r.G;
+  const bool positions = gpoints != nullptr;
   // coefficient gradient = direct seeds + cumulative chain (+ isovalue)
-  DArray<double> gcoef(K, s), work(K, s);
+  DArray<double> gcoef(K, s), work(K, s), gsums;
   gcoef.zero();
   work.zero();
+  if (positions) {
+    gsums.reset(3 * (size_t)G, s);
+    gsums.zero();
+  }
   if (gcoef_in) HIP_CHECK(hipMemcpyAsync(gcoef.data(), gcoef_in, sizeof(double) * K, hipMemcpyDeviceToDevice, s));
   if (gcum_in) HIP_CHECK(hipMemcpyAsync(work.data(), gcum_in, sizeof(double) * K, hipMemcpyDeviceToDevice, s));
-  if (iso_seed != 0.0 && r.G) {
+  if (iso_seed != 0.0 && G) {
     DArray<double> mass(1, s);
-    hipLaunchKernelGGL(k_sum_weights, dim3(1), dim3(64), 0, s, (const void*)r.weights.data(), sizeof(T) == 8 ? 1 : 0, r.G,
+    hipLaunchKernelGGL(k_sum_weights, dim3(1), dim3(64), 0, s, (const void*)r.weights.data(), sizeof(T) == 8 ? 1 : 0, G,
                        mass.data());
     KERNEL_CHECK();
     double mass_h = 0.0;
     read_bytes(&mass_h, mass.data(), sizeof(double), s);
-    hipLaunchKernelGGL(k_iso_vjp<T>, dim3(blocks_for(r.G, 256)), dim3(256), 0, s, field_index(r), D, (const T*)r.coefficients.data(),
-                       (const T*)r.cumulative.data(), (const T*)r.position_sums.data(), (const T*)r.weights.data(), r.G,
-                       iso_seed / mass_h, gcoef.data(), work.data());
+    hipLaunchKernelGGL(k_iso_vjp<T>, dim3(blocks_for(G, 256)), dim3(256), 0, s, field_index(r), D, (const T*)r.coefficients.data(),
+                       (const T*)r.cumulative.data(), (const T*)r.position_sums.data(), (const T*)r.weights.data(), G,
+                       iso_seed / mass_h, gcoef.data(), work.data(), positions ? gsums.data() : (double*)nullptr);
     KERNEL_CHECK();
   }
   const FieldIndex f = field_index(r);
@@ -289,29 +561,103 @@ void backward_normals(Result<T>& r, const double* gcoef_in, const double* gcum_i
   DArray<double> grec(3 * (size_t)records, s);
   grec.zero();
   solve_backward(r, (const double*)gcoef.data(), grec.data(), s);
-  // normal splats -> normal sums -> input normals
+  // normal splats -> normal sums (-> input normals), centroids and (depth, weight)
   DArray<int64_t> noff(r.normal_offsets.size(), s);
   std::vector<int64_t> ho(r.normal_offsets.begin(), r.normal_offsets.end());
   HIP_CHECK(hipMemcpyAsync(noff.data(), ho.data(), sizeof(int64_t) * ho.size(), hipMemcpyHostToDevice, s));
-  DArray<double> gns(3 * (size_t)r.G, s);
-  gns.zero();
-  hipLaunchKernelGGL(k_normal_splat_vjp<T>, dim3(blocks_for(r.G, 256)), dim3(256), 0, s, r.G, (const T*)r.position_sums.data(),
+  DArray<double> gns, gdw;
+  if (gnormals) {
+    gns.reset(3 * (size_t)G, s);
+    gns.zero();
+  }
+  if (positions) {
+    gdw.reset(2 * (size_t)G, s);
+    gdw.zero();
+  }
+  hipLaunchKernelGGL(k_normal_splat_vjp<T>, dim3(blocks_for(G, 256)), dim3(256), 0, s, G, (const T*)r.position_sums.data(),
                      (const T*)r.normal_sums.data(), (const T*)r.weights.data(), (const T*)r.dw.data(), r.prm.base_depth, D,
-                     (const int64_t*)r.normal_keys.data(), (const int64_t*)noff.data(), (const double*)grec.data(), gns.data());
+                     (const int64_t*)r.normal_keys.data(), (const int64_t*)noff.data(), (const double*)grec.data(),
+                     gnormals ? gns.data() : (double*)nullptr, positions ? gsums.data() : (double*)nullptr,
+                     positions ? gdw.data() : (double*)nullptr);
   KERNEL_CHECK();
-  hipLaunchKernelGGL(k_unit_normal_vjp<T>, dim3(blocks_for(n, 256)), dim3(256), 0, s, normals, n, r.transform[4],
-                     (const int*)r.input_group.data(), (const int*)r.input_status.data(), (const double*)gns.data(), gnormals);
-  KERNEL_CHECK();
-  HIP_CHECK(hipStreamSynchronize(s));  // the temporaries above return to the pool
+  if (gnormals) {
+    hipLaunchKernelGGL(k_unit_normal_vjp<T>, dim3(blocks_for(n, 256)), dim3(256), 0, s, normals, n, r.transform[4],
+                       (const int*)r.input_group.data(), (const int*)r.input_status.data(), (const double*)gns.data(), gnormals);
+    KERNEL_CHECK();
+  }
+  if (positions) {
+    // splat depth and weight -> densities (density.py:152-182)
+    const int kd = r.prm.kernel_depth, stride = kd + 1;
+    DArray<double> gdens((size_t)G * stride, s);
+    gdens.zero();
+    hipLaunchKernelGGL(k_depths_vjp<T>, dim3(blocks_for(G, 256)), dim3(256), 0, s, (const T*)r.densities.data(), G, kd,
+                       T(r.prm.samples_per_node), (const double*)gdw.data(), gdens.data());
+    KERNEL_CHECK();
+    // densities -> centroids and density splat records -> aggregates -> groups
+    for (int d = 0; d <= kd; ++d) {
+      LevelState<T>& L = r.levels[d];
+      const int64_t lcount = (int64_t)L.dens_keys.size(), m = (int64_t)L.agg_keys.size();
+      if (!lcount || !m) continue;
+      DArray<double> glvals(lcount, s), gagg(3 * (size_t)m, s);
+      glvals.zero();
+      hipLaunchKernelGGL(k_density_eval_vjp<T>, dim3(blocks_for(G, 256)), dim3(256), 0, s, (const int64_t*)L.dens_keys.data(),
+                         (const T*)L.dens_vals.data(), lcount, (const T*)r.position_sums.data(), (const T*)r.weights.data(), G,
+                         d, stride, (const double*)gdens.data(), gsums.data(), glvals.data());
+      KERNEL_CHECK();
+      hipLaunchKernelGGL(k_density_splat_vjp<T>, dim3(blocks_for(m, 256)), dim3(256), 0, s, (const int64_t*)L.agg_keys.data(),
+                         (const T*)L.agg_vals.data(), m, d, r.density_scale, (const int64_t*)L.dens_keys.data(), lcount,
+                         (const double*)glvals.data(), gagg.data());
+      KERNEL_CHECK();
+      hipLaunchKernelGGL(k_aggregate_to_groups, dim3(blocks_for(G, 256)), dim3(256), 0, s, (const int64_t*)r.sample_keys.data(),
+                         G, D, (const int64_t*)L.agg_keys.data(), m, d, (const double*)gagg.data(), gsums.data());
+      KERNEL_CHECK();
+    }
+    // group position sums -> input positions and the transform (samples.py:54-134)
+    DArray<double> acc(4, s);
+    acc.zero();
+    if (gtransform) HIP_CHECK(hipMemcpyAsync(acc.data(), gtransform, sizeof(double) * 4, hipMemcpyDeviceToDevice, s));
+    hipLaunchKernelGGL(k_sample_position_vjp<T>, dim3(blocks_for(n, 256)), dim3(256), 0, s, points, n, double(r.transform[0]),
+                       (const int*)r.input_group.data(), (const int*)r.input_status.data(), (const double*)gsums.data(),
+                       gpoints, acc.data());
+    KERNEL_CHECK();
+    double acc_h[4], gb[6];
+    read_bytes(acc_h, acc.data(), sizeof(acc_h), s);
+    transform_vjp<T>(r.bounds, r.prm.scale, acc_h, gb);
+    DArray<T> bounds(6, s);
+    DArray<int> owners(6, s);
+    DArray<double> gbd(6, s);
+    HIP_CHECK(hipMemcpyAsync(bounds.data(), r.bounds, sizeof(T) * 6, hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipMemcpyAsync(gbd.data(), gb, sizeof(gb), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_fill_owners, dim3(1), dim3(64), 0, s, owners.data());
+    KERNEL_CHECK();
+    hipLaunchKernelGGL(k_extrema_owners<T>, dim3(blocks_for(n, 256)), dim3(256), 0, s, points, n, (const T*)bounds.data(),
+                       owners.data());
+    KERNEL_CHECK();
+    hipLaunchKernelGGL(k_add_bound_grads<T>, dim3(1), dim3(64), 0, s, (const int*)owners.data(), (const double*)gbd.data(),
+                       gpoints);
+    KERNEL_CHECK();
+  }
+  HIP_CHECK(hipStreamSynchronize(s));  // the temporaries above (and host staging) return to the pool
+}
+
+template <class T>
+void backward_normals(Result<T>& r, const double* gcoef_in, const double* gcum_in, double iso_seed, T* gnormals,
+                      const T* normals, int64_t n, hipStream_t s) {
+  backward_inputs<T>(r, gcoef_in, gcum_in, iso_seed, (const double*)nullptr, (const T*)nullptr, normals, n, (T*)nullptr,
+                     gnormals, s);
 }
 
 template void field_vjp<float>(Result<float>&, const float*, int64_t, const float*, const float*, double*, double*, float*,
-                               hipStream_t);
+                               double*, hipStream_t);
 template void field_vjp<double>(Result<double>&, const double*, int64_t, const double*, const double*, double*, double*, double*,
-                                hipStream_t);
+                                double*, hipStream_t);
 template void backward_normals<float>(Result<float>&, const double*, const double*, double, float*, const float*, int64_t,
                                       hipStream_t);
 template void backward_normals<double>(Result<double>&, const double*, const double*, double, double*, const double*, int64_t,
                                        hipStream_t);
+template void backward_inputs<float>(Result<float>&, const double*, const double*, double, const double*, const float*,
+                                     const float*, int64_t, float*, float*, hipStream_t);
+template void backward_inputs<double>(Result<double>&, const double*, const double*, double, const double*, const double*,
+                                      const double*, int64_t, double*, double*, hipStream_t);
 
 }  // namespace psr

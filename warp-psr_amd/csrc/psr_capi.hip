@@ -531,8 +531,8 @@ int psr_field_evaluate(const PsrField* field, const void* queries, int64_t m, vo
   }
 }
 
-int psr_field_vjp(PsrResult* result, const void* queries, int64_t m, const void* value_seed, const void* grad_seed,
-                  double* coef_grad, double* cum_grad, void* query_grad, void* stream) {
+int psr_field_vjp_inputs(PsrResult* result, const void* queries, int64_t m, const void* value_seed, const void* grad_seed,
+                         double* coef_grad, double* cum_grad, void* query_grad, double* transform_grad, void* stream) {
   try {
     if (!result) throw PsrError(2, "null result");
     if (m < 0) throw PsrError(2, "query count must be nonnegative");
@@ -541,14 +541,22 @@ int psr_field_vjp(PsrResult* result, const void* queries, int64_t m, const void*
     hipStream_t s = (hipStream_t)stream;
     if (result->dtype == 0)
       field_vjp(*(Result<float>*)result->impl, (const float*)queries, m, (const float*)value_seed, (const float*)grad_seed,
-                coef_grad, cum_grad, (float*)query_grad, s);
+                coef_grad, cum_grad, (float*)query_grad, transform_grad, s);
     else
       field_vjp(*(Result<double>*)result->impl, (const double*)queries, m, (const double*)value_seed, (const double*)grad_seed,
-                coef_grad, cum_grad, (double*)query_grad, s);
+                coef_grad, cum_grad, (double*)query_grad, transform_grad, s);
     return 0;
   } catch (const PsrError& e) {
     return fail(e);
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return 1;
   }
+}
+
+int psr_field_vjp(PsrResult* result, const void* queries, int64_t m, const void* value_seed, const void* grad_seed,
+                  double* coef_grad, double* cum_grad, void* query_grad, void* stream) {
+  return psr_field_vjp_inputs(result, queries, m, value_seed, grad_seed, coef_grad, cum_grad, query_grad, nullptr, stream);
 }
 
 int psr_backward(PsrResult* result, const double* coef_grad, const double* cum_grad, double iso_seed, const void* normals,
@@ -566,6 +574,36 @@ int psr_backward(PsrResult* result, const double* coef_grad, const double* cum_g
       if (n != r->n) throw PsrError(2, "normals must be the reconstruction's input normals");
       backward_normals(*r, coef_grad, cum_grad, iso_seed, (double*)grad_normals, (const double*)normals, n, s);
     }
+    return 0;
+  } catch (const PsrError& e) {
+    return fail(e);
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return 1;
+  }
+}
+
+int psr_backward_inputs(PsrResult* result, const double* coef_grad, const double* cum_grad, double iso_seed,
+                        const double* transform_grad, const void* points, const void* normals, int64_t n, void* grad_points,
+                        void* grad_normals, void* stream) {
+  try {
+    if (!result) throw PsrError(2, "null result");
+    if (!grad_points && !grad_normals) throw PsrError(2, "grad_points and grad_normals are both null");
+    if (grad_points && !points) throw PsrError(2, "grad_points needs the input points");
+    if (grad_normals && !normals) throw PsrError(2, "grad_normals needs the input normals");
+    hipStream_t s = (hipStream_t)stream;
+    auto run = [&](auto* r, auto tag) {
+      using T = decltype(tag);
+      if (n != r->n) throw PsrError(2, "points and normals must be the reconstruction's inputs");
+      if (grad_points && r->prm.point_weight > 0 && !r->prm.exact_interpolation)
+        throw PsrError(2, "gradients with respect to positions need point_weight=0 or exact_interpolation=True");
+      backward_inputs(*r, coef_grad, cum_grad, iso_seed, transform_grad, (const T*)points, (const T*)normals, n, (T*)grad_points,
+                      (T*)grad_normals, s);
+    };
+    if (result->dtype == 0)
+      run((Result<float>*)result->impl, 0.0f);
+    else
+      run((Result<double>*)result->impl, 0.0);
     return 0;
   } catch (const PsrError& e) {
     return fail(e);

@@ -142,6 +142,8 @@ PSR_HD Quad quadratic(double x) {
   return q;
 }
 PSR_HD double quad_at(const Quad& q, int i) { return i == 0 ? q.b0 : (i == 1 ? q.b1 : q.b2); }
+// d/dx of quadratic(x)'s three values: b0 = (1-x)^2/2, b1 = (1 + 2x - 2x^2)/2, b2 = x^2/2
+PSR_HD double quad_deriv(double x, int i) { return i == 0 ? x - 1.0 : (i == 1 ? 1.0 - 2.0 * x : x); }
 
 // Deterministic natural logarithm: double-double evaluation of
 // k*ln2 + 2*atanh((m-1)/(m+1)) rounded once to double.  Identical on host and

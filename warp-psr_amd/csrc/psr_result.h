@@ -154,12 +154,19 @@ template <class T> void stage_tree(Result<T>& r);
 // (atomically accumulated) and query gradients (accumulated, may be null).
 template <class T>
 void field_vjp(Result<T>& r, const T* queries, int64_t m, const T* gvalue, const T* ggrad, double* gcoef, double* gcum,
-               T* gquery, hipStream_t s);
+               T* gquery, double* gtransform, hipStream_t s);
 // Coefficient / cumulative / isovalue seeds -> input-normal gradients
 // (accumulated into gnormals, n x 3).
 template <class T>
 void backward_normals(Result<T>& r, const double* gcoef, const double* gcum, double iso_seed, T* gnormals, const T* normals,
                       int64_t n, hipStream_t s);
+// The same seeds (plus gtransform[4], the loss's gradient w.r.t. the
+// normalisation (inv, t1, t2, t3) through the query mapping, or null) ->
+// gradients w.r.t. the input positions (gpoints, runs without screening)
+// and/or normals (gnormals); either may be null.  Both are accumulated (n x 3).
+template <class T>
+void backward_inputs(Result<T>& r, const double* gcoef, const double* gcum, double iso_seed, const double* gtransform,
+                     const T* points, const T* normals, int64_t n, T* gpoints, T* gnormals, hipStream_t s);
 // Reverse of the cascadic solve and normal loads: gcoef (float64, one per
 // concatenated coefficient) gives float64 gradients of the normal records'
 // three values (accumulated into grecords).

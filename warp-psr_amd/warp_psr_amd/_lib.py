@@ -11,7 +11,8 @@ LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libwarp_psr_amd.so"
 EXPORTED = (
     "psr_params_default", "psr_reconstruct", "psr_evaluate", "psr_result_iso", "psr_result_field",
     "psr_result_metadata_json", "psr_result_array", "psr_result_free", "psr_relax_sliced", "psr_last_error",
-    "psr_release_cache", "psr_field_evaluate", "psr_field_vjp", "psr_backward", "psr_reconstruct_batch",
+    "psr_release_cache", "psr_field_evaluate", "psr_field_vjp", "psr_backward", "psr_field_vjp_inputs", "psr_backward_inputs",
+    "psr_reconstruct_batch",
 )
 
 
@@ -74,6 +75,10 @@ def load():
     lib.psr_field_vjp.argtypes = [vp, vp, i64, vp, vp, vp, vp, vp, vp]
     lib.psr_backward.restype = c_int
     lib.psr_backward.argtypes = [vp, vp, vp, c_double, vp, i64, vp, vp]
+    lib.psr_backward_inputs.restype = c_int
+    lib.psr_backward_inputs.argtypes = [vp, vp, vp, c_double, vp, vp, vp, i64, vp, vp, vp]
+    lib.psr_field_vjp_inputs.restype = c_int
+    lib.psr_field_vjp_inputs.argtypes = [vp, vp, i64, vp, vp, vp, vp, vp, vp, vp]
     lib.psr_field_evaluate.restype = c_int
     lib.psr_field_evaluate.argtypes = [ctypes.POINTER(PsrField), vp, i64, vp, vp, vp, vp]
     lib.psr_relax_sliced.restype = c_int

@@ -10,9 +10,12 @@ solve (reverse Gauss-Seidel sweeps, base multigrid) -> normal loads -> normal
 splats -> unit normals -> input normals.  Discrete choices (octree, sample
 groups, splat depths, solver branches) are held fixed, as in the reference.
 
-Gradients with respect to positions are not provided: positions must not
-require grad.  Queries may require grad (d value / d query is the field
-gradient; d gradient / d query uses the trilinear second derivatives).
+Gradients with respect to positions cover runs without screening
+(point_weight 0 or exact_interpolation): they go through the isovalue
+centroids, the splats' basis, depth and weight, the density levels and
+aggregates, and the sample transform's extrema owners.  Queries may require
+grad (d value / d query is the field gradient; d gradient / d query uses the
+trilinear second derivatives).
 """
 import ctypes
 
@@ -33,7 +36,7 @@ def _function():
 
     class FieldQuery(torch.autograd.Function):
         @staticmethod
-        def forward(ctx, normals, queries, recon, centered):
+        def forward(ctx, positions, normals, queries, recon, centered):
             values, gradients, valid = recon._primal.evaluate(queries.detach())
             if centered:
                 values = values - recon.iso_value
@@ -52,23 +55,34 @@ def _function():
             gcoef = torch.zeros(count, dtype=torch.float64, device=dev)
             gcum = torch.zeros(count, dtype=torch.float64, device=dev)
             q = queries.detach().contiguous()
-            gquery = torch.zeros_like(q) if ctx.needs_input_grad[1] else None
+            gquery = torch.zeros_like(q) if ctx.needs_input_grad[2] else None
             gv = gvalue.contiguous() if gvalue is not None else None
             gg = ggrad.contiguous() if ggrad is not None else None
             lib = _lib.load()
             stream = _stream_handle(dev)
             handle = recon._primal._owner._handle
-            _lib.check(lib.psr_field_vjp(handle, q.data_ptr(), q.shape[0], gv.data_ptr() if gv is not None else None,
-                                         gg.data_ptr() if gg is not None else None, gcoef.data_ptr(), gcum.data_ptr(),
-                                         gquery.data_ptr() if gquery is not None else None, stream))
-            gnormals = None
-            if ctx.needs_input_grad[0]:
+            want_p, want_n = ctx.needs_input_grad[0], ctx.needs_input_grad[1]
+            gtransform = torch.zeros(4, dtype=torch.float64, device=dev) if want_p else None
+            _lib.check(lib.psr_field_vjp_inputs(
+                handle, q.data_ptr(), q.shape[0], gv.data_ptr() if gv is not None else None,
+                gg.data_ptr() if gg is not None else None, gcoef.data_ptr(), gcum.data_ptr(),
+                gquery.data_ptr() if gquery is not None else None,
+                gtransform.data_ptr() if gtransform is not None else None, stream))
+            gpositions = gnormals = None
+            if want_p or want_n:
                 iso_seed = -float(gv.sum().item()) if (ctx.centered and gv is not None) else 0.0
+                positions = recon._positions.detach().contiguous()
                 normals = recon._normals.detach().contiguous()
-                gnormals = torch.zeros_like(normals)
-                _lib.check(lib.psr_backward(handle, gcoef.data_ptr(), gcum.data_ptr(), ctypes.c_double(iso_seed),
-                                            normals.data_ptr(), normals.shape[0], gnormals.data_ptr(), stream))
-            return gnormals, gquery, None, None
+                if want_p:
+                    gpositions = torch.zeros_like(positions)
+                if want_n:
+                    gnormals = torch.zeros_like(normals)
+                _lib.check(lib.psr_backward_inputs(
+                    handle, gcoef.data_ptr(), gcum.data_ptr(), ctypes.c_double(iso_seed),
+                    gtransform.data_ptr() if gtransform is not None else None, positions.data_ptr(),
+                    normals.data_ptr(), normals.shape[0], gpositions.data_ptr() if want_p else None,
+                    gnormals.data_ptr() if want_n else None, stream))
+            return gpositions, gnormals, gquery, None, None
 
     _FieldQuery.fn = FieldQuery
     return FieldQuery
@@ -77,12 +91,13 @@ def _function():
 class DifferentiableReconstruction:
     """Reference DifferentiableReconstruction (reconstruction_adjoint.py:44-58)."""
 
-    def __init__(self, primal, normals):
+    def __init__(self, primal, positions, normals):
         self._primal = primal
+        self._positions = positions
         self._normals = normals
         self.metadata = dict(primal.metadata)
         self.metadata["differentiability"] = {
-            "mode": "finite_algorithm_fixed_assignments", "inputs": ["normals"],
+            "mode": "finite_algorithm_fixed_assignments", "inputs": ["positions", "normals"],
             "constant_options": ["point_weight", "scale", "samples_per_node", "cg_accuracy", "depths", "iterations"],
             "first_order_only": True}
         self.diagnostics = primal.diagnostics
@@ -97,27 +112,30 @@ class DifferentiableReconstruction:
         if (not isinstance(queries, torch.Tensor) or queries.dtype != self._normals.dtype
                 or queries.device != self._normals.device or queries.ndim != 2 or queries.shape[1] != 3):
             raise ValueError("queries must match field precision and device")
-        return _function().apply(self._normals, queries, self, bool(centered))
+        return _function().apply(self._positions, self._normals, queries, self, bool(centered))
 
     def device_field(self):
         return self._primal.device_field()
 
 
 def reconstruct_differentiable(positions, normals, **options):
-    """Reconstruct, then differentiate field losses with respect to ``normals``.
+    """Reconstruct, then differentiate field losses with respect to the inputs
+    (reconstruction_adjoint.py:61-185).
 
-    ``normals`` must require grad; ``positions`` must not (their reverse mode
-    is not provided).  Options are reconstruct()'s keywords (constants).
+    ``positions`` and/or ``normals`` must require grad.  Position gradients
+    need a run without screening (``point_weight=0`` or
+    ``exact_interpolation=True``).  Options are reconstruct()'s keywords
+    (constants).
     """
     torch = _torch()
     if not isinstance(positions, torch.Tensor) or not isinstance(normals, torch.Tensor):
         raise TypeError("positions and normals must be tensors")
-    if positions.requires_grad:
-        raise ValueError("gradients with respect to positions are not provided; pass positions.detach()")
-    if not normals.requires_grad:
-        raise ValueError("track normals with requires_grad=True")
+    if not positions.requires_grad and not normals.requires_grad:
+        raise ValueError("track positions and/or normals with requires_grad=True")
     for name, value in options.items():
         if isinstance(value, torch.Tensor):
-            raise ValueError(f"{name} must be a constant scalar; only normals support gradients")
+            raise ValueError(f"{name} must be a constant scalar; only positions and normals support gradients")
+    if positions.requires_grad and options.get("point_weight", 2.0) > 0 and not options.get("exact_interpolation", False):
+        raise ValueError("gradients with respect to positions need point_weight=0 or exact_interpolation=True")
     primal = reconstruct(positions.detach(), normals.detach(), **options)
-    return DifferentiableReconstruction(primal, normals)
+    return DifferentiableReconstruction(primal, positions, normals)
