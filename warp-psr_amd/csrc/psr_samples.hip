@@ -49,22 +49,40 @@ __global__ void k_bounds_partial(const T* __restrict__ pts, int64_t n, T* __rest
   }
 }
 
+// One workgroup folds the per-block partials (min/max of finite values are
+// order-independent).
 template <class T>
-__global__ void k_bounds_final(const T* __restrict__ partial, const int* __restrict__ bad, int blocks, T* __restrict__ out,
-                               int* __restrict__ out_bad) {
-  if (threadIdx.x != 0) return;
+__global__ void __launch_bounds__(256) k_bounds_final(const T* __restrict__ partial, const int* __restrict__ bad, int blocks,
+                                                      T* __restrict__ out, int* __restrict__ out_bad) {
+  __shared__ T sm[6][256];
+  __shared__ int sb[256];
   T b[6];
   for (int a = 0; a < 3; ++a) { b[a] = T(INFINITY); b[a + 3] = -T(INFINITY); }
   int invalid = 0;
-  for (int k = 0; k < blocks; ++k) {
+  for (int k = threadIdx.x; k < blocks; k += 256) {
     for (int a = 0; a < 3; ++a) {
       b[a] = tmin(b[a], partial[k * 6 + a]);
       b[a + 3] = tmax(b[a + 3], partial[k * 6 + a + 3]);
     }
     invalid += bad[k];
   }
-  for (int a = 0; a < 6; ++a) out[a] = b[a];
-  *out_bad = invalid;
+  for (int a = 0; a < 6; ++a) sm[a][threadIdx.x] = b[a];
+  sb[threadIdx.x] = invalid;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) {
+      for (int a = 0; a < 3; ++a) {
+        sm[a][threadIdx.x] = tmin(sm[a][threadIdx.x], sm[a][threadIdx.x + st]);
+        sm[a + 3][threadIdx.x] = tmax(sm[a + 3][threadIdx.x], sm[a + 3][threadIdx.x + st]);
+      }
+      sb[threadIdx.x] += sb[threadIdx.x + st];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    for (int a = 0; a < 6; ++a) out[a] = sm[a][0];
+    *out_bad = sb[0];
+  }
 }
 
 template <class T> struct Xform { T t[5]; };
@@ -104,34 +122,102 @@ __global__ void k_classify(const T* __restrict__ pts, const T* __restrict__ nrm,
   status[i] = st;
 }
 
-// samples.py:149-188 (_groups): serial sums over each run of equal leaf keys
+// samples.py:149-188 (_groups): serial sums over each run of equal leaf keys.
+// A run is [head_pos[g], end): the next head, or for the last group the end
+// of its equal keys.
+__device__ __forceinline__ int64_t group_end(const uint64_t* __restrict__ keys, const int* __restrict__ head_pos, int g, int G,
+                                             int64_t n) {
+  if (g + 1 < G) return head_pos[g + 1];
+  const uint64_t key = keys[head_pos[g]];
+  int64_t lo = head_pos[g], hi = n;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (keys[mid] <= key) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+template <class T>
+__device__ __forceinline__ void group_fields(const uint64_t key, int g, int G, int64_t i, int64_t end, int depth, const int* order,
+                                             int64_t* unique, int* first, int* goff) {
+  const uint64_t mask = ((uint64_t)1 << depth) - 1;
+  unique[g] = basis_key(depth, (int)(key >> (2 * depth)), (int)((key >> depth) & mask), (int)(key & mask));
+  first[g] = order[i];
+  goff[g] = (int)i;
+  if (g == G - 1) goff[G] = (int)end;
+}
+
+constexpr int GROUP_SERIAL_MAX = 16;  // longer runs are summed by a wave (k_groups_wide)
+
 template <class T>
 __global__ void k_groups(const uint64_t* __restrict__ keys, const int* __restrict__ order, const int* __restrict__ head_pos,
                          int G, int64_t n, int depth, const T* __restrict__ q, const T* __restrict__ unit,
                          int64_t* __restrict__ unique, T* __restrict__ sums, T* __restrict__ nsums, T* __restrict__ weights,
-                         int* __restrict__ first, int* __restrict__ goff, int* __restrict__ group_of) {
+                         int* __restrict__ first, int* __restrict__ goff, int* __restrict__ group_of, int* __restrict__ wide,
+                         int* __restrict__ wide_count) {
   int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= G) return;
   const int64_t i = head_pos[g];
-  const uint64_t key = keys[i];
+  const int64_t end = group_end(keys, head_pos, g, G, n);
+  if (end - i > GROUP_SERIAL_MAX) {
+    wide[atomicAdd(wide_count, 1)] = g;
+    return;
+  }
   T p0 = T(0), p1 = T(0), p2 = T(0), s0 = T(0), s1 = T(0), s2 = T(0), w = T(0);
-  int64_t j = i;
-  while (j < n && keys[j] == key) {
+  for (int64_t j = i; j < end; ++j) {
     const int o = order[j];
     p0 = p0 + q[3 * o]; p1 = p1 + q[3 * o + 1]; p2 = p2 + q[3 * o + 2];
     s0 = s0 + unit[3 * o]; s1 = s1 + unit[3 * o + 1]; s2 = s2 + unit[3 * o + 2];
     w = w + T(1);
     group_of[o] = g;
-    ++j;
   }
-  const uint64_t mask = ((uint64_t)1 << depth) - 1;
-  unique[g] = basis_key(depth, (int)(key >> (2 * depth)), (int)((key >> depth) & mask), (int)(key & mask));
+  group_fields<T>(keys[i], g, G, i, end, depth, order, unique, first, goff);
   sums[3 * g] = p0; sums[3 * g + 1] = p1; sums[3 * g + 2] = p2;
   nsums[3 * g] = s0; nsums[3 * g + 1] = s1; nsums[3 * g + 2] = s2;
   weights[g] = w;
-  first[g] = order[i];
-  goff[g] = (int)i;
-  if (g == G - 1) goff[G] = (int)j;
+}
+
+// Long runs, one wave each: the lanes load 64 members at a time and every
+// lane adds them in run order through lane broadcasts, so the sums equal the
+// serial ones bit for bit.
+template <class T>
+__global__ void __launch_bounds__(64) k_groups_wide(const uint64_t* __restrict__ keys, const int* __restrict__ order,
+                                                    const int* __restrict__ head_pos, int G, int64_t n, int depth,
+                                                    const T* __restrict__ q, const T* __restrict__ unit,
+                                                    int64_t* __restrict__ unique, T* __restrict__ sums, T* __restrict__ nsums,
+                                                    T* __restrict__ weights, int* __restrict__ first, int* __restrict__ goff,
+                                                    int* __restrict__ group_of, const int* __restrict__ wide,
+                                                    const int* __restrict__ wide_count) {
+  const int lane = threadIdx.x;
+  const int count = *wide_count;
+  for (int k = blockIdx.x; k < count; k += gridDim.x) {
+    const int g = wide[k];
+    const int64_t i = head_pos[g];
+    const int64_t end = group_end(keys, head_pos, g, G, n);
+    T p0 = T(0), p1 = T(0), p2 = T(0), s0 = T(0), s1 = T(0), s2 = T(0), w = T(0);
+    for (int64_t base = i; base < end; base += 64) {
+      const int64_t j = base + lane;
+      T v[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
+      if (j < end) {
+        const int o = order[j];
+        v[0] = q[3 * o]; v[1] = q[3 * o + 1]; v[2] = q[3 * o + 2];
+        v[3] = unit[3 * o]; v[4] = unit[3 * o + 1]; v[5] = unit[3 * o + 2];
+        group_of[o] = g;
+      }
+      const int m = (int)(end - base < 64 ? end - base : 64);
+      for (int t = 0; t < m; ++t) {
+        p0 = p0 + __shfl(v[0], t, 64); p1 = p1 + __shfl(v[1], t, 64); p2 = p2 + __shfl(v[2], t, 64);
+        s0 = s0 + __shfl(v[3], t, 64); s1 = s1 + __shfl(v[4], t, 64); s2 = s2 + __shfl(v[5], t, 64);
+        w = w + T(1);
+      }
+    }
+    if (lane == 0) {
+      group_fields<T>(keys[i], g, G, i, end, depth, order, unique, first, goff);
+      sums[3 * g] = p0; sums[3 * g + 1] = p1; sums[3 * g + 2] = p2;
+      nsums[3 * g] = s0; nsums[3 * g + 1] = s1; nsums[3 * g + 2] = s2;
+      weights[g] = w;
+    }
+  }
 }
 
 __global__ void k_fill_i32(int* p, int64_t n, int v) {
@@ -149,7 +235,7 @@ template <class T> void stage_samples(Result<T>& r, const T* pts, const T* nrm) 
   DArray<int> bad(blocks, s), bad_total(1, s);
   hipLaunchKernelGGL(k_bounds_partial<T>, dim3(blocks), dim3(256), 0, s, pts, n, partial.data(), bad.data());
   KERNEL_CHECK();
-  hipLaunchKernelGGL(k_bounds_final<T>, dim3(1), dim3(64), 0, s, (const T*)partial.data(), (const int*)bad.data(), blocks,
+  hipLaunchKernelGGL(k_bounds_final<T>, dim3(1), dim3(256), 0, s, (const T*)partial.data(), (const int*)bad.data(), blocks,
                      bounds.data(), bad_total.data());
   KERNEL_CHECK();
   struct { T b[6]; int bad; } host;
@@ -189,10 +275,19 @@ template <class T> void stage_samples(Result<T>& r, const T* pts, const T* nrm) 
   r.input_group.reset(n, s);
   hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(n, 256)), dim3(256), 0, s, r.input_group.data(), n, -1);
   KERNEL_CHECK();
+  DArray<int> wide(G, s), wide_count(1, s);
+  wide_count.zero();
   hipLaunchKernelGGL(k_groups<T>, dim3(blocks_for(G, 256)), dim3(256), 0, s, (const uint64_t*)keys.data(),
                      (const int*)order.data(), (const int*)head_pos.data(), G, n, depth, (const T*)r.q.data(),
                      (const T*)r.unit_normals.data(), r.sample_keys.data(), r.position_sums.data(), r.normal_sums.data(),
-                     r.weights.data(), r.first_indices.data(), r.group_offsets.data(), r.input_group.data());
+                     r.weights.data(), r.first_indices.data(), r.group_offsets.data(), r.input_group.data(), wide.data(),
+                     wide_count.data());
+  KERNEL_CHECK();
+  hipLaunchKernelGGL(k_groups_wide<T>, dim3(256), dim3(64), 0, s, (const uint64_t*)keys.data(), (const int*)order.data(),
+                     (const int*)head_pos.data(), G, n, depth, (const T*)r.q.data(), (const T*)r.unit_normals.data(),
+                     r.sample_keys.data(), r.position_sums.data(), r.normal_sums.data(), r.weights.data(),
+                     r.first_indices.data(), r.group_offsets.data(), r.input_group.data(), (const int*)wide.data(),
+                     (const int*)wide_count.data());
   KERNEL_CHECK();
   r.input_order = std::move(order);
 }
@@ -505,18 +600,21 @@ __global__ void __launch_bounds__(256) k_normal_splat_count(const int* __restric
   if (threadIdx.x == 0) block_counts[blockIdx.x] = count;
 }
 
-// The valid records, compacted in record order: key top<<3D | x<<2D | y<<D | z
-// and the record id (54 * group + slot).
+// The valid records, compacted in record order: key top<<3D | x<<2D | y<<D | z,
+// the compacted position (the sort payload) and the record's value.
 template <class T, class K>
 __global__ void __launch_bounds__(256) k_normal_splat_keys(const int* __restrict__ order, int G, const T* __restrict__ sums,
                                                            const T* __restrict__ nsums, const T* __restrict__ weights,
                                                            const T* __restrict__ dw, int minimum, int maximum,
                                                            const int* __restrict__ block_offsets, int64_t capacity,
-                                                           K* __restrict__ out, int* __restrict__ ids) {
+                                                           K* __restrict__ out, int* __restrict__ ids, T* __restrict__ vals) {
   __shared__ int wave_count[4];
   const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
   NormalRecord<T> rec;
-  const bool valid = normal_record_valid(idx, G, order, sums, nsums, weights, dw, minimum, maximum, &rec);
+  rec.valid = false;
+  if (idx < 54 * (int64_t)G)
+    rec = normal_record<T>(order[idx / 54], (int)(idx % 54), sums, nsums, weights, dw, minimum, maximum, true);
+  const bool valid = rec.valid;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint64_t ballot = __ballot(valid);
   if (lane == 0) wave_count[wave] = __popcll(ballot);
@@ -527,24 +625,24 @@ __global__ void __launch_bounds__(256) k_normal_splat_keys(const int* __restrict
   if (pos >= capacity) return;  // the counting pass saw the same records
   const int D = maximum;
   out[pos] = ((K)rec.top << (3 * D)) | ((K)rec.x << (2 * D)) | ((K)rec.y << D) | (K)rec.z;
-  ids[pos] = (int)idx;
+  ids[pos] = pos;
+  vals[3 * (int64_t)pos] = rec.v0;
+  vals[3 * (int64_t)pos + 1] = rec.v1;
+  vals[3 * (int64_t)pos + 2] = rec.v2;
 }
 
 template <class T, class K>
-__global__ void k_normal_splat_reduce(const K* __restrict__ keys, const int* __restrict__ ids,
-                                      const int* __restrict__ head_pos, int count, int64_t total,
-                                      const int* __restrict__ order, const T* __restrict__ sums,
-                                      const T* __restrict__ nsums, const T* __restrict__ weights, const T* __restrict__ dw,
-                                      int minimum, int maximum, int64_t* __restrict__ out_keys, T* __restrict__ out_vals) {
+__global__ void k_normal_splat_reduce(const K* __restrict__ keys, const int* __restrict__ ids, const T* __restrict__ vals,
+                                      const int* __restrict__ head_pos, int count, int64_t total, int maximum,
+                                      int64_t* __restrict__ out_keys, T* __restrict__ out_vals) {
   int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= count) return;
   int64_t j = head_pos[g];
   const K key = keys[j];
   T a = T(0), b = T(0), c = T(0);
   while (j < total && keys[j] == key) {
-    const int id = ids[j];
-    NormalRecord<T> rec = normal_record<T>(order[id / 54], id % 54, sums, nsums, weights, dw, minimum, maximum, true);
-    a = a + rec.v0; b = b + rec.v1; c = c + rec.v2;
+    const T* v = vals + 3 * (int64_t)ids[j];
+    a = a + v[0]; b = b + v[1]; c = c + v[2];
     ++j;
   }
   const int D = maximum;
@@ -652,10 +750,11 @@ template <class T, class K> void normal_splats(Result<T>& r, int minimum, int ma
   const int64_t valid = read_int(total.data(), s);
   DArray<K> rkeys(valid, s);
   DArray<int> rids(valid, s), heads(valid, s), prefix(valid, s);
+  DArray<T> rvals(3 * (size_t)valid, s);
   hipLaunchKernelGGL((k_normal_splat_keys<T, K>), dim3(blocks), dim3(256), 0, s, (const int*)r.normal_order.data(), G,
                      (const T*)r.position_sums.data(), (const T*)r.normal_sums.data(), (const T*)r.weights.data(),
                      (const T*)r.dw.data(), minimum, maximum, (const int*)block_offsets.data(), valid, rkeys.data(),
-                     rids.data());
+                     rids.data(), rvals.data());
   KERNEL_CHECK();
   sort_pairs(rkeys.data(), rids.data(), valid, bits_for((uint64_t)sentinel), s);
   mark_heads(rkeys.data(), valid, sentinel, heads.data(), s);
@@ -667,10 +766,8 @@ template <class T, class K> void normal_splats(Result<T>& r, int minimum, int ma
   r.normal_vals.reset(3 * (size_t)count, s);
   if (count) {
     hipLaunchKernelGGL((k_normal_splat_reduce<T, K>), dim3(blocks_for(count, 256)), dim3(256), 0, s,
-                       (const K*)rkeys.data(), (const int*)rids.data(), (const int*)head_pos.data(), count, valid,
-                       (const int*)r.normal_order.data(), (const T*)r.position_sums.data(),
-                       (const T*)r.normal_sums.data(), (const T*)r.weights.data(), (const T*)r.dw.data(), minimum,
-                       maximum, r.normal_keys.data(), r.normal_vals.data());
+                       (const K*)rkeys.data(), (const int*)rids.data(), (const T*)rvals.data(), (const int*)head_pos.data(),
+                       count, valid, maximum, r.normal_keys.data(), r.normal_vals.data());
     KERNEL_CHECK();
   }
 }
