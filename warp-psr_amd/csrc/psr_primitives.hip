@@ -7,6 +7,9 @@
 #include "psr_runtime.h"
 
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <tuple>
 
 namespace psr {
 
@@ -432,6 +435,15 @@ void read_bytes(void* host, const void* dev, size_t bytes, hipStream_t s) {
     HIP_CHECK(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
   }
+}
+
+hipStream_t companion_stream(int dev, hipStream_t s, int role) {
+  static std::mutex mu;
+  static std::map<std::tuple<int, hipStream_t, int>, hipStream_t> streams;
+  std::lock_guard<std::mutex> g(mu);
+  hipStream_t& c = streams[std::make_tuple(dev, s, role)];
+  if (!c) HIP_CHECK(hipStreamCreateWithFlags(&c, hipStreamNonBlocking));
+  return c;
 }
 
 int read_int(const int* dev, hipStream_t s) {

@@ -76,7 +76,7 @@ template <class T> struct Result {
   DArray<int> normal_order;
   // The serial normal average runs on a side stream, overlapped with the
   // tree stage; its input stays alive until the main stream joins it.
-  hipStream_t side = nullptr;
+  hipStream_t side = nullptr;  // a companion stream kept for the process (not owned)
   hipEvent_t average_done = nullptr;
   bool average_pending = false;
   DArray<T> average_contrib;
@@ -132,10 +132,6 @@ template <class T> struct Result {
     if (average_done) {
       (void)hipEventSynchronize(average_done);
       (void)hipEventDestroy(average_done);
-    }
-    if (side) {
-      (void)hipStreamSynchronize(side);
-      (void)hipStreamDestroy(side);
     }
     for (auto& st : relax_stats) {
       (void)hipEventDestroy(st.start);

@@ -165,6 +165,11 @@ inline void mark_heads(const uint32_t* keys, int64_t n, uint32_t sentinel, int* 
 inline void mark_heads(const uint64_t* keys, int64_t n, uint64_t sentinel, int* heads, hipStream_t s) {
   mark_heads_u64(keys, n, sentinel, heads, s);
 }
+// A companion stream of caller stream s on device dev for one role (0: the
+// serial normal average, 1: finer-level operator assembly): non-blocking,
+// default priority, created on first use and kept for the process, so its
+// pool blocks and queues are reused from call to call.
+hipStream_t companion_stream(int dev, hipStream_t s, int role);
 // Reads one int from device memory (synchronises the stream).
 int read_int(const int* dev, hipStream_t s);
 void read_bytes(void* host, const void* dev, size_t bytes, hipStream_t s);

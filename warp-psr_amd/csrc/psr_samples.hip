@@ -798,7 +798,11 @@ template <class T> void stage_normals(Result<T>& r) {
                      (const int*)r.normal_order.data(), G, (const T*)r.normal_sums.data(), (const T*)r.weights.data(),
                      (const T*)r.dw.data(), r.average_contrib.data());
   KERNEL_CHECK();
-  if (!r.side) HIP_CHECK(hipStreamCreateWithFlags(&r.side, hipStreamNonBlocking));
+  if (!r.side) {
+    int dev = 0;
+    HIP_CHECK(hipGetDevice(&dev));
+    r.side = companion_stream(dev, s, 0);
+  }
   if (!r.average_done) HIP_CHECK(hipEventCreateWithFlags(&r.average_done, hipEventDisableTiming));
   {
     hipEvent_t ready;

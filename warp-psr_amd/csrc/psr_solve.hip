@@ -1850,14 +1850,7 @@ template <class V, class T> static Stencil<V>& finer_operator(LevelState<T>& L, 
 
 // One assembly stream per (device, solve stream), kept for the process: its
 // pool blocks and the host thread's launches are reused by every call.
-static hipStream_t assembly_stream(int dev, hipStream_t s) {
-  static std::mutex mu;
-  static std::map<std::pair<int, hipStream_t>, hipStream_t> streams;
-  std::lock_guard<std::mutex> g(mu);
-  hipStream_t& as = streams[{dev, s}];
-  if (!as) HIP_CHECK(hipStreamCreateWithFlags(&as, hipStreamNonBlocking));
-  return as;
-}
+static hipStream_t assembly_stream(int dev, hipStream_t s) { return companion_stream(dev, s, 1); }
 
 // Builds the finer levels' operators on the assembly stream from a host
 // thread while the solve stream runs the base solve and the coarser levels'
