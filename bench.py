@@ -82,6 +82,9 @@ def relax_roofline(metadata, value_bytes, vector_bytes, kernel=None):
         "avg_launch_us": round(seconds / launches * 1e6, 3),
         "finest": {"depth": fine["depth"], "rows": fine["rows"], "nnz": fine["nnz"], "launches": fine["launches"],
                    "seconds": fine["seconds"]},
+        "per_level": [{"depth": st["depth"], "launches": st["launches"], "us_per_launch": round(st["seconds"] / st["launches"] * 1e6, 3),
+                       "GBps": round((st["nnz"] * (4 + v) + (st["rows"] + 1) * 4 + 3 * st["rows"] * s) * st["sweeps"]
+                                     / st["seconds"] / 1e9, 1)} for st in stats],
     }
 
 
