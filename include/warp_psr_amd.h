@@ -140,9 +140,12 @@ int psr_relax_sliced(int dtype, const int64_t* keys, int64_t rows, const int32_t
  * and `normals` are the forward's inputs.  Position gradients go through the
  * isovalue centroids, the normal splats' basis, depth and weight, the density
  * levels and aggregates, and the sample transform with its extrema owners
- * (sample_adjoint.py:16-102, splat_adjoint.py:104-188); they need a run
- * without screening (point_weight 0 or exact_interpolation), else the call
- * returns 2. */
+ * (sample_adjoint.py:16-102, splat_adjoint.py:104-188) and, in screened
+ * runs, the constraint points' screening terms in each level's operator,
+ * cross operator and point load (assembly_adjoint.py:44-91,
+ * constraints.py:55-84), the base hierarchy's Galerkin products and the
+ * normal average's weight.  Screened position gradients replay every
+ * relaxation once (about the forward solve's cost again). */
 int psr_field_vjp(PsrResult* result, const void* queries, int64_t m, const void* value_seed, const void* grad_seed,
                   double* coef_grad, double* cum_grad, void* query_grad, void* stream);
 int psr_backward(PsrResult* result, const double* coef_grad, const double* cum_grad, double iso_seed, const void* normals,

@@ -122,16 +122,25 @@ def test_gradient_accumulates_and_repeats(cuda):
     np.testing.assert_allclose(tn.grad.cpu().numpy(), 2 * first.cpu().numpy(), rtol=1e-9, atol=1e-12)
 
 
-# Unscreened runs: positions enter through the sample transform (bounds and
-# their extrema owners), the group sums, the density levels and depth/weight,
-# the normal splats' B-spline weights and the isovalue centroids
-# (reconstruction_adjoint.py:89-177, sample_adjoint.py, splat_adjoint.py).
+# Positions enter through the sample transform (bounds and their extrema
+# owners), the group sums, the density levels and depth/weight, the normal
+# splats' B-spline weights and the isovalue centroids
+# (reconstruction_adjoint.py:89-177, sample_adjoint.py, splat_adjoint.py);
+# screened runs add the constraint points: their screening terms in every
+# level's operator, cross operator and point load, the base hierarchy's
+# Galerkin products, the root conjugate gradient's matrix and the normal
+# average's weight (assembly_adjoint.py:44-91, constraints.py:55-84).
 POSITION_CASES = [
     ("psr", "sphere", 512, "double", dict(depth=4, full_depth=2, point_weight=0.0)),
     ("exact", "sphere", 512, "double", dict(depth=4, full_depth=2, point_weight=2.0, exact_interpolation=True)),
     ("root-padded-psr", "sphere", 512, "double", dict(depth=4, full_depth=2, base_depth=0, point_weight=0.0)),
     ("torus-psr", "torus", 8192, "double", dict(depth=6, full_depth=3, point_weight=0.0)),
     ("psr-float", "sphere", 512, "float", dict(depth=4, full_depth=2, point_weight=0.0)),
+    ("screened", "sphere", 512, "double", dict(depth=4, full_depth=2, point_weight=2.0)),
+    ("cycles", "sphere", 512, "double", dict(depth=4, full_depth=2, point_weight=2.0, base_v_cycles=2, iters=3)),
+    ("root-screened", "sphere", 512, "double", dict(depth=4, full_depth=0, base_depth=0, point_weight=2.0)),
+    ("torus-screened", "torus", 8192, "double", dict(depth=6, full_depth=3, point_weight=2.0)),
+    ("screened-float", "sphere", 512, "float", dict(depth=4, full_depth=2, point_weight=2.0)),
 ]
 
 

@@ -111,16 +111,14 @@ def test_metadata_parser_shapes():
 
 
 def test_differentiable_validates_tracked_inputs():
-    """Tracked positions need an unscreened run; something must be tracked
-    (reconstruction_adjoint.py:70-85); options are constants."""
+    """Something must be tracked (reconstruction_adjoint.py:70-85); options
+    are constants."""
     import torch
     from warp_psr_amd import reconstruct_differentiable
     p = torch.zeros((4, 3), dtype=torch.float64, requires_grad=True)
     n = torch.ones((4, 3), dtype=torch.float64, requires_grad=True)
-    with pytest.raises(ValueError, match="point_weight=0 or exact_interpolation"):
-        reconstruct_differentiable(p, n)
-    with pytest.raises(ValueError, match="point_weight=0 or exact_interpolation"):
-        reconstruct_differentiable(p, n.detach(), point_weight=1.0)
+    with pytest.raises(TypeError, match="tensors"):
+        reconstruct_differentiable(p.detach().numpy(), n)
     with pytest.raises(ValueError, match="requires_grad"):
         reconstruct_differentiable(p.detach(), n.detach())
     with pytest.raises(ValueError, match="constant scalar"):

@@ -382,6 +382,23 @@ __global__ void k_aggregate_to_groups(const int64_t* __restrict__ gkeys, int G, 
   for (int a = 0; a < 3; ++a) gsums[3 * g + a] += gagg[3 * idx + a];
 }
 
+// normals.py:111-163 reversed for the weight component: average[1] =
+// sum(dw[2s+1] w_s) / sum(w_s) over the groups with a nonzero mean normal,
+// so its gradient reaches each such group's splat weight as w_s / sum(w_s).
+template <class T>
+__global__ void k_average_vjp(int G, const T* __restrict__ nsums, const T* __restrict__ weights, const T* __restrict__ average,
+                              const double* __restrict__ gavg1, double* __restrict__ gdw) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= G) return;
+  const double ga = gavg1[0];
+  const T total = average[2];
+  if (ga == 0.0 || !(total > T(0))) return;
+  const T w = weights[g];
+  const T n0 = nsums[3 * g] / w, n1 = nsums[3 * g + 1] / w, n2 = nsums[3 * g + 2] / w;
+  if (!(sqrt(n0 * n0 + n1 * n1 + n2 * n2) > T(0))) return;
+  gdw[2 * g + 1] += ga * double(w) / double(total);
+}
+
 // samples.py:90-134 reversed for the positions: q = (P1, P2, P0) * inv +
 // (t1, t2, t3) summed per group.  The transform's own gradients (inv, t1..t3)
 // are reduced into acc[4] for the bounds' extrema owners.
@@ -560,7 +577,11 @@ void backward_inputs(Result<T>& r, const double* gcoef_in, const double* gcum_in
   const int64_t records = (int64_t)r.normal_keys.size();
   DArray<double> grec(3 * (size_t)records, s);
   grec.zero();
-  solve_backward(r, (const double*)gcoef.data(), grec.data(), s);
+  // screened runs: the positions also reach the operators through the
+  // constraint points (screening_stencils.py:34-70, constraints.py:55-84)
+  const bool screened = r.prm.point_weight > 0 && !r.prm.exact_interpolation;
+  SolveGrads mg;
+  solve_backward(r, (const double*)gcoef.data(), grec.data(), s, positions && screened ? &mg : (SolveGrads*)nullptr);
   // normal splats -> normal sums (-> input normals), centroids and (depth, weight)
   DArray<int64_t> noff(r.normal_offsets.size(), s);
   std::vector<int64_t> ho(r.normal_offsets.begin(), r.normal_offsets.end());
@@ -583,6 +604,22 @@ void backward_inputs(Result<T>& r, const double* gcoef_in, const double* gcum_in
   if (gnormals) {
     hipLaunchKernelGGL(k_unit_normal_vjp<T>, dim3(blocks_for(n, 256)), dim3(256), 0, s, normals, n, r.transform[4],
                        (const int*)r.input_group.data(), (const int*)r.input_status.data(), (const double*)gns.data(), gnormals);
+    KERNEL_CHECK();
+  }
+  if (positions && screened) {
+    std::vector<DArray<double>> gagg;
+    DArray<double> gavg1(1, s);
+    gavg1.zero();
+    screening_backward(r, mg, gagg, gavg1.data(), s);
+    for (int d = r.prm.base_depth; d <= mg.top && d < (int)gagg.size(); ++d) {
+      const int64_t m = (int64_t)r.levels[d].agg_keys.size();
+      if (gagg[d].empty() || !m) continue;
+      hipLaunchKernelGGL(k_aggregate_to_groups, dim3(blocks_for(G, 256)), dim3(256), 0, s, (const int64_t*)r.sample_keys.data(),
+                         G, D, (const int64_t*)r.levels[d].agg_keys.data(), m, d, (const double*)gagg[d].data(), gsums.data());
+      KERNEL_CHECK();
+    }
+    hipLaunchKernelGGL(k_average_vjp<T>, dim3(blocks_for(G, 256)), dim3(256), 0, s, G, (const T*)r.normal_sums.data(),
+                       (const T*)r.weights.data(), (const T*)r.normal_average.data(), (const double*)gavg1.data(), gdw.data());
     KERNEL_CHECK();
   }
   if (positions) {

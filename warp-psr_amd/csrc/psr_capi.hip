@@ -595,8 +595,6 @@ int psr_backward_inputs(PsrResult* result, const double* coef_grad, const double
     auto run = [&](auto* r, auto tag) {
       using T = decltype(tag);
       if (n != r->n) throw PsrError(2, "points and normals must be the reconstruction's inputs");
-      if (grad_points && r->prm.point_weight > 0 && !r->prm.exact_interpolation)
-        throw PsrError(2, "gradients with respect to positions need point_weight=0 or exact_interpolation=True");
       backward_inputs(*r, coef_grad, cum_grad, iso_seed, transform_grad, (const T*)points, (const T*)normals, n, (T*)grad_points,
                       (T*)grad_normals, s);
     };

@@ -163,10 +163,27 @@ void backward_normals(Result<T>& r, const double* gcoef, const double* gcum, dou
 template <class T>
 void backward_inputs(Result<T>& r, const double* gcoef, const double* gcum, double iso_seed, const double* gtransform,
                      const T* points, const T* normals, int64_t n, T* gpoints, T* gnormals, hipStream_t s);
+// Matrix-side gradients of a solve, per level d in [base, top]: w.r.t. the
+// operator values (gA: the level's storage layout, schedule positions for
+// finer levels and canonical rows for the base level), the finer levels'
+// cross operators (gC, schedule positions) and the right-hand sides (grhs,
+// canonical).
+struct SolveGrads {
+  int top = -1;
+  std::vector<DArray<double>> gA, gC, grhs;
+};
 // Reverse of the cascadic solve and normal loads: gcoef (float64, one per
 // concatenated coefficient) gives float64 gradients of the normal records'
-// three values (accumulated into grecords).
-template <class T> void solve_backward(Result<T>& r, const double* gcoef, double* grecords, hipStream_t s);
+// three values (accumulated into grecords); with mg, also the matrix-side
+// gradients (the relaxations are replayed once for the x they saw).
+template <class T>
+void solve_backward(Result<T>& r, const double* gcoef, double* grecords, hipStream_t s, SolveGrads* mg = nullptr);
+// Screening reversed (assembly_adjoint.py:44-91, constraints.py:55-84): the
+// operator, cross-operator and point-load gradients of mg -> the constraint
+// points' aggregates (gagg[d], 3 per aggregate of level d, accumulated) and
+// the normal average's weight component (gavg1, device scalar, accumulated).
+template <class T>
+void screening_backward(Result<T>& r, const SolveGrads& mg, std::vector<DArray<double>>& gagg, double* gavg1, hipStream_t s);
 template <class T> void stage_solve(Result<T>& r);
 template <class T> void stage_field(Result<T>& r);
 template <class T>

@@ -987,10 +987,13 @@ __global__ void k_root_solve(const int* __restrict__ cols, const T* __restrict__
 // padded 64 rows with the mean term, else R) enters as the gradient w.r.t.
 // the solution and leaves as the gradient w.r.t. the initial x; grhs
 // (canonical) accumulates the right-hand-side gradient.
+// With gvals, the root operator's value gradients (its canonical 27-slot
+// layout) are accumulated from every A-product of the iteration.
 template <class T>
 __global__ void k_root_cg_vjp(const int* __restrict__ cols, const T* __restrict__ vals, int R, const int64_t* __restrict__ keys,
                               const T* __restrict__ rhs, const T* __restrict__ x_init, int dc, int base, int cycle, double acc,
-                              int maxiter, double* __restrict__ store, double* __restrict__ gx, double* __restrict__ grhs) {
+                              int maxiter, double* __restrict__ store, double* __restrict__ gx, double* __restrict__ grhs,
+                              double* __restrict__ gvals) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   const int m = dc ? 64 : R;
   T b[64], xx[64], r[64], p[64], ap[64];
@@ -1060,6 +1063,15 @@ __global__ void k_root_cg_vjp(const int* __restrict__ cols, const T* __restrict_
       for (int i = 0; i < 64; ++i) out[i] += mean;
     }
   };
+  // gvals[k][i] += a * u[row(i)] * v[row(col)] for out = A v contributions
+  auto outer = [&](const double* u, const double* v, double a) {
+    for (int i = 0; i < R; ++i)
+      for (int k = 0; k < 27; ++k) {
+        const int c = cols[k * R + i];
+        if (c < 0) continue;
+        gvals[k * R + i] += a * u[dc ? root_row(keys, i) : i] * v[dc ? root_row(keys, c) : c];
+      }
+  };
   double xb[64], rb[64], pb[64], pk[64], apb[64], tmp[64];
   for (int i = 0; i < m; ++i) { xb[i] = gx[i]; rb[i] = 0.0; pb[i] = 0.0; }
   double rzb = 0.0;
@@ -1091,12 +1103,18 @@ __global__ void k_root_cg_vjp(const int* __restrict__ cols, const T* __restrict_
     // ap_k = A p_k
     apply_t(apb, tmp);
     for (int i = 0; i < m; ++i) pb[i] = pk[i] + tmp[i];
+    if (gvals) outer(apb, pk_, 1.0);
     rzb = rzbk;
   }
   // p_0 = r_0, rz_0 = r_0 . r_0, r_0 = b - A x_init
   const double* r0 = vec(0, 0);
   for (int i = 0; i < m; ++i) rb[i] += pb[i] + 2.0 * rzb * r0[i];
   apply_t(rb, tmp);
+  if (gvals) {
+    double xi[64];
+    for (int i = 0; i < m; ++i) xi[i] = double(x_init[i]);
+    outer(rb, xi, -1.0);
+  }
   for (int i = 0; i < m; ++i) gx[i] = xb[i] - tmp[i];
   for (int i = 0; i < R; ++i) grhs[i] += rb[dc ? root_row(keys, i) : i];
 }
@@ -2124,6 +2142,315 @@ __global__ void __launch_bounds__(256) k_gs_reverse(const int* __restrict__ orde
   }
 }
 
+// ---- matrix-value gradients (relaxation_adjoint.py:29-73, multigrid_adjoint.py,
+// assembly_adjoint.py:44-91): the reverse sweeps need the x each row saw, so
+// the forward relaxations are replayed once with x snapshots.
+
+// colour of a row in the sliced order: (x&1) + 2(y&1); in the 8-colour order
+// also + 4(z&1)
+__device__ __forceinline__ int sliced_colour(int64_t key) { return (key_x(key) & 1) + 2 * (key_y(key) & 1); }
+__device__ __forceinline__ int colour8(int64_t key) { return (key_x(key) & 1) + 2 * (key_y(key) & 1) + 4 * (key_z(key) & 1); }
+
+// One launch of relax_sliced that also writes each updated row's new x to
+// snap[t * R + row] (t = visit of the slice: first_visit + slice index).
+template <class T, class V>
+__global__ void __launch_bounds__(256) k_relax_wave_snap(const int* __restrict__ order, const int* __restrict__ goff,
+                                                         const int* __restrict__ cols, const V* __restrict__ vals,
+                                                         const T* __restrict__ b, const T* __restrict__ inv_diag,
+                                                         const unsigned char* __restrict__ has_diag, T* __restrict__ x, int R,
+                                                         int first_z, int color, int first_visit, T* __restrict__ snap) {
+  const int group = 4 * (first_z + 2 * (int)blockIdx.y) + color;
+  const int j = goff[group] + blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= goff[group + 1]) return;
+  gs_update(j, R, order, cols, vals, b, inv_diag, has_diag, x);
+  const int i = order[j];
+  snap[(int64_t)(first_visit + (int)blockIdx.y) * R + i] = x[i];
+}
+
+// Reverse of one sliced launch with the matrix terms: row i's update at visit
+// t saw x_c from visit t for neighbours earlier in the visit (the slice above,
+// or an earlier colour of its own slice) and from visit t-1 otherwise; the
+// diagonal entry also carries -rbar * (x_new - x_old) through 1/A_ii.
+// gvals is in the operator's (schedule-position) layout.
+template <class T, class V>
+__global__ void __launch_bounds__(256) k_gs_reverse_sliced_m(const int* __restrict__ order, const int* __restrict__ goff,
+                                                             int first_z, int color, int first_visit, const int* __restrict__ cols,
+                                                             const V* __restrict__ vals, const T* __restrict__ inv_diag,
+                                                             const unsigned char* __restrict__ has_diag, int R,
+                                                             const int64_t* __restrict__ keys, const T* __restrict__ snap,
+                                                             double* __restrict__ gx, double* __restrict__ gb,
+                                                             double* __restrict__ gvals) {
+  const int group = 4 * (first_z + 2 * (int)blockIdx.y) + color;
+  const int j = goff[group] + (int)blockIdx.x * blockDim.x + (int)threadIdx.x;
+  if (j >= goff[group + 1] || !has_diag[j]) return;
+  const int t = first_visit + (int)blockIdx.y;
+  const int i = order[j];
+  const double rbar = gx[i] * double(inv_diag[j]);
+  if (rbar == 0.0) return;
+  gb[i] += rbar;
+  const int64_t ki = keys[i];
+  const int zi = key_z(ki), ci = sliced_colour(ki);
+  const T* now = snap + (int64_t)t * R;
+  const T* before = t > 0 ? snap + (int64_t)(t - 1) * R : nullptr;
+  for (int k = 0; k < 27; ++k) {
+    const int c = cols[(int64_t)k * R + j];
+    if (c < 0) continue;
+    atomicAdd(&gx[c], -rbar * double(vals[(int64_t)k * R + j]));
+    const int64_t kc = keys[c];
+    const int zc = key_z(kc);
+    const bool earlier = zc > zi || (zc == zi && sliced_colour(kc) < ci);
+    double seen = earlier ? double(now[c]) : (before ? double(before[c]) : 0.0);
+    if (c == i) seen = double(now[i]);  // x_old + (x_new - x_old)
+    gvals[(int64_t)k * R + j] -= rbar * seen;
+  }
+}
+
+// relax_colored on one workgroup with x in global memory, writing x before
+// the call and after every sweep to snap[(sweep + 1) * R + i] (snap[0..R):
+// the input x).
+template <class T>
+__global__ void __launch_bounds__(1024) k_relax_block_snap(const int* __restrict__ order, const int* __restrict__ goff,
+                                                           const int* __restrict__ cols, const T* __restrict__ vals,
+                                                           const T* __restrict__ b, const T* __restrict__ inv_diag,
+                                                           const unsigned char* __restrict__ has_diag, T* __restrict__ x, int R,
+                                                           int iters, int forward, T* __restrict__ snap) {
+  for (int i = threadIdx.x; i < R; i += blockDim.x) snap[i] = x[i];
+  __syncthreads();
+  for (int it = 0; it < iters; ++it) {
+    for (int c = 0; c < 8; ++c) {
+      const int g = forward ? c : 7 - c;
+      for (int j = goff[g] + threadIdx.x; j < goff[g + 1]; j += blockDim.x)
+        gs_update(j, R, order, cols, vals, b, inv_diag, has_diag, x);
+      __syncthreads();
+    }
+    for (int i = threadIdx.x; i < R; i += blockDim.x) snap[(int64_t)(it + 1) * R + i] = x[i];
+    __syncthreads();
+  }
+}
+
+// Reverse of one colour group of an 8-colour sweep with the matrix terms
+// (canonical layout: gvals[k * R + order[j]]).
+template <class T>
+__global__ void __launch_bounds__(256) k_gs_reverse_colored_m(const int* __restrict__ order, const int* __restrict__ goff, int g,
+                                                              int forward, const int* __restrict__ cols, const T* __restrict__ vals,
+                                                              const T* __restrict__ inv_diag,
+                                                              const unsigned char* __restrict__ has_diag, int R,
+                                                              const int64_t* __restrict__ keys, const T* __restrict__ before,
+                                                              const T* __restrict__ after, double* __restrict__ gx,
+                                                              double* __restrict__ gb, double* __restrict__ gvals) {
+  const int j = goff[g] + (int)blockIdx.x * blockDim.x + (int)threadIdx.x;
+  if (j >= goff[g + 1] || !has_diag[j]) return;
+  const int i = order[j];
+  const double rbar = gx[i] * double(inv_diag[j]);
+  if (rbar == 0.0) return;
+  gb[i] += rbar;
+  const int pos = forward ? g : 7 - g;
+  for (int k = 0; k < 27; ++k) {
+    const int c = cols[(int64_t)k * R + j];
+    if (c < 0) continue;
+    atomicAdd(&gx[c], -rbar * double(vals[(int64_t)k * R + j]));
+    const int gc = colour8(keys[c]);
+    const bool earlier = (forward ? gc : 7 - gc) < pos;
+    double seen = earlier ? double(after[c]) : double(before[c]);
+    if (c == i) seen = double(after[i]);
+    gvals[(int64_t)k * R + i] -= rbar * seen;
+  }
+}
+
+// out[j] = 0.5 * g[order[j]] (or g[j])
+__global__ void k_half_gather(const int* __restrict__ order, int R, const double* __restrict__ g, double* __restrict__ out) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < R) out[j] = 0.5 * g[order ? order[j] : j];
+}
+
+// gvals[k * R + j] += alpha * gy[row] * x[col] for a stencil's present slots
+// (the matrix gradient of y += alpha * A x), rows at positions j of canonical
+// row order[j] (or j)
+template <class T>
+__global__ void k_stencil_outer(const int* __restrict__ order, const int* __restrict__ cols, int R, int slots,
+                                const double* __restrict__ gy, const T* __restrict__ x, double alpha, double* __restrict__ gvals) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= R) return;
+  const double g = gy[order ? order[j] : j] * alpha;
+  if (g == 0.0) return;
+  for (int k = 0; k < slots; ++k) {
+    const int c = cols[(int64_t)k * R + j];
+    if (c >= 0) gvals[(int64_t)k * R + j] += g * double(x[c]);
+  }
+}
+
+// Reverse of k_galerkin (one workgroup per coarse row r): the coarse slot
+// gradients reach the fine operator entries through the same R and P weights.
+template <class T>
+__global__ void __launch_bounds__(128) k_galerkin_vjp(const int* __restrict__ rcols, int C, const int* __restrict__ acols, int F,
+                                                      const int* __restrict__ pcols, const T* __restrict__ pvals,
+                                                      const int64_t* __restrict__ ckeys, const double* __restrict__ gcoarse,
+                                                      double* __restrict__ gfine) {
+  const int r = blockIdx.x;
+  const int t = threadIdx.x;
+  if (t >= 125) return;
+  const int64_t rkey = ckeys[r];
+  const int rx = key_x(rkey), ry = key_y(rkey), rz = key_z(rkey);
+  const int tx = t / 25, ty = (t / 5) % 5, tz = t % 5;
+  int col = -1;
+  for (int kr = 0; kr < 27; ++kr) {
+    const int k = rcols[(int64_t)kr * C + r];
+    if (k < 0) continue;
+    const int jx = tx - kr / 9, jy = ty - (kr / 3) % 3, jz = tz - kr % 3;
+    if ((unsigned)jx > 2u || (unsigned)jy > 2u || (unsigned)jz > 2u) continue;
+    const int c = acols[(int64_t)(jx * 9 + jy * 3 + jz) * F + k];
+    if (c >= 0) col = c;
+  }
+  if (col < 0) return;
+  double gra = 0.0;  // gradient of (R A)_{r,t}
+  for (int kp = 0; kp < 8; ++kp) {
+    const int pc = pcols[(int64_t)kp * F + col];
+    if (pc < 0) continue;
+    const int64_t pkey = ckeys[pc];
+    const int ox = key_x(pkey) - rx + 1, oy = key_y(pkey) - ry + 1, oz = key_z(pkey) - rz + 1;
+    if ((unsigned)ox > 2u || (unsigned)oy > 2u || (unsigned)oz > 2u) continue;
+    gra += gcoarse[(int64_t)(ox * 9 + oy * 3 + oz) * C + r] * double(pvals[(int64_t)kp * F + col]);
+  }
+  if (gra == 0.0) return;
+  for (int kr = 0; kr < 27; ++kr) {
+    const int k = rcols[(int64_t)kr * C + r];
+    if (k < 0) continue;
+    const int jx = tx - kr / 9, jy = ty - (kr / 3) % 3, jz = tz - kr % 3;
+    if ((unsigned)jx > 2u || (unsigned)jy > 2u || (unsigned)jz > 2u) continue;
+    const int j = jx * 9 + jy * 3 + jz;
+    if (acols[(int64_t)j * F + k] < 0) continue;
+    atomicAdd(&gfine[(int64_t)j * F + k], restriction_weight(kr) * gra);
+  }
+}
+
+// d/dp of max(0, 1 - |p s - c|): -sign(t) s for 0 < |t| < 1, zero at the
+// kink t = 0 and where clamped (assembly_adjoint.py:38-47)
+template <class T> __device__ __forceinline__ double hat_deriv(T p, T s, T c) {
+  const T t = p * s - c;
+  if (!(fabs(t) < T(1)) || t == T(0)) return 0.0;
+  return t < T(0) ? double(s) : -double(s);
+}
+
+// screening_stencils.py:34-70 reversed at one level: row j's slot gradients
+// of the operator (gA, trial depth = depth) and of the cross operator (gC,
+// trial depth = depth - 1, may be null) and its point-load gradient (gload,
+// may be null) reach the constraint points' positions and weights.
+template <class T>
+__global__ void __launch_bounds__(128) k_screening_vjp(const int64_t* __restrict__ rows, int R, const int* __restrict__ row_order,
+                                                       int depth, PointBins<T> pb, const double* __restrict__ gA,
+                                                       const double* __restrict__ gC, const double* __restrict__ gload,
+                                                       double* __restrict__ gpos, double* __restrict__ gw) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= R) return;
+  const int i = row_order ? row_order[j] : j;
+  const int64_t key = rows[i];
+  const int cx = key_x(key), cy = key_y(key), cz = key_z(key);
+  const int n = 1 << depth;
+  double GA[27], GC[27];
+  bool any = false;
+  for (int k = 0; k < 27; ++k) {
+    GA[k] = gA[(int64_t)k * R + j];
+    GC[k] = gC ? gC[(int64_t)k * R + j] : 0.0;
+    any = any || GA[k] != 0.0 || GC[k] != 0.0;
+  }
+  const double GL = gload ? gload[j] : 0.0;
+  if (!any && GL == 0.0) return;
+  const T s = T(1 << depth);
+  for (int a = 0; a < 2; ++a)
+    for (int bb = 0; bb < 2; ++bb)
+      for (int kz = 0; kz < 2; ++kz) {
+        const int bx = cx - a, by = cy - bb, bz = cz - kz;
+        if (bx < 0 || by < 0 || bz < 0 || bx >= n || by >= n || bz >= n) continue;
+        int lo, hi;
+        if (pb.cell_start) {
+          const int64_t c = ((int64_t)bz * n + bx) * n + by;
+          lo = pb.cell_start[c];
+          hi = pb.cell_end[c];
+        } else {
+          const uint64_t want = ((uint64_t)bx << (2 * depth)) | ((uint64_t)by << depth) | (uint64_t)bz;
+          int l = 0, h = pb.P;
+          if (pb.bin_cols.start) {
+            l = pb.bin_cols.start[bx * n + by];
+            h = pb.bin_cols.end[bx * n + by];
+          }
+          while (l < h) {
+            const int mid = (l + h) >> 1;
+            if (pb.bin_keys[mid] < want) l = mid + 1; else h = mid;
+          }
+          lo = l;
+          hi = l;
+          while (hi < pb.P && pb.bin_keys[hi] == want) ++hi;
+        }
+        for (int jj = lo; jj < hi; ++jj) {
+          const int index = pb.bin_ids[jj];
+          const T weight = pb.w[index];
+          if (weight == T(0)) continue;
+          const T p[3] = {pb.pos[3 * index], pb.pos[3 * index + 1], pb.pos[3 * index + 2]};
+          const T cc[3] = {T(cx), T(cy), T(cz)};
+          double h[3], dh[3];
+          for (int ax = 0; ax < 3; ++ax) {
+            const T v = T(1) - fabs(p[ax] * s - cc[ax]);
+            h[ax] = v > T(0) ? double(v) : 0.0;
+            dh[ax] = hat_deriv(p[ax], s, cc[ax]);
+          }
+          const double phi = h[0] * h[1] * h[2];
+          const double dphi[3] = {dh[0] * h[1] * h[2], h[0] * dh[1] * h[2], h[0] * h[1] * dh[2]};
+          double S = GL, Sp[3] = {0.0, 0.0, 0.0};  // sum_k G_k trial_k and its derivatives (load: trial = 1)
+          for (int which = 0; which < 2; ++which) {
+            const double* G = which == 0 ? GA : GC;
+            if (which == 1 && !gC) break;
+            const int td = which == 0 ? depth : depth - 1;
+            const int shift = depth - td;
+            const int px = (cx >> shift) - 1, py = (cy >> shift) - 1, pz = (cz >> shift) - 1;
+            const T st = T(1 << td);
+            const T pc[3] = {T(px), T(py), T(pz)};
+            double hx[3][3], dx[3][3];
+            for (int ax = 0; ax < 3; ++ax)
+              for (int o = 0; o < 3; ++o) {
+                const T v = T(1) - fabs(p[ax] * st - (pc[ax] + T(o)));
+                hx[ax][o] = v > T(0) ? double(v) : 0.0;
+                dx[ax][o] = hat_deriv(p[ax], st, pc[ax] + T(o));
+              }
+            for (int k = 0; k < 27; ++k) {
+              if (G[k] == 0.0) continue;
+              const int ox = k / 9, oy = (k / 3) % 3, oz = k % 3;
+              S += G[k] * hx[0][ox] * hx[1][oy] * hx[2][oz];
+              Sp[0] += G[k] * dx[0][ox] * hx[1][oy] * hx[2][oz];
+              Sp[1] += G[k] * hx[0][ox] * dx[1][oy] * hx[2][oz];
+              Sp[2] += G[k] * hx[0][ox] * hx[1][oy] * dx[2][oz];
+            }
+          }
+          const double w = double(weight);
+          atomicAdd(&gw[index], phi * S);
+          for (int ax = 0; ax < 3; ++ax) atomicAdd(&gpos[3 * index + ax], w * (dphi[ax] * S + phi * Sp[ax]));
+        }
+      }
+}
+
+// constraints.py:55-84 reversed: centroid p = v[0:3] / v[3] and weight
+// v[3] 2^d (point_weight avg[1]) (zero where the forward zeroed it).
+template <class T>
+__global__ void __launch_bounds__(256) k_points_vjp(const T* __restrict__ avals, const T* __restrict__ w, int P, int depth,
+                                                    double point_weight, const double* __restrict__ gpos,
+                                                    const double* __restrict__ gw, double* __restrict__ gagg,
+                                                    double* __restrict__ gavg1) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  double part = 0.0;
+  if (a < P && w[a] != T(0)) {
+    const double v3 = double(avals[4 * (int64_t)a + 3]);
+    for (int c = 0; c < 3; ++c) gagg[3 * (int64_t)a + c] += gpos[3 * (int64_t)a + c] / v3;
+    part = gw[a] * v3 * double(1 << depth) * point_weight;
+  }
+  __shared__ double red[256];
+  red[threadIdx.x] = part;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && red[0] != 0.0) atomicAdd(gavg1, red[0]);
+}
+
 // gx[col] += alpha * A[row, col] * gy[row] (A^T gy), rows at positions j of
 // canonical row order[j] (or j)
 template <class V>
@@ -2291,7 +2618,8 @@ void stencil_scatter(const int* order, const Stencil<V>& m, int slots, const dou
 // solution of the level below).
 template <class T, class V>
 void finer_backward(Result<T>& r, int d, const Stencil<V>& Aop, const Stencil<V>& Cop, DArray<double>& gx, DArray<double>& grhs,
-                    DArray<double>& gcum_prev, hipStream_t s) {
+                    DArray<double>& gcum_prev, hipStream_t s, DArray<double>* gA = nullptr, DArray<double>* gC = nullptr,
+                    const T* cum_prev = nullptr) {
   LevelState<T>& L = r.levels[d];
   const int R = L.basis_count;
   Schedule<T> sc = make_schedule<T>(L.basis_keys, R, d, true, s);
@@ -2301,10 +2629,81 @@ void finer_backward(Result<T>& r, int d, const Stencil<V>& Aop, const Stencil<V>
                      (const int*)Aop.cols.data(), (const V*)Aop.vals.data(), inv_diag.data(), has_diag.data());
   KERNEL_CHECK();
   grhs = zeros_d(R, s);
-  gs_sliced_reverse<T, V>(sc, (const int*)Aop.cols.data(), (const V*)Aop.vals.data(), (const T*)inv_diag.data(),
-                          (const unsigned char*)has_diag.data(), R, d, r.prm.iters, gx.data(), grhs.data(), s);
+  if (!gA) {
+    gs_sliced_reverse<T, V>(sc, (const int*)Aop.cols.data(), (const V*)Aop.vals.data(), (const T*)inv_diag.data(),
+                            (const unsigned char*)has_diag.data(), R, d, r.prm.iters, gx.data(), grhs.data(), s);
+  } else {
+    // replay the level's relaxation from zero (relaxation_adjoint.py:29-46:
+    // the x each visit saw), then the reverse with the matrix terms
+    const int n = 1 << d, iters = r.prm.iters;
+    const unsigned bx = (unsigned)std::max(1, (sc.max_group + 255) / 256);
+    struct Step { int z0, slices, first; };
+    std::vector<Step> program;
+    for (int q = 0; q <= n + 2 * (iters - 1); ++q) {
+      const int first = std::max(0, floordiv(q - n + 1, 2));
+      const int last = std::min(iters - 1, q / 2);
+      if (last < first) continue;
+      program.push_back({n - q + 2 * first, last - first + 1, first});
+    }
+    DArray<T> x(R, s), snap((size_t)iters * R, s);
+    x.zero();
+    for (const Step& st : program)
+      for (int color = 0; color < 4; ++color) {
+        hipLaunchKernelGGL((k_relax_wave_snap<T, V>), dim3(bx, (unsigned)st.slices), dim3(256), 0, s, (const int*)sc.order.data(),
+                           (const int*)sc.offsets.data(), (const int*)Aop.cols.data(), (const V*)Aop.vals.data(),
+                           (const T*)L.rhs.data(), (const T*)inv_diag.data(), (const unsigned char*)has_diag.data(), x.data(), R,
+                           st.z0, color, st.first, snap.data());
+      }
+    KERNEL_CHECK();
+    gA->reset(27 * (size_t)R, s);
+    gA->zero();
+    for (auto it = program.rbegin(); it != program.rend(); ++it)
+      for (int color = 3; color >= 0; --color) {
+        hipLaunchKernelGGL((k_gs_reverse_sliced_m<T, V>), dim3(bx, (unsigned)it->slices), dim3(256), 0, s,
+                           (const int*)sc.order.data(), (const int*)sc.offsets.data(), it->z0, color, it->first,
+                           (const int*)Aop.cols.data(), (const V*)Aop.vals.data(), (const T*)inv_diag.data(),
+                           (const unsigned char*)has_diag.data(), R, (const int64_t*)L.basis_keys.data(), (const T*)snap.data(),
+                           gx.data(), grhs.data(), gA->data());
+      }
+    KERNEL_CHECK();
+  }
   // b = rhs - cross * cumulative(d-1)
   stencil_scatter<V>((const int*)sc.order.data(), Cop, 27, (const double*)grhs.data(), -1.0, gcum_prev.data(), s);
+  if (gC && R) {
+    gC->reset(27 * (size_t)R, s);
+    gC->zero();
+    hipLaunchKernelGGL(k_stencil_outer<T>, dim3(blocks_for(R, 256)), dim3(256), 0, s, (const int*)sc.order.data(),
+                       (const int*)Cop.cols.data(), R, 27, (const double*)grhs.data(), cum_prev, -1.0, gC->data());
+    KERNEL_CHECK();
+  }
+}
+
+// Reverse of relax_colored with the matrix terms: sweeps from the last, each
+// sweep's colours in reverse, from the replay's snapshots (snap[0]: x before
+// the call, snap[s + 1]: after sweep s); gvals in the canonical layout.
+template <class T>
+void gs_colored_reverse_m(const GsOperator<T>& op, const Schedule<T>& sc, int iters, bool forward, const int64_t* keys,
+                          const T* snap, double* gx, double* gb, double* gvals, hipStream_t s) {
+  if (!op.R) return;
+  const unsigned bx = (unsigned)std::max(1, (sc.max_group + 255) / 256);
+  for (int it = iters - 1; it >= 0; --it)
+    for (int c = 7; c >= 0; --c) {
+      const int g = forward ? c : 7 - c;
+      hipLaunchKernelGGL(k_gs_reverse_colored_m<T>, dim3(bx), dim3(256), 0, s, op.update_order, (const int*)sc.offsets.data(), g,
+                         forward ? 1 : 0, (const int*)op.cols.data(), (const T*)op.vals.data(), (const T*)op.inv_diag.data(),
+                         (const unsigned char*)op.has_diag.data(), op.R, keys, snap + (size_t)it * op.R,
+                         snap + (size_t)(it + 1) * op.R, gx, gb, gvals);
+    }
+  KERNEL_CHECK();
+}
+
+template <class T>
+void relax_colored_snap(const GsOperator<T>& op, const Schedule<T>& sc, T* x, int iters, bool forward, T* snap, hipStream_t s) {
+  if (!op.R) return;
+  hipLaunchKernelGGL(k_relax_block_snap<T>, dim3(1), dim3(1024), 0, s, op.update_order, (const int*)sc.offsets.data(),
+                     (const int*)op.cols.data(), (const T*)op.vals.data(), (const T*)op.b.data(), (const T*)op.inv_diag.data(),
+                     (const unsigned char*)op.has_diag.data(), x, op.R, iters, forward ? 1 : 0, snap);
+  KERNEL_CHECK();
 }
 
 }  // namespace
@@ -2389,13 +2788,33 @@ void normal_loads_backward(Result<T>& r, std::vector<DArray<double>>& gload, int
   }
 }
 
-template <class T> void solve_backward(Result<T>& r, const double* gcoef, double* grecords, hipStream_t s) {
+template <class T> void solve_backward(Result<T>& r, const double* gcoef, double* grecords, hipStream_t s, SolveGrads* mg) {
   const int D = r.prm.depth, base = r.prm.base_depth;
   int top = base - 1;
   for (int d = base; d <= r.prm.solve_depth && d <= D; ++d)
     if (r.levels[d].solved) top = d;
   if (top < base) return;
   std::vector<DArray<double>> grhs(D + 1);
+  std::vector<DArray<T>> cum;
+  if (mg) {
+    mg->top = top;
+    mg->gA.clear();
+    mg->gC.clear();
+    mg->gA.resize(D + 1);
+    mg->gC.resize(D + 1);
+    // the forward's cumulative solutions: cum(base) = x_base, cum(d) = x_d + P_d cum(d-1)
+    cum.resize(D + 1);
+    for (int d = base; d <= top; ++d) {
+      const int R = r.levels[d].basis_count;
+      cum[d].reset(R, s);
+      if (R) HIP_CHECK(hipMemcpyAsync(cum[d].data(), r.levels[d].correction.data(), sizeof(T) * R, hipMemcpyDeviceToDevice, s));
+      if (d > base && R) {
+        hipLaunchKernelGGL(k_spmv<T>, dim3(blocks_for(R, 256)), dim3(256), 0, s, (const int*)r.levels[d].P.cols.data(),
+                           (const T*)r.levels[d].P.vals.data(), R, 8, (const T*)cum[d - 1].data(), cum[d].data(), T(1), T(1));
+        KERNEL_CHECK();
+      }
+    }
+  }
   // finer levels: cumulative(d) = P_d cumulative(d-1) + x_d
   DArray<double> gcum;
   for (int d = top; d > base; --d) {
@@ -2410,12 +2829,14 @@ template <class T> void solve_backward(Result<T>& r, const double* gcoef, double
     }
     DArray<double> gprev = zeros_d(C, s);
     if (!gcum.empty()) stencil_scatter<T>(nullptr, L.P, 8, (const double*)gcum.data(), 1.0, gprev.data(), s);
-    if (r.prm.float_operators) finer_backward<T, float>(r, d, L.A32, L.cross32, gx, grhs[d], gprev, s);
-    else finer_backward<T, T>(r, d, L.A, L.cross, gx, grhs[d], gprev, s);
+    DArray<double>* gA = mg ? &mg->gA[d] : nullptr;
+    DArray<double>* gC = mg ? &mg->gC[d] : nullptr;
+    const T* cprev = mg ? (const T*)cum[d - 1].data() : nullptr;
+    if (r.prm.float_operators) finer_backward<T, float>(r, d, L.A32, L.cross32, gx, grhs[d], gprev, s, gA, gC, cprev);
+    else finer_backward<T, T>(r, d, L.A, L.cross, gx, grhs[d], gprev, s, gA, gC, cprev);
     gcum = std::move(gprev);
   }
-  // base level: regular multigrid (linear in its right-hand side while the
-  // root solve is skipped)
+  // base level: regular multigrid
   {
     std::vector<int> Rn(base + 1);
     for (int d = 0; d <= base; ++d) Rn[d] = r.levels[d].basis_count;
@@ -2427,13 +2848,17 @@ template <class T> void solve_backward(Result<T>& r, const double* gcoef, double
       KERNEL_CHECK();
     }
     grhs[base] = zeros_d(Rn[base], s);
+    if (mg) {
+      mg->gA[base].reset(27 * (size_t)Rn[base], s);
+      mg->gA[base].zero();
+    }
     auto ran = [&](int cycle) { return std::find(r.cg_runs.begin(), r.cg_runs.end(), cycle) != r.cg_runs.end(); };
     const int maxiter = Rn[0];
     DArray<double> cg_store((size_t)(maxiter + 1) * 3 * 64 + 4 * (size_t)(maxiter + 1) + 64, s);
-    auto root_reverse = [&](int cycle, const int* cols0, const T* vals0, double* gx0, double* grhs0, int dc) {
+    auto root_reverse = [&](int cycle, const int* cols0, const T* vals0, double* gx0, double* grhs0, int dc, double* gvals0) {
       hipLaunchKernelGGL(k_root_cg_vjp<T>, dim3(1), dim3(64), 0, s, cols0, vals0, Rn[0], (const int64_t*)r.levels[0].basis_keys.data(),
                          (const T*)r.root_b[cycle].data(), (const T*)r.root_x[cycle].data(), dc, base, cycle, r.prm.cg_accuracy,
-                         maxiter, cg_store.data(), gx0, grhs0);
+                         maxiter, cg_store.data(), gx0, grhs0, gvals0);
       KERNEL_CHECK();
     };
     if (base == 0 && Rn[0] && !r.cg_runs.empty()) {
@@ -2449,7 +2874,7 @@ template <class T> void solve_backward(Result<T>& r, const double* gcoef, double
       }
       for (int cycle = r.prm.base_v_cycles - 1; cycle >= 0; --cycle)
         if (ran(cycle)) root_reverse(cycle, (const int*)r.levels[0].A.cols.data(), (const T*)r.levels[0].A.vals.data(), g,
-                                     grhs[base].data(), dc);
+                                     grhs[base].data(), dc, mg ? mg->gA[0].data() : (double*)nullptr);
     }
     if (base > 0) {
       // rebuild the Galerkin hierarchy, schedules and operators
@@ -2472,13 +2897,66 @@ template <class T> void solve_backward(Result<T>& r, const double* gcoef, double
           KERNEL_CHECK();
         }
       }
-      DArray<T> zero_b(Rn[base], s);
+      // gather_operator reads b over each level's rows
+      DArray<T> zero_b(std::max(1, *std::max_element(Rn.begin(), Rn.end())), s);
       zero_b.zero();
       std::vector<Schedule<T>> sched;
       std::vector<GsOperator<T>> ops;
       for (int d = 0; d <= base; ++d) {
         sched.push_back(make_schedule<T>(r.levels[d].basis_keys, Rn[d], d, false, s));
         ops.push_back(gather_operator<T>(sched[d], mcols[d], mvals[d], zero_b.data(), Rn[d], s));
+      }
+      const int iters = r.prm.iters, cycles = r.prm.base_v_cycles;
+      // replay of the V-cycles (regular_multigrid) with x snapshots
+      std::vector<std::vector<DArray<T>>> snap_down(cycles), snap_up(cycles);
+      std::vector<DArray<double>> gM(base + 1);
+      if (mg) {
+        std::vector<DArray<T>> rhs(base + 1), x(base + 1);
+        for (int d = 0; d <= base; ++d) {
+          rhs[d].reset(Rn[d], s);
+          rhs[d].zero();
+          x[d].reset(Rn[d], s);
+          x[d].zero();
+          gM[d].reset(27 * (size_t)Rn[d], s);
+          gM[d].zero();
+        }
+        if (Rn[base]) HIP_CHECK(hipMemcpyAsync(rhs[base].data(), B.rhs.data(), sizeof(T) * Rn[base], hipMemcpyDeviceToDevice, s));
+        DArray<T> pad_x(64, s), work(256, s), residual;
+        pad_x.zero();
+        DArray<int> flag(1, s);
+        for (int cycle = 0; cycle < cycles; ++cycle) {
+          snap_down[cycle].resize(base + 1);
+          snap_up[cycle].resize(base + 1);
+          for (int d = base; d > 0; --d) {
+            refresh_rhs(ops[d], sched[d], rhs[d].data(), s);
+            snap_down[cycle][d].reset((size_t)(iters + 1) * Rn[d], s);
+            relax_colored_snap(ops[d], sched[d], x[d].data(), iters, true, snap_down[cycle][d].data(), s);
+            residual.reset(Rn[d], s);
+            HIP_CHECK(hipMemcpyAsync(residual.data(), rhs[d].data(), sizeof(T) * Rn[d], hipMemcpyDeviceToDevice, s));
+            hipLaunchKernelGGL(k_spmv<T>, dim3(blocks_for(Rn[d], 256)), dim3(256), 0, s, (const int*)mcols[d].data(),
+                               (const T*)mvals[d].data(), Rn[d], 27, (const T*)x[d].data(), residual.data(), T(-1), T(1));
+            KERNEL_CHECK();
+            if (Rn[d - 1]) {
+              hipLaunchKernelGGL(k_restrict_apply<T>, dim3(blocks_for(Rn[d - 1], 256)), dim3(256), 0, s,
+                                 (const int*)r.levels[d - 1].restriction.data(), Rn[d - 1], (const T*)residual.data(),
+                                 rhs[d - 1].data(), 0);
+              KERNEL_CHECK();
+            }
+            x[d - 1].zero();
+          }
+          hipLaunchKernelGGL(k_root_solve<T>, dim3(1), dim3(64), 0, s, (const int*)mcols[0].data(), (const T*)mvals[0].data(), Rn[0],
+                             (const int64_t*)r.levels[0].basis_keys.data(), (const T*)rhs[0].data(), x[0].data(), pad_x.data(),
+                             work.data(), 0, base, cycle, r.prm.cg_accuracy, Rn[0], flag.data());
+          KERNEL_CHECK();
+          for (int d = 1; d <= base; ++d) {
+            hipLaunchKernelGGL(k_spmv<T>, dim3(blocks_for(Rn[d], 256)), dim3(256), 0, s, (const int*)r.levels[d].P.cols.data(),
+                               (const T*)r.levels[d].P.vals.data(), Rn[d], 8, (const T*)x[d - 1].data(), x[d].data(), T(1), T(1));
+            KERNEL_CHECK();
+            x[d - 1].zero();
+            snap_up[cycle][d].reset((size_t)(iters + 1) * Rn[d], s);
+            relax_colored_snap(ops[d], sched[d], x[d].data(), iters, false, snap_up[cycle][d].data(), s);
+          }
+        }
       }
       std::vector<DArray<double>> gxl(base + 1), grl(base + 1);
       for (int d = 0; d < base; ++d) {
@@ -2487,8 +2965,7 @@ template <class T> void solve_backward(Result<T>& r, const double* gcoef, double
       }
       gxl[base] = std::move(gxb);
       grl[base].reset(Rn[base], s);
-      const int iters = r.prm.iters;
-      for (int cycle = r.prm.base_v_cycles - 1; cycle >= 0; --cycle) {
+      for (int cycle = cycles - 1; cycle >= 0; --cycle) {
         for (int d = 0; d < base; ++d) {
           gxl[d].zero();
           grl[d].zero();
@@ -2496,12 +2973,16 @@ template <class T> void solve_backward(Result<T>& r, const double* gcoef, double
         grl[base].zero();
         // up: x_d += P_d x_{d-1}, then reverse-colour sweeps
         for (int d = base; d >= 1; --d) {
-          gs_colored_reverse<T>(ops[d], sched[d], iters, false, gxl[d].data(), grl[d].data(), s);
+          if (mg)
+            gs_colored_reverse_m<T>(ops[d], sched[d], iters, false, (const int64_t*)r.levels[d].basis_keys.data(),
+                                    (const T*)snap_up[cycle][d].data(), gxl[d].data(), grl[d].data(), gM[d].data(), s);
+          else
+            gs_colored_reverse<T>(ops[d], sched[d], iters, false, gxl[d].data(), grl[d].data(), s);
           stencil_scatter<T>(nullptr, r.levels[d].P, 8, (const double*)gxl[d].data(), 1.0, gxl[d - 1].data(), s);
         }
         // root: when skipped x_0 stays zero; when its CG ran, reverse it
         if (ran(cycle) && Rn[0]) root_reverse(cycle, (const int*)mcols[0].data(), (const T*)mvals[0].data(), gxl[0].data(),
-                                              grl[0].data(), 0);
+                                              grl[0].data(), 0, mg ? gM[0].data() : (double*)nullptr);
         // down: forward sweeps, residual, restriction
         for (int d = 1; d <= base; ++d) {
           DArray<double> gres = zeros_d(Rn[d], s);
@@ -2520,8 +3001,18 @@ template <class T> void solve_backward(Result<T>& r, const double* gcoef, double
                                (const int*)mcols[d].data(), (const T*)mvals[d].data(), Rn[d], 27, (const double*)gres.data(), -1.0,
                                gxl[d].data());
             KERNEL_CHECK();
+            if (mg) {  // residual = rhs - A x_after_down
+              hipLaunchKernelGGL(k_stencil_outer<T>, dim3(blocks_for(Rn[d], 256)), dim3(256), 0, s, (const int*)nullptr,
+                                 (const int*)mcols[d].data(), Rn[d], 27, (const double*)gres.data(),
+                                 (const T*)snap_down[cycle][d].data() + (size_t)iters * Rn[d], -1.0, gM[d].data());
+              KERNEL_CHECK();
+            }
           }
-          gs_colored_reverse<T>(ops[d], sched[d], iters, true, gxl[d].data(), grl[d].data(), s);
+          if (mg)
+            gs_colored_reverse_m<T>(ops[d], sched[d], iters, true, (const int64_t*)r.levels[d].basis_keys.data(),
+                                    (const T*)snap_down[cycle][d].data(), gxl[d].data(), grl[d].data(), gM[d].data(), s);
+          else
+            gs_colored_reverse<T>(ops[d], sched[d], iters, true, gxl[d].data(), grl[d].data(), s);
         }
         if (Rn[base]) {
           hipLaunchKernelGGL(k_axpy_d, dim3(blocks_for(Rn[base], 256)), dim3(256), 0, s, grhs[base].data(),
@@ -2529,10 +3020,55 @@ template <class T> void solve_backward(Result<T>& r, const double* gcoef, double
           KERNEL_CHECK();
         }
       }
+      if (mg) {
+        // Galerkin products A_{d-1} = R A_d P, coarse to fine
+        for (int d = 1; d <= base; ++d) {
+          const int C = Rn[d - 1], F = Rn[d];
+          if (!C || !F) continue;
+          hipLaunchKernelGGL(k_galerkin_vjp<T>, dim3(C), dim3(128), 0, s, (const int*)r.levels[d - 1].restriction.data(), C,
+                             (const int*)mcols[d].data(), F, (const int*)r.levels[d].P.cols.data(),
+                             (const T*)r.levels[d].P.vals.data(), (const int64_t*)r.levels[d - 1].basis_keys.data(),
+                             (const double*)gM[d - 1].data(), gM[d].data());
+          KERNEL_CHECK();
+        }
+        mg->gA[base] = std::move(gM[base]);
+      }
     }
   }
   // right-hand side = fp(load) + fp(0.5) * fp(point_load) (solver.py:119-126)
   normal_loads_backward(r, grhs, top, grecords, s);
+  if (mg) mg->grhs = std::move(grhs);
+}
+
+// Screening reversed per level (see the declaration).
+template <class T>
+void screening_backward(Result<T>& r, const SolveGrads& mg, std::vector<DArray<double>>& gagg, double* gavg1, hipStream_t s) {
+  const int base = r.prm.base_depth;
+  gagg.resize(r.prm.depth + 1);
+  for (int d = base; d <= mg.top; ++d) {
+    LevelState<T>& L = r.levels[d];
+    const int R = L.basis_count;
+    if (!R || mg.gA[d].empty() || mg.grhs[d].empty()) continue;
+    PointSet<T> pts = point_constraints(r, d, s, false);
+    if (!pts.P) continue;
+    const int* ord = d > base ? (const int*)L.order.data() : (const int*)nullptr;
+    // the point load enters b = T(load) + T(0.5) T(point_load), in the load's layout
+    DArray<double> gload(R, s);
+    hipLaunchKernelGGL(k_half_gather, dim3(blocks_for(R, 256)), dim3(256), 0, s, ord, R, (const double*)mg.grhs[d].data(),
+                       gload.data());
+    KERNEL_CHECK();
+    DArray<double> gpos = zeros_d(3 * (size_t)pts.P, s), gw = zeros_d(pts.P, s);
+    hipLaunchKernelGGL(k_screening_vjp<T>, dim3(blocks_for(R, 128)), dim3(128), 0, s, (const int64_t*)L.basis_keys.data(), R, ord, d,
+                       point_bins(pts), (const double*)mg.gA[d].data(),
+                       d > base && !mg.gC[d].empty() ? (const double*)mg.gC[d].data() : (const double*)nullptr,
+                       (const double*)gload.data(), gpos.data(), gw.data());
+    KERNEL_CHECK();
+    if (gagg[d].empty()) gagg[d] = zeros_d(3 * (size_t)pts.P, s);
+    hipLaunchKernelGGL(k_points_vjp<T>, dim3(blocks_for(pts.P, 256)), dim3(256), 0, s, (const T*)L.agg_vals.data(),
+                       (const T*)pts.w.data(), pts.P, d, r.prm.point_weight, (const double*)gpos.data(), (const double*)gw.data(),
+                       gagg[d].data(), gavg1);
+    KERNEL_CHECK();
+  }
 }
 
 template void relax_sliced_csr<float>(const int64_t*, int, const int*, const int*, const float*, const float*, float*, int, int,
@@ -2542,7 +3078,9 @@ template void relax_sliced_csr<double>(const int64_t*, int, const int*, const in
 
 template void stage_solve<float>(Result<float>&);
 template void stage_solve<double>(Result<double>&);
-template void solve_backward<float>(Result<float>&, const double*, double*, hipStream_t);
-template void solve_backward<double>(Result<double>&, const double*, double*, hipStream_t);
+template void solve_backward<float>(Result<float>&, const double*, double*, hipStream_t, SolveGrads*);
+template void solve_backward<double>(Result<double>&, const double*, double*, hipStream_t, SolveGrads*);
+template void screening_backward<float>(Result<float>&, const SolveGrads&, std::vector<DArray<double>>&, double*, hipStream_t);
+template void screening_backward<double>(Result<double>&, const SolveGrads&, std::vector<DArray<double>>&, double*, hipStream_t);
 
 }  // namespace psr

@@ -10,10 +10,12 @@ solve (reverse Gauss-Seidel sweeps, base multigrid) -> normal loads -> normal
 splats -> unit normals -> input normals.  Discrete choices (octree, sample
 groups, splat depths, solver branches) are held fixed, as in the reference.
 
-Gradients with respect to positions cover runs without screening
-(point_weight 0 or exact_interpolation): they go through the isovalue
-centroids, the splats' basis, depth and weight, the density levels and
-aggregates, and the sample transform's extrema owners.  Queries may require
+Gradients with respect to positions go through the isovalue centroids, the
+splats' basis, depth and weight, the density levels and aggregates, the
+sample transform's extrema owners and, in screened runs, the constraint
+points: their screening terms in every level's operator, cross operator and
+point load, reached by replaying each relaxation once for the x it saw, the
+Galerkin products of the base hierarchy and the normal average's weight.  Queries may require
 grad (d value / d query is the field gradient; d gradient / d query uses the
 trilinear second derivatives).
 """
@@ -122,10 +124,8 @@ def reconstruct_differentiable(positions, normals, **options):
     """Reconstruct, then differentiate field losses with respect to the inputs
     (reconstruction_adjoint.py:61-185).
 
-    ``positions`` and/or ``normals`` must require grad.  Position gradients
-    need a run without screening (``point_weight=0`` or
-    ``exact_interpolation=True``).  Options are reconstruct()'s keywords
-    (constants).
+    ``positions`` and/or ``normals`` must require grad.  Options are
+    reconstruct()'s keywords (constants).
     """
     torch = _torch()
     if not isinstance(positions, torch.Tensor) or not isinstance(normals, torch.Tensor):
@@ -135,7 +135,5 @@ def reconstruct_differentiable(positions, normals, **options):
     for name, value in options.items():
         if isinstance(value, torch.Tensor):
             raise ValueError(f"{name} must be a constant scalar; only positions and normals support gradients")
-    if positions.requires_grad and options.get("point_weight", 2.0) > 0 and not options.get("exact_interpolation", False):
-        raise ValueError("gradients with respect to positions need point_weight=0 or exact_interpolation=True")
     primal = reconstruct(positions.detach(), normals.detach(), **options)
     return DifferentiableReconstruction(primal, positions, normals)
