@@ -588,6 +588,65 @@ __global__ void k_normal_load(const int64_t* __restrict__ keys, int R, const int
   out[i] = value;
 }
 
+// k_normal_load over a dense record index with each (x, y) column's loads
+// batched: the column's W dense entries, then all W records' values, then the
+// terms in the same (x, y, z, component) order as k_normal_load, so the sum is
+// identical.  Unconditional, clamped addresses keep the loads independent of
+// the presence tests (k_normal_load issues them one record at a time).
+template <class V, int W>
+__global__ void __launch_bounds__(128) k_normal_load_dense(const int64_t* __restrict__ keys, int R, DenseIndex dn,
+                                                           const V* __restrict__ nvals, int nstride, int test_depth,
+                                                           int normal_depth, int first, const D2* __restrict__ table,
+                                                           double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= R) return;
+  const int64_t key = keys[i];
+  const int c0 = key_x(key), c1 = key_y(key), c2 = key_z(key);
+  const int e0 = normal_center(c0, test_depth, normal_depth), e1 = normal_center(c1, test_depth, normal_depth),
+            e2 = normal_center(c2, test_depth, normal_depth);
+  const int lim = 1 << normal_depth;
+  const int zlo = imax(e2 + first, 0), zhi = imin(e2 + first + W - 1, lim - 1);
+  D2 iz[W];
+#pragma unroll
+  for (int k = 0; k < W; ++k) iz[k] = table[c2 * W + k];
+  const int64_t dim = dn.dim;
+  double value = 0.0;
+  for (int x = 0; x < W; ++x) {
+    const int nx = e0 + x + first;
+    if (nx < 0 || nx >= lim) continue;
+    const D2 ix = table[c0 * W + x];
+    for (int y = 0; y < W; ++y) {
+      const int ny = e1 + y + first;
+      if (ny < 0 || ny >= lim) continue;
+      const D2 iy = table[c1 * W + y];
+      int jn[W];
+#pragma unroll
+      for (int z = 0; z < W; ++z) {
+        const int nz = e2 + z + first;
+        const bool in = nz >= zlo && nz <= zhi;
+        int64_t cell = dn.zmajor ? ((int64_t)nz * dim + nx) * dim + ny : ((int64_t)nx * dim + ny) * dim + nz;
+        cell = in ? cell : 0;
+        jn[z] = in ? dn.idx[cell] : -1;
+      }
+      V v[W][3];
+#pragma unroll
+      for (int z = 0; z < W; ++z) {
+        const int64_t j = jn[z] >= 0 ? jn[z] : 0;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) v[z][c] = jn[z] >= 0 ? nvals[(int64_t)nstride * j + c] : V(0);
+      }
+#pragma unroll
+      for (int z = 0; z < W; ++z) {
+        if (jn[z] < 0) continue;
+        value += double(v[z][0]) * ix.v1 * iy.v0 * iz[z].v0;
+        value += double(v[z][1]) * ix.v0 * iy.v1 * iz[z].v0;
+        value += double(v[z][2]) * ix.v0 * iy.v0 * iz[z].v1;
+      }
+    }
+  }
+  out[i] = value;
+}
+
 // normal_loads.py:27-63 (prolong degree-2 coefficients with degree-1 weights, issue #35)
 template <class T>
 __global__ void k_normal_prolong(const int64_t* __restrict__ keys, int R, const int64_t* __restrict__ own_keys, int64_t own_count,
@@ -1593,8 +1652,17 @@ static void normal_stencil_load(const LevelState<T>& testL, int test_depth, cons
   hipLaunchKernelGGL(k_normal_table, dim3(blocks_for(rows * width, 256)), dim3(256), 0, s, test_depth, normal_depth, first, width,
                      table.data());
   KERNEL_CHECK();
-  hipLaunchKernelGGL(k_normal_load<T>, dim3(blocks_for(R, 128)), dim3(128), 0, s, (const int64_t*)testL.basis_keys.data(), R,
-                     nkeys, ncount, ct, dn, nvals, 3, test_depth, normal_depth, first, width, (const D2*)table.data(), out);
+  if (dn.idx && width == 4)
+    hipLaunchKernelGGL((k_normal_load_dense<T, 4>), dim3(blocks_for(R, 128)), dim3(128), 0, s,
+                       (const int64_t*)testL.basis_keys.data(), R, dn, nvals, 3, test_depth, normal_depth, first,
+                       (const D2*)table.data(), out);
+  else if (dn.idx && width == 6)
+    hipLaunchKernelGGL((k_normal_load_dense<T, 6>), dim3(blocks_for(R, 128)), dim3(128), 0, s,
+                       (const int64_t*)testL.basis_keys.data(), R, dn, nvals, 3, test_depth, normal_depth, first,
+                       (const D2*)table.data(), out);
+  else
+    hipLaunchKernelGGL(k_normal_load<T>, dim3(blocks_for(R, 128)), dim3(128), 0, s, (const int64_t*)testL.basis_keys.data(), R,
+                       nkeys, ncount, ct, dn, nvals, 3, test_depth, normal_depth, first, width, (const D2*)table.data(), out);
   KERNEL_CHECK();
 }
 
