@@ -21,6 +21,8 @@
  *                                                                 src/warp_psr/reconstruction.py:303-349
  *   psr_relax_sliced    -> relax_sliced(A, b, x, schedule, depth, iters)
  *                                                                 src/warp_psr/relaxation.py:189-238
+ *   psr_relax_scheduled -> relax_scheduled(A, b, x, schedule, iters, forward, z)
+ *                                                                 src/warp_psr/relaxation.py:221-238,261-271
  *   psr_result_free     -> (Python garbage collection of Reconstruction)
  *   psr_field_vjp       -> DifferentiableReconstruction.evaluate backward
  *                                                                 src/warp_psr/field_adjoint.py:255-288, field.py:214-286
@@ -114,6 +116,14 @@ void psr_result_free(PsrResult* result);
  * columns/values (<= 27 entries per row), b[rows], x[rows] updated in place. */
 int psr_relax_sliced(int dtype, const int64_t* keys, int64_t rows, const int32_t* offsets, const int32_t* columns,
                      const void* values, const void* b, void* x, int depth, int iters, void* stream);
+
+/* 8-colour Gauss-Seidel on the same caller CSR arguments: z < 0 runs `iters`
+ * sweeps over the colour groups (x&1)+2(y&1)+4(z&1) of row_schedule(keys,
+ * depth), colours 0..7 (forward != 0) or 7..0; z in [0, 2**depth] relaxes slice
+ * z of the sliced schedule instead, its colours in the same order (the z mode
+ * of relax_scheduled).  Returns 2 for z beyond 2**depth. */
+int psr_relax_scheduled(int dtype, const int64_t* keys, int64_t rows, const int32_t* offsets, const int32_t* columns,
+                        const void* values, const void* b, void* x, int depth, int iters, int forward, int z, void* stream);
 
 /* Reverse mode (first order, octree / grouping / splat depths / solver branches
  * fixed; src/warp_psr/reconstruction_adjoint.py:61-185).

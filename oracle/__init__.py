@@ -51,6 +51,9 @@ def _lib(openmp=False):
         lib.oracle_relax_sliced.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                             ctypes.c_int, ctypes.c_int]
+        lib.oracle_relax_scheduled.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
+                                               ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                               ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         lib.oracle_hat_integrals.argtypes = [ctypes.c_int] * 4 + [ctypes.c_void_p]
         lib.oracle_normal_integral.argtypes = [ctypes.c_int] * 4 + [ctypes.c_void_p]
         _LIBS[name] = lib
@@ -160,6 +163,21 @@ def relax_sliced(keys, offsets, columns, values, b, x, depth, iters):
     x = np.array(x, dtype=dtype, copy=True)
     _lib().oracle_relax_sliced(1 if dtype == np.float64 else 0, keys.ctypes.data, len(keys), offsets.ctypes.data,
                                columns.ctypes.data, values.ctypes.data, b.ctypes.data, x.ctypes.data, depth, iters)
+    return x
+
+
+def relax_scheduled(keys, offsets, columns, values, b, x, depth, iters, forward=True, z=-1):
+    """Serial 8-colour Gauss-Seidel (or one slice of the sliced schedule) on caller CSR arrays; returns the new x."""
+    values = np.ascontiguousarray(values)
+    dtype = values.dtype
+    keys = np.ascontiguousarray(keys, dtype=np.int64)
+    offsets = np.ascontiguousarray(offsets, dtype=np.int32)
+    columns = np.ascontiguousarray(columns, dtype=np.int32)
+    b = np.ascontiguousarray(b, dtype=dtype)
+    x = np.array(x, dtype=dtype, copy=True)
+    _lib().oracle_relax_scheduled(1 if dtype == np.float64 else 0, keys.ctypes.data, len(keys), offsets.ctypes.data,
+                                  columns.ctypes.data, values.ctypes.data, b.ctypes.data, x.ctypes.data, depth, iters,
+                                  int(bool(forward)), int(z))
     return x
 
 

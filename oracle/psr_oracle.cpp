@@ -1719,6 +1719,38 @@ int oracle_relax_sliced(int dtype, const int64_t* keys, int64_t rows, const int*
   return 0;
 }
 
+// Stand-alone 8-colour Gauss-Seidel (relaxation.py:261-271, relax_scheduled):
+// z < 0 sweeps the 8 colour groups of row_schedule(keys, depth) (0..7 or
+// 7..0) iters times; z >= 0 relaxes slice z of the sliced schedule, taking the
+// colours whose bit 2 equals z & 1 in the same order (group z*4 + (colour & 3)).
+int oracle_relax_scheduled(int dtype, const int64_t* keys, int64_t rows, const int* offsets, const int* columns,
+                           const void* values, const void* b, void* x, int depth, int iters, int forward, int z) {
+  std::vector<int64_t> k(keys, keys + rows);
+  Schedule s = row_schedule(k, depth, z >= 0);
+  auto go = [&](auto tag) {
+    using T = decltype(tag);
+    CSR<T> A;
+    A.rows = (int)rows;
+    A.offsets.assign(offsets, offsets + rows + 1);
+    A.columns.assign(columns, columns + offsets[rows]);
+    A.values.assign((const T*)values, (const T*)values + offsets[rows]);
+    std::vector<T> bb((const T*)b, (const T*)b + rows), xx((T*)x, (T*)x + rows);
+    if (z < 0) {
+      relax_scheduled(A, bb, xx, s, iters, forward != 0);
+    } else {
+      for (int it = 0; it < iters; ++it)
+        for (int k8 = 0; k8 < 8; ++k8) {
+          const int color = forward ? k8 : 7 - k8;
+          if ((color >> 2) != (z & 1)) continue;
+          relax_group(A, bb, xx, s, z * 4 + (color & 3));
+        }
+    }
+    std::copy(xx.begin(), xx.end(), (T*)x);
+  };
+  if (dtype == 0) go(float()); else go(double());
+  return 0;
+}
+
 // 1D integral tables, exported for closed-form checks.
 void oracle_hat_integrals(int depth, int index, int trial_depth, int trial_index, double* out) {
   D2 r = hat_integrals(depth, index, trial_depth, trial_index);

@@ -7,7 +7,9 @@
 * psr_relax_sliced (relaxation.py:189-238) on caller CSR operators with
   random values equals the oracle's serial sliced Gauss-Seidel bit for bit
   (tests/test_relaxation.py:110-147 restated: any values on the 27-point
-  sparsity of a level give the same bits in every legal order).
+  sparsity of a level give the same bits in every legal order), and
+  psr_relax_scheduled (relaxation.py:261-271) equals the oracle's 8-colour
+  sweeps and one-slice relaxations (tests/test_relaxation.py:9-34).
 """
 import numpy as np
 import pytest
@@ -83,5 +85,45 @@ def test_relax_sliced_matches_oracle(cuda, precision):
     t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(cuda)
     x = t(x0)
     relax_sliced(t(keys), t(offsets), t(columns), t(values), t(b), x, depth, 3)
+    assert same(x.cpu().numpy(), expected)
+    assert not same(x0, expected)
+
+
+@pytest.mark.parametrize("precision", ["float", "double"])
+@pytest.mark.parametrize("shape,count,depth", [("torus", 8192, 6), ("ellipsoids", 60000, 7)])
+def test_relax_scheduled_matches_oracle(cuda, precision, shape, count, depth):
+    """One workgroup (torus level 6) and one launch per colour group (ellipsoid
+    union level 7, more than 65,536 rows) give the oracle's bits."""
+    import torch
+    from warp_psr_amd import synthetic
+    from warp_psr_amd.reconstruction import relax_scheduled
+    dt = np.float32 if precision == "float" else np.float64
+    if shape == "torus":
+        p, n = fixture_arrays(shape, count, precision)
+    else:
+        p, n = synthetic.workload_ellipsoids(count, dtype=dt)
+    ref = oracle.reconstruct(p, n, depth=depth, full_depth=3)
+    keys = ref.array("basis_keys", depth)
+    offsets = ref.array("matrix_offsets", depth).astype(np.int32)
+    columns = ref.array("matrix_columns", depth).astype(np.int32)
+    print(f"relax_scheduled {shape} depth {depth} rows {len(keys)} {precision}", flush=True)
+    rng = np.random.default_rng(13)
+    values = rng.uniform(-1.0, 1.0, size=len(columns)).astype(dt)
+    rows = np.repeat(np.arange(len(keys)), np.diff(offsets))
+    values[columns == rows] = dt(30.0)
+    b = rng.normal(size=len(keys)).astype(dt)
+    x0 = rng.normal(size=len(keys)).astype(dt)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(cuda)
+    dk, do, dc, dv, db = t(keys), t(offsets), t(columns), t(values), t(b)
+    for forward in (True, False):
+        expected = oracle.relax_scheduled(keys, offsets, columns, values, b, x0, depth, 3, forward=forward)
+        x = t(x0)
+        relax_scheduled(dk, do, dc, dv, db, x, depth, 3, forward=forward)
+        assert same(x.cpu().numpy(), expected)
+    expected = x0.copy()
+    x = t(x0)
+    for z in ((1 << depth), (1 << depth) // 2 + 1, (1 << depth) // 2, 0):
+        expected = oracle.relax_scheduled(keys, offsets, columns, values, b, expected, depth, 2, z=z)
+        relax_scheduled(dk, do, dc, dv, db, x, depth, 2, z=z)
     assert same(x.cpu().numpy(), expected)
     assert not same(x0, expected)

@@ -76,6 +76,9 @@ def test_c_abi_rejects_empty_and_bad_dtype():
     assert "nonempty" in lib.psr_last_error().decode()
     assert lib.psr_reconstruct(None, None, 4, 7, ctypes.byref(_params()), None, ctypes.byref(out)) == 2
     assert lib.psr_relax_sliced(5, None, 1, None, None, None, None, None, 3, 8, None) == 3
+    assert lib.psr_relax_scheduled(5, None, 1, None, None, None, None, None, 3, 8, 1, -1, None) == 3
+    assert lib.psr_relax_scheduled(0, None, 0, None, None, None, None, None, 3, 8, 1, 9, None) == 2
+    assert "slice z" in lib.psr_last_error().decode()
 
 
 def test_c_abi_mixed_requires_float64():

@@ -231,6 +231,59 @@ def test_wavefront_order_equals_sequential():
                         assert wave[(z, t)] < wave[(z2, t2)], (depth, iters, z, t, z2, t2)
 
 
+def _brute_relax(keys, offsets, columns, values, b, x, iters, forward, z):
+    """solver.py:84-107 (_gs) and :227-238 (relax): full row scans per colour,
+    rows in index order, in the values' precision."""
+    dt = values.dtype.type
+    x = x.copy()
+    c = np.stack([(keys >> 32) & 0xFFFF, (keys >> 16) & 0xFFFF, keys & 0xFFFF], axis=1)
+    colour = (c[:, 0] & 1) + 2 * (c[:, 1] & 1) + 4 * (c[:, 2] & 1)
+    for _ in range(iters):
+        for col in (range(8) if forward else range(7, -1, -1)):
+            if z >= 0 and (col >> 2) != (z & 1):
+                continue
+            for i in range(len(keys)):
+                if colour[i] != col or (z >= 0 and c[i, 2] != z):
+                    continue
+                r, diagonal = b[i], dt(0)
+                for e in range(offsets[i], offsets[i + 1]):
+                    r = dt(r - dt(x[columns[e]] * values[e]))
+                    if columns[e] == i:
+                        diagonal = values[e]
+                if diagonal != dt(0):
+                    x[i] = dt(x[i] + dt(r * dt(dt(1) / diagonal)))
+    return x
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_relax_scheduled_equals_full_row_scans(dt):
+    """tests/test_relaxation.py:9-34 restated: the scheduled 8-colour sweeps
+    (forward and reverse) and the one-slice z mode give the bits of the
+    full-row-scan relaxation."""
+    p, n = synthetic.fixture_input("sphere", 512, synthetic.SEED, "double")
+    ref = oracle.reconstruct(p, n, depth=4, full_depth=2)
+    depth = 3
+    keys = ref.array("basis_keys", depth)
+    offsets = ref.array("matrix_offsets", depth).astype(np.int32)
+    columns = ref.array("matrix_columns", depth).astype(np.int32)
+    rng = np.random.default_rng(5)
+    values = rng.uniform(-1.0, 1.0, size=len(columns)).astype(dt)
+    rows = np.repeat(np.arange(len(keys)), np.diff(offsets))
+    values[columns == rows] = dt(30.0)
+    b = rng.normal(size=len(keys)).astype(dt)
+    x0 = rng.normal(size=len(keys)).astype(dt)
+    for forward in (True, False):
+        got = oracle.relax_scheduled(keys, offsets, columns, values, b, x0, depth, 2, forward=forward)
+        assert np.array_equal(got, _brute_relax(keys, offsets, columns, values, b, x0, 2, forward, -1))
+        assert not np.array_equal(got, x0)
+    x = x0.copy()
+    want = x0.copy()
+    for z in range((1 << depth), -1, -1):
+        x = oracle.relax_scheduled(keys, offsets, columns, values, b, x, depth, 1, z=z)
+        want = _brute_relax(keys, offsets, columns, values, b, want, 1, True, z)
+    assert np.array_equal(x, want)
+
+
 def test_mixed_precision_error_bound():
     """Float32 finer-level operators keep the float64 octree exactly and the
     field within 1e-6 of the float64 run (inputs and interface queries)."""

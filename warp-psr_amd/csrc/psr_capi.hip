@@ -631,6 +631,27 @@ int psr_relax_sliced(int dtype, const int64_t* keys, int64_t rows, const int32_t
   }
 }
 
+int psr_relax_scheduled(int dtype, const int64_t* keys, int64_t rows, const int32_t* offsets, const int32_t* columns,
+                        const void* values, const void* b, void* x, int depth, int iters, int forward, int z, void* stream) {
+  try {
+    if (depth < 0 || depth > 15 || iters < 1) throw PsrError(2, "Require 0 <= depth <= 15 and iters >= 1");
+    if (rows < 0 || rows > INT32_MAX) throw PsrError(2, "row count out of range");
+    hipStream_t s = (hipStream_t)stream;
+    if (dtype == 0)
+      relax_scheduled_csr<float>(keys, (int)rows, offsets, columns, (const float*)values, (const float*)b, (float*)x, depth, iters,
+                                 forward, z, s);
+    else if (dtype == 1)
+      relax_scheduled_csr<double>(keys, (int)rows, offsets, columns, (const double*)values, (const double*)b, (double*)x, depth,
+                                  iters, forward, z, s);
+    else
+      throw PsrError(3, "dtype must be 0 (float32) or 1 (float64)");
+    HIP_CHECK(hipStreamSynchronize(s));
+    return 0;
+  } catch (const PsrError& e) {
+    return fail(e);
+  }
+}
+
 const char* psr_last_error(void) { return g_error.c_str(); }
 
 void psr_release_cache(void) { Pool::instance().release_all(); }

@@ -194,6 +194,10 @@ template <class T>
 void relax_sliced_csr(const int64_t* keys, int R, const int* offsets, const int* columns, const T* values, const T* b, T* x,
                       int depth, int iters, hipStream_t s);
 
+template <class T>
+void relax_scheduled_csr(const int64_t* keys, int R, const int* offsets, const int* columns, const T* values, const T* b, T* x,
+                         int depth, int iters, int forward, int z, hipStream_t s);
+
 // shared device helpers used by several stages
 template <class T> void sort_records_u64(DArray<uint64_t>& keys, DArray<int>& ids, int64_t n, int bits, hipStream_t s);
 

@@ -2180,6 +2180,46 @@ void relax_sliced_csr(const int64_t* keys, int R, const int* offsets, const int*
   (void)relax_sliced_any(op, sc, x, depth, iters, s);
 }
 
+// relaxation.py:261-271 (relax_scheduled) on caller CSR arrays: z < 0 runs
+// iters 8-colour sweeps of row_schedule(keys, depth) (colours 0..7, or 7..0);
+// z >= 0 relaxes slice z of the sliced schedule, taking the colours whose
+// bit 2 equals z & 1 in that order (group z*4 + (colour & 3)).  Small
+// operators run in one workgroup; larger ones launch each colour group.
+template <class T>
+void relax_scheduled_csr(const int64_t* keys, int R, const int* offsets, const int* columns, const T* values, const T* b, T* x,
+                         int depth, int iters, int forward, int z, hipStream_t s) {
+  if (z > (1 << depth)) throw PsrError(2, "slice z must lie in [0, 2**depth] (or be negative for 8-colour sweeps)");
+  if (R <= 0) return;
+  DArray<int64_t> k(R, s);
+  HIP_CHECK(hipMemcpyAsync(k.data(), keys, sizeof(int64_t) * R, hipMemcpyDeviceToDevice, s));
+  DArray<int> cols(27 * (size_t)R, s), flag(1, s);
+  DArray<T> vals(27 * (size_t)R, s);
+  flag.zero();
+  hipLaunchKernelGGL(k_csr_to_slots<T>, dim3(blocks_for(R, 256)), dim3(256), 0, s, offsets, columns, values, R, cols.data(),
+                     vals.data(), flag.data());
+  KERNEL_CHECK();
+  if (read_int(flag.data(), s)) throw PsrError(2, "relaxation rows hold at most 27 entries");
+  Schedule<T> sc = make_schedule<T>(k, R, depth, z >= 0, s);
+  GsOperator<T> op = gather_operator<T>(sc, cols, vals, b, R, s);
+  if (z < 0 && R <= 65536) {
+    relax_colored(op, sc, x, iters, forward != 0, s);
+    return;
+  }
+  const unsigned bx = (unsigned)std::max(1, (sc.max_group + 255) / 256);
+  auto group = [&](int g) {
+    hipLaunchKernelGGL((k_relax_wave<T, T>), dim3(bx, 1), dim3(256), 0, s, op.update_order, (const int*)sc.offsets.data(),
+                       (const int*)op.cols.data(), (const T*)op.vals.data(), (const T*)op.b.data(), (const T*)op.inv_diag.data(),
+                       (const unsigned char*)op.has_diag.data(), x, op.R, g >> 2, g & 3);
+  };
+  for (int it = 0; it < iters; ++it)
+    for (int k8 = 0; k8 < 8; ++k8) {
+      const int color = forward ? k8 : 7 - k8;
+      if (z < 0) group(color);
+      else if ((color >> 2) == (z & 1)) group(4 * z + (color & 3));
+    }
+  KERNEL_CHECK();
+}
+
 // ================================================================ adjoint
 // Reverse of the cascadic solve and of the normal loads with respect to the
 // normal-record values, holding the octree, matrices, schedules and the root
@@ -3141,6 +3181,10 @@ void screening_backward(Result<T>& r, const SolveGrads& mg, std::vector<DArray<d
 
 template void relax_sliced_csr<float>(const int64_t*, int, const int*, const int*, const float*, const float*, float*, int, int,
                                       hipStream_t);
+template void relax_scheduled_csr<float>(const int64_t*, int, const int*, const int*, const float*, const float*, float*, int,
+                                          int, int, int, hipStream_t);
+template void relax_scheduled_csr<double>(const int64_t*, int, const int*, const int*, const double*, const double*, double*,
+                                           int, int, int, int, hipStream_t);
 template void relax_sliced_csr<double>(const int64_t*, int, const int*, const int*, const double*, const double*, double*, int,
                                        int, hipStream_t);
 

@@ -10,7 +10,7 @@ LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libwarp_psr_amd.so"
 
 EXPORTED = (
     "psr_params_default", "psr_reconstruct", "psr_evaluate", "psr_result_iso", "psr_result_field",
-    "psr_result_metadata_json", "psr_result_array", "psr_result_free", "psr_relax_sliced", "psr_last_error",
+    "psr_result_metadata_json", "psr_result_array", "psr_result_free", "psr_relax_sliced", "psr_relax_scheduled", "psr_last_error",
     "psr_release_cache", "psr_field_evaluate", "psr_field_vjp", "psr_backward", "psr_field_vjp_inputs", "psr_backward_inputs",
     "psr_reconstruct_batch",
 )
@@ -68,6 +68,7 @@ def load():
     lib.psr_result_free.argtypes = [vp]
     lib.psr_result_free.restype = None
     lib.psr_relax_sliced.argtypes = [c_int, vp, i64, vp, vp, vp, vp, vp, c_int, c_int, vp]
+    lib.psr_relax_scheduled.argtypes = [c_int, vp, i64, vp, vp, vp, vp, vp, c_int, c_int, c_int, c_int, vp]
     lib.psr_reconstruct_batch.restype = c_int
     lib.psr_reconstruct_batch.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(i64), c_int, c_int,
                                           ctypes.POINTER(PsrParams), c_int, c_int, vp, ctypes.POINTER(vp)]
@@ -82,6 +83,7 @@ def load():
     lib.psr_field_evaluate.restype = c_int
     lib.psr_field_evaluate.argtypes = [ctypes.POINTER(PsrField), vp, i64, vp, vp, vp, vp]
     lib.psr_relax_sliced.restype = c_int
+    lib.psr_relax_scheduled.restype = c_int
     lib.psr_last_error.argtypes = []
     lib.psr_last_error.restype = ctypes.c_char_p
     lib.psr_release_cache.argtypes = []

@@ -283,6 +283,24 @@ def reconstruct_batch(clouds, *, max_concurrency=8, **options):
     return [Reconstruction(out[i], dtype, device, prepared[i][3]) for i in range(count)]
 
 
+def relax_scheduled(keys, offsets, columns, values, b, x, depth, iters, forward=True, z=-1):
+    """8-colour Gauss-Seidel on device CSR tensors (relaxation.py:261-271); updates x.
+
+    ``z < 0`` sweeps the colour groups of ``row_schedule(keys, depth)``; ``z >= 0``
+    relaxes slice ``z`` of the sliced schedule, as the reference's ``z`` argument.
+    """
+    torch = _torch()
+    for t in (keys, offsets, columns, values, b, x):
+        if not isinstance(t, torch.Tensor) or t.device.type != "cuda" or not t.is_contiguous():
+            raise ValueError("relax_scheduled requires contiguous ROCm tensors")
+    dtype = 1 if values.dtype == torch.float64 else 0
+    with torch.cuda.device(x.device):
+        _lib.check(_lib.load().psr_relax_scheduled(dtype, keys.data_ptr(), keys.shape[0], offsets.data_ptr(),
+                                                   columns.data_ptr(), values.data_ptr(), b.data_ptr(), x.data_ptr(),
+                                                   depth, iters, int(bool(forward)), int(z), _stream_handle(x.device)))
+    return x
+
+
 def relax_sliced(keys, offsets, columns, values, b, x, depth, iters):
     """Sliced Gauss-Seidel on device CSR tensors in the reference order (relaxation.py:189-210); updates x."""
     torch = _torch()
