@@ -505,10 +505,21 @@ template <class V>
 __global__ void k_restrict_apply(const int* __restrict__ cols, int C, const V* __restrict__ x, V* __restrict__ y, int accumulate) {
   int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= C) return;
-  V v = V(0);
+  int c[27];
+#pragma unroll
+  for (int k = 0; k < 27; ++k) c[k] = cols[(int64_t)k * C + r];
+  V xv[27];
+#pragma unroll
   for (int k = 0; k < 27; ++k) {
-    const int c = cols[(int64_t)k * C + r];
-    if (c >= 0) v = v + V(restriction_weight(k)) * x[c];
+    // x may be empty (a finer level without nodes): absent slots read nothing
+    xv[k] = V(0);
+    if (c[k] >= 0) xv[k] = x[c[k]];
+  }
+  V v = V(0);
+#pragma unroll
+  for (int k = 0; k < 27; ++k) {
+    const V t = v + V(restriction_weight(k)) * xv[k];
+    v = c[k] >= 0 ? t : v;
   }
   if (accumulate) v = v + y[r];
   y[r] = v;
@@ -516,15 +527,48 @@ __global__ void k_restrict_apply(const int* __restrict__ cols, int C, const V* _
 
 // y = alpha*(A x) + beta*y with A a stencil (slots in column order)
 // (values of type V, widened exactly to T)
+// The slot sum with every load issued before the additions (S known at
+// compile time); absent slots are skipped by a select, so the sum is the
+// sequential one.
+template <class T, class V, int S>
+__device__ __forceinline__ T stencil_row_sum(const int* __restrict__ cols, const V* __restrict__ vals, int R, int i,
+                                             const T* __restrict__ x) {
+  int c[S];
+#pragma unroll
+  for (int k = 0; k < S; ++k) c[k] = cols[(int64_t)k * R + i];
+  V w[S];
+  T xv[S];
+#pragma unroll
+  for (int k = 0; k < S; ++k) {
+    w[k] = vals[(int64_t)k * R + i];
+    // x may be empty (a level without nodes): absent slots read nothing
+    xv[k] = T(0);
+    if (c[k] >= 0) xv[k] = x[c[k]];
+  }
+  T v = T(0);
+#pragma unroll
+  for (int k = 0; k < S; ++k) {
+    const T t = v + T(w[k]) * xv[k];
+    v = c[k] >= 0 ? t : v;
+  }
+  return v;
+}
+
 template <class T, class V = T>
 __global__ void k_spmv(const int* __restrict__ cols, const V* __restrict__ vals, int R, int slots, const T* __restrict__ x,
                        T* __restrict__ y, T alpha, T beta) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= R) return;
   T v = T(0);
-  for (int k = 0; k < slots; ++k) {
-    const int c = cols[(int64_t)k * R + i];
-    if (c >= 0) v = v + T(vals[(int64_t)k * R + i]) * x[c];
+  if (slots == 27) {
+    v = stencil_row_sum<T, V, 27>(cols, vals, R, i, x);
+  } else if (slots == 8) {
+    v = stencil_row_sum<T, V, 8>(cols, vals, R, i, x);
+  } else {
+    for (int k = 0; k < slots; ++k) {
+      const int c = cols[(int64_t)k * R + i];
+      if (c >= 0) v = v + T(vals[(int64_t)k * R + i]) * x[c];
+    }
   }
   v = v * alpha;
   if (beta != T(0)) v = v + beta * y[i];
