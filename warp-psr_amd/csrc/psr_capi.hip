@@ -199,9 +199,9 @@ template <class T> bool export_array(Result<T>& r, const std::string& name, int 
     else stencil_csr<T>(L().A, 27, name.c_str() + 7, out, inv);
     return true;
   }
-  if (lvl && name.rfind("cross_", 0) == 0) {
-    if (L().cross32.rows) stencil_csr<T>(L().cross32, 27, name.c_str() + 6, out, inv);
-    else stencil_csr<T>(L().cross, 27, name.c_str() + 6, out, inv);
+  if (lvl && name.rfind("cross_", 0) == 0) {  // canonical rows
+    if (L().cross32.rows) stencil_csr<T>(L().cross32, 27, name.c_str() + 6, out, {});
+    else stencil_csr<T>(L().cross, 27, name.c_str() + 6, out, {});
     return true;
   }
   if (lvl && name.rfind("prolong_", 0) == 0) { stencil_csr<T>(L().P, 8, name.c_str() + 8, out, {}); return true; }

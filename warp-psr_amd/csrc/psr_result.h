@@ -44,8 +44,8 @@ template <class T> struct LevelState {
   Stencil<T> P;             // prolongation rows (8 slots, x-major corner order)
   DArray<double> normal_load, point_load;
   DArray<T> rhs, correction;
-  // finer levels keep A, cross, rhs and point_load in sliced schedule order:
-  // position j holds row order[j]
+  // finer levels keep A, rhs and point_load in sliced schedule order
+  // (position j holds row order[j]); cross keeps canonical rows
   DArray<int> order;
   bool scheduled = false;
   bool solved = false;
@@ -166,7 +166,7 @@ void backward_inputs(Result<T>& r, const double* gcoef, const double* gcum, doub
 // Matrix-side gradients of a solve, per level d in [base, top]: w.r.t. the
 // operator values (gA: the level's storage layout, schedule positions for
 // finer levels and canonical rows for the base level), the finer levels'
-// cross operators (gC, schedule positions) and the right-hand sides (grhs,
+// cross operators (gC, canonical rows) and the right-hand sides (grhs,
 // canonical).
 struct SolveGrads {
   int top = -1;

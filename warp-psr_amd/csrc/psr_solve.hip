@@ -395,7 +395,7 @@ __global__ void __launch_bounds__(128) k_level_operator(const int64_t* __restric
   const int j = blockIdx.x * blockDim.x + threadIdx.x;  // output position
   int count = 0;
   if (j < R) {
-    const int i = row_order[j];
+    const int i = row_order ? row_order[j] : j;
     const int64_t key = rows[i];
     double acc[27], rhs = 0.0;
     if (SCREEN) {
@@ -1238,6 +1238,17 @@ __global__ void k_root_extract(const int64_t* __restrict__ keys, int R, const T*
   x[i] = pad_x[cz * 16 + (cx & 1) + 2 * (cy & 1) + 2 * (cx & 2) + 4 * (cy & 2)];
 }
 
+// b[j] = (sum[order[j]] * -1) + 1 * b[j]: k_spmv's alpha = -1, beta = 1 step
+// for a sum computed over canonical rows
+template <class T>
+__global__ void k_subtract_coupling(const int* __restrict__ order, int R, const T* __restrict__ sum, T* __restrict__ b) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= R) return;
+  T v = sum[order[j]] * T(-1);
+  v = v + T(1) * b[j];
+  b[j] = v;
+}
+
 // solver.py:119-126 (_rhs) written in schedule order
 template <class T>
 __global__ void k_rhs_sched(const double* __restrict__ load, const double* __restrict__ point_load_sched,
@@ -1640,7 +1651,8 @@ template <class T> static Stencil<T> cast_stencil(Stencil<double>&& m, hipStream
 
 // A sliced finer level's operator (colsL = rowsL, trial_depth = depth; op and
 // present given) or cross operator (colsL one level coarser) in one pass of
-// k_level_operator, rows in schedule order ord (inv = its inverse).
+// k_level_operator, rows in schedule order ord (inv = its inverse) or, with ord
+// null, in canonical order.
 template <class T, class V>
 static void level_operator(const LevelState<T>& rowsL, const LevelState<T>& colsL, int depth, int trial_depth,
                            const PointSet<T>* pts, double* load, const int* ord, const int* inv, Stencil<V>& out,
@@ -1921,8 +1933,10 @@ static void prepare_finer(Result<T>& r, int d, Stencil<V>& Aop, Stencil<V>& Cop,
   op.R = R;
   level_operator<T, V>(L, L, d, d, screened ? &pts : nullptr, L.point_load.data(), ord, inv.data(), Aop, &op, f.nnz.data(), s);
   op.vals = std::move(Aop.vals);
-  level_operator<T, V>(L, r.levels[d - 1], d, d - 1, screened ? &pts : nullptr, (double*)nullptr, ord, (const int*)nullptr,
-                       Cop, (GsOperator<T, V>*)nullptr, (unsigned long long*)nullptr, s);
+  // the cross operator keeps canonical rows: its product with the coarser
+  // cumulative solution gathers from neighbouring parents in key order
+  level_operator<T, V>(L, r.levels[d - 1], d, d - 1, screened ? &pts : nullptr, (double*)nullptr, (const int*)nullptr,
+                       (const int*)nullptr, Cop, (GsOperator<T, V>*)nullptr, (unsigned long long*)nullptr, s);
 }
 
 // One finer level's solve (solver.py:343-360): right-hand side, b -= cross *
@@ -1939,9 +1953,16 @@ static void solve_finer(Result<T>& r, int d, Stencil<V>& Aop, Stencil<V>& Cop, F
   hipLaunchKernelGGL(k_rhs_sched<T>, dim3(blocks_for(R, 256)), dim3(256), 0, s, (const double*)L.normal_load.data(),
                      (const double*)L.point_load.data(), ord, R, L.rhs.data());
   KERNEL_CHECK();
-  hipLaunchKernelGGL((k_spmv<T, V>), dim3(blocks_for(R, 256)), dim3(256), 0, s, (const int*)Cop.cols.data(),
-                     (const V*)Cop.vals.data(), R, 27, (const T*)cumulative.data(), L.rhs.data(), T(-1), T(1));
-  KERNEL_CHECK();
+  {  // b -= cross * cumulative(d-1): the canonical-row sum, then k_spmv's
+     // alpha = -1, beta = 1 combination at each schedule position
+    DArray<T> coupling(R, s);
+    hipLaunchKernelGGL((k_spmv<T, V>), dim3(blocks_for(R, 256)), dim3(256), 0, s, (const int*)Cop.cols.data(),
+                       (const V*)Cop.vals.data(), R, 27, (const T*)cumulative.data(), coupling.data(), T(1), T(0));
+    KERNEL_CHECK();
+    hipLaunchKernelGGL(k_subtract_coupling<T>, dim3(blocks_for(R, 256)), dim3(256), 0, s, ord, R, (const T*)coupling.data(),
+                       L.rhs.data());
+    KERNEL_CHECK();
+  }
   op.b = std::move(L.rhs);
   DArray<T> xs(R, s);
   xs.zero();  // the level's correction starts at zero (solver.py:343)
@@ -2485,8 +2506,8 @@ template <class T> __device__ __forceinline__ double hat_deriv(T p, T s, T c) {
 
 // screening_stencils.py:34-70 reversed at one level: row j's slot gradients
 // of the operator (gA, trial depth = depth) and of the cross operator (gC,
-// trial depth = depth - 1, may be null) and its point-load gradient (gload,
-// may be null) reach the constraint points' positions and weights.
+// trial depth = depth - 1) and its point-load gradient (gload) reach the
+// constraint points' positions and weights; each of the three may be null.
 template <class T>
 __global__ void __launch_bounds__(128) k_screening_vjp(const int64_t* __restrict__ rows, int R, const int* __restrict__ row_order,
                                                        int depth, PointBins<T> pb, const double* __restrict__ gA,
@@ -2501,7 +2522,7 @@ __global__ void __launch_bounds__(128) k_screening_vjp(const int64_t* __restrict
   double GA[27], GC[27];
   bool any = false;
   for (int k = 0; k < 27; ++k) {
-    GA[k] = gA[(int64_t)k * R + j];
+    GA[k] = gA ? gA[(int64_t)k * R + j] : 0.0;
     GC[k] = gC ? gC[(int64_t)k * R + j] : 0.0;
     any = any || GA[k] != 0.0 || GC[k] != 0.0;
   }
@@ -2550,7 +2571,7 @@ __global__ void __launch_bounds__(128) k_screening_vjp(const int64_t* __restrict
           double S = GL, Sp[3] = {0.0, 0.0, 0.0};  // sum_k G_k trial_k and its derivatives (load: trial = 1)
           for (int which = 0; which < 2; ++which) {
             const double* G = which == 0 ? GA : GC;
-            if (which == 1 && !gC) break;
+            if ((which == 0 && !gA) || (which == 1 && !gC)) continue;
             const int td = which == 0 ? depth : depth - 1;
             const int shift = depth - td;
             const int px = (cx >> shift) - 1, py = (cy >> shift) - 1, pz = (cz >> shift) - 1;
@@ -2819,12 +2840,12 @@ void finer_backward(Result<T>& r, int d, const Stencil<V>& Aop, const Stencil<V>
       }
     KERNEL_CHECK();
   }
-  // b = rhs - cross * cumulative(d-1)
-  stencil_scatter<V>((const int*)sc.order.data(), Cop, 27, (const double*)grhs.data(), -1.0, gcum_prev.data(), s);
+  // b = rhs - cross * cumulative(d-1); the cross operator has canonical rows
+  stencil_scatter<V>((const int*)nullptr, Cop, 27, (const double*)grhs.data(), -1.0, gcum_prev.data(), s);
   if (gC && R) {
     gC->reset(27 * (size_t)R, s);
     gC->zero();
-    hipLaunchKernelGGL(k_stencil_outer<T>, dim3(blocks_for(R, 256)), dim3(256), 0, s, (const int*)sc.order.data(),
+    hipLaunchKernelGGL(k_stencil_outer<T>, dim3(blocks_for(R, 256)), dim3(256), 0, s, (const int*)nullptr,
                        (const int*)Cop.cols.data(), R, 27, (const double*)grhs.data(), cum_prev, -1.0, gC->data());
     KERNEL_CHECK();
   }
@@ -3210,11 +3231,18 @@ void screening_backward(Result<T>& r, const SolveGrads& mg, std::vector<DArray<d
                        gload.data());
     KERNEL_CHECK();
     DArray<double> gpos = zeros_d(3 * (size_t)pts.P, s), gw = zeros_d(pts.P, s);
+    // the operator and point load at schedule positions (finer levels), the
+    // cross operator at canonical rows
     hipLaunchKernelGGL(k_screening_vjp<T>, dim3(blocks_for(R, 128)), dim3(128), 0, s, (const int64_t*)L.basis_keys.data(), R, ord, d,
-                       point_bins(pts), (const double*)mg.gA[d].data(),
-                       d > base && !mg.gC[d].empty() ? (const double*)mg.gC[d].data() : (const double*)nullptr,
-                       (const double*)gload.data(), gpos.data(), gw.data());
+                       point_bins(pts), (const double*)mg.gA[d].data(), (const double*)nullptr, (const double*)gload.data(),
+                       gpos.data(), gw.data());
     KERNEL_CHECK();
+    if (d > base && !mg.gC[d].empty()) {
+      hipLaunchKernelGGL(k_screening_vjp<T>, dim3(blocks_for(R, 128)), dim3(128), 0, s, (const int64_t*)L.basis_keys.data(), R,
+                         (const int*)nullptr, d, point_bins(pts), (const double*)nullptr, (const double*)mg.gC[d].data(),
+                         (const double*)nullptr, gpos.data(), gw.data());
+      KERNEL_CHECK();
+    }
     if (gagg[d].empty()) gagg[d] = zeros_d(3 * (size_t)pts.P, s);
     hipLaunchKernelGGL(k_points_vjp<T>, dim3(blocks_for(pts.P, 256)), dim3(256), 0, s, (const T*)L.agg_vals.data(),
                        (const T*)pts.w.data(), pts.P, d, r.prm.point_weight, (const double*)gpos.data(), (const double*)gw.data(),
