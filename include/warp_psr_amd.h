@@ -166,6 +166,43 @@ int psr_backward_inputs(PsrResult* result, const double* coef_grad, const double
                         const double* transform_grad, const void* points, const void* normals, int64_t n, void* grad_points,
                         void* grad_normals, void* stream);
 
+/* Marching cubes on a uniform model-space grid (src/warp_psr/meshing.py:150-280,
+ * the sparse backend's steps), all buffers on the device and on `stream`:
+ *   psr_mesh_begin       grid of shape[0] x shape[1] x shape[2] nodes between
+ *                        lower and upper (float32 spacing (upper-lower)/(n-1));
+ *                        returns 2 with the reference's messages for bad grids
+ *   psr_mesh_slab_nodes  node count of planes [start, start + width]
+ *   psr_mesh_slab_points their model-space points (3 floats each, x-major)
+ *   psr_mesh_slab_values classify the cells of planes [start, start + width)
+ *                        from the caller's values at those points
+ *   psr_mesh_select      count and list the crossing cells (value >= 0 is
+ *                        positive); invalid != 0 if a value was not finite
+ *   psr_mesh_corner_points  the 8 corner points of every retained cell, in the
+ *                        corner order (000,100,110,010,001,101,111,011)
+ *   psr_mesh_triangulate from the caller's values at those corners: one
+ *                        vertex per crossing grid edge, triangles from the
+ *                        library's case table (psr_mesh_case_table); faces
+ *                        point toward increasing values
+ *   psr_mesh_copy        vertices (3 floats each) and indices (3 per triangle)
+ *   psr_mesh_vertex_vjp  vertex gradients -> corner-value gradients (added)
+ * The corner values given to psr_mesh_triangulate must stay valid for
+ * psr_mesh_vertex_vjp.  Each mesh is used from one stream at a time. */
+typedef struct PsrMesh PsrMesh;
+int psr_mesh_begin(const int32_t shape[3], const float lower[3], const float upper[3], void* stream, PsrMesh** out);
+int psr_mesh_slab_nodes(PsrMesh* mesh, int32_t start, int32_t width, int64_t* nodes);
+int psr_mesh_slab_points(PsrMesh* mesh, int32_t start, int32_t width, float* points);
+int psr_mesh_slab_values(PsrMesh* mesh, int32_t start, int32_t width, const float* values);
+int psr_mesh_select(PsrMesh* mesh, int64_t* retained, int32_t* invalid);
+int psr_mesh_corner_points(PsrMesh* mesh, float* points);
+int psr_mesh_triangulate(PsrMesh* mesh, const float* corner_values, int64_t* vertices, int64_t* triangles);
+int psr_mesh_copy(PsrMesh* mesh, float* vertices, int32_t* indices);
+int psr_mesh_vertex_vjp(PsrMesh* mesh, const float* corner_values, const float* grad_vertices, float* grad_values);
+/* The 256-case table: counts[256] triangles per case, edges[256][36] cube
+ * edges per triangle; edge k joins corners (lower, upper) =
+ * (0,1),(1,2),(3,2),(0,3),(4,5),(5,6),(7,6),(4,7),(0,4),(1,5),(2,6),(3,7). */
+int psr_mesh_case_table(int8_t* counts, int8_t* edges);
+void psr_mesh_free(PsrMesh* mesh);
+
 /* Evaluates any PsrField view (psr_result_field) at m device queries with the
  * header-only psr_evaluate_field of warp_psr_amd_field.h -- the kernel-side
  * evaluate_field(field, point) of src/warp_psr/device_field.py:91-112 --

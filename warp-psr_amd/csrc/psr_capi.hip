@@ -5,6 +5,7 @@
 #include "../../include/warp_psr_amd.h"
 #include "../../include/warp_psr_amd_field.h"
 #include "psr_result.h"
+#include "psr_mesh.h"
 
 #include <cmath>
 #include <cstring>
@@ -650,6 +651,132 @@ int psr_relax_scheduled(int dtype, const int64_t* keys, int64_t rows, const int3
   } catch (const PsrError& e) {
     return fail(e);
   }
+}
+
+// ---------------------------------------------------------------- marching cubes (meshing.py)
+struct PsrMesh {
+  McState* state = nullptr;
+};
+
+int psr_mesh_begin(const int32_t shape[3], const float lower[3], const float upper[3], void* stream, PsrMesh** out) {
+  try {
+    if (!shape || !lower || !upper || !out) throw PsrError(2, "null argument");
+    int sh[3] = {shape[0], shape[1], shape[2]};
+    auto* m = new PsrMesh();
+    try {
+      m->state = mc_create(sh, lower, upper, (hipStream_t)stream);
+    } catch (...) {
+      delete m;
+      throw;
+    }
+    *out = m;
+    return 0;
+  } catch (const PsrError& e) {
+    return fail(e);
+  }
+}
+
+int psr_mesh_slab_nodes(PsrMesh* m, int32_t start, int32_t width, int64_t* nodes) {
+  try {
+    if (!m || !nodes) throw PsrError(2, "null argument");
+    *nodes = mc_slab_nodes(*m->state, start, width);
+    return 0;
+  } catch (const PsrError& e) {
+    return fail(e);
+  }
+}
+
+int psr_mesh_slab_points(PsrMesh* m, int32_t start, int32_t width, float* points) {
+  try {
+    if (!m || !points) throw PsrError(2, "null argument");
+    mc_slab_points(*m->state, start, width, points);
+    return 0;
+  } catch (const PsrError& e) {
+    return fail(e);
+  }
+}
+
+int psr_mesh_slab_values(PsrMesh* m, int32_t start, int32_t width, const float* values) {
+  try {
+    if (!m || !values) throw PsrError(2, "null argument");
+    mc_slab_values(*m->state, start, width, values);
+    return 0;
+  } catch (const PsrError& e) {
+    return fail(e);
+  }
+}
+
+int psr_mesh_select(PsrMesh* m, int64_t* retained, int32_t* invalid) {
+  try {
+    if (!m || !retained || !invalid) throw PsrError(2, "null argument");
+    int bad = 0;
+    *retained = mc_select(*m->state, &bad);
+    *invalid = bad;
+    return 0;
+  } catch (const PsrError& e) {
+    return fail(e);
+  }
+}
+
+int psr_mesh_corner_points(PsrMesh* m, float* points) {
+  try {
+    if (!m) throw PsrError(2, "null argument");
+    mc_corner_points(*m->state, points);
+    return 0;
+  } catch (const PsrError& e) {
+    return fail(e);
+  }
+}
+
+int psr_mesh_triangulate(PsrMesh* m, const float* corner_values, int64_t* vertices, int64_t* triangles) {
+  try {
+    if (!m || !vertices || !triangles) throw PsrError(2, "null argument");
+    int V = 0, T = 0;
+    mc_triangulate(*m->state, corner_values, &V, &T);
+    *vertices = V;
+    *triangles = T;
+    return 0;
+  } catch (const PsrError& e) {
+    return fail(e);
+  }
+}
+
+int psr_mesh_copy(PsrMesh* m, float* vertices, int32_t* indices) {
+  try {
+    if (!m) throw PsrError(2, "null argument");
+    mc_copy(*m->state, vertices, indices);
+    return 0;
+  } catch (const PsrError& e) {
+    return fail(e);
+  }
+}
+
+int psr_mesh_vertex_vjp(PsrMesh* m, const float* corner_values, const float* grad_vertices, float* grad_values) {
+  try {
+    if (!m) throw PsrError(2, "null argument");
+    mc_vertex_vjp(*m->state, corner_values, grad_vertices, grad_values);
+    return 0;
+  } catch (const PsrError& e) {
+    return fail(e);
+  }
+}
+
+int psr_mesh_case_table(int8_t* counts, int8_t* edges) {
+  try {
+    if (!counts || !edges) throw PsrError(2, "null argument");
+    const McTable& t = mc_table();
+    std::memcpy(counts, t.count, sizeof(t.count));
+    std::memcpy(edges, t.edges, sizeof(t.edges));
+    return 0;
+  } catch (const PsrError& e) {
+    return fail(e);
+  }
+}
+
+void psr_mesh_free(PsrMesh* m) {
+  if (!m) return;
+  mc_free(m->state);
+  delete m;
 }
 
 const char* psr_last_error(void) { return g_error.c_str(); }

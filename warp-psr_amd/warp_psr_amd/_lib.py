@@ -12,7 +12,9 @@ EXPORTED = (
     "psr_params_default", "psr_reconstruct", "psr_evaluate", "psr_result_iso", "psr_result_field",
     "psr_result_metadata_json", "psr_result_array", "psr_result_free", "psr_relax_sliced", "psr_relax_scheduled", "psr_last_error",
     "psr_release_cache", "psr_field_evaluate", "psr_field_vjp", "psr_backward", "psr_field_vjp_inputs", "psr_backward_inputs",
-    "psr_reconstruct_batch",
+    "psr_reconstruct_batch", "psr_mesh_begin", "psr_mesh_slab_nodes", "psr_mesh_slab_points", "psr_mesh_slab_values",
+    "psr_mesh_select", "psr_mesh_corner_points", "psr_mesh_triangulate", "psr_mesh_copy", "psr_mesh_vertex_vjp",
+    "psr_mesh_case_table", "psr_mesh_free",
 )
 
 
@@ -84,6 +86,24 @@ def load():
     lib.psr_field_evaluate.argtypes = [ctypes.POINTER(PsrField), vp, i64, vp, vp, vp, vp]
     lib.psr_relax_sliced.restype = c_int
     lib.psr_relax_scheduled.restype = c_int
+    i32p, i64p = ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(i64)
+    lib.psr_mesh_begin.argtypes = [i32p, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float), vp,
+                                   ctypes.POINTER(vp)]
+    lib.psr_mesh_slab_nodes.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, i64p]
+    lib.psr_mesh_slab_points.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, vp]
+    lib.psr_mesh_slab_values.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, vp]
+    lib.psr_mesh_select.argtypes = [vp, i64p, i32p]
+    lib.psr_mesh_corner_points.argtypes = [vp, vp]
+    lib.psr_mesh_triangulate.argtypes = [vp, vp, i64p, i64p]
+    lib.psr_mesh_copy.argtypes = [vp, vp, vp]
+    lib.psr_mesh_vertex_vjp.argtypes = [vp, vp, vp, vp]
+    lib.psr_mesh_case_table.argtypes = [vp, vp]
+    for name in ("psr_mesh_begin", "psr_mesh_slab_nodes", "psr_mesh_slab_points", "psr_mesh_slab_values", "psr_mesh_select",
+                 "psr_mesh_corner_points", "psr_mesh_triangulate", "psr_mesh_copy", "psr_mesh_vertex_vjp",
+                 "psr_mesh_case_table"):
+        getattr(lib, name).restype = c_int
+    lib.psr_mesh_free.argtypes = [vp]
+    lib.psr_mesh_free.restype = None
     lib.psr_last_error.argtypes = []
     lib.psr_last_error.restype = ctypes.c_char_p
     lib.psr_release_cache.argtypes = []
