@@ -88,6 +88,32 @@ def relax_roofline(metadata, value_bytes, vector_bytes, kernel=None):
     }
 
 
+def mesh_throughput(torch, recon, points, dev, resolution=256, repeats=3):
+    """Marching cubes of the float32 depth-9 field on a resolution^3 node grid
+    over the input's bounding box (sparse backend, 16-layer slabs): every node
+    evaluated once for the classification, the retained cells' corners again
+    for the vertices.  Median of `repeats` after one warm-up."""
+    from warp_psr_amd.meshing import extract_mesh
+    lo, hi = points.min(axis=0).astype(np.float64), points.max(axis=0).astype(np.float64)
+    pad = 0.02 * (hi - lo)
+    bounds = (tuple(lo - pad), tuple(hi + pad))
+    mesh = extract_mesh(recon, resolution=resolution, bounds=bounds)
+    times = []
+    for _ in range(repeats):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        mesh = extract_mesh(recon, resolution=resolution, bounds=bounds)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t)
+    dt = sorted(times)[len(times) // 2]
+    cells = (resolution - 1) ** 3
+    log(f"mesh: {resolution}^3 nodes, {mesh.metadata['retained_cells']} retained cells, {len(mesh.vertices)} vertices, "
+        f"{dt * 1e3:.1f} ms")
+    return {"resolution": resolution, "backend": "sparse", "seconds": round(dt, 4), "Mcells_s": round(cells / dt / 1e6, 2),
+            "field_evaluations": mesh.metadata["field_evaluations"], "retained_cells": mesh.metadata["retained_cells"],
+            "vertices": int(len(mesh.vertices)), "triangles": int(len(mesh.indices) // 3)}
+
+
 def batch_throughput(torch, synthetic, dev, concurrency=8):
     """64 independent float32 clouds (5k-200k points, depth 7/8): one batched
     call on `concurrency` worker streams vs the same clouds one by one."""
@@ -144,6 +170,7 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--skip-batch", action="store_true")
+    ap.add_argument("--skip-mesh", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (read 1 GiB, write 1 GiB) three times, for scaling "
                          "FETCH_SIZE/WRITE_SIZE counters")
@@ -236,6 +263,8 @@ def main():
 
         out["field_error_vs_fp64"] = error(rmx, q64)
         out["float32"]["field_error_vs_fp64"] = error(r32, q64.astype(np.float32))
+    if not args.skip_mesh:
+        out["mesh"] = mesh_throughput(torch, r32, p32, dev)
     if not args.skip_batch:
         out["batch"] = batch_throughput(torch, synthetic, dev)
     if not args.skip_cpu:
