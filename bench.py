@@ -171,6 +171,7 @@ def main():
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--skip-batch", action="store_true")
     ap.add_argument("--skip-mesh", action="store_true")
+    ap.add_argument("--skip-float32", action="store_true", help="no float32 leg (implies --skip-mesh)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (read 1 GiB, write 1 GiB) three times, for scaling "
                          "FETCH_SIZE/WRITE_SIZE counters")
@@ -238,9 +239,11 @@ def main():
     out["roofline"] = relax_roofline(rmx.metadata, 4, 8, "k_relax_wave_args<double, float, 64>")
     out["stage_seconds"] = rmx.metadata["timings_seconds"]
     out["normal_average_wait_seconds"] = rmx.metadata["normal_average_wait_seconds"]
-    r32, dt32 = timed(tp32, tn32, max(1, args.steps // 2), 1, "float32")
-    out["float32"] = {"Mpts_s": round(args.points / dt32 / 1e6, 4), "ms_per_step": round(dt32 * 1e3, 3),
-                      "relax_roofline": relax_roofline(r32.metadata, 4, 4)}
+    r32 = None
+    if not args.skip_float32:
+        r32, dt32 = timed(tp32, tn32, max(1, args.steps // 2), 1, "float32")
+        out["float32"] = {"Mpts_s": round(args.points / dt32 / 1e6, 4), "ms_per_step": round(dt32 * 1e3, 3),
+                          "relax_roofline": relax_roofline(r32.metadata, 4, 4)}
     if not args.skip_fp64:
         r64, dt64 = timed(tp64, tn64, max(1, args.steps // 2), 1, "float64")
         out["float64"] = {"Mpts_s": round(args.points / dt64 / 1e6, 4), "ms_per_step": round(dt64 * 1e3, 3),
@@ -262,8 +265,9 @@ def main():
                     "within_1e-6": bool(err.max() <= 1e-6)}
 
         out["field_error_vs_fp64"] = error(rmx, q64)
-        out["float32"]["field_error_vs_fp64"] = error(r32, q64.astype(np.float32))
-    if not args.skip_mesh:
+        if r32 is not None:
+            out["float32"]["field_error_vs_fp64"] = error(r32, q64.astype(np.float32))
+    if not args.skip_mesh and r32 is not None:
         out["mesh"] = mesh_throughput(torch, r32, p32, dev)
     if not args.skip_batch:
         out["batch"] = batch_throughput(torch, synthetic, dev)

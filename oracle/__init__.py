@@ -21,7 +21,7 @@ _INT_ARRAYS = {
     "prolong_offsets", "prolong_columns", "cg_runs",
 }
 _I64_ARRAYS = {"sample_keys", "aggregate_keys", "density_keys", "normal_keys", "tree_keys", "basis_keys", "keys"}
-_F64_ARRAYS = {"normal_load", "point_load"}
+_F64_ARRAYS = {"normal_load", "ordered_normal_load", "point_load"}
 
 
 def build():

@@ -23,6 +23,7 @@
 //                                                 (isovalue goldens), bitwise
 //                                                 parity unpinned.
 #include <algorithm>
+#include <type_traits>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -953,8 +954,158 @@ std::vector<double> normal_stencil_load(const std::vector<int64_t>& keys, const 
   return out;
 }
 
-// src/warp_psr/normal_loads.py:66-190 (stencil_loads=True path)
+// src/warp_psr/ordered_normal_loads.py:86-94 (normal_stencil) and :137-152
+// (contribution): the record's three values against the float64 stencil
+// rounded to T, summed in T
+template <class T>
+T ordered_contribution(int test_depth, const I3& test, int normal_depth, const I3& n, const int64_t* keys, int64_t count,
+                       const T* values, int stride) {
+  T dot = T(0);
+  if (std::min(std::min(n[0], n[1]), n[2]) >= 0 && std::max(std::max(n[0], n[1]), n[2]) < (1 << normal_depth)) {
+    const int64_t j = find_key(keys, count, basis_key(normal_depth, n[0], n[1], n[2]));
+    if (j >= 0) {
+      const D2 x = normal_integral(test_depth, test[0], normal_depth, n[0]);
+      const D2 y = normal_integral(test_depth, test[1], normal_depth, n[1]);
+      const D2 z = normal_integral(test_depth, test[2], normal_depth, n[2]);
+      const double st[3] = {x.v[1] * y.v[0] * z.v[0], x.v[0] * y.v[1] * z.v[0], x.v[0] * y.v[0] * z.v[1]};
+      for (int a = 0; a < 3; ++a) dot = dot + values[stride * j + a] * T(st[a]);
+    }
+  }
+  return dot;
+}
+
+// src/warp_psr/ordered_normal_loads.py:97-103 (morton_xy)
+inline int64_t morton_xy(const I3& c) {
+  int64_t key = 0;
+  for (int bit = 0; bit < 15; ++bit) {
+    key |= (int64_t)((c[0] >> bit) & 1) << (2 * bit);
+    key |= (int64_t)((c[1] >> bit) & 1) << (2 * bit + 1);
+  }
+  return key;
+}
+
+// src/warp_psr/ordered_normal_loads.py:263-337 (ordered_normal_loads): the
+// loads of a float32 run with base depth 0 (normal_loads.py:84-92), summed in
+// T in the pinned serial order, then widened to float64.
+template <class T>
+std::vector<std::vector<double>> ordered_normal_loads(const State<T>& s, int max_depth, int base_depth) {
+  const auto& rk = s.normal_records.keys;
+  const auto& rv = s.normal_records.values;
+  std::vector<T> rvals(4 * rv.size());
+  for (size_t i = 0; i < rv.size(); ++i)
+    for (int a = 0; a < 4; ++a) rvals[4 * i + a] = rv[i][a];
+  auto own_keys = [&](int d) { return rk.data() + s.normal_offsets[d]; };
+  auto own_count = [&](int d) { return (int64_t)(s.normal_offsets[d + 1] - s.normal_offsets[d]); };
+  auto own_vals = [&](int d) { return rvals.data() + 4 * s.normal_offsets[d]; };
+  std::vector<std::vector<T>> loads(max_depth + 1);
+  std::vector<T> finer(s.basis[max_depth].size(), T(0));
+  for (int d = max_depth; d >= base_depth; --d) {
+    const std::vector<int64_t>& keys = s.basis[d];
+    std::vector<T> load(keys.size());
+    for (size_t i = 0; i < keys.size(); ++i) {  // :156-171 (same)
+      const I3 c = coords(keys[i]);
+      T value = finer[i];
+      for (int k = 0; k < 64; ++k) {
+        const I3 n = {{c[0] + k / 16 - 2, c[1] + (k / 4) % 4 - 2, c[2] + k % 4 - 2}};
+        value = value + ordered_contribution<T>(d, c, d, n, own_keys(d), own_count(d), own_vals(d), 4);
+      }
+      load[i] = value;
+    }
+    loads[d] = load;
+    if (d > base_depth) {
+      const std::vector<int64_t>& coarse = s.basis[d - 1];
+      std::vector<T> next(coarse.size(), T(0));
+      if (own_count(d)) {
+        for (size_t i = 0; i < coarse.size(); ++i) {  // :105-122 (order) and :173-188 (parent)
+          const I3 c = coords(coarse[i]);
+          std::vector<std::pair<int64_t, int>> order(216);
+          for (int k = 0; k < 216; ++k) {
+            const I3 cc = {{2 * c[0] + k % 6 - 3, 2 * c[1] + (k / 6) % 6 - 3, 2 * c[2] + k / 36 - 3}};
+            order[k] = {std::min(std::min(cc[0], cc[1]), cc[2]) >= 0 ? morton_xy(cc) : (int64_t)2147483647, k};
+          }
+          for (int z = 0; z < 6; ++z) std::stable_sort(order.begin() + 36 * z, order.begin() + 36 * (z + 1),
+                                                       [](const auto& a, const auto& b) { return a.first < b.first; });
+          T value = T(0);
+          for (int j = 0; j < 216; ++j) {
+            const int k = order[j].second;
+            const I3 n = {{2 * c[0] + k % 6 - 3, 2 * c[1] + (k / 6) % 6 - 3, 2 * c[2] + k / 36 - 3}};
+            value = value + ordered_contribution<T>(d - 1, c, d, n, own_keys(d), own_count(d), own_vals(d), 4);
+          }
+          next[i] = value;
+        }
+      }
+      if (d < max_depth) {
+        for (size_t i = 0; i < coarse.size(); ++i) {  // :190-212 (restrict)
+          const I3 c = coords(coarse[i]);
+          T value = next[i];
+          for (int k = 0; k < 27; ++k) {
+            const I3 off = {{k / 9 - 1, (k / 3) % 3 - 1, k % 3 - 1}};
+            const I3 n = {{2 * c[0] + off[0], 2 * c[1] + off[1], 2 * c[2] + off[2]}};
+            if (std::min(std::min(n[0], n[1]), n[2]) >= 0 && std::max(std::max(n[0], n[1]), n[2]) <= (1 << d)) {
+              const int64_t j = find_key(keys, basis_key(d, n[0], n[1], n[2]));
+              if (j >= 0) {
+                T weight = T(1);
+                for (int a = 0; a < 3; ++a)
+                  if (off[a] != 0) weight = weight * T(0.5);
+                value = value + finer[j] * weight;
+              }
+            }
+          }
+          next[i] = value;
+        }
+      }
+      finer = std::move(next);
+    }
+  }
+  std::vector<int64_t> previous_keys;
+  std::vector<T> previous;
+  for (int d = 0; d < max_depth; ++d) {
+    const std::vector<int64_t>& keys = s.basis[d];
+    std::vector<T> cumulative(4 * keys.size());
+    for (size_t i = 0; i < keys.size(); ++i) {  // :214-238 (prolong)
+      const I3 c = coords(keys[i]);
+      T value[4] = {T(0), T(0), T(0), T(0)};
+      const int64_t own = find_key(own_keys(d), own_count(d), keys[i]);
+      if (own >= 0)
+        for (int a = 0; a < 4; ++a) value[a] = own_vals(d)[4 * own + a];
+      if (d > 0) {
+        for (int k = 0; k < 8; ++k) {
+          const I3 p = {{c[0] / 2 + (k >> 2), c[1] / 2 + ((k >> 1) & 1), c[2] / 2 + (k & 1)}};
+          T weight = T(1);
+          for (int a = 0; a < 3; ++a) weight = weight * (T(1) - std::fabs(T(c[a]) * T(0.5) - T(p[a])));
+          if (weight > T(0)) {
+            const int64_t j = find_key(previous_keys, basis_key(d - 1, p[0], p[1], p[2]));
+            if (j >= 0)
+              for (int a = 0; a < 4; ++a) value[a] = value[a] + previous[4 * j + a] * weight;
+          }
+        }
+      }
+      for (int a = 0; a < 4; ++a) cumulative[4 * i + a] = value[a];
+    }
+    if (d + 1 >= base_depth) {  // :240-258 (from_coarse)
+      const std::vector<int64_t>& fine = s.basis[d + 1];
+      for (size_t i = 0; i < fine.size(); ++i) {
+        const I3 c = coords(fine[i]);
+        T value = T(0);
+        for (int k = 0; k < 64; ++k) {
+          const I3 n = {{c[0] / 2 + k / 16 - 2, c[1] / 2 + (k / 4) % 4 - 2, c[2] / 2 + k % 4 - 2}};
+          value = value + ordered_contribution<T>(d + 1, c, d, n, keys.data(), (int64_t)keys.size(), cumulative.data(), 4);
+        }
+        loads[d + 1][i] = loads[d + 1][i] + value;
+      }
+    }
+    previous_keys = keys;
+    previous = std::move(cumulative);
+  }
+  std::vector<std::vector<double>> out(max_depth + 1);
+  for (int d = base_depth; d <= max_depth; ++d) out[d].assign(loads[d].begin(), loads[d].end());
+  return out;
+}
+
+// src/warp_psr/normal_loads.py:66-190 (stencil_loads=True path); float32 runs
+// with base depth 0 take the ordered path (normal_loads.py:84-92)
 template <class T> std::vector<std::vector<double>> normal_loads(const State<T>& s, int max_depth, int base_depth) {
+  if (std::is_same<T, float>::value && base_depth == 0) return ordered_normal_loads(s, max_depth, base_depth);
   std::vector<std::vector<double>> same(max_depth + 1), from_child(max_depth + 1), from_parent(max_depth + 1);
   std::vector<bool> has_parent(max_depth + 1, false);
   std::vector<int64_t> previous_keys;
@@ -1597,6 +1748,11 @@ template <class T> bool export_array(State<T>& s, const std::string& name, int l
   if (name == "tree_keys" && lvl_ok(s.tree_levels.size())) { put(out, s.tree_levels[level].data(), s.tree_levels[level].size()); return true; }
   if (name == "basis_keys" && lvl_ok(s.basis.size())) { put(out, s.basis[level].data(), s.basis[level].size()); return true; }
   if (name == "normal_load" && lvl_ok(s.normal_loads.size())) { put(out, s.normal_loads[level].data(), s.normal_loads[level].size()); return true; }
+  if (name == "ordered_normal_load" && lvl_ok(s.basis.size())) {  // the ordered path at any precision, base depth 0
+    std::vector<std::vector<double>> loads = ordered_normal_loads(s, (int)s.basis.size() - 1, 0);
+    put(out, loads[level].data(), loads[level].size());
+    return true;
+  }
   if (name == "point_load" && lvl_ok(s.point_loads.size())) { put(out, s.point_loads[level].data(), s.point_loads[level].size()); return true; }
   if (name == "rhs" && lvl_ok(s.rhs.size())) { put(out, s.rhs[level].data(), s.rhs[level].size()); return true; }
   if (name == "corrections" && lvl_ok(s.corrections.size())) { put(out, s.corrections[level].data(), s.corrections[level].size()); return true; }

@@ -102,6 +102,7 @@ FIXTURES = [
     ("disconnected", "disconnected", 1024, "float", dict(depth=5, full_depth=2, point_weight=2.0)),
     ("base-zero-psr-double", "sphere", 512, "double", dict(depth=4, full_depth=2, base_depth=0, point_weight=0.0)),
     ("root-solve-float", "sphere", 512, "float", dict(depth=4, full_depth=0, base_depth=0, point_weight=2.0)),
+    ("torus-base-zero-float", "torus", 8192, "float", dict(depth=6, full_depth=3, base_depth=0, point_weight=2.0)),
     ("exact", "sphere", 512, "float", dict(depth=4, full_depth=2, point_weight=2.0, exact_interpolation=True)),
     ("cycles", "sphere", 512, "double", dict(depth=4, full_depth=2, point_weight=2.0, base_v_cycles=2, iters=3)),
     ("torus-mixed", "torus", 8192, "double", dict(depth=6, full_depth=3, point_weight=2.0, mixed=True)),

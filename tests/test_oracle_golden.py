@@ -284,6 +284,32 @@ def test_relax_scheduled_equals_full_row_scans(dt):
     assert np.array_equal(x, want)
 
 
+def test_ordered_normal_loads_match_fem_loads():
+    """tests/test_ordered_normal_loads.py:46-106 restated: in float64 the
+    ordered path (ordered_normal_loads.py:263-337, taken by float32 runs with
+    base depth 0) gives the stencil path's loads to 1e-10 of their scale, on
+    every level."""
+    p, n = synthetic.fixture_input("sphere", 512, synthetic.SEED, "double")
+    ref = oracle.reconstruct(p, n, depth=4, full_depth=0, base_depth=0)
+    for d in range(5):
+        expected = ref.array("normal_load", d)
+        actual = ref.array("ordered_normal_load", d)
+        scale = np.max(np.abs(expected)) or 1.0
+        np.testing.assert_allclose(actual / scale, expected / scale, atol=1e-10, rtol=1e-9)
+    assert np.any(ref.array("normal_load", 4) != 0)
+
+
+def test_float_base_zero_takes_the_ordered_path():
+    """normal_loads.py:84-92: float32 runs with base depth 0 use the ordered
+    float32 loads (rounded in float32, widened to float64)."""
+    p, n = synthetic.fixture_input("sphere", 512, synthetic.SEED, "float")
+    ref = oracle.reconstruct(p, n, depth=4, full_depth=0, base_depth=0)
+    for d in range(5):
+        load = ref.array("normal_load", d)
+        assert np.array_equal(load, ref.array("ordered_normal_load", d))
+        assert np.array_equal(load, load.astype(np.float32).astype(np.float64))
+
+
 def test_mixed_precision_error_bound():
     """Float32 finer-level operators keep the float64 octree exactly and the
     field within 1e-6 of the float64 run (inputs and interface queries)."""

@@ -1722,9 +1722,244 @@ static void normal_stencil_load(const LevelState<T>& testL, int test_depth, cons
   KERNEL_CHECK();
 }
 
-// normal_loads.py:66-190 (stencil path)
+// ---------------------------------------------------------------- ordered normal loads
+// ordered_normal_loads.py:263-337: float32 runs with base depth 0 sum their
+// normal loads in T in a pinned serial order (normal_loads.py:84-92).  One
+// thread per row, each following the reference's loop order.
+
+// ordered_normal_loads.py:86-94 (normal_stencil) and :137-152 (contribution)
+template <class T>
+__device__ T ordered_contribution(int td, int tx, int ty, int tz, int nd, int nx, int ny, int nz, const int64_t* __restrict__ keys,
+                                  int64_t count, const T* __restrict__ vals) {
+  T dot = T(0);
+  const int lim = 1 << nd;
+  if (imin(imin(nx, ny), nz) >= 0 && imax(imax(nx, ny), nz) < lim) {
+    const int64_t j = find_key(keys, count, basis_key(nd, nx, ny, nz));
+    if (j >= 0) {
+      const D2 x = normal_integral(td, tx, nd, nx), y = normal_integral(td, ty, nd, ny), z = normal_integral(td, tz, nd, nz);
+      const double st[3] = {x.v1 * y.v0 * z.v0, x.v0 * y.v1 * z.v0, x.v0 * y.v0 * z.v1};
+#pragma unroll
+      for (int a = 0; a < 3; ++a) dot = dot + vals[3 * j + a] * T(st[a]);
+    }
+  }
+  return dot;
+}
+
+// :156-171 (same): finer + the 64 same-level records, x-major offsets -2..1
+template <class T>
+__global__ void k_ol_same(const int64_t* __restrict__ keys, int R, const int64_t* __restrict__ nkeys, int64_t ncount,
+                          const T* __restrict__ nvals, const T* __restrict__ finer, int depth, T* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= R) return;
+  const int64_t key = keys[i];
+  const int cx = key_x(key), cy = key_y(key), cz = key_z(key);
+  T value = finer[i];
+  for (int k = 0; k < 64; ++k)
+    value = value + ordered_contribution<T>(depth, cx, cy, cz, depth, cx + k / 16 - 2, cy + (k / 4) % 4 - 2, cz + k % 4 - 2, nkeys,
+                                            ncount, nvals);
+  out[i] = value;
+}
+
+// :97-103 (morton_xy)
+__device__ __forceinline__ uint64_t morton_xy(int x, int y) {
+  uint64_t key = 0;
+  for (int bit = 0; bit < 15; ++bit) {
+    key |= (uint64_t)((x >> bit) & 1) << (2 * bit);
+    key |= (uint64_t)((y >> bit) & 1) << (2 * bit + 1);
+  }
+  return key;
+}
+
+// :105-122 (parent_order, segmented sort) and :173-188 (parent): the 216
+// child-level records of a coarse row, each z layer of 36 in (morton_xy,
+// offset index) order; offsets with a negative coordinate sort last
+template <class T>
+__global__ void k_ol_parent(const int64_t* __restrict__ keys, int C, const int64_t* __restrict__ nkeys, int64_t ncount,
+                            const T* __restrict__ nvals, int depth, T* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= C) return;
+  const int64_t key = keys[i];
+  const int cx = key_x(key), cy = key_y(key), cz = key_z(key);
+  T value = T(0);
+  for (int zl = 0; zl < 6; ++zl) {
+    uint64_t order[36];
+    for (int m = 0; m < 36; ++m) {
+      const int k = 36 * zl + m;
+      const int x = 2 * cx + k % 6 - 3, y = 2 * cy + (k / 6) % 6 - 3, z = 2 * cz + k / 36 - 3;
+      const uint64_t o = imin(imin(x, y), z) >= 0 ? morton_xy(x, y) : (uint64_t)2147483647;
+      order[m] = (o << 8) | (uint64_t)m;
+    }
+    for (int a = 1; a < 36; ++a) {  // insertion sort: stable by construction of the keys
+      const uint64_t v = order[a];
+      int b = a - 1;
+      while (b >= 0 && order[b] > v) {
+        order[b + 1] = order[b];
+        --b;
+      }
+      order[b + 1] = v;
+    }
+    for (int m = 0; m < 36; ++m) {
+      const int k = 36 * zl + (int)(order[m] & 255);
+      value = value + ordered_contribution<T>(depth, cx, cy, cz, depth + 1, 2 * cx + k % 6 - 3, 2 * cy + (k / 6) % 6 - 3,
+                                              2 * cz + k / 36 - 3, nkeys, ncount, nvals);
+    }
+  }
+  out[i] = value;
+}
+
+// :190-212 (restrict): out += the 27 finer values with weights 1/2 per
+// nonzero offset axis
+template <class T>
+__global__ void k_ol_restrict(const int64_t* __restrict__ fine_keys, int64_t F, const T* __restrict__ fine,
+                              const int64_t* __restrict__ coarse, int C, int depth, T* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= C) return;
+  const int64_t key = coarse[i];
+  const int cx = key_x(key), cy = key_y(key), cz = key_z(key);
+  T value = out[i];
+  for (int k = 0; k < 27; ++k) {
+    const int ox = k / 9 - 1, oy = (k / 3) % 3 - 1, oz = k % 3 - 1;
+    const int nx = 2 * cx + ox, ny = 2 * cy + oy, nz = 2 * cz + oz;
+    if (imin(imin(nx, ny), nz) >= 0 && imax(imax(nx, ny), nz) <= (1 << depth)) {
+      const int64_t j = find_key(fine_keys, F, basis_key(depth, nx, ny, nz));
+      if (j >= 0) {
+        T weight = T(1);
+        if (ox != 0) weight = weight * T(0.5);
+        if (oy != 0) weight = weight * T(0.5);
+        if (oz != 0) weight = weight * T(0.5);
+        value = value + fine[j] * weight;
+      }
+    }
+  }
+  out[i] = value;
+}
+
+// :214-238 (prolong): own records plus the 8 coarser cumulative values
+// (corner bits x, y, z = 4, 2, 1)
+template <class T>
+__global__ void k_ol_prolong(const int64_t* __restrict__ keys, int R, const int64_t* __restrict__ own_keys, int64_t own_count,
+                             const T* __restrict__ own, const int64_t* __restrict__ prev_keys, int64_t prev_count,
+                             const T* __restrict__ prev, int depth, T* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= R) return;
+  const int64_t key = keys[i];
+  const int c[3] = {key_x(key), key_y(key), key_z(key)};
+  T v[3] = {T(0), T(0), T(0)};
+  const int64_t j0 = find_key(own_keys, own_count, key);
+  if (j0 >= 0)
+    for (int a = 0; a < 3; ++a) v[a] = own[3 * j0 + a];
+  if (depth > 0) {
+    for (int k = 0; k < 8; ++k) {
+      const int p[3] = {c[0] / 2 + (k >> 2), c[1] / 2 + ((k >> 1) & 1), c[2] / 2 + (k & 1)};
+      T weight = T(1);
+      for (int a = 0; a < 3; ++a) weight = weight * (T(1) - fabs(T(c[a]) * T(0.5) - T(p[a])));
+      if (weight > T(0)) {
+        const int64_t j = find_key(prev_keys, prev_count, basis_key(depth - 1, p[0], p[1], p[2]));
+        if (j >= 0)
+          for (int a = 0; a < 3; ++a) v[a] = v[a] + prev[3 * j + a] * weight;
+      }
+    }
+  }
+  for (int a = 0; a < 3; ++a) out[3 * (int64_t)i + a] = v[a];
+}
+
+// :240-258 (from_coarse): loads += the 64 coarser cumulative values around
+// the parent, x-major offsets -2..1
+template <class T>
+__global__ void k_ol_from_coarse(const int64_t* __restrict__ keys, int R, const int64_t* __restrict__ ckeys, int64_t ccount,
+                                 const T* __restrict__ cum, int depth, T* __restrict__ loads) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= R) return;
+  const int64_t key = keys[i];
+  const int cx = key_x(key), cy = key_y(key), cz = key_z(key);
+  T value = T(0);
+  for (int k = 0; k < 64; ++k)
+    value = value + ordered_contribution<T>(depth, cx, cy, cz, depth - 1, cx / 2 + k / 16 - 2, cy / 2 + (k / 4) % 4 - 2,
+                                            cz / 2 + k % 4 - 2, ckeys, ccount, cum);
+  loads[i] = loads[i] + value;
+}
+
+template <class T>
+__global__ void k_widen(const T* __restrict__ in, double* __restrict__ out, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = double(in[i]);
+}
+
+template <class T> static void ordered_normal_loads(Result<T>& r, hipStream_t s) {
+  const int D = r.prm.depth;
+  auto own_keys = [&](int d) { return (const int64_t*)r.normal_keys.data() + r.normal_offsets[d]; };
+  auto own_count = [&](int d) { return (int64_t)(r.normal_offsets[d + 1] - r.normal_offsets[d]); };
+  auto own_vals = [&](int d) { return (const T*)r.normal_vals.data() + 3 * (int64_t)r.normal_offsets[d]; };
+  std::vector<DArray<T>> loads(D + 1);
+  DArray<T> finer(r.levels[D].basis_count, s);
+  finer.zero();
+  for (int d = D; d >= 0; --d) {
+    LevelState<T>& L = r.levels[d];
+    const int R = L.basis_count;
+    loads[d].reset(R, s);
+    if (R) {
+      hipLaunchKernelGGL(k_ol_same<T>, dim3(blocks_for(R, 128)), dim3(128), 0, s, (const int64_t*)L.basis_keys.data(), R,
+                         own_keys(d), own_count(d), own_vals(d), (const T*)finer.data(), d, loads[d].data());
+      KERNEL_CHECK();
+    }
+    if (d > 0) {
+      LevelState<T>& Cl = r.levels[d - 1];
+      const int C = Cl.basis_count;
+      DArray<T> next(C, s);
+      next.zero();
+      if (C && own_count(d)) {
+        hipLaunchKernelGGL(k_ol_parent<T>, dim3(blocks_for(C, 64)), dim3(64), 0, s, (const int64_t*)Cl.basis_keys.data(), C,
+                           own_keys(d), own_count(d), own_vals(d), d - 1, next.data());
+        KERNEL_CHECK();
+      }
+      if (C && d < D) {
+        hipLaunchKernelGGL(k_ol_restrict<T>, dim3(blocks_for(C, 128)), dim3(128), 0, s, (const int64_t*)L.basis_keys.data(),
+                           (int64_t)R, (const T*)finer.data(), (const int64_t*)Cl.basis_keys.data(), C, d, next.data());
+        KERNEL_CHECK();
+      }
+      finer = std::move(next);
+    }
+  }
+  DArray<T> previous;
+  for (int d = 0; d < D; ++d) {
+    LevelState<T>& L = r.levels[d];
+    const int R = L.basis_count;
+    DArray<T> cumulative(3 * (size_t)R, s);
+    if (R) {
+      const LevelState<T>* P = d > 0 ? &r.levels[d - 1] : nullptr;
+      hipLaunchKernelGGL(k_ol_prolong<T>, dim3(blocks_for(R, 128)), dim3(128), 0, s, (const int64_t*)L.basis_keys.data(), R,
+                         own_keys(d), own_count(d), own_vals(d), P ? (const int64_t*)P->basis_keys.data() : (const int64_t*)nullptr,
+                         (int64_t)(P ? P->basis_count : 0), (const T*)previous.data(), d, cumulative.data());
+      KERNEL_CHECK();
+    }
+    const int F = r.levels[d + 1].basis_count;
+    if (F) {
+      hipLaunchKernelGGL(k_ol_from_coarse<T>, dim3(blocks_for(F, 128)), dim3(128), 0, s,
+                         (const int64_t*)r.levels[d + 1].basis_keys.data(), F, (const int64_t*)L.basis_keys.data(), (int64_t)R,
+                         (const T*)cumulative.data(), d + 1, loads[d + 1].data());
+      KERNEL_CHECK();
+    }
+    previous = std::move(cumulative);
+  }
+  for (int d = 0; d <= D; ++d) {
+    LevelState<T>& L = r.levels[d];
+    L.normal_load.reset(L.basis_count, s);
+    if (L.basis_count) {
+      hipLaunchKernelGGL(k_widen<T>, dim3(blocks_for(L.basis_count, 256)), dim3(256), 0, s, (const T*)loads[d].data(),
+                         L.normal_load.data(), (int64_t)L.basis_count);
+      KERNEL_CHECK();
+    }
+  }
+}
+
+// normal_loads.py:66-190 (stencil path); float32 runs with base depth 0 take
+// the ordered path (normal_loads.py:84-92)
 template <class T> static void normal_loads(Result<T>& r, std::vector<DArray<int>>& restr, hipStream_t s) {
   const int D = r.prm.depth, base = r.prm.base_depth;
+  if (std::is_same<T, float>::value && base == 0) {
+    ordered_normal_loads(r, s);
+    return;
+  }
   std::vector<DArray<double>> same(D + 1), from_child(D + 1), from_parent(D + 1);
   DArray<T> previous;
   bool has_cumulative = false;
