@@ -203,6 +203,24 @@ int psr_mesh_vertex_vjp(PsrMesh* mesh, const float* corner_values, const float* 
 int psr_mesh_case_table(int8_t* counts, int8_t* edges);
 void psr_mesh_free(PsrMesh* mesh);
 
+/* Tree metadata (src/warp_psr/tree_metadata.py:95-153, build_tree_metadata):
+ * every retained brood of every level (the active support nodes and the
+ * broods of the finer level's parents, the samples, the density splat
+ * records and the normal records), in global lattice coordinates (local +
+ * 2^(d+1), tree-key encoded, sorted), after two padding levels (local depths
+ * -2 and -1).  Per node: flags (1|4 SPACE and Dirichlet validity, 2 Neumann
+ * validity, 64 ghost children), ghost mark, parent index in the previous
+ * level and 8 child indices in the next level (x bit first; -1 if absent).
+ * psr_tree_metadata_copy copies one level into caller device buffers (any may
+ * be NULL): keys[n], flags[n], ghost[n], parents[n], children[8 n]. */
+typedef struct PsrTreeMetadata PsrTreeMetadata;
+int psr_tree_metadata(PsrResult* result, void* stream, PsrTreeMetadata** out);
+int psr_tree_metadata_levels(const PsrTreeMetadata* meta, int32_t* count);
+int psr_tree_metadata_level(const PsrTreeMetadata* meta, int32_t index, int32_t* local_depth, int64_t* count);
+int psr_tree_metadata_copy(const PsrTreeMetadata* meta, int32_t index, int64_t* keys, uint8_t* flags, uint8_t* ghost,
+                           int32_t* parents, int32_t* children, void* stream);
+void psr_tree_metadata_free(PsrTreeMetadata* meta);
+
 /* Evaluates any PsrField view (psr_result_field) at m device queries with the
  * header-only psr_evaluate_field of warp_psr_amd_field.h -- the kernel-side
  * evaluate_field(field, point) of src/warp_psr/device_field.py:91-112 --

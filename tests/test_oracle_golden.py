@@ -26,8 +26,8 @@ from warp_psr_amd import synthetic
 REPORTS = json.loads((Path(__file__).parent / "golden" / "reference_reports.json").read_text())
 FIXTURES = [f for f in REPORTS["fixtures"] if f["params"]["shape"] != "external" and not f["params"].get("variant")]
 # base_depth 0 float32 uses the reference's ordered serial normal loads
-# (ordered_normal_loads.py), which the oracle does not restate; its tree is
-# still checked, its isovalue against PoissonRecon's with the field tolerance.
+# (ordered_normal_loads.py:263-337); the isovalue is checked against both
+# warp-psr's and PoissonRecon's with the float32 field tolerance.
 ORDERED_ROOT = {f["name"] for f in FIXTURES if f["params"]["base_depth"] == 0 and f["params"]["precision"] == "float"}
 
 
@@ -60,8 +60,9 @@ def test_fixture_tree_iso_transform(f):
     np.testing.assert_array_equal(ctm, np.asarray(f["cube_to_model"], dtype=ctm.dtype))
     if f["params"]["precision"] == "float":
         # float32 budget of tools/check_reconstruction.py:23 (atol 1e-5, rtol 1e-4)
-        target = f["reference_iso_value"] if f["name"] in ORDERED_ROOT else f["iso_value"]
-        assert abs(res.iso_value - target) <= 1e-5 + 1e-4 * abs(target)
+        targets = [f["iso_value"]] + ([f["reference_iso_value"]] if f["name"] in ORDERED_ROOT else [])
+        for target in targets:
+            assert abs(res.iso_value - target) <= 1e-5 + 1e-4 * abs(target)
     else:
         # float64: equal to warp-psr to the last few bits (base-level FEM order differs)
         assert abs(res.iso_value - f["iso_value"]) <= 1e-13

@@ -6,6 +6,7 @@
 #include "../../include/warp_psr_amd_field.h"
 #include "psr_result.h"
 #include "psr_mesh.h"
+#include "psr_meta.h"
 
 #include <cmath>
 #include <cstring>
@@ -778,6 +779,67 @@ void psr_mesh_free(PsrMesh* m) {
   mc_free(m->state);
   delete m;
 }
+
+// ---------------------------------------------------------------- tree metadata (tree_metadata.py)
+struct PsrTreeMetadata {
+  TreeMetadata meta;
+};
+
+int psr_tree_metadata(PsrResult* result, void* stream, PsrTreeMetadata** out) {
+  try {
+    if (!result || !out) throw PsrError(2, "null argument");
+    auto* m = new PsrTreeMetadata();
+    try {
+      hipStream_t s = (hipStream_t)stream;
+      if (result->dtype == 0) build_tree_metadata(*(Result<float>*)result->impl, m->meta, s);
+      else build_tree_metadata(*(Result<double>*)result->impl, m->meta, s);
+    } catch (...) {
+      delete m;
+      throw;
+    }
+    *out = m;
+    return 0;
+  } catch (const PsrError& e) {
+    return fail(e);
+  }
+}
+
+int psr_tree_metadata_levels(const PsrTreeMetadata* m, int32_t* count) {
+  if (!m || !count) return fail(PsrError(2, "null argument"));
+  *count = (int32_t)m->meta.levels.size();
+  return 0;
+}
+
+int psr_tree_metadata_level(const PsrTreeMetadata* m, int32_t index, int32_t* local_depth, int64_t* count) {
+  if (!m || !local_depth || !count) return fail(PsrError(2, "null argument"));
+  if (index < 0 || index >= (int32_t)m->meta.levels.size()) return fail(PsrError(2, "level index out of range"));
+  *local_depth = m->meta.levels[index].local_depth;
+  *count = (int64_t)m->meta.levels[index].keys.size();
+  return 0;
+}
+
+int psr_tree_metadata_copy(const PsrTreeMetadata* m, int32_t index, int64_t* keys, uint8_t* flags, uint8_t* ghost,
+                           int32_t* parents, int32_t* children, void* stream) {
+  try {
+    if (!m) throw PsrError(2, "null argument");
+    if (index < 0 || index >= (int32_t)m->meta.levels.size()) throw PsrError(2, "level index out of range");
+    const TreeMetadata::Level& lv = m->meta.levels[index];
+    const size_t n = lv.keys.size();
+    hipStream_t s = (hipStream_t)stream;
+    if (n) {
+      if (keys) HIP_CHECK(hipMemcpyAsync(keys, lv.keys.data(), sizeof(int64_t) * n, hipMemcpyDeviceToDevice, s));
+      if (flags) HIP_CHECK(hipMemcpyAsync(flags, lv.flags.data(), n, hipMemcpyDeviceToDevice, s));
+      if (ghost) HIP_CHECK(hipMemcpyAsync(ghost, lv.ghost.data(), n, hipMemcpyDeviceToDevice, s));
+      if (parents) HIP_CHECK(hipMemcpyAsync(parents, lv.parents.data(), sizeof(int) * n, hipMemcpyDeviceToDevice, s));
+      if (children) HIP_CHECK(hipMemcpyAsync(children, lv.children.data(), sizeof(int) * 8 * n, hipMemcpyDeviceToDevice, s));
+    }
+    return 0;
+  } catch (const PsrError& e) {
+    return fail(e);
+  }
+}
+
+void psr_tree_metadata_free(PsrTreeMetadata* m) { delete m; }
 
 const char* psr_last_error(void) { return g_error.c_str(); }
 

@@ -14,7 +14,8 @@ EXPORTED = (
     "psr_release_cache", "psr_field_evaluate", "psr_field_vjp", "psr_backward", "psr_field_vjp_inputs", "psr_backward_inputs",
     "psr_reconstruct_batch", "psr_mesh_begin", "psr_mesh_slab_nodes", "psr_mesh_slab_points", "psr_mesh_slab_values",
     "psr_mesh_select", "psr_mesh_corner_points", "psr_mesh_triangulate", "psr_mesh_copy", "psr_mesh_vertex_vjp",
-    "psr_mesh_case_table", "psr_mesh_free",
+    "psr_mesh_case_table", "psr_mesh_free", "psr_tree_metadata", "psr_tree_metadata_levels", "psr_tree_metadata_level",
+    "psr_tree_metadata_copy", "psr_tree_metadata_free",
 )
 
 
@@ -102,6 +103,14 @@ def load():
                  "psr_mesh_corner_points", "psr_mesh_triangulate", "psr_mesh_copy", "psr_mesh_vertex_vjp",
                  "psr_mesh_case_table"):
         getattr(lib, name).restype = c_int
+    lib.psr_tree_metadata.argtypes = [vp, vp, ctypes.POINTER(vp)]
+    lib.psr_tree_metadata_levels.argtypes = [vp, i32p]
+    lib.psr_tree_metadata_level.argtypes = [vp, ctypes.c_int32, i32p, i64p]
+    lib.psr_tree_metadata_copy.argtypes = [vp, ctypes.c_int32, vp, vp, vp, vp, vp, vp]
+    for name in ("psr_tree_metadata", "psr_tree_metadata_levels", "psr_tree_metadata_level", "psr_tree_metadata_copy"):
+        getattr(lib, name).restype = c_int
+    lib.psr_tree_metadata_free.argtypes = [vp]
+    lib.psr_tree_metadata_free.restype = None
     lib.psr_mesh_free.argtypes = [vp]
     lib.psr_mesh_free.restype = None
     lib.psr_last_error.argtypes = []
