@@ -1952,98 +1952,164 @@ template <class T> static void ordered_normal_loads(Result<T>& r, hipStream_t s)
   }
 }
 
-// normal_loads.py:66-190 (stencil path); float32 runs with base depth 0 take
-// the ordered path (normal_loads.py:84-92)
-template <class T> static void normal_loads(Result<T>& r, std::vector<DArray<int>>& restr, hipStream_t s) {
+// normal_loads.py:66-190 (stencil path), split by what the base solve needs.
+// The base level's load takes the from-child loads of every finer level,
+// restricted down level by level: that part runs first, on the solve stream
+// (normal_loads_base).  Each finer level's same-level and from-parent loads,
+// and the prolonged cumulative normal field behind the latter, feed only that
+// level's right-hand side: normal_loads_level computes them level by level,
+// on the assembly stream ahead of the level's operators.  Every array is
+// summed in the order of a single pass over the levels.  Float32 runs with
+// base depth 0 take the ordered path (normal_loads.py:84-92) in one go.
+template <class T> struct NormalLoadPlan {
+  bool active = false;  // false: every level's load is already set
+  int base = 0, depth = 0;
+  std::vector<ColumnIndex> records;         // per level, the normal records' column index
+  std::vector<DArray<int>> rdense;          // and their dense (x, y, z)-major index
+  std::vector<DenseIndex> rdn;
+  std::vector<DArray<double>> finer;        // loads restricted to level d from the finer levels
+  std::vector<DArray<double>> from_parent;  // from-parent loads, filled by the chain
+  DArray<T> previous;                       // the chain's last cumulative coefficients
+  int chain = 0;                            // next level of the chain
+  bool has_cumulative = false;
+  int next_level = 0;                       // next level whose load is still to be set
+  hipEvent_t ready = nullptr;               // solve stream, after normal_loads_base
+  NormalLoadPlan() = default;
+  NormalLoadPlan(const NormalLoadPlan&) = delete;
+  NormalLoadPlan& operator=(const NormalLoadPlan&) = delete;
+  ~NormalLoadPlan() {
+    if (ready) (void)hipEventDestroy(ready);
+  }
+};
+
+template <class T>
+static void normal_loads_base(Result<T>& r, std::vector<DArray<int>>& restr, NormalLoadPlan<T>& plan, hipStream_t s) {
   const int D = r.prm.depth, base = r.prm.base_depth;
+  plan.base = base;
+  plan.depth = D;
   if (std::is_same<T, float>::value && base == 0) {
     ordered_normal_loads(r, s);
+    plan.active = false;
     return;
   }
-  std::vector<DArray<double>> same(D + 1), from_child(D + 1), from_parent(D + 1);
-  DArray<T> previous;
-  bool has_cumulative = false;
+  plan.active = true;
+  plan.records.resize(D + 1);
+  plan.rdense.resize(D + 1);
+  plan.rdn.assign(D + 1, DenseIndex{});
+  plan.finer.resize(D + 1);
+  plan.from_parent.resize(D + 1);
+  std::vector<DArray<double>> from_child(D + 1);
+  DArray<double> same_base;
   for (int d = 0; d <= D; ++d) {
     const int64_t r0 = r.normal_offsets[d], r1 = r.normal_offsets[d + 1];
     const int64_t* nk = r.normal_keys.data() + r0;
     const T* nv = r.normal_vals.data() + 3 * r0;
-    has_cumulative = has_cumulative || (r1 > r0);
-    LevelState<T>& L = r.levels[d];
-    ColumnIndex records;
-    build_column_index(nk, r1 - r0, (1 << d) + 1, records, s);
+    build_column_index(nk, r1 - r0, (1 << d) + 1, plan.records[d], s);
     // dense (x, y, z)-major index of the level's records for the tests taken
     // in canonical order (consecutive z)
-    DArray<int> rdense;
-    DenseIndex rdn;
     const int64_t rdim = (1 << d) + 1;
     if (r1 > r0 && d >= 5 && rdim * rdim * rdim <= DENSE_INDEX_MAX_CELLS) {
-      rdense.reset((size_t)(rdim * rdim * rdim), s);
-      HIP_CHECK(hipMemsetAsync(rdense.data(), 0xff, sizeof(int) * (size_t)(rdim * rdim * rdim), s));
-      hipLaunchKernelGGL(k_dense_fill_xyz, dim3(blocks_for(r1 - r0, 256)), dim3(256), 0, s, nk, r1 - r0, (int)rdim, rdense.data());
+      plan.rdense[d].reset((size_t)(rdim * rdim * rdim), s);
+      HIP_CHECK(hipMemsetAsync(plan.rdense[d].data(), 0xff, sizeof(int) * (size_t)(rdim * rdim * rdim), s));
+      hipLaunchKernelGGL(k_dense_fill_xyz, dim3(blocks_for(r1 - r0, 256)), dim3(256), 0, s, nk, r1 - r0, (int)rdim,
+                         plan.rdense[d].data());
       KERNEL_CHECK();
-      rdn.idx = rdense.data();
-      rdn.dim = (int)rdim;
+      plan.rdn[d].idx = plan.rdense[d].data();
+      plan.rdn[d].dim = (int)rdim;
     }
-    if (d >= base) {
-      same[d].reset(L.basis_count, s);
-      normal_stencil_load(L, d, nk, r1 - r0, records.view(), rdn, nv, d, same[d].data(), s);
-      if (d > base) {
-        from_child[d - 1].reset(r.levels[d - 1].basis_count, s);
-        normal_stencil_load(r.levels[d - 1], d - 1, nk, r1 - r0, records.view(), rdn, nv, d, from_child[d - 1].data(), s);
-      }
+    if (d == base) {
+      same_base.reset(r.levels[d].basis_count, s);
+      normal_stencil_load(r.levels[d], d, nk, r1 - r0, plan.records[d].view(), plan.rdn[d], nv, d, same_base.data(), s);
     }
-    if (d < D) {
-      DArray<T> cumulative(3 * (size_t)L.basis_count, s);
-      if (L.basis_count) {
-        const LevelState<T>* P = d > 0 ? &r.levels[d - 1] : nullptr;
-        hipLaunchKernelGGL(k_normal_prolong<T>, dim3(blocks_for(L.basis_count, 256)), dim3(256), 0, s,
-                           (const int64_t*)L.basis_keys.data(), L.basis_count, nk, r1 - r0, records.view(), nv,
-                           P ? (const int64_t*)P->basis_keys.data() : (const int64_t*)nullptr,
-                           (int64_t)(P ? P->basis_count : 0), P ? P->columns.view() : ColumnTable{}, (const T*)previous.data(), d,
-                           cumulative.data());
-        KERNEL_CHECK();
-      }
-      if (d >= base && has_cumulative) {
-        from_parent[d + 1].reset(r.levels[d + 1].basis_count, s);
-        DenseIndex bdn;
-        if (L.dense_dim) {
-          bdn.idx = L.dense.data();
-          bdn.dim = L.dense_dim;
-          bdn.zmajor = 1;
-        }
-        normal_stencil_load(r.levels[d + 1], d + 1, (const int64_t*)L.basis_keys.data(), (int64_t)L.basis_count,
-                            L.columns.view(), bdn, (const T*)cumulative.data(), d, from_parent[d + 1].data(), s);
-      }
-      previous = std::move(cumulative);
+    if (d > base) {
+      from_child[d - 1].reset(r.levels[d - 1].basis_count, s);
+      normal_stencil_load(r.levels[d - 1], d - 1, nk, r1 - r0, plan.records[d].view(), plan.rdn[d], nv, d,
+                          from_child[d - 1].data(), s);
     }
   }
   DArray<double> finer(r.levels[D].basis_count, s);
   finer.zero();
-  for (int d = D; d >= base; --d) {
-    LevelState<T>& L = r.levels[d];
-    DArray<double> load = std::move(same[d]);
-    const int R = L.basis_count;
-    if (R) {
-      hipLaunchKernelGGL(k_add_f64, dim3(blocks_for(R, 256)), dim3(256), 0, s, load.data(), (const double*)finer.data(), R);
+  for (int d = D; d > base; --d) {
+    DArray<double> next = std::move(from_child[d - 1]);
+    const int C = r.levels[d - 1].basis_count;
+    if (C) {
+      hipLaunchKernelGGL(k_restrict_apply<double>, dim3(blocks_for(C, 256)), dim3(256), 0, s, (const int*)restr[d - 1].data(),
+                         C, (const double*)finer.data(), next.data(), 1);
       KERNEL_CHECK();
-      if (!from_parent[d].empty()) {
-        hipLaunchKernelGGL(k_add_f64, dim3(blocks_for(R, 256)), dim3(256), 0, s, load.data(),
-                           (const double*)from_parent[d].data(), R);
-        KERNEL_CHECK();
-      }
     }
-    L.normal_load = std::move(load);
-    if (d > base) {
-      DArray<double> next = std::move(from_child[d - 1]);
-      const int C = r.levels[d - 1].basis_count;
-      if (C) {
-        hipLaunchKernelGGL(k_restrict_apply<double>, dim3(blocks_for(C, 256)), dim3(256), 0, s, (const int*)restr[d - 1].data(),
-                           C, (const double*)finer.data(), next.data(), 1);
-        KERNEL_CHECK();
+    plan.finer[d] = std::move(finer);
+    finer = std::move(next);
+  }
+  LevelState<T>& B = r.levels[base];
+  if (B.basis_count) {  // no from-parent load reaches the base level
+    hipLaunchKernelGGL(k_add_f64, dim3(blocks_for(B.basis_count, 256)), dim3(256), 0, s, same_base.data(),
+                       (const double*)finer.data(), B.basis_count);
+    KERNEL_CHECK();
+  }
+  B.normal_load = std::move(same_base);
+  plan.finer[base] = std::move(finer);
+  plan.next_level = base + 1;
+  HIP_CHECK(hipEventCreateWithFlags(&plan.ready, hipEventDisableTiming));
+  HIP_CHECK(hipEventRecord(plan.ready, s));
+}
+
+// Level d's load (d = plan.next_level): the chain through level d - 1, then
+// same-level + restricted finer + from-parent, in that order.  Runs on s,
+// which must be ordered after plan.ready.
+template <class T> static void normal_loads_level(Result<T>& r, NormalLoadPlan<T>& plan, int d, hipStream_t s) {
+  if (!plan.active || d != plan.next_level || d > plan.depth) return;
+  while (plan.chain <= d - 1) {
+    const int c = plan.chain;
+    const int64_t r0 = r.normal_offsets[c], r1 = r.normal_offsets[c + 1];
+    const int64_t* nk = r.normal_keys.data() + r0;
+    const T* nv = r.normal_vals.data() + 3 * r0;
+    plan.has_cumulative = plan.has_cumulative || (r1 > r0);
+    LevelState<T>& L = r.levels[c];
+    DArray<T> cumulative(3 * (size_t)L.basis_count, s);
+    if (L.basis_count) {
+      const LevelState<T>* P = c > 0 ? &r.levels[c - 1] : nullptr;
+      hipLaunchKernelGGL(k_normal_prolong<T>, dim3(blocks_for(L.basis_count, 256)), dim3(256), 0, s,
+                         (const int64_t*)L.basis_keys.data(), L.basis_count, nk, r1 - r0, plan.records[c].view(), nv,
+                         P ? (const int64_t*)P->basis_keys.data() : (const int64_t*)nullptr, (int64_t)(P ? P->basis_count : 0),
+                         P ? P->columns.view() : ColumnTable{}, (const T*)plan.previous.data(), c, cumulative.data());
+      KERNEL_CHECK();
+    }
+    if (c >= plan.base && plan.has_cumulative) {
+      plan.from_parent[c + 1].reset(r.levels[c + 1].basis_count, s);
+      DenseIndex bdn;
+      if (L.dense_dim) {
+        bdn.idx = L.dense.data();
+        bdn.dim = L.dense_dim;
+        bdn.zmajor = 1;
       }
-      finer = std::move(next);
+      normal_stencil_load(r.levels[c + 1], c + 1, (const int64_t*)L.basis_keys.data(), (int64_t)L.basis_count,
+                          L.columns.view(), bdn, (const T*)cumulative.data(), c, plan.from_parent[c + 1].data(), s);
+    }
+    plan.previous = std::move(cumulative);
+    ++plan.chain;
+  }
+  const int64_t r0 = r.normal_offsets[d], r1 = r.normal_offsets[d + 1];
+  LevelState<T>& L = r.levels[d];
+  const int R = L.basis_count;
+  DArray<double> load(R, s);
+  normal_stencil_load(L, d, r.normal_keys.data() + r0, r1 - r0, plan.records[d].view(), plan.rdn[d],
+                      r.normal_vals.data() + 3 * r0, d, load.data(), s);
+  if (R) {
+    hipLaunchKernelGGL(k_add_f64, dim3(blocks_for(R, 256)), dim3(256), 0, s, load.data(), (const double*)plan.finer[d].data(), R);
+    KERNEL_CHECK();
+    if (!plan.from_parent[d].empty()) {
+      hipLaunchKernelGGL(k_add_f64, dim3(blocks_for(R, 256)), dim3(256), 0, s, load.data(),
+                         (const double*)plan.from_parent[d].data(), R);
+      KERNEL_CHECK();
     }
   }
+  L.normal_load = std::move(load);
+  ++plan.next_level;
+}
+
+// Every level's load not set yet, on s (ordered after plan.ready).
+template <class T> static void normal_loads_rest(Result<T>& r, NormalLoadPlan<T>& plan, hipStream_t s) {
+  while (plan.active && plan.next_level <= plan.depth) normal_loads_level(r, plan, plan.next_level, s);
 }
 
 // solver.py:241-317 (regular_multigrid)
@@ -2293,7 +2359,15 @@ template <class T, class V> class FinerAssembly {
     gate_open_ = true;
     cv_.notify_all();
   }
-  // Level d's operators, once the solve stream is ordered after them.
+  // The finer levels' loads: each is set on the assembly stream before its
+  // level's event, after the solve stream's plan.ready.
+  void set_loads(NormalLoadPlan<T>* plan) {
+    if (!thread_.joinable()) return;
+    std::lock_guard<std::mutex> g(mu_);
+    loads_ = plan;
+    cv_.notify_all();
+  }
+  // Level d's operators and load, once the solve stream is ordered after them.
   FinerLevel<T, V>& wait(int d) {
     const int k = d - first_;
     std::unique_lock<std::mutex> g(mu_);
@@ -2327,6 +2401,18 @@ template <class T, class V> class FinerAssembly {
         if (!L.basis_count) break;
         prepare_finer<T, V>(r_, d, finer_operator<V>(L, false), finer_operator<V>(L, true), screened,
                             levels_[d - first_], as_);
+        NormalLoadPlan<T>* plan = nullptr;
+        {
+          std::unique_lock<std::mutex> g(mu_);
+          cv_.wait(g, [&] { return loads_ != nullptr || cancel_; });
+          if (cancel_) break;
+          plan = loads_;
+        }
+        if (!loads_waited_) {
+          if (plan->ready) HIP_CHECK(hipStreamWaitEvent(as_, plan->ready, 0));
+          loads_waited_ = true;
+        }
+        normal_loads_level(r_, *plan, d, as_);
         HIP_CHECK(hipEventRecord(events_[d - first_], as_));
         std::lock_guard<std::mutex> g(mu_);
         ready_ = d - first_ + 1;
@@ -2352,16 +2438,16 @@ template <class T, class V> class FinerAssembly {
   std::condition_variable cv_;
   int ready_ = 0;
   bool cancel_ = false, done_ = false, gate_open_ = false;
+  NormalLoadPlan<T>* loads_ = nullptr;
+  bool loads_waited_ = false;  // assembly thread only
   std::exception_ptr error_;
 };
 
-// Normal loads, then the cascadic solve from the base level up
-// (solver.py:332-360); the finer levels' operators are assembled alongside.
+// The base solve, then each finer level once its operators and load are ready.
 template <class T, class V>
-static void solve_levels(Result<T>& r, std::vector<DArray<int>>& restr, bool screened, hipStream_t s) {
+static void solve_cascade(Result<T>& r, std::vector<DArray<int>>& restr, bool screened, FinerAssembly<T, V>& finer,
+                          hipStream_t s) {
   const int base = r.prm.base_depth;
-  FinerAssembly<T, V> finer(r, base + 1, r.prm.solve_depth, screened, s);
-  normal_loads(r, restr, s);
   DArray<T> cumulative;
   for (int d = base; d <= r.prm.solve_depth; ++d) {
     LevelState<T>& L = r.levels[d];
@@ -2389,6 +2475,22 @@ static void solve_levels(Result<T>& r, std::vector<DArray<int>>& restr, bool scr
     }
     L.solved = true;
   }
+}
+
+// Normal loads, then the cascadic solve from the base level up
+// (solver.py:332-360); the finer levels' operators and loads are assembled
+// alongside.
+template <class T, class V>
+static void solve_levels(Result<T>& r, std::vector<DArray<int>>& restr, bool screened, hipStream_t s) {
+  const int base = r.prm.base_depth;
+  NormalLoadPlan<T> plan;  // outlives the assembly, whose stream reads its arrays
+  {
+    FinerAssembly<T, V> finer(r, base + 1, r.prm.solve_depth, screened, s);
+    normal_loads_base(r, restr, plan, s);
+    finer.set_loads(&plan);
+    solve_cascade<T, V>(r, restr, screened, finer, s);
+  }  // the solve stream is ordered after the assembly stream here
+  normal_loads_rest(r, plan, s);
 }
 
 template <class T> void stage_solve(Result<T>& r) {
