@@ -150,3 +150,15 @@ def test_c_abi_batch_validates_before_device_work():
     assert "Require 0 <= base_depth" in lib.psr_last_error().decode()
     assert lib.psr_reconstruct_batch(pts, pts, counts, count, 0, params, 3, 4, None, out) == 2
     assert lib.psr_reconstruct_batch(pts, pts, counts, 0, 0, params, 1, 4, None, out) == 0
+
+
+def test_c_abi_tree_metadata_validates_arguments():
+    lib = _lib.load()
+    out = ctypes.c_void_p()
+    count, depth, n = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int64()
+    assert lib.psr_tree_metadata(None, None, ctypes.byref(out)) == 2
+    assert "null argument" in lib.psr_last_error().decode()
+    assert lib.psr_tree_metadata_levels(None, ctypes.byref(count)) == 2
+    assert lib.psr_tree_metadata_level(None, 0, ctypes.byref(depth), ctypes.byref(n)) == 2
+    assert lib.psr_tree_metadata_copy(None, 0, None, None, None, None, None, None) == 2
+    lib.psr_tree_metadata_free(None)
