@@ -1,11 +1,12 @@
 """Extract the parity anchors from the reference's published result reports.
 
-Reads docs/results/{validation,profiles,geometry-profiles,cancelled-normals}.json
-of the reference checkout (default /root/reference) and writes
-reference_reports.json beside this script: per fixture the generator
+Reads docs/results/{validation,profiles,geometry-profiles,cancelled-normals,
+tree-metadata}.json of the reference checkout (default /root/reference) and
+writes reference_reports.json beside this script: per fixture the generator
 parameters, the input SHA-256, per-level support/active key counts and SHA-256s,
 the isovalues (warp-psr's and PoissonRecon's), cube_to_model and the basis
-counts.  Only data is copied; run once, the output is committed.
+counts; for the topology fixtures also the complete tree's node and ghost
+counts per level.  Only data is copied; run once, the output is committed.
 """
 import json
 import sys
@@ -50,8 +51,14 @@ def main():
     for c in cancelled["profiles"]:
         cases.append({k: c.get(k) for k in ("case", "input_sha256", "parameters", "iso_value", "reference_iso_value",
                                             "normal_average", "tree")})
-    OUT.write_text(json.dumps({"fixtures": out, "cancelled_normals": cases}, indent=1) + "\n")
-    print(f"wrote {OUT} ({len(out)} fixtures, {len(cases)} cancelled-normal cases)")
+    topology = []
+    meta = json.loads((ROOT / "docs/results/tree-metadata.json").read_text())
+    for p in meta["profiles"]:
+        e = entry(p["fixture"], "tree-metadata.json", p["manifest"], p["reconstruction"])
+        e["complete_tree"] = p["reconstruction"]["complete_tree"]["levels"]
+        topology.append(e)
+    OUT.write_text(json.dumps({"fixtures": out, "cancelled_normals": cases, "complete_tree": topology}, indent=1) + "\n")
+    print(f"wrote {OUT} ({len(out)} fixtures, {len(cases)} cancelled-normal cases, {len(topology)} complete trees)")
 
 
 if __name__ == "__main__":

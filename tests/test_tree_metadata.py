@@ -1,15 +1,20 @@
 """Tree metadata (tree_metadata.py:95-153).
 
 CPU: the restatement in oracle/tree_metadata.py, fed the oracle's trees, is
-pinned by what the reference's metadata must satisfy (the reference checks it
-against PoissonRecon's instrumented topology, tests/test_tree_metadata.py:30-60,
-which needs the unavailable PoissonRecon build): every level is made of whole
-broods and contains the active support nodes; the non-ghost nodes of a depth
-are exactly its active nodes; parent and child links are mutually consistent;
-the padding root holds the eight level -1 nodes; flags follow their
+pinned by the reference's published complete-tree counts (nodes and ghost
+nodes per level of the four topology fixtures of docs/results/tree-metadata.json,
+where warp-psr matched PoissonRecon's instrumented topology exactly) and by what
+the metadata must satisfy (tests/test_tree_metadata.py:56-73): every level is
+made of whole broods and contains the active support nodes; the non-ghost nodes
+of a depth are exactly its active nodes; parent and child links are mutually
+consistent; the padding root holds the eight level -1 nodes; flags follow their
 definitions.  GPU: the library's metadata equals the restatement fed the same
 reconstruction's arrays, level by level.
 """
+import hashlib
+import json
+from pathlib import Path
+
 import numpy as np
 import pytest
 
@@ -100,6 +105,41 @@ def _check_invariants(levels, tree_keys):
             np.testing.assert_array_equal(flags & 7, 0)
 
 
+TOPOLOGY = json.loads((Path(__file__).parent / "golden" / "reference_reports.json").read_text())["complete_tree"]
+
+
+def _topology_run(f):
+    p = f["params"]
+    pts, nrm = synthetic.fixture_input(p["shape"], p["count"], p["seed"], p["precision"], p["input_scale"],
+                                       p["translation"])
+    assert hashlib.sha256(synthetic.fixture_input_text(pts, nrm)).hexdigest() == f["input_sha256"]
+    options = dict(depth=p["depth"], full_depth=p["full_depth"], base_depth=p["base_depth"],
+                   solve_depth=p["solve_depth"], kernel_depth=p["kernel_depth"], samples_per_node=p["samples_per_node"],
+                   scale=p["scale"], point_weight=p["weight"], exact_interpolation=p["exact"], iters=p["iters"],
+                   base_v_cycles=p["base_v_cycles"], cg_accuracy=p["cg_accuracy"])
+    return pts, nrm, options
+
+
+def _counts(levels):
+    return [{"local_depth": int(d), "nodes": len(keys), "ghosts": int(np.asarray(ghost).sum())}
+            for d, keys, _, ghost, _, _ in levels]
+
+
+@pytest.mark.parametrize("f", TOPOLOGY, ids=[f["name"] for f in TOPOLOGY])
+def test_restatement_matches_reference_complete_tree(f):
+    pts, nrm, options = _topology_run(f)
+    r = oracle.reconstruct(pts, nrm, **options)
+    depth = options["depth"]
+    ins = _inputs(r, depth)
+    levels = tm.build(depth, *ins)
+    assert _counts(levels) == f["complete_tree"]
+    for lv in f["tree"]:
+        d = lv["depth"]
+        assert hashlib.sha256(r.array("tree_keys", d).astype("<i8").tobytes()).hexdigest() == lv["support_sha256"]
+    _check_invariants(levels, ins[0])
+    _check_sources(levels, depth, *ins)
+
+
 @pytest.mark.parametrize("shape,count,precision,params", [
     ("sphere", 512, "float", dict(depth=4, full_depth=2)),
     ("torus", 8192, "double", dict(depth=5, full_depth=3, point_weight=0.0)),
@@ -144,3 +184,11 @@ def test_library_metadata_matches_restatement(cuda):
             np.testing.assert_array_equal(level.ghost.cpu().numpy(), ghost)
             np.testing.assert_array_equal(level.parents.cpu().numpy(), parents)
             np.testing.assert_array_equal(level.children.cpu().numpy(), children)
+    for f in TOPOLOGY:
+        print(f"tree metadata: {f['name']}", flush=True)
+        pts, nrm, options = _topology_run(f)
+        r = reconstruct(torch.from_numpy(pts).to(cuda), torch.from_numpy(nrm).to(cuda), **options)
+        meta = build_tree_metadata(r)
+        counts = [{"local_depth": lv.local_depth, "nodes": int(lv.keys.numel()), "ghosts": int(lv.ghost.sum().item())}
+                  for lv in meta.levels]
+        assert counts == f["complete_tree"], f["name"]
