@@ -6,7 +6,7 @@ writes reference_reports.json beside this script: per fixture the generator
 parameters, the input SHA-256, per-level support/active key counts and SHA-256s,
 the isovalues (warp-psr's and PoissonRecon's), cube_to_model and the basis
 counts; for the topology fixtures also the complete tree's node and ghost
-counts per level.  Only data is copied; run once, the output is committed.
+counts per level and the input status / aggregate counts.  Only data is copied; run once, the output is committed.
 """
 import json
 import sys
@@ -56,6 +56,7 @@ def main():
     for p in meta["profiles"]:
         e = entry(p["fixture"], "tree-metadata.json", p["manifest"], p["reconstruction"])
         e["complete_tree"] = p["reconstruction"]["complete_tree"]["levels"]
+        e["samples"] = p["reconstruction"].get("samples")
         topology.append(e)
     OUT.write_text(json.dumps({"fixtures": out, "cancelled_normals": cases, "complete_tree": topology}, indent=1) + "\n")
     print(f"wrote {OUT} ({len(out)} fixtures, {len(cases)} cancelled-normal cases, {len(topology)} complete trees)")

@@ -133,6 +133,13 @@ def test_restatement_matches_reference_complete_tree(f):
     ins = _inputs(r, depth)
     levels = tm.build(depth, *ins)
     assert _counts(levels) == f["complete_tree"]
+    # tools/check_samples.py's summary of the same run
+    status = r.array("input_status")
+    samples = f["samples"]
+    assert len(status) == samples["input_count"] and int((status == 0).sum()) == samples["accepted"]
+    assert len(r.array("sample_keys")) == samples["aggregates"]
+    codes, counts = np.unique(status[status != 0], return_counts=True)
+    assert {str(int(c)): int(k) for c, k in zip(codes, counts)} == samples["rejected_by_status"]
     for lv in f["tree"]:
         d = lv["depth"]
         assert hashlib.sha256(r.array("tree_keys", d).astype("<i8").tobytes()).hexdigest() == lv["support_sha256"]
