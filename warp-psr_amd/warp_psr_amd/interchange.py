@@ -109,7 +109,8 @@ def _tree_key(x, y, z):
 class Stages:
     """Stage arrays of one reconstruction: ``array(name, level=-1)`` and the
     resolved ``params``.  ``Stages.of(result)`` accepts a ``Reconstruction``
-    (its diagnostics) or any object with ``array`` and ``params``."""
+    (its diagnostics), a differentiable reconstruction (its forward result) or
+    any object with ``array`` and ``params``."""
 
     def __init__(self, array, params):
         self.array = array
@@ -119,6 +120,7 @@ class Stages:
     def of(cls, result):
         if isinstance(result, cls):
             return result
+        result = getattr(result, "_primal", result)  # a differentiable reconstruction's forward result
         diagnostics = getattr(result, "diagnostics", None)
         if diagnostics is not None:
             return cls(diagnostics.array, diagnostics.params)
