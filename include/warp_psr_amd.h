@@ -58,6 +58,9 @@ typedef struct PsrParams {
   int keep_intermediates;  /* keep per-stage arrays for psr_result_array (default 1) */
   int float_operators;     /* 1: mixed precision -- a float64 run whose finer-level Laplacian +
                               screening operators are stored as float32 (default 0; float64 only) */
+  int relax_tile;          /* finer-level sliced Gauss-Seidel: side of the (x, y) tiles that one
+                              workgroup relaxes per wave (8..40), 0 = library default, -1 = one
+                              launch per colour and wave; every choice gives the same bits */
 } PsrParams;
 
 typedef struct PsrResult PsrResult;
@@ -220,6 +223,22 @@ int psr_tree_metadata_level(const PsrTreeMetadata* meta, int32_t index, int32_t*
 int psr_tree_metadata_copy(const PsrTreeMetadata* meta, int32_t index, int64_t* keys, uint8_t* flags, uint8_t* ghost,
                            int32_t* parents, int32_t* children, void* stream);
 void psr_tree_metadata_free(PsrTreeMetadata* meta);
+
+/* First key of each level d = 0..depth+1 of sorted device keys (d<<48 | ...)
+ * into the device array offsets (depth + 2 int32): the level_offsets of a
+ * PsrField built by a caller from its own keys and coefficients, as the
+ * reference's LevelField(keys, coefficients, ...) is (src/warp_psr/field.py:
+ * 122-136; its find_key searches all keys, src/warp_psr/field.py:19-33).
+ * Depth outside 0..15 fails with code 2 and the reference's message. */
+int psr_field_level_offsets(const int64_t* keys, int64_t count, int depth, int32_t* offsets, void* stream);
+
+/* The isovalue reduction _compute_isovalue (src/warp_psr/reconstruction.py:
+ * 101-116) on caller device arrays: out (device, one value of dtype) =
+ * sum(double(values[i] * weights[i])) / sum(double(weights[i])), one serial
+ * double chain for n <= 1024, else double partials of 1024-element tiles
+ * combined by halving trees, rounded to dtype.  Partial buffers come from the
+ * library's pool. */
+int psr_isovalue(int dtype, const void* values, const void* weights, int64_t n, void* out, void* stream);
 
 /* Evaluates any PsrField view (psr_result_field) at m device queries with the
  * header-only psr_evaluate_field of warp_psr_amd_field.h -- the kernel-side

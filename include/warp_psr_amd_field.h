@@ -7,7 +7,11 @@
  * (src/warp_psr/device_field.py:91-112) with the reference evaluation
  * (src/warp_psr/field.py:289-359): leaf search from the finest level, the
  * 1e-6 cube-boundary nudge, and coefficient(leaf) + cumulative(leaf-1) +
- * coefficient(leaf+1), with the gradient mapped back to model space.
+ * coefficient(leaf+1), with the gradient mapped back to model space.  A view
+ * whose cumulative pointer is NULL (a LevelField built without cumulative
+ * coefficients) takes the reference's direct level summation
+ * (src/warp_psr/field.py:37-75) instead, as the reference DeviceField does
+ * (src/warp_psr/device_field.py:31-41).
  *
  * Returns 1 for a valid query; 0 outside the unit cube (value and gradient
  * NaN).  T must match field.dtype (float for 0, double for 1).  The field's
@@ -72,6 +76,41 @@ __device__ __forceinline__ void add_level(const PsrField& f, const T* coef, cons
   }
 }
 
+/* field.py:37-75 (evaluation_function): direct summation over levels
+ * 0..depth of a field without cumulative coefficients; left-owned cells, so
+ * knots take left derivatives; no boundary nudge.  res = value and the
+ * cube-space gradient. */
+template <class T>
+__device__ __forceinline__ void direct_levels(const PsrField& f, const T* coef, const T p[3], T res[4]) {
+  for (int d = 0; d <= f.depth; ++d) {
+    const int n = 1 << d;
+    int cell[3];
+    T fr[3];
+    for (int a = 0; a < 3; ++a) {
+      const T t = p[a] * T(n);
+      cell[a] = clampi((int)ceil(t) - 1, 0, n - 1);
+      fr[a] = t - T(cell[a]);
+    }
+    const int64_t lo = f.level_offsets[d], hi = f.level_offsets[d + 1];
+    for (int corner = 0; corner < 8; ++corner) {
+      const int o[3] = {corner / 4, (corner / 2) % 2, corner % 2};
+      const int64_t idx = find(f.keys, lo, hi, pack(d, cell[0] + o[0], cell[1] + o[1], cell[2] + o[2]));
+      if (idx < 0) continue;
+      T h[3], sg[3];
+      for (int a = 0; a < 3; ++a) {
+        h[a] = o[a] ? fr[a] : T(1) - fr[a];
+        sg[a] = o[a] ? T(1) : T(-1);
+      }
+      const T c = coef[idx];
+      res[0] = res[0] + c * h[0] * h[1] * h[2];
+      const T cn = c * T(n);
+      res[1] = res[1] + cn * (sg[0] * h[1] * h[2]);
+      res[2] = res[2] + cn * (h[0] * sg[1] * h[2]);
+      res[3] = res[3] + cn * (h[0] * h[1] * sg[2]);
+    }
+  }
+}
+
 }  // namespace psr_field_detail
 
 template <class T>
@@ -92,6 +131,14 @@ __device__ __forceinline__ int psr_evaluate_field(const PsrField& f, const T poi
   }
   const T* coef = (const T*)f.coefficients;
   const T* cum = (const T*)f.cumulative;
+  if (!cum) { /* a field without cumulative coefficients: direct summation */
+    T res[4] = {T(0), T(0), T(0), T(0)};
+    direct_levels(f, coef, p, res);
+    *value = res[0];
+    if (gradient)
+      for (int c = 0; c < 3; ++c) gradient[c] = L[c] * res[1] + L[3 + c] * res[2] + L[6 + c] * res[3];
+    return 1;
+  }
   int leaf = 0;
   for (int d = f.depth; d >= 0; --d) {
     const int n = 1 << d;

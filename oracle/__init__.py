@@ -191,3 +191,126 @@ def normal_integral(test_depth, test_index, normal_depth, normal_index):
     out = np.zeros(2)
     _lib().oracle_normal_integral(test_depth, test_index, normal_depth, normal_index, out.ctypes.data)
     return out
+
+
+def relax_sliced_tiled(keys, offsets, columns, values, b, x, depth, iters, tile, seed=0, plant=0):
+    """The tile-parallel form of the sliced schedule (tiles of each wave in a
+    seeded random order, two value buffers); returns the new x.  Equals
+    relax_sliced bit for bit on degree-one operators.  ``plant`` (tests only)
+    injects a known mistake: 1 = one halo ring short, 2 = one value buffer."""
+    values = np.ascontiguousarray(values)
+    dtype = values.dtype
+    keys = np.ascontiguousarray(keys, dtype=np.int64)
+    offsets = np.ascontiguousarray(offsets, dtype=np.int32)
+    columns = np.ascontiguousarray(columns, dtype=np.int32)
+    b = np.ascontiguousarray(b, dtype=dtype)
+    x = np.array(x, dtype=dtype, copy=True)
+    lib = _lib()
+    fn = lib.oracle_relax_sliced_tiled
+    fn.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                   ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_int]
+    if fn(1 if dtype == np.float64 else 0, keys.ctypes.data, len(keys), offsets.ctypes.data, columns.ctypes.data,
+          values.ctypes.data, b.ctypes.data, x.ctypes.data, depth, iters, tile, seed, plant):
+        raise RuntimeError(lib.oracle_last_error().decode())
+    return x
+
+
+# ---- known-answer hooks (stand-alone pipeline pieces on caller data)
+
+def _code(dtype):
+    return 1 if np.dtype(dtype) == np.float64 else 0
+
+
+def field_evaluate(keys, coefficients, depth, linear, translation, queries, cumulative=None):
+    """field.py:364-421 on a caller field; returns (values, gradients, valid)."""
+    coefficients = np.ascontiguousarray(coefficients)
+    dt = coefficients.dtype
+    keys = np.ascontiguousarray(keys, dtype=np.int64)
+    q = np.ascontiguousarray(queries, dtype=dt)
+    cum = None if cumulative is None else np.ascontiguousarray(cumulative, dtype=dt)
+    m = len(q)
+    values = np.empty(m, dt)
+    grads = np.empty((m, 3), dt)
+    valid = np.empty(m, np.int32)
+    lin = np.ascontiguousarray(np.asarray(linear, np.float64).reshape(9))
+    tr = np.ascontiguousarray(np.asarray(translation, np.float64).reshape(3))
+    fn = _lib().oracle_field_evaluate
+    fn.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                   ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p,
+                   ctypes.c_void_p]
+    fn(_code(dt), keys.ctypes.data, len(keys), coefficients.ctypes.data, None if cum is None else cum.ctypes.data, depth,
+       lin.ctypes.data, tr.ctypes.data, q.ctypes.data, m, values.ctypes.data, grads.ctypes.data, valid.ctypes.data)
+    return values, grads, valid
+
+
+def cumulative(keys, coefficients, depth):
+    """field.py:187-210 (cumulative_coefficients) on caller keys and coefficients."""
+    coefficients = np.ascontiguousarray(coefficients)
+    keys = np.ascontiguousarray(keys, dtype=np.int64)
+    out = np.empty_like(coefficients)
+    fn = _lib().oracle_cumulative
+    fn.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+    fn(_code(coefficients.dtype), keys.ctypes.data, len(keys), coefficients.ctypes.data, depth, out.ctypes.data)
+    return out
+
+
+def isovalue(values, weights):
+    """reconstruction.py:76-116 (_compute_isovalue) on caller values and weights."""
+    values = np.ascontiguousarray(values)
+    weights = np.ascontiguousarray(weights, dtype=values.dtype)
+    out = np.zeros(1, values.dtype)
+    fn = _lib().oracle_isovalue
+    fn.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p]
+    fn(_code(values.dtype), values.ctypes.data, weights.ctypes.data, len(values), out.ctypes.data)
+    return out[0]
+
+
+def screening_dense(positions, weights, depth):
+    """screening_stencils.py:80-101 on a uniform level (all nodes of `depth`,
+    keys in sorted order): the dense screening matrix and the point load."""
+    positions = np.ascontiguousarray(positions)
+    weights = np.ascontiguousarray(weights, dtype=positions.dtype)
+    k = ((1 << depth) + 1) ** 3
+    dense = np.zeros((k, k))
+    load = np.zeros(k)
+    lib = _lib()
+    fn = lib.oracle_screening_dense
+    fn.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_void_p,
+                   ctypes.c_void_p]
+    if fn(_code(positions.dtype), positions.ctypes.data, weights.ctypes.data, len(positions), depth, dense.ctypes.data,
+          load.ctypes.data):
+        raise RuntimeError(lib.oracle_last_error().decode())
+    return dense, load
+
+
+def _csr_args(offsets, columns, values):
+    return (np.ascontiguousarray(offsets, dtype=np.int32), np.ascontiguousarray(columns, dtype=np.int32),
+            np.ascontiguousarray(values))
+
+
+def root_padded_apply(keys, offsets, columns, values, v):
+    """solver.py:167-238: the 64-row padded root system applied with the
+    constant-mode term to v (64 values)."""
+    offsets, columns, values = _csr_args(offsets, columns, values)
+    keys = np.ascontiguousarray(keys, dtype=np.int64)
+    v = np.ascontiguousarray(v, dtype=values.dtype)
+    out = np.empty(64, values.dtype)
+    fn = _lib().oracle_root_padded_apply
+    fn.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_int64] + [ctypes.c_void_p] * 5
+    fn(_code(values.dtype), keys.ctypes.data, len(keys), offsets.ctypes.data, columns.ctypes.data, values.ctypes.data,
+       v.ctypes.data, out.ctypes.data)
+    return out
+
+
+def root_solve(keys, offsets, columns, values, b, iters=8, cycles=1, accuracy=1e-3, dc=True):
+    """solver.py:241-317 with base depth 0 on a caller root operator."""
+    offsets, columns, values = _csr_args(offsets, columns, values)
+    keys = np.ascontiguousarray(keys, dtype=np.int64)
+    b = np.ascontiguousarray(b, dtype=values.dtype)
+    x = np.zeros(len(keys), values.dtype)
+    fn = _lib().oracle_root_solve
+    fn.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_int64] + [ctypes.c_void_p] * 5 + [
+        ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int]
+    fn(_code(values.dtype), keys.ctypes.data, len(keys), offsets.ctypes.data, columns.ctypes.data, values.ctypes.data,
+       b.ctypes.data, x.ctypes.data, iters, cycles, accuracy, int(bool(dc)))
+    return x

@@ -1290,6 +1290,118 @@ template <class T> void relax_sliced(const CSR<T>& A, const std::vector<T>& b, s
       for (int color = 0; color < 4; ++color) relax_group(A, b, x, s, z * 4 + color);
 }
 
+// Tile-parallel execution of the sliced schedule, the form the HIP library
+// runs (not a reference function).  It is checked here against relax_sliced
+// above, bit for bit, to show that the decomposition keeps every update's
+// inputs and arithmetic order.
+//
+// Wave q of wavefront_groups (relaxation.py:128-143) visits slices z two
+// apart (visit t = (q + z - n) / 2 of slice z), so during a wave the slices
+// z - 1 and z + 1 are not written.  Each slice's (x, y) plane is cut into
+// ts x ts tiles; a tile updates colours 0..3 of its rows in order.  Colour c
+// at Chebyshev ring r outside the tile needs colours < c at ring r + 1, so a
+// tile recomputes colour c on rings up to 3 - c from old values it loaded on
+// rings up to 4, and writes only its interior rows.  Two buffers keep loads
+// and stores of one wave apart: after t visits a slice's values sit in
+// buffer (iters - t) % 2, so visit t of slice z reads slices z - 1 and z from
+// buffer (iters - t) % 2, slice z + 1 (t + 1 visits done) from the other,
+// and writes slice z there.  Tiles of a wave are run in a seeded random
+// order; the result must not depend on it.  Needs same-level couplings
+// within one cell in x, y and z (degree-one operators); throws otherwise.
+// `plant` injects a known mistake for the tests that show the check has
+// teeth: 1 computes each colour on one ring fewer, 2 uses one buffer.
+template <class T> void relax_sliced_tiled(const std::vector<int64_t>& keys, const CSR<T>& A, const std::vector<T>& b,
+                                           std::vector<T>& x, int depth, int iters, int ts, uint64_t seed, int plant) {
+  if (ts < 8) throw std::runtime_error("tile size must be at least 8");
+  const int n = 1 << depth, R = (int)keys.size(), nt = (n + 1 + ts - 1) / ts, g = ts + 8;
+  std::vector<I3> c(R);
+  std::vector<std::vector<int>> slice(n + 1);
+  for (int i = 0; i < R; ++i) {
+    c[i] = coords(keys[i]);
+    slice[c[i][2]].push_back(i);
+  }
+  std::vector<T> buf[2] = {x, x};
+  auto cur = [&](int t) -> std::vector<T>& { return buf[plant == 2 ? 0 : ((iters - t) % 2 + 2) % 2]; };
+  uint64_t rng = seed * 0x9E3779B97F4A7C15ull + 1;
+  auto next = [&]() { rng ^= rng << 13; rng ^= rng >> 7; rng ^= rng << 17; return rng; };
+  auto ring = [&](int v, int t0) { return std::max(0, std::max(t0 - v, v - (t0 + ts - 1))); };
+  for (int q = 0; q <= n + 2 * (iters - 1); ++q) {
+    const int d = q - n + 1;  // Python's (q - n + 1) // 2
+    const int first = std::max(0, d >= 0 ? d / 2 : -((-d + 1) / 2)), last = std::min(iters - 1, q / 2);
+    if (last < first) continue;
+    const int z0 = n - q + 2 * first;
+    // every tile of every slice of the wave, in a random order
+    std::vector<std::pair<int, int>> tiles;  // (slice index, tile)
+    std::vector<std::vector<char>> has(last - first + 1, std::vector<char>((size_t)nt * nt, 0));
+    for (int si = 0; si <= last - first; ++si)
+      for (int i : slice[z0 + 2 * si]) has[si][(size_t)(c[i][1] / ts) * nt + c[i][0] / ts] = 1;
+    for (int si = 0; si <= last - first; ++si)
+      for (int k = 0; k < nt * nt; ++k)
+        if (has[si][k]) tiles.push_back({si, k});
+    for (size_t k = tiles.size(); k > 1; --k) std::swap(tiles[k - 1], tiles[next() % k]);
+    for (auto [si, tile] : tiles) {
+      const int z = z0 + 2 * si, t = first + si;
+      const std::vector<T>& P = cur(t);
+      std::vector<T>& Q = cur(t + 1);
+      const int tx0 = (tile % nt) * ts, ty0 = (tile / nt) * ts;
+      std::vector<T> grid((size_t)g * g, T(0));
+      std::vector<char> loaded((size_t)g * g, 0);
+      auto cell = [&](int i) { return (size_t)(c[i][1] - ty0 + 4) * g + (c[i][0] - tx0 + 4); };
+      std::vector<int> rows;
+      for (int i : slice[z]) {
+        const int r = std::max(ring(c[i][0], tx0), ring(c[i][1], ty0));
+        if (r > 4) continue;
+        grid[cell(i)] = P[i];
+        loaded[cell(i)] = 1;
+        rows.push_back(i);
+      }
+      for (int color = 0; color < 4; ++color) {
+        std::vector<std::pair<int, T>> upd;
+        for (int i : rows) {
+          if ((c[i][0] & 1) + 2 * (c[i][1] & 1) != color) continue;
+          const int reach = plant == 1 ? std::max(0, 2 - color) : 3 - color;
+          if (std::max(ring(c[i][0], tx0), ring(c[i][1], ty0)) > reach) continue;
+          T r = b[i];
+          T diagonal = T(0);
+          for (int e = A.offsets[i]; e < A.offsets[i + 1]; ++e) {
+            const int col = A.columns[e];
+            const I3 cc = c[col];
+            if (std::abs(cc[0] - c[i][0]) > 1 || std::abs(cc[1] - c[i][1]) > 1 || std::abs(cc[2] - z) > 1)
+              throw std::runtime_error("tiled relaxation needs couplings within one cell");
+            T xv;
+            if (cc[2] == z) {
+              if (!loaded[cell(col)]) throw std::runtime_error("tiled relaxation: neighbour not loaded");
+              xv = grid[cell(col)];
+            } else {
+              xv = cc[2] < z ? P[col] : Q[col];
+            }
+            r = r - xv * A.values[e];
+            if (col == i) diagonal = A.values[e];
+          }
+          T v = grid[cell(i)];
+          if (diagonal != T(0)) v = v + r * (T(1) / diagonal);
+          upd.push_back({i, v});
+        }
+        for (auto [i, v] : upd) {
+          grid[cell(i)] = v;
+          if (c[i][0] / ts == tx0 / ts && c[i][1] / ts == ty0 / ts) Q[i] = v;
+        }
+      }
+    }
+  }
+  x = cur(iters);
+}
+
+// A caller CSR operator (host arrays) as the oracle's CSR.
+template <class T> CSR<T> csr_from(int64_t rows, const int* offsets, const int* columns, const void* values) {
+  CSR<T> A;
+  A.rows = A.cols = (int)rows;
+  A.offsets.assign(offsets, offsets + rows + 1);
+  A.columns.assign(columns, columns + offsets[rows]);
+  A.values.assign((const T*)values, (const T*)values + offsets[rows]);
+  return A;
+}
+
 // solver.py:30-33
 inline int root_row(int64_t key) {
   I3 c = coords(key);
@@ -1328,6 +1440,41 @@ void conjugate_gradient(Op apply, const std::vector<T>& b, std::vector<T>& x, do
   }
 }
 
+// solver.py:167-202 (padded_root_system): the root operator's rows and
+// columns moved to root_row(key) in a 64-row system, row entries in column
+// order (as a triplet build sorts them).
+template <class T> CSR<T> padded_root_system(const CSR<T>& K, const std::vector<int64_t>& keys) {
+  CSR<T> pad;
+  pad.rows = pad.cols = 64;
+  std::vector<std::vector<std::pair<int, T>>> rowsv(64);
+  for (size_t i = 0; i < keys.size(); ++i)
+    for (int e = K.offsets[i]; e < K.offsets[i + 1]; ++e)
+      rowsv[root_row(keys[i])].push_back({root_row(keys[K.columns[e]]), K.values[e]});
+  pad.offsets.assign(65, 0);
+  for (int r = 0; r < 64; ++r) {
+    std::sort(rowsv[r].begin(), rowsv[r].end(), [](const std::pair<int, T>& a, const std::pair<int, T>& b) { return a.first < b.first; });
+    for (auto& e : rowsv[r]) { pad.columns.push_back(e.first); pad.values.push_back(e.second); }
+    pad.offsets[r + 1] = (int)pad.columns.size();
+  }
+  return pad;
+}
+
+// solver.py:205-238 (constant_mode_operator): out = M v, plus mean(v) in
+// every row when the constant-mode term is on.
+template <class T> void constant_mode_apply(const CSR<T>& M, bool dc, const std::vector<T>& v, std::vector<T>& out) {
+  for (int i = 0; i < M.rows; ++i) {
+    T a = T(0);
+    for (int e = M.offsets[i]; e < M.offsets[i + 1]; ++e) a = a + M.values[e] * v[M.columns[e]];
+    out[i] = a;
+  }
+  if (dc) {
+    T mean = T(0);
+    for (int i = 0; i < M.rows; ++i) mean = mean + v[i];
+    mean = mean / T(M.rows);
+    for (int i = 0; i < M.rows; ++i) out[i] = out[i] + T(1) * mean;
+  }
+}
+
 // solver.py:241-317 (regular_multigrid)
 template <class T> std::vector<T> regular_multigrid(State<T>& s, const CSR<T>& A, const std::vector<T>& b, int base) {
   std::vector<CSR<T>> mats(base + 1);
@@ -1354,35 +1501,12 @@ template <class T> std::vector<T> regular_multigrid(State<T>& s, const CSR<T>& A
   CSR<T> pad;
   std::vector<T> pad_rhs, pad_x;
   if (dc) {
-    pad.rows = pad.cols = 64;
-    std::vector<std::vector<std::pair<int, T>>> rowsv(64);
-    for (size_t i = 0; i < m0; ++i)
-      for (int e = K.offsets[i]; e < K.offsets[i + 1]; ++e)
-        rowsv[root_row(s.basis[0][i])].push_back({root_row(s.basis[0][K.columns[e]]), K.values[e]});
-    pad.offsets.assign(65, 0);
-    for (int r = 0; r < 64; ++r) {
-      std::sort(rowsv[r].begin(), rowsv[r].end(), [](const std::pair<int, T>& a, const std::pair<int, T>& b) { return a.first < b.first; });
-      for (auto& e : rowsv[r]) { pad.columns.push_back(e.first); pad.values.push_back(e.second); }
-      pad.offsets[r + 1] = (int)pad.columns.size();
-    }
+    pad = padded_root_system(K, s.basis[0]);
     pad_rhs.assign(64, T(0));
     pad_x.assign(64, T(0));
     for (size_t i = 0; i < m0; ++i) pad_rhs[root_row(s.basis[0][i])] = rhs[0][i];
   }
-  auto apply = [&](const std::vector<T>& v, std::vector<T>& out) {
-    const CSR<T>& M = dc ? pad : K;
-    for (int i = 0; i < M.rows; ++i) {
-      T a = T(0);
-      for (int e = M.offsets[i]; e < M.offsets[i + 1]; ++e) a = a + M.values[e] * v[M.columns[e]];
-      out[i] = a;
-    }
-    if (dc) {
-      T mean = T(0);
-      for (int i = 0; i < 64; ++i) mean = mean + v[i];
-      mean = mean / T(64);
-      for (int i = 0; i < 64; ++i) out[i] = out[i] + T(1) * mean;
-    }
-  };
+  auto apply = [&](const std::vector<T>& v, std::vector<T>& out) { constant_mode_apply(dc ? pad : K, dc, v, out); };
   for (int cycle = 0; cycle < s.prm.base_v_cycles; ++cycle) {
     for (int d = base; d > 0; --d) {
       relax_scheduled(mats[d], rhs[d], x[d], sched[d], iters, true);
@@ -1529,6 +1653,53 @@ template <class T> void add_level(const std::vector<int64_t>& keys, const std::v
       }
     }
   }
+}
+
+// field.py:37-75 (evaluation_function): direct level summation of a field
+// without cumulative coefficients -- every level 0..depth, the left-owned
+// cell (ceil - 1) and its local coordinates, so knots take left derivatives;
+// no boundary nudge.
+template <class T>
+void evaluate_direct(const State<T>& s, const V3<T>& q, const T* L, const T* tr, T* value, T* grad, int* valid) {
+  V3<T> p;
+  for (int i = 0; i < 3; ++i) p[i] = L[3 * i] * q[0] + L[3 * i + 1] * q[1] + L[3 * i + 2] * q[2] + tr[i];
+  bool ok = std::isfinite(p[0]) && std::isfinite(p[1]) && std::isfinite(p[2]) &&
+            std::min(std::min(p[0], p[1]), p[2]) >= T(0) && std::max(std::max(p[0], p[1]), p[2]) <= T(1);
+  *valid = ok ? 1 : 0;
+  if (!ok) {
+    *value = std::numeric_limits<T>::quiet_NaN();
+    grad[0] = grad[1] = grad[2] = *value;
+    return;
+  }
+  T v = T(0), g[3] = {T(0), T(0), T(0)};
+  for (int d = 0; d <= s.prm.depth; ++d) {
+    const int n = 1 << d;
+    int c[3];
+    T f[3];
+    for (int a = 0; a < 3; ++a) {
+      const T t = p[a] * T(n);
+      c[a] = std::min(std::max((int)std::ceil(t) - 1, 0), n - 1);
+      f[a] = t - T(c[a]);
+    }
+    for (int corner = 0; corner < 8; ++corner) {
+      const int o[3] = {corner / 4, (corner / 2) % 2, corner % 2};
+      const int64_t idx = find_key(s.keys, basis_key(d, c[0] + o[0], c[1] + o[1], c[2] + o[2]));
+      if (idx < 0) continue;
+      T h[3], sg[3];
+      for (int a = 0; a < 3; ++a) {
+        h[a] = o[a] ? f[a] : T(1) - f[a];
+        sg[a] = o[a] ? T(1) : T(-1);
+      }
+      const T co = s.coefficients[idx];
+      v = v + co * h[0] * h[1] * h[2];
+      const T cn = co * T(n);
+      g[0] = g[0] + cn * (sg[0] * h[1] * h[2]);
+      g[1] = g[1] + cn * (h[0] * sg[1] * h[2]);
+      g[2] = g[2] + cn * (h[0] * h[1] * sg[2]);
+    }
+  }
+  *value = v;
+  for (int i = 0; i < 3; ++i) grad[i] = L[i] * g[0] + L[3 + i] * g[1] + L[6 + i] * g[2];
 }
 
 // field.py:395-420 (reference evaluation kernel)
@@ -1902,6 +2073,185 @@ int oracle_relax_scheduled(int dtype, const int64_t* keys, int64_t rows, const i
         }
     }
     std::copy(xx.begin(), xx.end(), (T*)x);
+  };
+  if (dtype == 0) go(float()); else go(double());
+  return 0;
+}
+
+// The tile-parallel sliced schedule (relax_sliced_tiled) on caller CSR arrays.
+int oracle_relax_sliced_tiled(int dtype, const int64_t* keys, int64_t rows, const int* offsets, const int* columns,
+                              const void* values, const void* b, void* x, int depth, int iters, int tile, uint64_t seed,
+                              int plant) {
+  try {
+    std::vector<int64_t> k(keys, keys + rows);
+    auto go = [&](auto tag) {
+      using T = decltype(tag);
+      CSR<T> A;
+      A.rows = (int)rows;
+      A.offsets.assign(offsets, offsets + rows + 1);
+      A.columns.assign(columns, columns + offsets[rows]);
+      A.values.assign((const T*)values, (const T*)values + offsets[rows]);
+      std::vector<T> bb((const T*)b, (const T*)b + rows), xx((T*)x, (T*)x + rows);
+      relax_sliced_tiled(k, A, bb, xx, depth, iters, tile, seed, plant);
+      std::copy(xx.begin(), xx.end(), (T*)x);
+    };
+    if (dtype == 0) go(float()); else go(double());
+    return 0;
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return 1;
+  }
+}
+
+// ---- known-answer hooks: stand-alone pieces of the pipeline on caller data
+
+// LevelField.evaluate (field.py:150-184) on a caller field (sorted keys of
+// levels 0..depth, level corrections, model -> cube linear map and
+// translation): the reference evaluation (field.py:364-421) with cumulative
+// coefficients, the direct level summation (field.py:37-75) without.
+int oracle_field_evaluate(int dtype, const int64_t* keys, int64_t count, const void* coefficients, const void* cumulative,
+                          int depth, const double* linear, const double* translation, const void* queries, int64_t m,
+                          void* values, void* grads, int32_t* valid) {
+  auto go = [&](auto tag) {
+    using T = decltype(tag);
+    State<T> st;
+    st.prm.depth = depth;
+    st.keys.assign(keys, keys + count);
+    st.coefficients.assign((const T*)coefficients, (const T*)coefficients + count);
+    if (cumulative) st.cumulative.assign((const T*)cumulative, (const T*)cumulative + count);
+    T L[9], tr[3];
+    for (int i = 0; i < 9; ++i) L[i] = T(linear[i]);
+    for (int i = 0; i < 3; ++i) tr[i] = T(translation[i]);
+    const T* q = (const T*)queries;
+    for (int64_t i = 0; i < m; ++i) {
+      V3<T> qq = {{q[3 * i], q[3 * i + 1], q[3 * i + 2]}};
+      int ok;
+      if (cumulative) evaluate_one(st, qq, L, tr, (T*)values + i, (T*)grads + 3 * i, &ok);
+      else evaluate_direct(st, qq, L, tr, (T*)values + i, (T*)grads + 3 * i, &ok);
+      valid[i] = ok;
+    }
+  };
+  if (dtype == 0) go(float()); else go(double());
+  return 0;
+}
+
+// field.py:187-210 on caller keys (sorted, levels 0..depth) and coefficients.
+int oracle_cumulative(int dtype, const int64_t* keys, int64_t count, const void* coefficients, int depth, void* out) {
+  auto go = [&](auto tag) {
+    using T = decltype(tag);
+    State<T> st;
+    st.prm.depth = depth;
+    st.keys.assign(keys, keys + count);
+    st.coefficients.assign((const T*)coefficients, (const T*)coefficients + count);
+    cumulative_coefficients(st);
+    std::copy(st.cumulative.begin(), st.cumulative.end(), (T*)out);
+  };
+  if (dtype == 0) go(float()); else go(double());
+  return 0;
+}
+
+// reconstruction.py:76-116 (_compute_isovalue) on caller values and weights.
+int oracle_isovalue(int dtype, const void* values, const void* weights, int64_t n, void* out) {
+  auto go = [&](auto tag) {
+    using T = decltype(tag);
+    std::vector<T> v((const T*)values, (const T*)values + n), w((const T*)weights, (const T*)weights + n);
+    *(T*)out = isovalue(v, w);
+  };
+  if (dtype == 0) go(float()); else go(double());
+  return 0;
+}
+
+// screening_stencils.py:80-101 on one uniform level: every node of depth
+// `depth` as a row, the 27-neighbour CSR pattern, points binned by floor cell
+// (positions in the unit cube); writes the dense count x count screening
+// matrix (row-major, keys in sorted order) and the point load.
+int oracle_screening_dense(int dtype, const void* positions, const void* weights, int64_t m, int depth, double* dense,
+                           double* load) {
+  try {
+    const int n = 1 << depth;
+    std::vector<int64_t> keys;
+    for (int x = 0; x <= n; ++x)
+      for (int y = 0; y <= n; ++y)
+        for (int z = 0; z <= n; ++z) keys.push_back(basis_key(depth, x, y, z));
+    const size_t K = keys.size();
+    CSR<double> A;
+    A.rows = A.cols = (int)K;
+    A.offsets.assign(K + 1, 0);
+    for (size_t i = 0; i < K; ++i) {
+      I3 c = coords(keys[i]);
+      for (int k = 0; k < 27; ++k) {
+        const int64_t nb = find_key(keys, basis_key(depth, c[0] + k / 9 - 1, c[1] + (k / 3) % 3 - 1, c[2] + k % 3 - 1));
+        if (nb >= 0) { A.columns.push_back((int)nb); A.values.push_back(0.0); }
+      }
+      A.offsets[i + 1] = (int)A.columns.size();
+    }
+    auto go = [&](auto tag) {
+      using T = decltype(tag);
+      Points<T> pts;
+      const T* p = (const T*)positions;
+      for (int64_t i = 0; i < m; ++i) {
+        pts.positions.push_back(V3<T>{{p[3 * i], p[3 * i + 1], p[3 * i + 2]}});
+        pts.weights.push_back(((const T*)weights)[i]);
+      }
+      std::vector<int64_t> bins(m);
+      for (int64_t i = 0; i < m; ++i) {
+        I3 c;
+        for (int a = 0; a < 3; ++a) c[a] = std::min(std::max((int)std::floor(pts.positions[i][a] * T(n)), 0), n - 1);
+        bins[i] = basis_key(depth, c[0], c[1], c[2]);
+      }
+      std::vector<int> ids = stable_order(bins);
+      for (int64_t i = 0; i < m; ++i) pts.bin_keys.push_back(bins[ids[i]]);
+      pts.bin_ids = ids;
+      std::vector<double> l = add_screening(A, keys, keys, pts, depth, depth);
+      std::copy(l.begin(), l.end(), load);
+    };
+    if (dtype == 0) go(float()); else go(double());
+    std::fill(dense, dense + K * K, 0.0);
+    for (size_t i = 0; i < K; ++i)
+      for (int e = A.offsets[i]; e < A.offsets[i + 1]; ++e) dense[i * K + A.columns[e]] = A.values[e];
+    return 0;
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return 1;
+  }
+}
+
+// solver.py:167-238 on a caller root operator: the 64-row padded system of
+// `keys` (depth-0 nodes) applied with the constant-mode term to v (64
+// values): out = P v + mean(v).
+int oracle_root_padded_apply(int dtype, const int64_t* keys, int64_t rows, const int* offsets, const int* columns,
+                             const void* values, const void* v, void* out) {
+  auto go = [&](auto tag) {
+    using T = decltype(tag);
+    CSR<T> K = csr_from<T>(rows, offsets, columns, values);
+    CSR<T> pad = padded_root_system(K, std::vector<int64_t>(keys, keys + rows));
+    std::vector<T> vv((const T*)v, (const T*)v + 64), o(64);
+    constant_mode_apply(pad, true, vv, o);
+    std::copy(o.begin(), o.end(), (T*)out);
+  };
+  if (dtype == 0) go(float()); else go(double());
+  return 0;
+}
+
+// solver.py:241-317 with base depth 0: `cycles` root solves (skip test, then
+// CG on the padded system with the constant-mode term when dc) of a caller
+// root operator; x receives the solution.
+int oracle_root_solve(int dtype, const int64_t* keys, int64_t rows, const int* offsets, const int* columns,
+                      const void* values, const void* b, void* x, int iters, int cycles, double accuracy, int dc) {
+  auto go = [&](auto tag) {
+    using T = decltype(tag);
+    State<T> st;
+    st.prm.depth = 0;
+    st.prm.base_depth = 0;
+    st.prm.point_weight = dc ? 0.0 : 2.0;
+    st.prm.iters = iters;
+    st.prm.base_v_cycles = cycles;
+    st.prm.cg_accuracy = accuracy;
+    st.basis.assign(1, std::vector<int64_t>(keys, keys + rows));
+    CSR<T> K = csr_from<T>(rows, offsets, columns, values);
+    std::vector<T> bb((const T*)b, (const T*)b + rows);
+    std::vector<T> sol = regular_multigrid(st, K, bb, 0);
+    std::copy(sol.begin(), sol.end(), (T*)x);
   };
   if (dtype == 0) go(float()); else go(double());
   return 0;

@@ -46,7 +46,7 @@ def test_params_default():
     p = _params()
     assert (p.depth, p.full_depth, p.base_depth, p.solve_depth, p.kernel_depth) == (6, 3, -1, -1, -1)
     assert (p.samples_per_node, p.scale, p.point_weight, p.iters, p.base_v_cycles, p.cg_accuracy) == (1.5, 1.1, 2.0, 8, 1, 1e-3)
-    assert (p.keep_intermediates, p.float_operators) == (1, 0)
+    assert (p.keep_intermediates, p.float_operators, p.relax_tile) == (1, 0, 0)
 
 
 @pytest.mark.parametrize("kw,message", [
@@ -58,6 +58,9 @@ def test_params_default():
     (dict(cg_accuracy=0.0), "positive cg_accuracy"),
     (dict(scale=1.0), "scale > 1"),
     (dict(samples_per_node=float("nan")), "samples_per_node"),
+    (dict(relax_tile=7), "relax_tile must be"),
+    (dict(relax_tile=41), "relax_tile must be"),
+    (dict(relax_tile=-2), "relax_tile must be"),
 ])
 def test_c_abi_rejects_invalid_parameters_before_device_work(kw, message):
     lib = _lib.load()
@@ -162,3 +165,43 @@ def test_c_abi_tree_metadata_validates_arguments():
     assert lib.psr_tree_metadata_level(None, 0, ctypes.byref(depth), ctypes.byref(n)) == 2
     assert lib.psr_tree_metadata_copy(None, 0, None, None, None, None, None, None) == 2
     lib.psr_tree_metadata_free(None)
+
+
+def _relax_operands(**changes):
+    """A valid 3-row CSR operator on the CPU, with `changes` applied."""
+    import torch
+    ops = dict(
+        keys=torch.tensor([1, 2, 3], dtype=torch.int64),
+        offsets=torch.tensor([0, 1, 2, 3], dtype=torch.int32),
+        columns=torch.tensor([0, 1, 2], dtype=torch.int32),
+        values=torch.ones(3, dtype=torch.float64),
+        b=torch.zeros(3, dtype=torch.float64),
+        x=torch.zeros(3, dtype=torch.float64),
+    )
+    ops.update(changes)
+    return ops
+
+
+@pytest.mark.parametrize("fn", ["relax_sliced", "relax_scheduled"])
+@pytest.mark.parametrize("change,message", [
+    (lambda t: dict(columns=t.tensor([0, 1, 2], dtype=t.int64)), "columns must be int32"),
+    (lambda t: dict(offsets=t.tensor([0, 1, 2, 3], dtype=t.int64)), "offsets must be int32"),
+    (lambda t: dict(keys=t.tensor([1, 2, 3], dtype=t.int32)), "keys must be int64"),
+    (lambda t: dict(b=t.zeros(3, dtype=t.float32)), "b must have the dtype of values"),
+    (lambda t: dict(x=t.zeros(3, dtype=t.float32)), "x must have the dtype of values"),
+    (lambda t: dict(values=t.ones(3, dtype=t.int32)), "values must be float32 or float64"),
+    (lambda t: dict(offsets=t.tensor([0, 1, 3], dtype=t.int32)), "offsets must hold rows"),
+    (lambda t: dict(x=t.zeros(4, dtype=t.float64)), "b and x must hold one value per row"),
+    (lambda t: dict(columns=t.tensor([0, 1], dtype=t.int32)), "same length"),
+    (lambda t: dict(x=t.zeros((3, 1), dtype=t.float64)), "one-dimensional"),
+    (lambda t: dict(b=[0.0, 0.0, 0.0]), "b must be a torch tensor"),
+    (lambda t: {}, "ROCm device"),
+])
+def test_relax_wrappers_reject_bad_operands(fn, change, message):
+    """The relaxation wrappers check dtypes, lengths and the device before any
+    pointer reaches the C ABI (which cannot see them)."""
+    import torch
+    from warp_psr_amd import reconstruction
+    ops = _relax_operands(**change(torch))
+    with pytest.raises(ValueError, match=message):
+        getattr(reconstruction, fn)(ops["keys"], ops["offsets"], ops["columns"], ops["values"], ops["b"], ops["x"], 3, 2)

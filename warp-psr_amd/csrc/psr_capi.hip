@@ -48,6 +48,8 @@ void check_params(const PsrParams& p) {
     throw PsrError(2, "Require finite nonnegative point_weight and positive cg_accuracy");
   if (!std::isfinite(p.samples_per_node) || p.samples_per_node <= 0 || !std::isfinite(p.scale) || p.scale <= 1)
     throw PsrError(2, "Require finite positive samples_per_node and scale > 1");
+  if (!(p.relax_tile == 0 || p.relax_tile == -1 || (p.relax_tile >= 8 && p.relax_tile <= 40)))
+    throw PsrError(2, "relax_tile must be 0 (default), -1 (one launch per colour) or a tile side in [8, 40]");
 }
 
 Params resolve(const PsrParams& in) {
@@ -62,6 +64,7 @@ Params resolve(const PsrParams& in) {
   q.point_weight = p.point_weight; q.exact_interpolation = p.exact_interpolation ? 1 : 0; q.iters = p.iters;
   q.base_v_cycles = p.base_v_cycles; q.cg_accuracy = p.cg_accuracy; q.keep_intermediates = p.keep_intermediates;
   q.float_operators = p.float_operators ? 1 : 0;
+  q.relax_tile = p.relax_tile;
   return q;
 }
 
@@ -247,7 +250,9 @@ template <class T> std::string metadata_json(const Result<T>& r) {
   for (size_t k = 0; k < r.relax_stats.size(); ++k) {
     const auto& st = r.relax_stats[k];
     o << (k ? ", " : "") << "{\"depth\": " << st.depth << ", \"rows\": " << st.rows << ", \"nnz\": " << st.nnz
-      << ", \"launches\": " << st.launches << ", \"sweeps\": " << p.iters << ", \"seconds\": " << st.seconds << "}";
+      << ", \"launches\": " << st.launches << ", \"sweeps\": " << p.iters << ", \"seconds\": " << st.seconds
+      << ", \"tile\": " << st.tile << ", \"threads_per_colour\": " << st.width << ", \"tiles\": " << st.tiles
+      << ", \"tile_entries\": " << st.entries << ", \"tile_max_colour_rows\": " << st.max_colour_rows << "}";
   }
   o << "], \"profile\": \"pinned cascadic degree-1 Neumann\", \"constant_mode\": \""
     << ((p.base_depth == 0 && p.point_weight == 0) ? "reference root mean term" : "reference solve without an added mean term")
@@ -312,6 +317,18 @@ __global__ void k_field_evaluate(PsrField f, const T* __restrict__ queries, int6
   if (grads) { grads[3 * i] = g[0]; grads[3 * i + 1] = g[1]; grads[3 * i + 2] = g[2]; }
   if (valid) valid[i] = ok;
 }
+// first key of each level d = 0..depth+1 in sorted keys
+__global__ void k_level_offsets(const int64_t* __restrict__ keys, int64_t count, int depth, int32_t* __restrict__ out) {
+  const int d = blockIdx.x * blockDim.x + threadIdx.x;
+  if (d > depth + 1) return;
+  const int64_t key = (int64_t)d << 48;
+  int64_t lo = 0, hi = count;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (keys[mid] < key) lo = mid + 1; else hi = mid;
+  }
+  out[d] = (int32_t)lo;
+}
 }  // namespace
 
 extern "C" {
@@ -331,6 +348,7 @@ void psr_params_default(PsrParams* p) {
   p->cg_accuracy = 1e-3;
   p->keep_intermediates = 1;
   p->float_operators = 0;
+  p->relax_tile = 0;
 }
 
 int psr_reconstruct(const void* points, const void* normals, int64_t n, int dtype, const PsrParams* params, void* stream,
@@ -441,6 +459,9 @@ int psr_evaluate(const PsrResult* result, const void* queries, int64_t m, void* 
     return 0;
   } catch (const PsrError& e) {
     return fail(e);
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return 1;
   }
 }
 
@@ -500,6 +521,9 @@ int psr_result_array(PsrResult* result, const char* name, int level, const void*
     return 0;
   } catch (const PsrError& e) {
     return fail(e);
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return 1;
   }
 }
 
@@ -530,6 +554,9 @@ int psr_field_evaluate(const PsrField* field, const void* queries, int64_t m, vo
     return 0;
   } catch (const PsrError& e) {
     return fail(e);
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return 1;
   }
 }
 
@@ -630,6 +657,9 @@ int psr_relax_sliced(int dtype, const int64_t* keys, int64_t rows, const int32_t
     return 0;
   } catch (const PsrError& e) {
     return fail(e);
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return 1;
   }
 }
 
@@ -651,6 +681,9 @@ int psr_relax_scheduled(int dtype, const int64_t* keys, int64_t rows, const int3
     return 0;
   } catch (const PsrError& e) {
     return fail(e);
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return 1;
   }
 }
 
@@ -674,6 +707,9 @@ int psr_mesh_begin(const int32_t shape[3], const float lower[3], const float upp
     return 0;
   } catch (const PsrError& e) {
     return fail(e);
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return 1;
   }
 }
 
@@ -684,6 +720,9 @@ int psr_mesh_slab_nodes(PsrMesh* m, int32_t start, int32_t width, int64_t* nodes
     return 0;
   } catch (const PsrError& e) {
     return fail(e);
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return 1;
   }
 }
 
@@ -694,6 +733,9 @@ int psr_mesh_slab_points(PsrMesh* m, int32_t start, int32_t width, float* points
     return 0;
   } catch (const PsrError& e) {
     return fail(e);
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return 1;
   }
 }
 
@@ -704,6 +746,9 @@ int psr_mesh_slab_values(PsrMesh* m, int32_t start, int32_t width, const float* 
     return 0;
   } catch (const PsrError& e) {
     return fail(e);
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return 1;
   }
 }
 
@@ -716,6 +761,9 @@ int psr_mesh_select(PsrMesh* m, int64_t* retained, int32_t* invalid) {
     return 0;
   } catch (const PsrError& e) {
     return fail(e);
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return 1;
   }
 }
 
@@ -726,6 +774,9 @@ int psr_mesh_corner_points(PsrMesh* m, float* points) {
     return 0;
   } catch (const PsrError& e) {
     return fail(e);
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return 1;
   }
 }
 
@@ -739,6 +790,9 @@ int psr_mesh_triangulate(PsrMesh* m, const float* corner_values, int64_t* vertic
     return 0;
   } catch (const PsrError& e) {
     return fail(e);
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return 1;
   }
 }
 
@@ -749,6 +803,9 @@ int psr_mesh_copy(PsrMesh* m, float* vertices, int32_t* indices) {
     return 0;
   } catch (const PsrError& e) {
     return fail(e);
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return 1;
   }
 }
 
@@ -759,6 +816,9 @@ int psr_mesh_vertex_vjp(PsrMesh* m, const float* corner_values, const float* gra
     return 0;
   } catch (const PsrError& e) {
     return fail(e);
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return 1;
   }
 }
 
@@ -771,6 +831,9 @@ int psr_mesh_case_table(int8_t* counts, int8_t* edges) {
     return 0;
   } catch (const PsrError& e) {
     return fail(e);
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return 1;
   }
 }
 
@@ -801,6 +864,9 @@ int psr_tree_metadata(PsrResult* result, void* stream, PsrTreeMetadata** out) {
     return 0;
   } catch (const PsrError& e) {
     return fail(e);
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return 1;
   }
 }
 
@@ -836,10 +902,46 @@ int psr_tree_metadata_copy(const PsrTreeMetadata* m, int32_t index, int64_t* key
     return 0;
   } catch (const PsrError& e) {
     return fail(e);
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return 1;
   }
 }
 
 void psr_tree_metadata_free(PsrTreeMetadata* m) { delete m; }
+
+int psr_field_level_offsets(const int64_t* keys, int64_t count, int depth, int32_t* offsets, void* stream) {
+  try {
+    if (depth < 0 || depth > 15) throw PsrError(2, "Keys support depths 0..15, including upper boundary nodes");
+    if (count < 0 || count > INT32_MAX) throw PsrError(2, "key count must lie in [0, 2**31)");
+    if (!offsets || (count && !keys)) throw PsrError(2, "null argument");
+    hipLaunchKernelGGL(k_level_offsets, dim3(1), dim3(64), 0, (hipStream_t)stream, keys, count, depth, offsets);
+    KERNEL_CHECK();
+    return 0;
+  } catch (const PsrError& e) {
+    return fail(e);
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return 1;
+  }
+}
+
+int psr_isovalue(int dtype, const void* values, const void* weights, int64_t n, void* out, void* stream) {
+  try {
+    if (dtype != 0 && dtype != 1) throw PsrError(3, "dtype must be 0 (float32) or 1 (float64)");
+    if (n < 0) throw PsrError(2, "value count must be nonnegative");
+    if (!out || (n && (!values || !weights))) throw PsrError(2, "null argument");
+    hipStream_t s = (hipStream_t)stream;
+    if (dtype == 0) compute_isovalue((const float*)values, (const float*)weights, n, (float*)out, s);
+    else compute_isovalue((const double*)values, (const double*)weights, n, (double*)out, s);
+    return 0;
+  } catch (const PsrError& e) {
+    return fail(e);
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return 1;
+  }
+}
 
 const char* psr_last_error(void) { return g_error.c_str(); }
 

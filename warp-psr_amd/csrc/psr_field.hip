@@ -187,6 +187,30 @@ __global__ void k_iso_serial(const T* __restrict__ v, const T* __restrict__ w, i
 
 }  // namespace
 
+template <class T>
+void isovalue_launch(const T* v, const T* w, int64_t G, double* tp, double* mp, T* out, hipStream_t s) {
+  if (G <= 1024) {
+    hipLaunchKernelGGL(k_iso_serial<T>, dim3(1), dim3(64), 0, s, v, w, G, out);
+  } else {
+    const int64_t blocks = isovalue_partials(G);
+    hipLaunchKernelGGL(k_iso_partials<T>, dim3((unsigned)blocks), dim3(256), 0, s, v, w, G, tp, mp);
+    KERNEL_CHECK();
+    hipLaunchKernelGGL(k_iso_finish<T>, dim3(1), dim3(256), 0, s, (const double*)tp, (const double*)mp, blocks, out);
+  }
+  KERNEL_CHECK();
+}
+
+template <class T> void compute_isovalue(const T* v, const T* w, int64_t G, T* out, hipStream_t s) {
+  const int64_t blocks = G > 1024 ? isovalue_partials(G) : 0;
+  DArray<double> tp(blocks, s), mp(blocks, s);
+  isovalue_launch(v, w, G, tp.data(), mp.data(), out, s);
+}
+
+template void isovalue_launch<float>(const float*, const float*, int64_t, double*, double*, float*, hipStream_t);
+template void isovalue_launch<double>(const double*, const double*, int64_t, double*, double*, double*, hipStream_t);
+template void compute_isovalue<float>(const float*, const float*, int64_t, float*, hipStream_t);
+template void compute_isovalue<double>(const double*, const double*, int64_t, double*, hipStream_t);
+
 // transforms.py:13-52 (cofactor inverse in the input precision)
 template <class T> static T det3(const T m[3][3]) {
   return m[0][0] * (m[1][1] * m[2][2] - m[1][2] * m[2][1]) - m[0][1] * (m[1][0] * m[2][2] - m[1][2] * m[2][0]) +
@@ -292,19 +316,7 @@ template <class T> void stage_field(Result<T>& r) {
                      (T*)nullptr, (int*)nullptr);
   KERNEL_CHECK();
   DArray<T> iso(1, s);
-  if (G <= 1024) {
-    hipLaunchKernelGGL(k_iso_serial<T>, dim3(1), dim3(64), 0, s, (const T*)r.centroid_values.data(), (const T*)r.weights.data(),
-                       (int64_t)G, iso.data());
-  } else {
-    const int64_t blocks = (G + 1023) / 1024;
-    DArray<double> tp(blocks, s), mp(blocks, s);
-    hipLaunchKernelGGL(k_iso_partials<T>, dim3((unsigned)blocks), dim3(256), 0, s, (const T*)r.centroid_values.data(),
-                       (const T*)r.weights.data(), (int64_t)G, tp.data(), mp.data());
-    KERNEL_CHECK();
-    hipLaunchKernelGGL(k_iso_finish<T>, dim3(1), dim3(256), 0, s, (const double*)tp.data(), (const double*)mp.data(), blocks,
-                       iso.data());
-  }
-  KERNEL_CHECK();
+  compute_isovalue((const T*)r.centroid_values.data(), (const T*)r.weights.data(), (int64_t)G, iso.data(), s);
   read_bytes(&r.iso, iso.data(), sizeof(T), s);
   // transforms.py:54-79 (host scalar work)
   const T sc = r.transform[0];

@@ -386,10 +386,42 @@ void compact_heads(const int* heads, const int* prefix, int64_t n, int* head_pos
 }
 
 namespace {
+// 64 KiB pinned staging buffers for small device-to-host reads, one per host
+// thread.  A thread returns its buffer to a process-wide free list when it
+// ends; the buffers and the list are never freed, so no HIP call runs from a
+// thread-local or static destructor (those can run after the HIP runtime has
+// shut down at process exit).
+struct PinnedList {
+  std::mutex mu;
+  std::vector<void*> free;
+};
+PinnedList& pinned_list() {
+  static PinnedList* list = new PinnedList();  // leaked on purpose
+  return *list;
+}
 struct PinnedScratch {
   void* p = nullptr;
-  PinnedScratch() { (void)hipHostMalloc(&p, 1 << 16, hipHostMallocDefault); }
-  ~PinnedScratch() { if (p) (void)hipHostFree(p); }
+  PinnedScratch() {
+    PinnedList& l = pinned_list();
+    {
+      std::lock_guard<std::mutex> g(l.mu);
+      if (!l.free.empty()) {
+        p = l.free.back();
+        l.free.pop_back();
+        return;
+      }
+    }
+    if (hipHostMalloc(&p, 1 << 16, hipHostMallocDefault) != hipSuccess) {
+      (void)hipGetLastError();
+      p = nullptr;
+    }
+  }
+  ~PinnedScratch() {
+    if (!p) return;
+    PinnedList& l = pinned_list();
+    std::lock_guard<std::mutex> g(l.mu);
+    l.free.push_back(p);
+  }
 };
 thread_local PinnedScratch g_pinned;
 }  // namespace

@@ -13,6 +13,7 @@ struct Params {
   double cg_accuracy = 1e-3;
   int keep_intermediates = 1;  // keep per-stage arrays for diagnostics
   int float_operators = 0;     // finer-level operators stored as float (double runs)
+  int relax_tile = 0;          // tiled relaxation side (0 default, -1 colour-per-launch form)
 };
 
 // 27-slot stencil matrix (slot k <-> neighbour offset (k/9-1, k/3%3-1, k%3-1)
@@ -119,6 +120,9 @@ template <class T> struct Result {
     int launches;
     hipEvent_t start, end;
     float seconds;
+    // tiled form: tile side, threads per colour, tiles and tile entries
+    // (0 for the colour-per-launch form)
+    int tile = 0, width = 0, tiles = 0, entries = 0, max_colour_rows = 0;
   };
   std::vector<RelaxStat> relax_stats;
   // device nonzero counters of the relaxed levels, read once the run is done
@@ -185,6 +189,15 @@ void solve_backward(Result<T>& r, const double* gcoef, double* grecords, hipStre
 template <class T>
 void screening_backward(Result<T>& r, const SolveGrads& mg, std::vector<DArray<double>>& gagg, double* gavg1, hipStream_t s);
 template <class T> void stage_solve(Result<T>& r);
+// reconstruction.py:101-116 (_compute_isovalue): *out (device) = the
+// weighted mean sum(double(v*w)) / sum(double(w)) of G aggregates, as one
+// serial double chain for G <= 1024, else double partials of 1024-element
+// tiles (256 lanes, elements lane + 256k in order, then a halving tree)
+// combined the same way.  isovalue_launch takes the partial buffers (tp, mp:
+// isovalue_partials(G) doubles each); compute_isovalue allocates them.
+inline int64_t isovalue_partials(int64_t G) { return (G + 1023) / 1024; }
+template <class T> void isovalue_launch(const T* v, const T* w, int64_t G, double* tp, double* mp, T* out, hipStream_t s);
+template <class T> void compute_isovalue(const T* v, const T* w, int64_t G, T* out, hipStream_t s);
 template <class T> void stage_field(Result<T>& r);
 template <class T>
 void evaluate_field(const Result<T>& r, const T* queries, int64_t m, const T* linear, const T* translation, T* values,

@@ -1290,6 +1290,257 @@ __global__ void k_diag_sched(const int* __restrict__ order, int R, const int* __
   inv_diag[j] = diagonal != T(0) ? T(1) / diagonal : T(0);
 }
 
+// ---------------------------------------------------------------- tiled sliced relaxation
+// Wave q of relaxation.py:128-143 updates slices two apart (visit t of slice
+// z at q = 2t + 2^d - z), so slices z - 1 and z + 1 are constant during the
+// wave.  One launch per wave: each workgroup takes one ts x ts (x, y) tile of
+// one slice and runs its four colours in order with workgroup barriers
+// between them.  Colour c at Chebyshev ring r outside the tile reads colours
+// < c at ring r + 1, so the workgroup recomputes colour c on rings up to
+// 3 - c and stores only its interior rows.  Old values of slices z - 1, z and
+// z + 1 on rings up to 4 are staged in three LDS grids ((ts + 8)^2 cells
+// each), so a row update reads its 27 neighbours from LDS by offset and needs
+// no column indices: absent neighbours have no grid entry (the grid is
+// cleared to +0) and their operator slots hold +0, so their term
+// r - (+0)(+0) leaves r unchanged, bit for bit, as skipping it does.  Every
+// update keeps its inputs and slot order, so the result equals the
+// colour-per-launch form.  Two value buffers keep the loads and stores of one
+// wave apart: after t visits a slice lives in buffer (iters - t) % 2, so
+// visit t of slice z reads slices z - 1 and z from buffer (iters - t) % 2 and
+// slice z + 1 from the other one, which it writes; the last visit lands in
+// buffer 0.
+//
+// Tile tables, built per level from the schedule: entries (schedule
+// position, cell | layer << 13 | interior << 15) grouped per tile by category
+// -- rows computed for colour 0..3 (categories 0..3), then the staged values
+// of slice z (4), z - 1 (5) and z + 1 (6) -- and per tile the category starts
+// and the end (8 ints).
+constexpr int TILE_CATEGORIES = 7;
+
+__device__ __forceinline__ int ring_of(int v, int t0, int ts) {
+  const int lo = t0 - v, hi = v - (t0 + ts - 1);
+  return lo > 0 ? lo : (hi > 0 ? hi : 0);
+}
+
+// flags[(z * nt + y / ts) * nt + x / ts] = 1 for every row (tile has interior rows)
+__global__ void k_tile_flags(const int* __restrict__ order, const int64_t* __restrict__ keys, int R, int ts, int nt,
+                             int* __restrict__ flags) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= R) return;
+  const int64_t k = keys[order[j]];
+  flags[((int64_t)key_z(k) * nt + key_y(k) / ts) * nt + key_x(k) / ts] = 1;
+}
+
+// The entries of row j in the tiles (with interior rows) of slices z - 1, z
+// and z + 1 whose ring-4 region holds it: counts, or the sort keys (compact
+// tile << 3 | category) and values (j) from pos[j] on.
+template <bool EMIT>
+__global__ void k_tile_members(const int* __restrict__ order, const int64_t* __restrict__ keys, int R, int n, int ts, int nt,
+                               const int* __restrict__ flags, const int* __restrict__ compact, int* __restrict__ count,
+                               uint32_t* __restrict__ out_keys, int* __restrict__ out_vals) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= R) return;
+  const int64_t k = keys[order[j]];
+  const int x = key_x(k), y = key_y(k), z = key_z(k);
+  const int colour = (x & 1) + 2 * (y & 1);
+  const int tx0 = max(0, (x - 4) / ts), tx1 = min(nt - 1, (x + 4) / ts);
+  const int ty0 = max(0, (y - 4) / ts), ty1 = min(nt - 1, (y + 4) / ts);
+  int c = 0, pos = EMIT ? count[j] : 0;
+  auto emit = [&](int64_t dense, int cat) {
+    if (EMIT) {
+      out_keys[pos] = ((uint32_t)compact[dense] << 3) | (uint32_t)cat;
+      out_vals[pos] = j;
+      ++pos;
+    }
+    ++c;
+  };
+  for (int dz = -1; dz <= 1; ++dz) {  // the tile's slice is z + dz
+    const int zt = z + dz;
+    if (zt < 0 || zt > n) continue;
+    for (int ty = ty0; ty <= ty1; ++ty)
+      for (int tx = tx0; tx <= tx1; ++tx) {
+        const int r = max(ring_of(x, tx * ts, ts), ring_of(y, ty * ts, ts));
+        const int64_t dense = ((int64_t)zt * nt + ty) * nt + tx;
+        if (r > 4 || !flags[dense]) continue;
+        if (dz == 0) {
+          if (r <= 3 - colour) emit(dense, colour);
+          emit(dense, 4);
+        } else {
+          emit(dense, dz > 0 ? 5 : 6);  // the tile's slice z - 1 (5) or z + 1 (6)
+        }
+      }
+  }
+  if (!EMIT) count[j] = c;
+}
+
+__global__ void k_tile_dense_of(const int* __restrict__ flags, const int* __restrict__ compact, int64_t ndense,
+                                int* __restrict__ dense_of) {
+  const int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (d < ndense && flags[d]) dense_of[compact[d]] = (int)d;
+}
+
+// Per tile: category starts (lower bounds of tile << 3 | c, c = 0..7);
+// stats[0] = most computed rows of a tile, stats[1] = most staged values.
+__global__ void k_tile_desc(const uint32_t* __restrict__ skeys, int E, int tiles, int* __restrict__ desc,
+                            int* __restrict__ stats) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= tiles) return;
+  int cs[8];
+  for (int c = 0; c < 8; ++c) {
+    const uint32_t key = ((uint32_t)t << 3) | (uint32_t)c;
+    int lo = 0, hi = E;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (skeys[mid] < key) lo = mid + 1; else hi = mid;
+    }
+    cs[c] = lo;
+    desc[8 * t + c] = lo;
+  }
+  atomicMax(&stats[0], cs[4] - cs[0]);
+  atomicMax(&stats[1], cs[7] - cs[4]);
+}
+
+__global__ void k_tile_entries(const uint32_t* __restrict__ skeys, const int* __restrict__ svals, int E,
+                               const int* __restrict__ dense_of, const int* __restrict__ order,
+                               const int64_t* __restrict__ keys, int ts, int nt, int2* __restrict__ entries) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= E) return;
+  const int t = (int)(skeys[e] >> 3), cat = (int)(skeys[e] & 7u), j = svals[e];
+  const int d = dense_of[t];
+  const int ox = (d % nt) * ts - 4, oy = ((d / nt) % nt) * ts - 4, g = ts + 8;
+  const int64_t k = keys[order[j]];
+  const int lx = key_x(k) - ox, ly = key_y(k) - oy;
+  const int layer = cat == 5 ? 0 : (cat == 6 ? 2 : 1);
+  const int interior = (cat < 4 && lx >= 4 && lx < 4 + ts && ly >= 4 && ly < 4 + ts) ? 1 : 0;
+  entries[e] = make_int2(j, (ly * g + lx) | (layer << 13) | (interior << 15));
+}
+
+// first tile of each slice z (z = 0..n + 1) from the exclusive scan of the flags
+__global__ void k_tile_slice_start(const int* __restrict__ compact, int n, int per_slice, int* __restrict__ out) {
+  const int z = blockIdx.x * blockDim.x + threadIdx.x;
+  if (z <= n + 1) out[z] = compact[(int64_t)z * per_slice];
+}
+
+template <class T, class V> struct TiledArgs {
+  const int2* entries;
+  const int* desc;
+  const V* vals;
+  const T* b;
+  const T* inv_diag;
+  const unsigned char* has_diag;
+  T* x0;
+  T* x1;
+  int R;
+  int g;  // grid side, ts + 8
+};
+
+// The slices of one wave: tiles [start[s], end[s]) of slice z0 + 2s; bit s of
+// flip selects buffer 1 as the slice's read buffer.
+struct WaveTiles {
+  int start[RELAX_MAX_SLICES], end[RELAX_MAX_SLICES];
+  unsigned flip;
+};
+
+// One computed row: operator values, right-hand side and diagonal factor.
+template <class T, class V> struct TiledRow {
+  int j, cell, colour, interior;
+  V v[27];
+  T b, inv;
+  unsigned char has;
+};
+
+template <class T, class V>
+__device__ __forceinline__ void tiled_fetch(const TiledArgs<T, V>& a, int2 en, int colour, TiledRow<T, V>& w) {
+  w.j = en.x;
+  w.cell = en.y & 8191;
+  w.interior = (en.y >> 15) & 1;
+  w.colour = colour;
+  const size_t R = (size_t)a.R;
+  const unsigned j = (unsigned)en.x;
+#pragma unroll
+  for (int k = 0; k < 27; ++k) w.v[k] = (a.vals + k * R)[j];
+  w.b = a.b[j];
+  w.inv = a.inv_diag[j];
+  w.has = a.has_diag[j];
+}
+
+template <class T, class V>
+__device__ __forceinline__ void tiled_update(T* grid, int g, int plane, T* Q, const TiledRow<T, V>& w) {
+  const T* mid = grid + plane + w.cell;
+  T r = w.b;
+#pragma unroll
+  for (int k = 0; k < 27; ++k) {
+    const int dx = k / 9 - 1, dy = (k / 3) % 3 - 1, dz = k % 3 - 1;
+    const T t = r - mid[dz * plane + dy * g + dx] * T(w.v[k]);
+    r = t;
+  }
+  const T xi = mid[0];
+  const T nv = w.has ? xi + r * w.inv : xi;
+  grid[plane + w.cell] = nv;
+  if (w.interior) Q[w.j] = nv;
+}
+
+// BS threads per tile: thread i computes the tile's i-th row (rows of colour
+// c are consecutive, so a colour's rows sit on consecutive threads) and
+// stages grid values i, i + BS, ...  Loads go out in dependency steps:
+// descriptor; entries; operator rows and staged values.  Tiles with more
+// than BS computed rows take further passes for the rest.
+constexpr int TILED_STAGE_UNROLL = 4;
+
+template <class T, class V, int BS>
+__global__ void __launch_bounds__(BS) k_relax_tiled(TiledArgs<T, V> a, WaveTiles wv) {
+  extern __shared__ unsigned char lds_raw[];
+  T* grid = (T*)lds_raw;
+  const int s = blockIdx.y;
+  const int tile = wv.start[s] + (int)blockIdx.x;
+  if (tile >= wv.end[s]) return;
+  const bool flip = (wv.flip >> s) & 1u;
+  const T* P = flip ? a.x1 : a.x0;
+  T* Q = flip ? a.x0 : a.x1;
+  const int4 d0 = *(const int4*)(a.desc + 8 * tile);
+  const int4 d1 = *(const int4*)(a.desc + 8 * tile + 4);
+  const int tid = (int)threadIdx.x, g = a.g, plane = g * g;
+  const int c0 = d0.x, c1 = d0.y, c2 = d0.z, c3 = d0.w, stage0 = d1.x, end = d1.w;
+  for (int i = tid; i < 3 * plane; i += BS) grid[i] = T(0);
+  // this thread's first computed row, and its first staged values
+  const int e = c0 + tid;
+  const bool have = e < stage0;
+  const int colour = e < c1 ? 0 : (e < c2 ? 1 : (e < c3 ? 2 : 3));
+  TiledRow<T, V> w;
+  int2 sen[TILED_STAGE_UNROLL];
+  T sval[TILED_STAGE_UNROLL];
+  const int2 en = have ? a.entries[e] : make_int2(0, 0);
+#pragma unroll
+  for (int u = 0; u < TILED_STAGE_UNROLL; ++u) {
+    const int se = stage0 + tid + u * BS;
+    sen[u] = se < end ? a.entries[se] : make_int2(-1, 0);
+  }
+  if (have) tiled_fetch(a, en, colour, w);
+#pragma unroll
+  for (int u = 0; u < TILED_STAGE_UNROLL; ++u)
+    if (sen[u].x >= 0) sval[u] = (((sen[u].y >> 13) & 3) == 2 ? (const T*)Q : P)[(unsigned)sen[u].x];
+  __syncthreads();  // grid cleared
+#pragma unroll
+  for (int u = 0; u < TILED_STAGE_UNROLL; ++u)
+    if (sen[u].x >= 0) grid[((sen[u].y >> 13) & 3) * plane + (sen[u].y & 8191)] = sval[u];
+  for (int se = stage0 + tid + TILED_STAGE_UNROLL * BS; se < end; se += BS) {
+    const int2 ge = a.entries[se];
+    grid[((ge.y >> 13) & 3) * plane + (ge.y & 8191)] = (((ge.y >> 13) & 3) == 2 ? (const T*)Q : P)[(unsigned)ge.x];
+  }
+  __syncthreads();  // grid staged
+  const int cstart[5] = {c0, c1, c2, c3, stage0};
+  for (int phase = 0; phase < 4; ++phase) {
+    if (have && colour == phase) tiled_update(grid, g, plane, Q, w);
+    for (int e2 = c0 + tid + BS; e2 < stage0; e2 += BS) {
+      if (e2 < cstart[phase] || e2 >= cstart[phase + 1]) continue;
+      TiledRow<T, V> w2;
+      tiled_fetch(a, a.entries[e2], phase, w2);
+      tiled_update(grid, g, plane, Q, w2);
+    }
+    __syncthreads();
+  }
+}
+
 inline int floordiv(int a, int b) { return (a >= 0) ? a / b : -((-a + b - 1) / b); }
 
 }  // namespace
@@ -1503,13 +1754,14 @@ int relax_sliced_graph(const GsOperator<T, V>& op, const Schedule<T>& sc, T* x, 
   return (int)g->nodes.size();
 }
 
-// PSR_RELAX=wave selects direct launches (same bits); the default replays a
-// captured graph of the launch sequence.
+// PSR_RELAX=wave selects direct colour-per-launch launches, PSR_RELAX=graph
+// their captured graph; the default is the tiled form (one launch per wave)
+// where a level has tile tables.  All give the same bits.
 static int relax_mode() {
   static int mode = -1;
   if (mode < 0) {
     const char* e = std::getenv("PSR_RELAX");
-    mode = (e && std::string(e) == "wave") ? 1 : 0;
+    mode = (e && std::string(e) == "wave") ? 1 : (e && std::string(e) == "graph") ? 2 : 0;
   }
   return mode;
 }
@@ -1518,10 +1770,200 @@ template <class T, class V>
 int relax_sliced_any(const GsOperator<T, V>& op, const Schedule<T>& sc, T* x, int depth, int iters, hipStream_t s) {
   switch (relax_mode()) {
     case 1: return relax_sliced(op, sc, x, depth, iters, s);
+    case 2:
     default:
       if (iters > RELAX_MAX_SLICES) return relax_sliced(op, sc, x, depth, iters, s);
       return relax_sliced_graph(op, sc, x, depth, iters, s);
   }
+}
+
+// ---------------------------------------------------------------- tiled sliced relaxation (host)
+// The tile tables of one level (see k_relax_tiled).
+struct TileTable {
+  int ts = 0, nt = 0, tiles = 0, entries_count = 0;
+  DArray<int2> entries;
+  DArray<int> desc;
+  std::vector<int> slice_start;  // first tile of slice z, z = 0..2^d + 1
+  int max_tiles = 0;             // most tiles of one slice
+  int max_rows = 0;              // most computed rows of one tile
+  int max_staged = 0;            // most staged values of one tile
+};
+
+// Tile side for a level: the run's relax_tile (8..40, which keeps the three
+// double grids within 64 KiB of LDS) or, for 0, the library default; -1 (or
+// the PSR_RELAX environment choice of a colour-per-launch form) selects no
+// tiles.
+constexpr int RELAX_TILE_MAX = 40;
+static int relax_tile_size(int requested, int depth) {
+  if (requested < 0 || relax_mode() != 0) return 0;
+  if (requested >= 8 && requested <= RELAX_TILE_MAX) return requested;
+  (void)depth;
+  return 12;
+}
+
+static void build_tile_table(const int64_t* keys, const int* order, int R, int depth, int ts, TileTable& tt,
+                             hipStream_t s) {
+  const int n = 1 << depth;
+  tt.ts = ts;
+  tt.nt = (n + 1 + ts - 1) / ts;
+  const int64_t per_slice = (int64_t)tt.nt * tt.nt;
+  const int64_t ndense = per_slice * (n + 1);
+  if (ndense + 1 > INT32_MAX) throw PsrError(1, "tiled relaxation: too many tiles");
+  DArray<int> flags(ndense + 1, s), compact(ndense + 1, s), total(1, s);
+  flags.zero();
+  hipLaunchKernelGGL(k_tile_flags, dim3(blocks_for(R, 256)), dim3(256), 0, s, order, keys, R, ts, tt.nt, flags.data());
+  KERNEL_CHECK();
+  scan_exclusive_i32(flags.data(), compact.data(), ndense + 1, total.data(), s);
+  DArray<int> count(R + 1, s), pos(R + 1, s), etotal(1, s);
+  hipLaunchKernelGGL(k_tile_members<false>, dim3(blocks_for(R, 256)), dim3(256), 0, s, order, keys, R, n, ts, tt.nt,
+                     (const int*)flags.data(), (const int*)compact.data(), count.data(), (uint32_t*)nullptr, (int*)nullptr);
+  KERNEL_CHECK();
+  HIP_CHECK(hipMemsetAsync(count.data() + R, 0, sizeof(int), s));
+  scan_exclusive_i32(count.data(), pos.data(), R + 1, etotal.data(), s);
+  int sizes[2];
+  HIP_CHECK(hipMemcpyAsync(&sizes[0], total.data(), sizeof(int), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipMemcpyAsync(&sizes[1], etotal.data(), sizeof(int), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  tt.tiles = sizes[0];
+  const int E = sizes[1];
+  tt.entries_count = E;
+  DArray<uint32_t> skeys(std::max(E, 1), s);
+  DArray<int> svals(std::max(E, 1), s);
+  hipLaunchKernelGGL(k_tile_members<true>, dim3(blocks_for(R, 256)), dim3(256), 0, s, order, keys, R, n, ts, tt.nt,
+                     (const int*)flags.data(), (const int*)compact.data(), pos.data(), skeys.data(), svals.data());
+  KERNEL_CHECK();
+  sort_pairs_u32(skeys.data(), svals.data(), E, bits_for((uint64_t)tt.tiles * 8), s);
+  DArray<int> dense_of(std::max(tt.tiles, 1), s);
+  hipLaunchKernelGGL(k_tile_dense_of, dim3(blocks_for(ndense, 256)), dim3(256), 0, s, (const int*)flags.data(),
+                     (const int*)compact.data(), ndense, dense_of.data());
+  KERNEL_CHECK();
+  // slice starts and stats in one buffer, read once
+  DArray<int> small(n + 2 + 2, s);
+  HIP_CHECK(hipMemsetAsync(small.data() + n + 2, 0, 2 * sizeof(int), s));
+  tt.desc.reset(8 * (size_t)std::max(tt.tiles, 1), s);
+  hipLaunchKernelGGL(k_tile_desc, dim3(blocks_for(tt.tiles, 256)), dim3(256), 0, s, (const uint32_t*)skeys.data(), E, tt.tiles,
+                     tt.desc.data(), small.data() + n + 2);
+  KERNEL_CHECK();
+  tt.entries.reset(std::max(E, 1), s);
+  hipLaunchKernelGGL(k_tile_entries, dim3(blocks_for(E, 256)), dim3(256), 0, s, (const uint32_t*)skeys.data(),
+                     (const int*)svals.data(), E, (const int*)dense_of.data(), order, keys, ts, tt.nt, tt.entries.data());
+  KERNEL_CHECK();
+  hipLaunchKernelGGL(k_tile_slice_start, dim3(blocks_for(n + 2, 256)), dim3(256), 0, s, (const int*)compact.data(), n,
+                     (int)per_slice, small.data());
+  KERNEL_CHECK();
+  std::vector<int> h(n + 4);
+  read_bytes(h.data(), small.data(), sizeof(int) * (n + 4), s);
+  tt.slice_start.assign(h.begin(), h.begin() + n + 2);
+  tt.max_rows = h[n + 2];
+  tt.max_staged = h[n + 3];
+  tt.max_tiles = 0;
+  for (int z = 0; z <= n; ++z) tt.max_tiles = std::max(tt.max_tiles, tt.slice_start[z + 1] - tt.slice_start[z]);
+}
+
+// Threads per tile: enough for most tiles' computed rows in one pass (256 at
+// most; larger tiles take further passes).
+static int tiled_width(const TileTable& tt) { return tt.max_rows <= 64 ? 64 : tt.max_rows <= 128 ? 128 : 256; }
+
+// The wave sequence of the tiled relaxation as a graph of kernel nodes (one
+// per wave), cached per (precision, depth, iters, width, grid x, device,
+// caller stream); each call writes its buffers and the waves' tile ranges
+// into the node arguments, as relax_sliced_graph does.
+struct TiledGraph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<hipGraphNode_t> nodes;
+  std::vector<int> first_z, slices, first_t;  // per node
+  hipKernelNodeParams params{};
+  hipEvent_t done = nullptr;
+  std::mutex mu;
+};
+
+template <class T, class V, int BS> static void* tiled_kernel() {
+  return reinterpret_cast<void*>(&k_relax_tiled<T, V, BS>);
+}
+
+template <class T, class V>
+static std::unique_ptr<TiledGraph> build_tiled_graph(int n, int iters, int width, unsigned bx, size_t lds) {
+  auto g = std::make_unique<TiledGraph>();
+  HIP_CHECK(hipGraphCreate(&g->graph, 0));
+  TiledArgs<T, V> a{};
+  WaveTiles wv{};
+  void* args[2] = {&a, &wv};
+  hipKernelNodeParams& kp = g->params;
+  kp.func = width == 64 ? tiled_kernel<T, V, 64>() : width == 128 ? tiled_kernel<T, V, 128>() : tiled_kernel<T, V, 256>();
+  kp.blockDim = dim3(width);
+  kp.sharedMemBytes = (unsigned)lds;
+  kp.extra = nullptr;
+  hipGraphNode_t prev = nullptr;
+  for (int q = 0; q <= n + 2 * (iters - 1); ++q) {
+    const int first = std::max(0, floordiv(q - n + 1, 2));
+    const int last = std::min(iters - 1, q / 2);
+    if (last < first) continue;
+    kp.gridDim = dim3(bx, (unsigned)(last - first + 1));
+    kp.kernelParams = args;
+    hipGraphNode_t node = nullptr;
+    HIP_CHECK(hipGraphAddKernelNode(&node, g->graph, prev ? &prev : nullptr, prev ? 1 : 0, &kp));
+    g->nodes.push_back(node);
+    g->first_z.push_back(n - q + 2 * first);
+    g->slices.push_back(last - first + 1);
+    g->first_t.push_back(first);
+    prev = node;
+  }
+  HIP_CHECK(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+  HIP_CHECK(hipEventCreateWithFlags(&g->done, hipEventDisableTiming));
+  return g;
+}
+
+// relax_sliced on the tile tables: x (schedule order) holds the start values
+// and receives the result; x1 is a second buffer of R values.  The operator's
+// absent slots must hold +0 (k_level_operator stores them so).
+template <class T, class V>
+int relax_tiled_graph(const GsOperator<T, V>& op, const TileTable& tt, T* x, T* x1, int depth, int iters, hipStream_t s) {
+  static std::mutex mu;
+  static std::map<std::tuple<int, int, int, int, unsigned, int, hipStream_t>, std::unique_ptr<TiledGraph>> cache;
+  const int n = 1 << depth;
+  if (iters > RELAX_MAX_SLICES) throw PsrError(1, "tiled relaxation supports at most 16 sweeps");
+  if (op.R <= 0 || tt.tiles <= 0) return 0;
+  const int width = tiled_width(tt);
+  unsigned bx = 1;
+  while (bx < (unsigned)std::max(1, tt.max_tiles)) bx <<= 1;
+  const int gside = tt.ts + 8;
+  const size_t lds = sizeof(T) * 3 * (size_t)gside * gside;
+  int dev = 0;
+  HIP_CHECK(hipGetDevice(&dev));
+  TiledGraph* g = nullptr;
+  {
+    std::lock_guard<std::mutex> guard(mu);
+    std::unique_ptr<TiledGraph>& slot = cache[std::make_tuple((int)sizeof(V), depth, iters, width, bx, dev, s)];
+    if (!slot) slot = build_tiled_graph<T, V>(n, iters, width, bx, lds);
+    g = slot.get();
+  }
+  // the first visit reads buffer iters % 2
+  if (iters % 2) HIP_CHECK(hipMemcpyAsync(x1, x, sizeof(T) * op.R, hipMemcpyDeviceToDevice, s));
+  std::lock_guard<std::mutex> guard(g->mu);
+  HIP_CHECK(hipEventSynchronize(g->done));
+  TiledArgs<T, V> a{tt.entries.data(), tt.desc.data(), op.vals.data(), op.b.data(), op.inv_diag.data(),
+                    op.has_diag.data(), x, x1, op.R, gside};
+  WaveTiles wv;
+  void* args[2] = {&a, &wv};
+  hipKernelNodeParams kp = g->params;
+  kp.kernelParams = args;
+  kp.sharedMemBytes = (unsigned)lds;
+  for (size_t i = 0; i < g->nodes.size(); ++i) {
+    for (int t = 0; t < RELAX_MAX_SLICES; ++t) wv.start[t] = wv.end[t] = 0;
+    wv.flip = 0;
+    for (int k = 0; k < g->slices[i]; ++k) {
+      const int z = g->first_z[i] + 2 * k, t = g->first_t[i] + k;
+      wv.start[k] = tt.slice_start[z];
+      wv.end[k] = tt.slice_start[z + 1];
+      if ((iters - t) % 2) wv.flip |= 1u << k;
+    }
+    kp.gridDim = dim3(bx, (unsigned)g->slices[i]);
+    HIP_CHECK(hipGraphExecKernelNodeSetParams(g->exec, g->nodes[i], &kp));
+  }
+  HIP_CHECK(hipGraphLaunch(g->exec, s));
+  HIP_CHECK(hipEventRecord(g->done, s));
+  return (int)g->nodes.size();
 }
 
 // x of up to 8,192 double (16,384 float) rows stays in LDS (64 KiB)
@@ -2210,6 +2652,7 @@ template <class T, class V> struct FinerLevel {
   Schedule<T> sc;
   GsOperator<T, V> op;  // cols (schedule positions), vals, inv_diag, has_diag; b is set by the solve
   DArray<unsigned long long> nnz;  // stored nonzeros, read after the run
+  TileTable tiles;                 // tiled relaxation (empty when the colour-per-launch form runs)
 };
 
 template <class T, class V>
@@ -2219,6 +2662,8 @@ static void prepare_finer(Result<T>& r, int d, Stencil<V>& Aop, Stencil<V>& Cop,
   const int R = L.basis_count;
   f.sc = make_schedule<T>(L.basis_keys, R, d, true, s);
   const int* ord = f.sc.order.data();
+  const int ts = relax_tile_size(r.prm.relax_tile, d);
+  if (ts && R && r.prm.iters <= RELAX_MAX_SLICES) build_tile_table(L.basis_keys.data(), ord, R, d, ts, f.tiles, s);
   L.point_load.reset(R, s);
   if (!screened) L.point_load.zero();
   PointSet<T> pts;
@@ -2269,10 +2714,23 @@ static void solve_finer(Result<T>& r, int d, Stencil<V>& Aop, Stencil<V>& Cop, F
   xs.zero();  // the level's correction starts at zero (solver.py:343)
   r.relax_nnz.push_back(std::move(f.nnz));
   typename Result<T>::RelaxStat st{d, R, 0, 0, nullptr, nullptr, 0.f};
+  const bool tiled = f.tiles.tiles > 0;
+  DArray<T> xs1;
+  if (tiled) {
+    xs1.reset(R, s);
+    st.tile = f.tiles.ts;
+    st.width = tiled_width(f.tiles);
+    st.tiles = f.tiles.tiles;
+    st.entries = f.tiles.entries_count;
+    st.max_colour_rows = f.tiles.max_rows;
+  }
   HIP_CHECK(hipEventCreate(&st.start));
   HIP_CHECK(hipEventCreate(&st.end));
   HIP_CHECK(hipEventRecord(st.start, s));
-  st.launches = relax_sliced_any(op, f.sc, xs.data(), d, r.prm.iters, s);
+  if (tiled)
+    st.launches = relax_tiled_graph(op, f.tiles, xs.data(), xs1.data(), d, r.prm.iters, s);
+  else
+    st.launches = relax_sliced_any(op, f.sc, xs.data(), d, r.prm.iters, s);
   HIP_CHECK(hipEventRecord(st.end, s));
   r.relax_stats.push_back(st);
   hipLaunchKernelGGL(k_scatter_rows<T>, dim3(blocks_for(R, 256)), dim3(256), 0, s, ord, R, (const T*)xs.data(),

@@ -15,7 +15,7 @@ EXPORTED = (
     "psr_reconstruct_batch", "psr_mesh_begin", "psr_mesh_slab_nodes", "psr_mesh_slab_points", "psr_mesh_slab_values",
     "psr_mesh_select", "psr_mesh_corner_points", "psr_mesh_triangulate", "psr_mesh_copy", "psr_mesh_vertex_vjp",
     "psr_mesh_case_table", "psr_mesh_free", "psr_tree_metadata", "psr_tree_metadata_levels", "psr_tree_metadata_level",
-    "psr_tree_metadata_copy", "psr_tree_metadata_free",
+    "psr_tree_metadata_copy", "psr_tree_metadata_free", "psr_field_level_offsets", "psr_isovalue",
 )
 
 
@@ -25,7 +25,7 @@ class PsrParams(ctypes.Structure):
         ("solve_depth", ctypes.c_int), ("kernel_depth", ctypes.c_int), ("samples_per_node", ctypes.c_double),
         ("scale", ctypes.c_double), ("point_weight", ctypes.c_double), ("exact_interpolation", ctypes.c_int),
         ("iters", ctypes.c_int), ("base_v_cycles", ctypes.c_int), ("cg_accuracy", ctypes.c_double),
-        ("keep_intermediates", ctypes.c_int), ("float_operators", ctypes.c_int),
+        ("keep_intermediates", ctypes.c_int), ("float_operators", ctypes.c_int), ("relax_tile", ctypes.c_int),
     ]
 
 
@@ -85,6 +85,10 @@ def load():
     lib.psr_field_vjp_inputs.argtypes = [vp, vp, i64, vp, vp, vp, vp, vp, vp, vp]
     lib.psr_field_evaluate.restype = c_int
     lib.psr_field_evaluate.argtypes = [ctypes.POINTER(PsrField), vp, i64, vp, vp, vp, vp]
+    lib.psr_field_level_offsets.restype = c_int
+    lib.psr_field_level_offsets.argtypes = [vp, i64, c_int, vp, vp]
+    lib.psr_isovalue.restype = c_int
+    lib.psr_isovalue.argtypes = [c_int, vp, vp, i64, vp, vp]
     lib.psr_relax_sliced.restype = c_int
     lib.psr_relax_scheduled.restype = c_int
     i32p, i64p = ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(i64)

@@ -75,9 +75,63 @@ class DeviceField:
 
 
 class LevelField:
-    """Per-level sparse degree-1 field (reference field.py:122-184)."""
+    """Per-level sparse degree-1 field (reference field.py:122-184).
 
-    def __init__(self, result, dtype, device):
+    ``LevelField(keys, coefficients, linear, translation, depth, iso_value,
+    cumulative=None)`` over an int64 key tensor and float32/float64
+    coefficient tensors on one ROCm device, validated as the reference's
+    dataclass (field.py:130-136).  With cumulative coefficients it evaluates
+    the reference evaluation (field.py:364-421, leaf search, boundary nudge);
+    without them the direct level summation (field.py:37-75, left derivatives
+    at knots).  ``Reconstruction.field`` is a LevelField over the result's
+    arrays.  Keep the tensors unmodified while the field or its device view is
+    in use.
+    """
+
+    def __init__(self, keys, coefficients, linear, translation, depth, iso_value, cumulative=None):
+        torch = _torch()
+        if (not isinstance(keys, torch.Tensor) or not isinstance(coefficients, torch.Tensor)
+                or coefficients.dtype not in (torch.float32, torch.float64) or keys.dtype != torch.int64):
+            raise ValueError("Field requires float32/float64 coefficients and int64 keys")
+        if not isinstance(depth, int) or not 0 <= depth <= 15:
+            raise ValueError("Keys support depths 0..15, including upper boundary nodes")
+        if keys.shape != coefficients.shape or keys.ndim != 1:
+            raise ValueError("One key per coefficient is required")
+        if cumulative is not None and (not isinstance(cumulative, torch.Tensor) or cumulative.dtype != coefficients.dtype
+                                       or cumulative.shape != coefficients.shape):
+            raise ValueError("Cumulative coefficients must match coefficient precision and shape")
+        device = coefficients.device
+        if device.type != "cuda" or keys.device != device or (cumulative is not None and cumulative.device != device):
+            raise ValueError("Keys and coefficients must share one ROCm device")
+        linear = np.asarray(linear, dtype=np.float64).reshape(-1)
+        translation = np.asarray(translation, dtype=np.float64).reshape(-1)
+        if linear.size != 9 or translation.size != 3:
+            raise ValueError("linear must hold 9 values and translation 3")
+        self._result = None
+        self.dtype, self.device = coefficients.dtype, device
+        self.keys, self.coefficients = keys.contiguous(), coefficients.contiguous()
+        self.cumulative = None if cumulative is None else cumulative.contiguous()
+        self.depth, self.iso_value = depth, float(iso_value)
+        self.linear, self.translation = tuple(float(v) for v in linear), tuple(float(v) for v in translation)
+        self._offsets = torch.empty(depth + 2, dtype=torch.int32, device=device)
+        with torch.cuda.device(device):
+            _lib.check(_lib.load().psr_field_level_offsets(self.keys.data_ptr(), self.keys.shape[0], depth,
+                                                           self._offsets.data_ptr(), _stream_handle(device)))
+        f = _lib.PsrField()
+        f.keys, f.coefficients = self.keys.data_ptr(), self.coefficients.data_ptr()
+        f.cumulative = None if self.cumulative is None else self.cumulative.data_ptr()
+        f.level_offsets, f.count, f.depth = self._offsets.data_ptr(), self.keys.shape[0], depth
+        f.dtype = 1 if self.dtype == torch.float64 else 0
+        for i in range(9):
+            f.linear[i] = self.linear[i]
+        for i in range(3):
+            f.translation[i] = self.translation[i]
+        f.iso_value = self.iso_value
+        self._field = f
+
+    @classmethod
+    def _from_result(cls, result, dtype, device):
+        self = cls.__new__(cls)
         self._result = result
         self.dtype = dtype
         self.device = device
@@ -88,15 +142,20 @@ class LevelField:
         self.iso_value = field.iso_value
         self.linear = tuple(field.linear)
         self.translation = tuple(field.translation)
+        return self
 
     def evaluate(self, queries, *, requires_grad=False):
         """Values, model-space gradients and validity; NaN outside the cube."""
         torch = _torch()
         if requires_grad:
-            raise ValueError("tracked cumulative fields require reconstruct_differentiable")
+            if self._result is not None or self.cumulative is not None:
+                raise ValueError("tracked cumulative fields require reconstruct_differentiable")
+            raise ValueError("tracked direct-field evaluation is not supported; use reconstruct_differentiable")
         if (not isinstance(queries, torch.Tensor) or queries.dtype != self.dtype or queries.device != self.device
                 or queries.ndim != 2 or queries.shape[1] != 3):
             raise ValueError("Query precision/device must match coefficients")
+        if self._result is None:
+            return self.device_field().evaluate(queries)
         q = queries.contiguous()
         m = q.shape[0]
         values = torch.empty(m, dtype=self.dtype, device=self.device)
@@ -108,7 +167,7 @@ class LevelField:
         return values, gradients, valid
 
     def device_field(self):
-        return DeviceField(self._field, self._result, self.dtype, self.device)
+        return DeviceField(self._field, self._result if self._result is not None else self, self.dtype, self.device)
 
 
 class _ResultHandle:
@@ -129,7 +188,7 @@ class Reconstruction:
 
     def __init__(self, handle, dtype, device, params):
         self._owner = _ResultHandle(handle)
-        self.field = LevelField(self._owner, dtype, device)
+        self.field = LevelField._from_result(self._owner, dtype, device)
         lib = _lib.load()
         need = ctypes.c_size_t()
         lib.psr_result_metadata_json(handle, None, 0, ctypes.byref(need))
@@ -174,7 +233,7 @@ class _Diagnostics:
 
 def _prepare(positions, normals, depth=6, full_depth=3, base_depth=None, solve_depth=None, kernel_depth=None,
              samples_per_node=1.5, scale=1.1, point_weight=2.0, exact_interpolation=False, iters=8, base_v_cycles=1,
-             cg_accuracy=1e-3, mixed=False):
+             cg_accuracy=1e-3, mixed=False, relax_tile=0):
     """Reference validation (reconstruction.py:160-212), then the C parameters."""
     torch = _torch()
     if base_depth is None:
@@ -206,6 +265,8 @@ def _prepare(positions, normals, depth=6, full_depth=3, base_depth=None, solve_d
         raise ValueError("points and normals must have equal nonempty 1D shapes")
     if mixed and positions.dtype != torch.float64:
         raise ValueError("mixed precision (float_operators) runs on float64 inputs")
+    if not isinstance(relax_tile, int) or not (relax_tile in (0, -1) or 8 <= relax_tile <= 40):
+        raise ValueError("relax_tile must be 0 (default), -1 (one launch per colour) or a tile side in [8, 40]")
     params = _lib.PsrParams()
     _lib.load().psr_params_default(ctypes.byref(params))
     params.depth, params.full_depth, params.base_depth = depth, full_depth, base_depth
@@ -214,6 +275,7 @@ def _prepare(positions, normals, depth=6, full_depth=3, base_depth=None, solve_d
     params.exact_interpolation = int(bool(exact_interpolation))
     params.iters, params.base_v_cycles, params.cg_accuracy = iters, base_v_cycles, cg_accuracy
     params.float_operators = int(bool(mixed))
+    params.relax_tile = relax_tile
     resolved = dict(depth=depth, full_depth=full_depth, base_depth=base_depth, solve_depth=solve_depth,
                     kernel_depth=kernel_depth, samples_per_node=samples_per_node, scale=scale,
                     point_weight=point_weight, exact_interpolation=bool(exact_interpolation), iters=iters,
@@ -223,7 +285,7 @@ def _prepare(positions, normals, depth=6, full_depth=3, base_depth=None, solve_d
 
 def reconstruct(positions, normals, *, depth=6, full_depth=3, base_depth=None, solve_depth=None, kernel_depth=None,
                 samples_per_node=1.5, scale=1.1, point_weight=2.0, exact_interpolation=False, iters=8, base_v_cycles=1,
-                cg_accuracy=1e-3, mixed=False):
+                cg_accuracy=1e-3, mixed=False, relax_tile=0):
     """Reconstruct oriented (n, 3) float32/float64 ROCm tensors; point_weight=0 selects PSR.
 
     Degree one, Neumann boundaries, confidence disabled, library alignDir=0.
@@ -231,13 +293,16 @@ def reconstruct(positions, normals, *, depth=6, full_depth=3, base_depth=None, s
     ``mixed=True`` (float64 tensors only) stores the finer levels' Laplacian +
     screening operators as float32 while every vector, the octree stages and
     the base-level solve stay float64.
+    ``relax_tile`` sets the side of the (x, y) tiles of the finer levels'
+    sliced Gauss-Seidel (8..40; 0 = library default; -1 = one launch per
+    colour); every choice gives the same bits.
     """
     torch = _torch()
     p, n, params, resolved = _prepare(
         positions, normals, depth=depth, full_depth=full_depth, base_depth=base_depth, solve_depth=solve_depth,
         kernel_depth=kernel_depth, samples_per_node=samples_per_node, scale=scale, point_weight=point_weight,
         exact_interpolation=exact_interpolation, iters=iters, base_v_cycles=base_v_cycles, cg_accuracy=cg_accuracy,
-        mixed=mixed)
+        mixed=mixed, relax_tile=relax_tile)
     handle = ctypes.c_void_p()
     dtype_code = 1 if p.dtype == torch.float64 else 0
     with torch.cuda.device(p.device):
@@ -283,18 +348,61 @@ def reconstruct_batch(clouds, *, max_concurrency=8, **options):
     return [Reconstruction(out[i], dtype, device, prepared[i][3]) for i in range(count)]
 
 
+def _relax_operands(name, keys, offsets, columns, values, b, x, depth, iters):
+    """Checks a caller CSR operator before its raw pointers reach the C ABI,
+    which cannot see dtypes, lengths or devices: int64 keys, int32 offsets and
+    columns, float32/float64 values with b and x of the same dtype, rows+1
+    offsets, offsets[-1] == len(columns) == len(values), columns in [0, rows),
+    one ROCm device.  Returns the dtype code (0 float32, 1 float64).  The
+    reference gets the same protection from Warp's typed arrays
+    (relaxation.py:38-64)."""
+    torch = _torch()
+    tensors = dict(keys=keys, offsets=offsets, columns=columns, values=values, b=b, x=x)
+    for label, t in tensors.items():
+        if not isinstance(t, torch.Tensor):
+            raise ValueError(f"{name}: {label} must be a torch tensor")
+        if t.dim() != 1 or not t.is_contiguous():
+            raise ValueError(f"{name}: {label} must be a contiguous one-dimensional tensor")
+    if keys.dtype != torch.int64:
+        raise ValueError(f"{name}: keys must be int64 basis keys")
+    for label in ("offsets", "columns"):
+        if tensors[label].dtype != torch.int32:
+            raise ValueError(f"{name}: {label} must be int32")
+    if values.dtype not in (torch.float32, torch.float64):
+        raise ValueError(f"{name}: values must be float32 or float64")
+    for label in ("b", "x"):
+        if tensors[label].dtype != values.dtype:
+            raise ValueError(f"{name}: {label} must have the dtype of values ({values.dtype})")
+    rows = keys.shape[0]
+    if offsets.shape[0] != rows + 1:
+        raise ValueError(f"{name}: offsets must hold rows + 1 = {rows + 1} entries")
+    if b.shape[0] != rows or x.shape[0] != rows:
+        raise ValueError(f"{name}: b and x must hold one value per row ({rows})")
+    if columns.shape[0] != values.shape[0]:
+        raise ValueError(f"{name}: columns and values must have the same length")
+    if int(depth) < 0 or int(iters) < 0:
+        raise ValueError(f"{name}: depth and iters must be non-negative")
+    device = x.device
+    if device.type != "cuda" or any(t.device != device for t in tensors.values()):
+        raise ValueError(f"{name} requires every tensor on one ROCm device")
+    if rows:
+        if int(offsets[0]) != 0 or int(offsets[-1]) != columns.shape[0]:
+            raise ValueError(f"{name}: offsets must start at 0 and end at len(columns)")
+        if bool((offsets[1:] < offsets[:-1]).any()):
+            raise ValueError(f"{name}: offsets must be non-decreasing")
+        if columns.shape[0] and (int(columns.min()) < 0 or int(columns.max()) >= rows):
+            raise ValueError(f"{name}: columns must lie in [0, rows)")
+    return 1 if values.dtype == torch.float64 else 0
+
+
 def relax_scheduled(keys, offsets, columns, values, b, x, depth, iters, forward=True, z=-1):
     """8-colour Gauss-Seidel on device CSR tensors (relaxation.py:261-271); updates x.
 
     ``z < 0`` sweeps the colour groups of ``row_schedule(keys, depth)``; ``z >= 0``
     relaxes slice ``z`` of the sliced schedule, as the reference's ``z`` argument.
     """
-    torch = _torch()
-    for t in (keys, offsets, columns, values, b, x):
-        if not isinstance(t, torch.Tensor) or t.device.type != "cuda" or not t.is_contiguous():
-            raise ValueError("relax_scheduled requires contiguous ROCm tensors")
-    dtype = 1 if values.dtype == torch.float64 else 0
-    with torch.cuda.device(x.device):
+    dtype = _relax_operands("relax_scheduled", keys, offsets, columns, values, b, x, depth, iters)
+    with _torch().cuda.device(x.device):
         _lib.check(_lib.load().psr_relax_scheduled(dtype, keys.data_ptr(), keys.shape[0], offsets.data_ptr(),
                                                    columns.data_ptr(), values.data_ptr(), b.data_ptr(), x.data_ptr(),
                                                    depth, iters, int(bool(forward)), int(z), _stream_handle(x.device)))
@@ -303,12 +411,8 @@ def relax_scheduled(keys, offsets, columns, values, b, x, depth, iters, forward=
 
 def relax_sliced(keys, offsets, columns, values, b, x, depth, iters):
     """Sliced Gauss-Seidel on device CSR tensors in the reference order (relaxation.py:189-210); updates x."""
-    torch = _torch()
-    for t in (keys, offsets, columns, values, b, x):
-        if not isinstance(t, torch.Tensor) or t.device.type != "cuda" or not t.is_contiguous():
-            raise ValueError("relax_sliced requires contiguous ROCm tensors")
-    dtype = 1 if values.dtype == torch.float64 else 0
-    with torch.cuda.device(x.device):
+    dtype = _relax_operands("relax_sliced", keys, offsets, columns, values, b, x, depth, iters)
+    with _torch().cuda.device(x.device):
         _lib.check(_lib.load().psr_relax_sliced(dtype, keys.data_ptr(), keys.shape[0], offsets.data_ptr(),
                                                 columns.data_ptr(), values.data_ptr(), b.data_ptr(), x.data_ptr(),
                                                 depth, iters, _stream_handle(x.device)))
