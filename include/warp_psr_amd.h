@@ -59,7 +59,7 @@ typedef struct PsrParams {
   int float_operators;     /* 1: mixed precision -- a float64 run whose finer-level Laplacian +
                               screening operators are stored as float32 (default 0; float64 only) */
   int relax_tile;          /* finer-level sliced Gauss-Seidel: side of the (x, y) tiles that one
-                              workgroup relaxes per wave (8..40), 0 = library default, -1 = one
+                              workgroup relaxes per wave (even, 8..40), 0 = library default, -1 = one
                               launch per colour and wave; every choice gives the same bits */
 } PsrParams;
 

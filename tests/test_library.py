@@ -60,6 +60,7 @@ def test_params_default():
     (dict(samples_per_node=float("nan")), "samples_per_node"),
     (dict(relax_tile=7), "relax_tile must be"),
     (dict(relax_tile=41), "relax_tile must be"),
+    (dict(relax_tile=13), "relax_tile must be"),
     (dict(relax_tile=-2), "relax_tile must be"),
 ])
 def test_c_abi_rejects_invalid_parameters_before_device_work(kw, message):

@@ -48,8 +48,8 @@ void check_params(const PsrParams& p) {
     throw PsrError(2, "Require finite nonnegative point_weight and positive cg_accuracy");
   if (!std::isfinite(p.samples_per_node) || p.samples_per_node <= 0 || !std::isfinite(p.scale) || p.scale <= 1)
     throw PsrError(2, "Require finite positive samples_per_node and scale > 1");
-  if (!(p.relax_tile == 0 || p.relax_tile == -1 || (p.relax_tile >= 8 && p.relax_tile <= 40)))
-    throw PsrError(2, "relax_tile must be 0 (default), -1 (one launch per colour) or a tile side in [8, 40]");
+  if (!(p.relax_tile == 0 || p.relax_tile == -1 || (p.relax_tile >= 8 && p.relax_tile <= 40 && p.relax_tile % 2 == 0)))
+    throw PsrError(2, "relax_tile must be 0 (default), -1 (one launch per colour) or an even tile side in [8, 40]");
 }
 
 Params resolve(const PsrParams& in) {

@@ -1310,33 +1310,28 @@ __global__ void k_diag_sched(const int* __restrict__ order, int R, const int* __
 // slice z + 1 from the other one, which it writes; the last visit lands in
 // buffer 0.
 //
-// Tile tables, built per level from the schedule: entries (schedule
-// position, cell | layer << 13 | interior << 15) grouped per tile by category
-// -- rows computed for colour 0..3 (categories 0..3), then the staged values
-// of slice z (4), z - 1 (5) and z + 1 (6) -- and per tile the category starts
-// and the end (8 ints).
-constexpr int TILE_CATEGORIES = 7;
+// Tile tables, built per level from the schedule, for every tile whose
+// ring-4 region holds rows of its slice: entries (schedule position, lx |
+// ly << 8 | interior << 16, lx and ly the row's grid coordinates, the tile
+// interior starting at 4) grouped per tile by category -- rows computed for colour
+// 0..3 (categories 0..3; tiles without interior rows have none), then the
+// staged values of the tile's slice (4) -- and per tile 12 ints: the five
+// category starts, the end, and the staged ranges of the tiles at the same
+// (x, y) in slices z + 1 and z - 1 (empty when absent), which are exactly the
+// values the tile needs from those slices.
+constexpr int TILE_DESC_INTS = 12;
 
 __device__ __forceinline__ int ring_of(int v, int t0, int ts) {
   const int lo = t0 - v, hi = v - (t0 + ts - 1);
   return lo > 0 ? lo : (hi > 0 ? hi : 0);
 }
 
-// flags[(z * nt + y / ts) * nt + x / ts] = 1 for every row (tile has interior rows)
-__global__ void k_tile_flags(const int* __restrict__ order, const int64_t* __restrict__ keys, int R, int ts, int nt,
-                             int* __restrict__ flags) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= R) return;
-  const int64_t k = keys[order[j]];
-  flags[((int64_t)key_z(k) * nt + key_y(k) / ts) * nt + key_x(k) / ts] = 1;
-}
-
-// The entries of row j in the tiles (with interior rows) of slices z - 1, z
-// and z + 1 whose ring-4 region holds it: counts, or the sort keys (compact
-// tile << 3 | category) and values (j) from pos[j] on.
-template <bool EMIT>
-__global__ void k_tile_members(const int* __restrict__ order, const int64_t* __restrict__ keys, int R, int n, int ts, int nt,
-                               const int* __restrict__ flags, const int* __restrict__ compact, int* __restrict__ count,
+// Tiles of row j's slice whose ring-4 region holds it: flags (FLAG), counts,
+// or the sort keys (compact tile << 3 | category) and values (j) from
+// count[j] on (EMIT).
+template <int MODE>  // 0 flag, 1 count, 2 emit
+__global__ void k_tile_members(const int* __restrict__ order, const int64_t* __restrict__ keys, int R, int ts, int nt,
+                               int* __restrict__ flags, const int* __restrict__ compact, int* __restrict__ count,
                                uint32_t* __restrict__ out_keys, int* __restrict__ out_vals) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= R) return;
@@ -1345,32 +1340,26 @@ __global__ void k_tile_members(const int* __restrict__ order, const int64_t* __r
   const int colour = (x & 1) + 2 * (y & 1);
   const int tx0 = max(0, (x - 4) / ts), tx1 = min(nt - 1, (x + 4) / ts);
   const int ty0 = max(0, (y - 4) / ts), ty1 = min(nt - 1, (y + 4) / ts);
-  int c = 0, pos = EMIT ? count[j] : 0;
-  auto emit = [&](int64_t dense, int cat) {
-    if (EMIT) {
-      out_keys[pos] = ((uint32_t)compact[dense] << 3) | (uint32_t)cat;
-      out_vals[pos] = j;
-      ++pos;
-    }
-    ++c;
-  };
-  for (int dz = -1; dz <= 1; ++dz) {  // the tile's slice is z + dz
-    const int zt = z + dz;
-    if (zt < 0 || zt > n) continue;
-    for (int ty = ty0; ty <= ty1; ++ty)
-      for (int tx = tx0; tx <= tx1; ++tx) {
-        const int r = max(ring_of(x, tx * ts, ts), ring_of(y, ty * ts, ts));
-        const int64_t dense = ((int64_t)zt * nt + ty) * nt + tx;
-        if (r > 4 || !flags[dense]) continue;
-        if (dz == 0) {
-          if (r <= 3 - colour) emit(dense, colour);
-          emit(dense, 4);
-        } else {
-          emit(dense, dz > 0 ? 5 : 6);  // the tile's slice z - 1 (5) or z + 1 (6)
-        }
+  int c = 0, pos = MODE == 2 ? count[j] : 0;
+  for (int ty = ty0; ty <= ty1; ++ty)
+    for (int tx = tx0; tx <= tx1; ++tx) {
+      const int r = max(ring_of(x, tx * ts, ts), ring_of(y, ty * ts, ts));
+      if (r > 4) continue;
+      const int64_t dense = ((int64_t)z * nt + ty) * nt + tx;
+      if (MODE == 0) {
+        flags[dense] = 1;
+        continue;
       }
-  }
-  if (!EMIT) count[j] = c;
+      const int n_here = r <= 3 - colour ? 2 : 1;
+      if (MODE == 2) {
+        const uint32_t t = (uint32_t)compact[dense] << 3;
+        if (n_here == 2) { out_keys[pos] = t | (uint32_t)colour; out_vals[pos++] = j; }
+        out_keys[pos] = t | 4u;
+        out_vals[pos++] = j;
+      }
+      c += n_here;
+    }
+  if (MODE == 1) count[j] = c;
 }
 
 __global__ void k_tile_dense_of(const int* __restrict__ flags, const int* __restrict__ compact, int64_t ndense,
@@ -1379,25 +1368,42 @@ __global__ void k_tile_dense_of(const int* __restrict__ flags, const int* __rest
   if (d < ndense && flags[d]) dense_of[compact[d]] = (int)d;
 }
 
-// Per tile: category starts (lower bounds of tile << 3 | c, c = 0..7);
-// stats[0] = most computed rows of a tile, stats[1] = most staged values.
-__global__ void k_tile_desc(const uint32_t* __restrict__ skeys, int E, int tiles, int* __restrict__ desc,
-                            int* __restrict__ stats) {
+__device__ __forceinline__ int lower_bound_u32(const uint32_t* __restrict__ a, int n, uint32_t key) {
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (a[mid] < key) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+// Per tile: the category starts, the end and the neighbouring slices'
+// staged ranges; stats[0] = most computed rows of a tile, stats[1] = most
+// staged values (all three slices).
+__global__ void k_tile_desc(const uint32_t* __restrict__ skeys, int E, int tiles, const int* __restrict__ dense_of,
+                            const int* __restrict__ flags, const int* __restrict__ compact, int64_t per_slice,
+                            int64_t ndense, int* __restrict__ desc, int* __restrict__ stats) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= tiles) return;
-  int cs[8];
-  for (int c = 0; c < 8; ++c) {
-    const uint32_t key = ((uint32_t)t << 3) | (uint32_t)c;
-    int lo = 0, hi = E;
-    while (lo < hi) {
-      const int mid = (lo + hi) >> 1;
-      if (skeys[mid] < key) lo = mid + 1; else hi = mid;
+  int* o = desc + TILE_DESC_INTS * t;
+  for (int c = 0; c < 6; ++c) o[c] = lower_bound_u32(skeys, E, ((uint32_t)t << 3) | (uint32_t)c);
+  const int64_t d = dense_of[t];
+  int staged = o[5] - o[4];
+  for (int side = 0; side < 2; ++side) {  // slice z + 1, then z - 1
+    const int64_t nd = side == 0 ? d + per_slice : d - per_slice;
+    int lo = 0, hi = 0;
+    if (nd >= 0 && nd < ndense && flags[nd]) {
+      const uint32_t tn = (uint32_t)compact[nd] << 3;
+      lo = lower_bound_u32(skeys, E, tn | 4u);
+      hi = lower_bound_u32(skeys, E, tn | 5u);
     }
-    cs[c] = lo;
-    desc[8 * t + c] = lo;
+    o[6 + 2 * side] = lo;
+    o[7 + 2 * side] = hi;
+    staged += hi - lo;
   }
-  atomicMax(&stats[0], cs[4] - cs[0]);
-  atomicMax(&stats[1], cs[7] - cs[4]);
+  o[10] = o[11] = 0;
+  atomicMax(&stats[0], o[4] - o[0]);
+  atomicMax(&stats[1], staged);
 }
 
 __global__ void k_tile_entries(const uint32_t* __restrict__ skeys, const int* __restrict__ svals, int E,
@@ -1410,9 +1416,9 @@ __global__ void k_tile_entries(const uint32_t* __restrict__ skeys, const int* __
   const int ox = (d % nt) * ts - 4, oy = ((d / nt) % nt) * ts - 4, g = ts + 8;
   const int64_t k = keys[order[j]];
   const int lx = key_x(k) - ox, ly = key_y(k) - oy;
-  const int layer = cat == 5 ? 0 : (cat == 6 ? 2 : 1);
   const int interior = (cat < 4 && lx >= 4 && lx < 4 + ts && ly >= 4 && ly < 4 + ts) ? 1 : 0;
-  entries[e] = make_int2(j, (ly * g + lx) | (layer << 13) | (interior << 15));
+  (void)g;
+  entries[e] = make_int2(j, lx | (ly << 8) | (interior << 16));
 }
 
 // first tile of each slice z (z = 0..n + 1) from the exclusive scan of the flags
@@ -1441,20 +1447,31 @@ struct WaveTiles {
   unsigned flip;
 };
 
-// One computed row: operator values, right-hand side and diagonal factor.
+// LDS grid layout: per slice layer, four parity planes ((y & 1) * 2 + (x & 1),
+// the row colours) of gh * gh cells, x-major inside a plane: rows of one
+// colour with consecutive y -- consecutive threads -- read and write
+// consecutive words, free of bank conflicts; a neighbour (dx, dy) of a row
+// of known colour lies at a constant offset in a known plane.  The tile side
+// is even, so grid parities equal key parities.
+__device__ __forceinline__ int grid_index(int lx, int ly, int gh) {
+  return (((ly & 1) << 1) | (lx & 1)) * gh * gh + (lx >> 1) * gh + (ly >> 1);
+}
+
+// One computed row: operator values, right-hand side and diagonal factor;
+// base = (lx >> 1) * gh + (ly >> 1), the row's offset inside its plane.
 template <class T, class V> struct TiledRow {
-  int j, cell, colour, interior;
+  int j, base, interior;
   V v[27];
   T b, inv;
   unsigned char has;
 };
 
 template <class T, class V>
-__device__ __forceinline__ void tiled_fetch(const TiledArgs<T, V>& a, int2 en, int colour, TiledRow<T, V>& w) {
+__device__ __forceinline__ void tiled_fetch(const TiledArgs<T, V>& a, int2 en, int gh, TiledRow<T, V>& w) {
+  const int lx = en.y & 255, ly = (en.y >> 8) & 255;
   w.j = en.x;
-  w.cell = en.y & 8191;
-  w.interior = (en.y >> 15) & 1;
-  w.colour = colour;
+  w.base = (lx >> 1) * gh + (ly >> 1);
+  w.interior = (en.y >> 16) & 1;
   const size_t R = (size_t)a.R;
   const unsigned j = (unsigned)en.x;
 #pragma unroll
@@ -1464,19 +1481,26 @@ __device__ __forceinline__ void tiled_fetch(const TiledArgs<T, V>& a, int2 en, i
   w.has = a.has_diag[j];
 }
 
-template <class T, class V>
-__device__ __forceinline__ void tiled_update(T* grid, int g, int plane, T* Q, const TiledRow<T, V>& w) {
-  const T* mid = grid + plane + w.cell;
+// The update of a row of colour C (x parity C & 1, y parity C >> 1):
+// neighbour (dx, dy, dz) sits in layer dz + 1, plane of parities
+// ((C & 1) + dx, (C >> 1) + dy), at base + ((px + dx) >> 1) * gh + ((py + dy) >> 1).
+template <int C, class T, class V>
+__device__ __forceinline__ void tiled_update(T* grid, int gh, T* Q, const TiledRow<T, V>& w) {
+  constexpr int px = C & 1, py = C >> 1;
+  const int pl = gh * gh, layer = 4 * pl;
   T r = w.b;
 #pragma unroll
   for (int k = 0; k < 27; ++k) {
     const int dx = k / 9 - 1, dy = (k / 3) % 3 - 1, dz = k % 3 - 1;
-    const T t = r - mid[dz * plane + dy * g + dx] * T(w.v[k]);
-    r = t;
+    const int qx = px + dx + 2, qy = py + dy + 2;  // shifted to stay nonnegative
+    const int plane = ((qy & 1) << 1) | (qx & 1);
+    const int off = (dz + 1) * layer + plane * pl + ((qx >> 1) - 1) * gh + ((qy >> 1) - 1);
+    r = r - grid[off + w.base] * T(w.v[k]);
   }
-  const T xi = mid[0];
+  const int self = layer + (py * 2 + px) * pl + w.base;
+  const T xi = grid[self];
   const T nv = w.has ? xi + r * w.inv : xi;
-  grid[plane + w.cell] = nv;
+  grid[self] = nv;
   if (w.interior) Q[w.j] = nv;
 }
 
@@ -1486,6 +1510,16 @@ __device__ __forceinline__ void tiled_update(T* grid, int g, int plane, T* Q, co
 // descriptor; entries; operator rows and staged values.  Tiles with more
 // than BS computed rows take further passes for the rest.
 constexpr int TILED_STAGE_UNROLL = 4;
+
+template <class T, class V>
+__device__ __forceinline__ void tiled_phase(int phase, T* grid, int gh, T* Q, const TiledRow<T, V>& w) {
+  switch (phase) {
+    case 0: tiled_update<0>(grid, gh, Q, w); break;
+    case 1: tiled_update<1>(grid, gh, Q, w); break;
+    case 2: tiled_update<2>(grid, gh, Q, w); break;
+    default: tiled_update<3>(grid, gh, Q, w); break;
+  }
+}
 
 template <class T, class V, int BS>
 __global__ void __launch_bounds__(BS) k_relax_tiled(TiledArgs<T, V> a, WaveTiles wv) {
@@ -1497,45 +1531,61 @@ __global__ void __launch_bounds__(BS) k_relax_tiled(TiledArgs<T, V> a, WaveTiles
   const bool flip = (wv.flip >> s) & 1u;
   const T* P = flip ? a.x1 : a.x0;
   T* Q = flip ? a.x0 : a.x1;
-  const int4 d0 = *(const int4*)(a.desc + 8 * tile);
-  const int4 d1 = *(const int4*)(a.desc + 8 * tile + 4);
-  const int tid = (int)threadIdx.x, g = a.g, plane = g * g;
-  const int c0 = d0.x, c1 = d0.y, c2 = d0.z, c3 = d0.w, stage0 = d1.x, end = d1.w;
-  for (int i = tid; i < 3 * plane; i += BS) grid[i] = T(0);
-  // this thread's first computed row, and its first staged values
+  const int4 d0 = *(const int4*)(a.desc + TILE_DESC_INTS * tile);
+  const int4 d1 = *(const int4*)(a.desc + TILE_DESC_INTS * tile + 4);
+  const int4 d2 = *(const int4*)(a.desc + TILE_DESC_INTS * tile + 8);
+  const int c0 = d0.x, c1 = d0.y, c2 = d0.z, c3 = d0.w, stage0 = d1.x;
+  if (c0 == stage0) return;  // no interior rows: staged data for the neighbouring slices only
+  const int tid = (int)threadIdx.x, gh = a.g >> 1, layer = 4 * gh * gh;
+  // staged values: this slice [d1.x, d1.y) (layer 1), slice z + 1 [d1.z, d1.w)
+  // (layer 2, read buffer of the write side) and slice z - 1 [d2.x, d2.y)
+  // (layer 0), as one index range
+  const int n_own = d1.y - d1.x, n_up = d1.w - d1.z, n_all = n_own + n_up + (d2.y - d2.x);
+  for (int i = tid; i < 3 * layer; i += BS) grid[i] = T(0);
   const int e = c0 + tid;
   const bool have = e < stage0;
   const int colour = e < c1 ? 0 : (e < c2 ? 1 : (e < c3 ? 2 : 3));
   TiledRow<T, V> w;
   int2 sen[TILED_STAGE_UNROLL];
+  int slay[TILED_STAGE_UNROLL];
   T sval[TILED_STAGE_UNROLL];
   const int2 en = have ? a.entries[e] : make_int2(0, 0);
+  auto staged = [&](int i, int& lay) {
+    if (i < n_own) { lay = 1; return d1.x + i; }
+    if (i < n_own + n_up) { lay = 2; return d1.z + (i - n_own); }
+    lay = 0;
+    return d2.x + (i - n_own - n_up);
+  };
 #pragma unroll
   for (int u = 0; u < TILED_STAGE_UNROLL; ++u) {
-    const int se = stage0 + tid + u * BS;
-    sen[u] = se < end ? a.entries[se] : make_int2(-1, 0);
+    const int i = tid + u * BS;
+    slay[u] = -1;
+    if (i < n_all) sen[u] = a.entries[staged(i, slay[u])];
   }
-  if (have) tiled_fetch(a, en, colour, w);
+  if (have) tiled_fetch(a, en, gh, w);
 #pragma unroll
   for (int u = 0; u < TILED_STAGE_UNROLL; ++u)
-    if (sen[u].x >= 0) sval[u] = (((sen[u].y >> 13) & 3) == 2 ? (const T*)Q : P)[(unsigned)sen[u].x];
+    if (slay[u] >= 0) sval[u] = (slay[u] == 2 ? (const T*)Q : P)[(unsigned)sen[u].x];
   __syncthreads();  // grid cleared
 #pragma unroll
   for (int u = 0; u < TILED_STAGE_UNROLL; ++u)
-    if (sen[u].x >= 0) grid[((sen[u].y >> 13) & 3) * plane + (sen[u].y & 8191)] = sval[u];
-  for (int se = stage0 + tid + TILED_STAGE_UNROLL * BS; se < end; se += BS) {
-    const int2 ge = a.entries[se];
-    grid[((ge.y >> 13) & 3) * plane + (ge.y & 8191)] = (((ge.y >> 13) & 3) == 2 ? (const T*)Q : P)[(unsigned)ge.x];
+    if (slay[u] >= 0) grid[slay[u] * layer + grid_index(sen[u].y & 255, (sen[u].y >> 8) & 255, gh)] = sval[u];
+  for (int i = tid + TILED_STAGE_UNROLL * BS; i < n_all; i += BS) {
+    int lay;
+    const int2 ge = a.entries[staged(i, lay)];
+    grid[lay * layer + grid_index(ge.y & 255, (ge.y >> 8) & 255, gh)] = (lay == 2 ? (const T*)Q : P)[(unsigned)ge.x];
   }
   __syncthreads();  // grid staged
+  // Further rows of a thread come after its first row in entry order, so
+  // their colours are at least its colour: they reuse the first row's
+  // registers once it is done.
   const int cstart[5] = {c0, c1, c2, c3, stage0};
   for (int phase = 0; phase < 4; ++phase) {
-    if (have && colour == phase) tiled_update(grid, g, plane, Q, w);
+    if (have && colour == phase) tiled_phase(phase, grid, gh, Q, w);
     for (int e2 = c0 + tid + BS; e2 < stage0; e2 += BS) {
       if (e2 < cstart[phase] || e2 >= cstart[phase + 1]) continue;
-      TiledRow<T, V> w2;
-      tiled_fetch(a, a.entries[e2], phase, w2);
-      tiled_update(grid, g, plane, Q, w2);
+      tiled_fetch(a, a.entries[e2], gh, w);
+      tiled_phase(phase, grid, gh, Q, w);
     }
     __syncthreads();
   }
@@ -1796,9 +1846,9 @@ struct TileTable {
 constexpr int RELAX_TILE_MAX = 40;
 static int relax_tile_size(int requested, int depth) {
   if (requested < 0 || relax_mode() != 0) return 0;
-  if (requested >= 8 && requested <= RELAX_TILE_MAX) return requested;
+  if (requested >= 8 && requested <= RELAX_TILE_MAX && requested % 2 == 0) return requested;
   (void)depth;
-  return 12;
+  return 16;
 }
 
 static void build_tile_table(const int64_t* keys, const int* order, int R, int depth, int ts, TileTable& tt,
@@ -1811,12 +1861,13 @@ static void build_tile_table(const int64_t* keys, const int* order, int R, int d
   if (ndense + 1 > INT32_MAX) throw PsrError(1, "tiled relaxation: too many tiles");
   DArray<int> flags(ndense + 1, s), compact(ndense + 1, s), total(1, s);
   flags.zero();
-  hipLaunchKernelGGL(k_tile_flags, dim3(blocks_for(R, 256)), dim3(256), 0, s, order, keys, R, ts, tt.nt, flags.data());
+  hipLaunchKernelGGL(k_tile_members<0>, dim3(blocks_for(R, 256)), dim3(256), 0, s, order, keys, R, ts, tt.nt, flags.data(),
+                     (const int*)nullptr, (int*)nullptr, (uint32_t*)nullptr, (int*)nullptr);
   KERNEL_CHECK();
   scan_exclusive_i32(flags.data(), compact.data(), ndense + 1, total.data(), s);
   DArray<int> count(R + 1, s), pos(R + 1, s), etotal(1, s);
-  hipLaunchKernelGGL(k_tile_members<false>, dim3(blocks_for(R, 256)), dim3(256), 0, s, order, keys, R, n, ts, tt.nt,
-                     (const int*)flags.data(), (const int*)compact.data(), count.data(), (uint32_t*)nullptr, (int*)nullptr);
+  hipLaunchKernelGGL(k_tile_members<1>, dim3(blocks_for(R, 256)), dim3(256), 0, s, order, keys, R, ts, tt.nt, flags.data(),
+                     (const int*)compact.data(), count.data(), (uint32_t*)nullptr, (int*)nullptr);
   KERNEL_CHECK();
   HIP_CHECK(hipMemsetAsync(count.data() + R, 0, sizeof(int), s));
   scan_exclusive_i32(count.data(), pos.data(), R + 1, etotal.data(), s);
@@ -1829,8 +1880,8 @@ static void build_tile_table(const int64_t* keys, const int* order, int R, int d
   tt.entries_count = E;
   DArray<uint32_t> skeys(std::max(E, 1), s);
   DArray<int> svals(std::max(E, 1), s);
-  hipLaunchKernelGGL(k_tile_members<true>, dim3(blocks_for(R, 256)), dim3(256), 0, s, order, keys, R, n, ts, tt.nt,
-                     (const int*)flags.data(), (const int*)compact.data(), pos.data(), skeys.data(), svals.data());
+  hipLaunchKernelGGL(k_tile_members<2>, dim3(blocks_for(R, 256)), dim3(256), 0, s, order, keys, R, ts, tt.nt, flags.data(),
+                     (const int*)compact.data(), pos.data(), skeys.data(), svals.data());
   KERNEL_CHECK();
   sort_pairs_u32(skeys.data(), svals.data(), E, bits_for((uint64_t)tt.tiles * 8), s);
   DArray<int> dense_of(std::max(tt.tiles, 1), s);
@@ -1840,8 +1891,9 @@ static void build_tile_table(const int64_t* keys, const int* order, int R, int d
   // slice starts and stats in one buffer, read once
   DArray<int> small(n + 2 + 2, s);
   HIP_CHECK(hipMemsetAsync(small.data() + n + 2, 0, 2 * sizeof(int), s));
-  tt.desc.reset(8 * (size_t)std::max(tt.tiles, 1), s);
+  tt.desc.reset(TILE_DESC_INTS * (size_t)std::max(tt.tiles, 1), s);
   hipLaunchKernelGGL(k_tile_desc, dim3(blocks_for(tt.tiles, 256)), dim3(256), 0, s, (const uint32_t*)skeys.data(), E, tt.tiles,
+                     (const int*)dense_of.data(), (const int*)flags.data(), (const int*)compact.data(), per_slice, ndense,
                      tt.desc.data(), small.data() + n + 2);
   KERNEL_CHECK();
   tt.entries.reset(std::max(E, 1), s);
@@ -1927,7 +1979,7 @@ int relax_tiled_graph(const GsOperator<T, V>& op, const TileTable& tt, T* x, T* 
   const int width = tiled_width(tt);
   unsigned bx = 1;
   while (bx < (unsigned)std::max(1, tt.max_tiles)) bx <<= 1;
-  const int gside = tt.ts + 8;
+  const int gside = tt.ts + 8;  // even: the parity planes are (gside / 2)^2 cells
   const size_t lds = sizeof(T) * 3 * (size_t)gside * gside;
   int dev = 0;
   HIP_CHECK(hipGetDevice(&dev));

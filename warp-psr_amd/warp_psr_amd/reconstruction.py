@@ -265,8 +265,8 @@ def _prepare(positions, normals, depth=6, full_depth=3, base_depth=None, solve_d
         raise ValueError("points and normals must have equal nonempty 1D shapes")
     if mixed and positions.dtype != torch.float64:
         raise ValueError("mixed precision (float_operators) runs on float64 inputs")
-    if not isinstance(relax_tile, int) or not (relax_tile in (0, -1) or 8 <= relax_tile <= 40):
-        raise ValueError("relax_tile must be 0 (default), -1 (one launch per colour) or a tile side in [8, 40]")
+    if not isinstance(relax_tile, int) or not (relax_tile in (0, -1) or (8 <= relax_tile <= 40 and relax_tile % 2 == 0)):
+        raise ValueError("relax_tile must be 0 (default), -1 (one launch per colour) or an even tile side in [8, 40]")
     params = _lib.PsrParams()
     _lib.load().psr_params_default(ctypes.byref(params))
     params.depth, params.full_depth, params.base_depth = depth, full_depth, base_depth
@@ -294,7 +294,7 @@ def reconstruct(positions, normals, *, depth=6, full_depth=3, base_depth=None, s
     screening operators as float32 while every vector, the octree stages and
     the base-level solve stay float64.
     ``relax_tile`` sets the side of the (x, y) tiles of the finer levels'
-    sliced Gauss-Seidel (8..40; 0 = library default; -1 = one launch per
+    sliced Gauss-Seidel (even, 8..40; 0 = library default; -1 = one launch per
     colour); every choice gives the same bits.
     """
     torch = _torch()
