@@ -240,6 +240,15 @@ int psr_field_level_offsets(const int64_t* keys, int64_t count, int depth, int32
  * library's pool. */
 int psr_isovalue(int dtype, const void* values, const void* weights, int64_t n, void* out, void* stream);
 
+/* psr_isovalue without any allocation: the same kernels and bits, with the
+ * double partials in a file-scope device array of the library (1 MiB per GPU,
+ * held until the process ends) instead of the pool.  No counterpart in the
+ * reference: for callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  At most 2**26
+ * values (more fails with code 2).  Every call on one GPU shares the array:
+ * calls that overlap on one GPU race. */
+int psr_isovalue_noalloc(int dtype, const void* values, const void* weights, int64_t n, void* out, void* stream);
+
 /* Evaluates any PsrField view (psr_result_field) at m device queries with the
  * header-only psr_evaluate_field of warp_psr_amd_field.h -- the kernel-side
  * evaluate_field(field, point) of src/warp_psr/device_field.py:91-112 --

@@ -943,6 +943,23 @@ int psr_isovalue(int dtype, const void* values, const void* weights, int64_t n, 
   }
 }
 
+int psr_isovalue_noalloc(int dtype, const void* values, const void* weights, int64_t n, void* out, void* stream) {
+  try {
+    if (dtype != 0 && dtype != 1) throw PsrError(3, "dtype must be 0 (float32) or 1 (float64)");
+    if (n < 0) throw PsrError(2, "value count must be nonnegative");
+    if (!out || (n && (!values || !weights))) throw PsrError(2, "null argument");
+    hipStream_t s = (hipStream_t)stream;
+    if (dtype == 0) isovalue_noalloc((const float*)values, (const float*)weights, n, (float*)out, s);
+    else isovalue_noalloc((const double*)values, (const double*)weights, n, (double*)out, s);
+    return 0;
+  } catch (const PsrError& e) {
+    return fail(e);
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return 1;
+  }
+}
+
 const char* psr_last_error(void) { return g_error.c_str(); }
 
 void psr_release_cache(void) { Pool::instance().release_all(); }

@@ -206,6 +206,46 @@ template <class T> void compute_isovalue(const T* v, const T* w, int64_t G, T* o
   isovalue_launch(v, w, G, tp.data(), mp.data(), out, s);
 }
 
+// The isovalue reduction without allocation, for callers that cannot call
+// hipMalloc (HIP graph capture, tight serving loops): the double partials
+// live in a file-scope device array, shared by every call on a GPU (calls
+// that overlap on one GPU race on it).  Every partial a call reads is written
+// by the same call first, so nothing needs clearing.  `scratch` is a
+// host-side stand-in for the array in host code; its device address is
+// looked up once per device (with that device current) and only that
+// address is passed to kernels.
+constexpr int ISO_SCRATCH_MAX_DEVICES = 64;
+static __device__ __align__(256) double scratch[2 * ISOVALUE_NOALLOC_PARTIALS];
+
+static double* scratch_address(int dev) {
+  static std::mutex mu;
+  static double* addr[ISO_SCRATCH_MAX_DEVICES] = {};
+  if (dev < 0 || dev >= ISO_SCRATCH_MAX_DEVICES) throw PsrError(1, "device index beyond the supported range");
+  std::lock_guard<std::mutex> g(mu);
+  if (!addr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      (void)hipGetLastError();
+      throw PsrError(1, std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+    }
+    addr[dev] = (double*)p;
+  }
+  return addr[dev];
+}
+
+template <class T> void isovalue_noalloc(const T* v, const T* w, int64_t G, T* out, hipStream_t s) {
+  if (G > ISOVALUE_NOALLOC_MAX)
+    throw PsrError(2, "psr_isovalue_noalloc handles at most " + std::to_string(ISOVALUE_NOALLOC_MAX) +
+                          " values (its partial sums live in a fixed device array)");
+  int dev = 0;
+  HIP_CHECK(hipGetDevice(&dev));
+  double* tp = scratch_address(dev);
+  isovalue_launch(v, w, G, tp, tp + ISOVALUE_NOALLOC_PARTIALS, out, s);
+}
+
+template void isovalue_noalloc<float>(const float*, const float*, int64_t, float*, hipStream_t);
+template void isovalue_noalloc<double>(const double*, const double*, int64_t, double*, hipStream_t);
 template void isovalue_launch<float>(const float*, const float*, int64_t, double*, double*, float*, hipStream_t);
 template void isovalue_launch<double>(const double*, const double*, int64_t, double*, double*, double*, hipStream_t);
 template void compute_isovalue<float>(const float*, const float*, int64_t, float*, hipStream_t);

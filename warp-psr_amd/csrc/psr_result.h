@@ -198,6 +198,12 @@ template <class T> void stage_solve(Result<T>& r);
 inline int64_t isovalue_partials(int64_t G) { return (G + 1023) / 1024; }
 template <class T> void isovalue_launch(const T* v, const T* w, int64_t G, double* tp, double* mp, T* out, hipStream_t s);
 template <class T> void compute_isovalue(const T* v, const T* w, int64_t G, T* out, hipStream_t s);
+// The same reduction with its partials in a file-scope device array of
+// 2 * ISOVALUE_NOALLOC_PARTIALS doubles (1 MiB per GPU): no allocation; at
+// most ISOVALUE_NOALLOC_MAX values; calls that overlap on one GPU race.
+constexpr int64_t ISOVALUE_NOALLOC_MAX = (int64_t)1 << 26;
+constexpr int64_t ISOVALUE_NOALLOC_PARTIALS = ISOVALUE_NOALLOC_MAX / 1024;
+template <class T> void isovalue_noalloc(const T* v, const T* w, int64_t G, T* out, hipStream_t s);
 template <class T> void stage_field(Result<T>& r);
 template <class T>
 void evaluate_field(const Result<T>& r, const T* queries, int64_t m, const T* linear, const T* translation, T* values,

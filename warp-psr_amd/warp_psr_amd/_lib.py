@@ -16,6 +16,7 @@ EXPORTED = (
     "psr_mesh_select", "psr_mesh_corner_points", "psr_mesh_triangulate", "psr_mesh_copy", "psr_mesh_vertex_vjp",
     "psr_mesh_case_table", "psr_mesh_free", "psr_tree_metadata", "psr_tree_metadata_levels", "psr_tree_metadata_level",
     "psr_tree_metadata_copy", "psr_tree_metadata_free", "psr_field_level_offsets", "psr_isovalue",
+    "psr_isovalue_noalloc",
 )
 
 
@@ -89,6 +90,8 @@ def load():
     lib.psr_field_level_offsets.argtypes = [vp, i64, c_int, vp, vp]
     lib.psr_isovalue.restype = c_int
     lib.psr_isovalue.argtypes = [c_int, vp, vp, i64, vp, vp]
+    lib.psr_isovalue_noalloc.restype = c_int
+    lib.psr_isovalue_noalloc.argtypes = [c_int, vp, vp, i64, vp, vp]
     lib.psr_relax_sliced.restype = c_int
     lib.psr_relax_scheduled.restype = c_int
     i32p, i64p = ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(i64)
