@@ -5,9 +5,10 @@
   tests/golden/reference_reports.json): the regenerated inputs hash to the
   recorded input SHA-256, every level's support and active key arrays hash to
   the recorded SHA-256s (tools/check_tree.py:9-21), cube_to_model is equal and
-  the isovalue is within the reference's float tolerance of warp-psr's.
+  the isovalue is within 1e-6 of warp-psr's (3 ulp for screened runs).
 * Known answers from the reference's tests (tests/test_samples.py:15-28,
-  tests/test_field.py, tests/test_cancelled_normals.py).
+  tests/test_cancelled_normals.py); tests/test_field.py and the other
+  known-answer tests are restated in test_oracle_known_answers.py.
 * Closed forms: hat and normal integral tables vs brute-force quadrature,
   stiffness row sums, the deterministic log vs math.log, and the wavefront
   relaxation order vs the sequential sliced order (tests/test_relaxation.py:94-107).
@@ -58,15 +59,36 @@ def test_fixture_tree_iso_transform(f):
         assert sha(res.array("basis_keys", d)) == lv["active_sha256"], f"active keys differ at depth {d}"
     ctm = res.array("cube_to_model").reshape(4, 4)
     np.testing.assert_array_equal(ctm, np.asarray(f["cube_to_model"], dtype=ctm.dtype))
-    if f["params"]["precision"] == "float":
-        # float32 budget of tools/check_reconstruction.py:23 (atol 1e-5, rtol 1e-4)
-        targets = [f["iso_value"]] + ([f["reference_iso_value"]] if f["name"] in ORDERED_ROOT else [])
-        for target in targets:
-            assert abs(res.iso_value - target) <= 1e-5 + 1e-4 * abs(target)
+    dt = np.float32 if f["params"]["precision"] == "float" else np.float64
+    iso, warp_iso = dt(res.iso_value), dt(f["iso_value"])
+    ulps = abs(float(iso) - float(warp_iso)) / float(np.spacing(abs(warp_iso)))
+    screened = f["params"]["weight"] > 0 and not f["params"]["exact"]
+    if dt == np.float32:
+        # Every float32 isovalue is within the 1e-6 field bound of warp-psr's.
+        # Screened runs agree to 3 ulp; unscreened and exact-interpolation
+        # runs leave the constant mode to the base-level smoother and move by
+        # up to 8.4e-7 (sphere-psr, sphere-exact).  The differing bits come
+        # from Warp's FEM quadrature order at the base level and below depth
+        # 4 (the oracle sums the same hat products row by row).
+        assert abs(float(iso) - float(warp_iso)) <= 1e-6
+        if screened:  # (an isovalue near zero, profile-root-solve: 2.7e-9)
+            assert ulps <= 3 or abs(float(iso) - float(warp_iso)) <= 1e-8
+        if f["name"] in ORDERED_ROOT:
+            assert abs(float(iso) - f["reference_iso_value"]) <= 1e-6
     else:
-        # float64: equal to warp-psr to the last few bits (base-level FEM order differs)
-        assert abs(res.iso_value - f["iso_value"]) <= 1e-13
+        assert abs(float(iso) - float(warp_iso)) <= 3e-15
         assert abs(res.iso_value - f["reference_iso_value"]) <= 1e-8 + 1e-6 * abs(f["reference_iso_value"])
+
+
+def test_fixture_isovalues_exact_count():
+    """How many float32 isovalues equal warp-psr's published value bit for bit
+    (7 of 28; every other one within 3 ulp when screened, 8.4e-7 otherwise)."""
+    exact = 0
+    floats = [f for f in FIXTURES if f["params"]["precision"] == "float"]
+    for f in floats:
+        _, _, res = run_fixture(f)
+        exact += int(np.float32(res.iso_value) == np.float32(f["iso_value"]))
+    assert len(floats) == 28 and exact >= 7
 
 
 CANCELLED = REPORTS["cancelled_normals"]

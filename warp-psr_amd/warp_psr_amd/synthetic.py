@@ -51,6 +51,37 @@ def fixture_input(shape, count, seed=SEED, precision="float", input_scale=1.0, t
     return p.astype(dtype), n.astype(dtype)
 
 
+def ownership_input(dtype):
+    """The split-plane ownership probes of the reference's threshold fixtures
+    (tools/check_thresholds.py:25-57): two corner points fixing an exact
+    scale-2 normalisation, then for every dyadic split of depths 1..4 inside
+    (1/4, 3/4), on each axis, a point just below, on and just above the split
+    (model = (cube[[2, 0, 1]] - 1/2) * 2), and five points exercising a
+    cancelling aggregate, multiplicity and a zero normal.  Returns (points,
+    normals) of ``dtype``; the points must be the reference's bit for bit, so
+    the construction follows it step for step."""
+    positions = [np.full(3, -.5, dtype=dtype), np.full(3, .5, dtype=dtype)]
+    normals = [[1., 0., 0.], [0., 1., 0.]]
+    for depth in range(1, 5):
+        for numerator in range(1, 1 << depth):
+            split = dtype(numerator / (1 << depth))
+            if not .25 < split < .75:
+                continue
+            for axis in range(3):
+                for coordinate in (np.nextafter(split, dtype(-np.inf)), split, np.nextafter(split, dtype(np.inf))):
+                    cube = np.array([.40625, .546875, .59375], dtype=dtype)
+                    cube[axis] = coordinate
+                    positions.append((cube[[2, 0, 1]] - dtype(.5)) * dtype(2))
+                    normals.append([0., 0., 2.])
+    for position, normal in [([.21, -.19, .23], [1, 0, 0]), ([.21, -.19, .23], [-3, 0, 0]),
+                             ([-.23, .19, -.21], [0, 2, 0]), ([-.23, .19, -.21], [0, 5, 0]),
+                             ([.11, .13, -.17], [0, 0, 0])]:
+        positions.append(np.array(position, dtype=dtype))
+        normals.append(normal)
+    data = np.column_stack([positions, normals]).astype(dtype)
+    return np.ascontiguousarray(data[:, :3]), np.ascontiguousarray(data[:, 3:])
+
+
 def fixture_input_text(points, normals):
     """The input.pts text whose SHA-256 the reference reports record."""
     import io
