@@ -13,8 +13,9 @@ same way; mixed and float32 fields are compared with the float64 field at
 interfaces).
 
 Prints one JSON line (metric/value per the benchmark contract) with a
-``roofline`` object for the sliced Gauss-Seidel relaxation of the finest level
-and a ``cpu_baseline`` object (the serial CPU oracle on a bounded sample).
+``roofline`` object for the sliced Gauss-Seidel relaxation of the finer levels
+and a ``cpu_baseline`` object (the CPU oracle on the host's cores on the same
+input, with the core count and its provenance).
 """
 import argparse
 import json
@@ -50,13 +51,16 @@ def measured_traffic(kernel):
 
 
 def relax_roofline(metadata, value_bytes, vector_bytes, kernel=None):
-    """Sliced Gauss-Seidel relaxation of every finer level (one kernel, one
-    launch per colour and pipeline step): algorithmic bytes over the HIP-event
-    span of its launches on the reconstruction stream.
+    """Sliced Gauss-Seidel relaxation of every finer level: algorithmic bytes
+    over the HIP-event span of its launches on the reconstruction stream.
+    The tiled form (k_relax_tiled) launches once per wave; the
+    colour-per-launch form (k_relax_wave_args) four times.
 
     Per level and sweep: nnz*(4 + v) (column index + operator value) +
     (R+1)*4 (row extent) + R*s (b) + 2*R*s (x read and written), v = operator
-    value bytes, s = vector bytes; times the sweeps, summed over levels.
+    value bytes, s = vector bytes; times the sweeps, summed over levels.  (The
+    tiled form reads no column indices; the CSR-equivalent count is kept so
+    both forms are measured against the same work.)
     """
     stats = metadata["relaxation"]
     v, s = value_bytes, vector_bytes
@@ -65,8 +69,10 @@ def relax_roofline(metadata, value_bytes, vector_bytes, kernel=None):
     launches = sum(st["launches"] for st in stats)
     achieved = total / seconds / 1e9
     fine = max(stats, key=lambda st: st["rows"])
+    tiled = all(st.get("tile", 0) > 0 for st in stats)
+    name = "k_relax_tiled" if tiled else "k_relax_wave_args"
     return {
-        "kernel": "k_relax_wave_args (sliced Gauss-Seidel, all finer levels)",
+        "kernel": f"{name} (sliced Gauss-Seidel, all finer levels)",
         "bound": "hbm",
         "achieved": round(achieved, 2),
         "peak": HBM_PEAK_GBS,
@@ -84,7 +90,8 @@ def relax_roofline(metadata, value_bytes, vector_bytes, kernel=None):
                    "seconds": fine["seconds"]},
         "per_level": [{"depth": st["depth"], "launches": st["launches"], "us_per_launch": round(st["seconds"] / st["launches"] * 1e6, 3),
                        "GBps": round((st["nnz"] * (4 + v) + (st["rows"] + 1) * 4 + 3 * st["rows"] * s) * st["sweeps"]
-                                     / st["seconds"] / 1e9, 1)} for st in stats],
+                                     / st["seconds"] / 1e9, 1), "tile": st.get("tile", 0),
+                       "tiles": st.get("tiles", 0), "tile_entries": st.get("tile_entries", 0)} for st in stats],
     }
 
 
@@ -143,20 +150,68 @@ def batch_throughput(torch, synthetic, dev, concurrency=8):
             "one_by_one_seconds": round(serial, 4), "one_by_one_Mpts_s": round(total / serial / 1e6, 3)}
 
 
-def cpu_baseline(points, normals, sample):
+def cpu_environment():
+    """Logical CPUs, this process's affinity and the visible cgroup-v2 cpu.max
+    quotas (the provenance the reference records with its CPU timings,
+    tools/cpu_environment.py:7-43), and the thread count they allow."""
+    quotas = []
+    try:
+        group = next(line[3:] for line in Path("/proc/self/cgroup").read_text().splitlines() if line.startswith("0::"))
+        root = Path("/sys/fs/cgroup")
+        current = root / group.lstrip("/")
+        while True:
+            limit = current / "cpu.max"
+            if limit.exists():
+                quota, period = limit.read_text().split()
+                quotas.append({"group": str(current.relative_to(root)), "cpu_max": f"{quota} {period}",
+                               "cpu_equivalents": None if quota == "max" else int(quota) / int(period)})
+            if current == root or root not in current.parents:
+                break
+            current = current.parent
+    except (OSError, StopIteration, ValueError):
+        pass
+    affinity = sorted(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None
+    finite = [q["cpu_equivalents"] for q in quotas if q["cpu_equivalents"] is not None]
+    limits = [os.cpu_count() or 1]
+    if affinity:
+        limits.append(len(affinity))
+    if finite:
+        limits.append(max(1, int(min(finite))))
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        limits.append(int(env))
+    return {"logical_cpus": os.cpu_count(), "affinity_cpus": len(affinity) if affinity else None,
+            "cgroup_quotas": quotas, "omp_num_threads_env": env, "threads": min(limits)}
+
+
+def cpu_baseline(points, normals, serial_sample):
+    """The CPU oracle on the host cores: its OpenMP build (parallel loops over
+    rows, queries and aggregates with unchanged per-row arithmetic; bit-equal
+    to the serial build) on the full benchmark input after one warm-up run on
+    a 50,000-point sample; and the serial build on that sample for the
+    one-core rate."""
     import oracle  # the checker; timed here as the CPU reference path
-    p = points[:sample]
-    n = normals[:sample]
+    env = cpu_environment()
+    os.environ["OMP_NUM_THREADS"] = str(env["threads"])
+    oracle.reconstruct(points[:serial_sample], normals[:serial_sample], mixed=True, openmp=True, **PARAMS)
     t = time.perf_counter()
-    oracle.reconstruct(p, n, mixed=True, **PARAMS)
+    oracle.reconstruct(points, normals, mixed=True, openmp=True, **PARAMS)
     dt = time.perf_counter() - t
+    t = time.perf_counter()
+    oracle.reconstruct(points[:serial_sample], normals[:serial_sample], mixed=True, **PARAMS)
+    ds = time.perf_counter() - t
+    log(f"cpu baseline: OpenMP oracle, {env['threads']} threads, {len(points)} points: {dt:.1f} s; serial oracle "
+        f"{serial_sample} points: {ds:.1f} s")
     return {
-        "value": round(sample / dt / 1e6, 6),
+        "value": round(len(points) / dt / 1e6, 6),
         "unit": "Mpts/s",
-        "cores": 1,
+        "cores": env["threads"],
         "kind": "port",
-        "sample": f"first {sample:,} of the {len(points):,}-point cloud, same depth-9 parameters and mixed "
-                  f"precision, serial C++ oracle, {dt:.2f} s",
+        "sample": f"the full {len(points):,}-point depth-9 workload in mixed precision, OpenMP C++ oracle on "
+                  f"{env['threads']} threads, one run after a warm-up on a {serial_sample:,}-point sample: {dt:.2f} s",
+        "environment": env,
+        "serial": {"points": serial_sample, "seconds": round(ds, 3), "Mpts_s": round(serial_sample / ds / 1e6, 6),
+                   "cores": 1},
     }
 
 
@@ -166,7 +221,7 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--points", type=int, default=800000)
-    ap.add_argument("--cpu-sample", type=int, default=50000)
+    ap.add_argument("--cpu-sample", type=int, default=50000, help="points of the CPU warm-up and serial runs")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--skip-batch", action="store_true")
@@ -236,7 +291,7 @@ def main():
                                 "float32 finer-level operators",
                    "basis_counts": rmx.metadata["basis_counts"]},
     }
-    out["roofline"] = relax_roofline(rmx.metadata, 4, 8, "k_relax_wave_args<double, float, 64>")
+    out["roofline"] = relax_roofline(rmx.metadata, 4, 8, "k_relax_tiled<double, float>")
     out["stage_seconds"] = rmx.metadata["timings_seconds"]
     out["normal_average_wait_seconds"] = rmx.metadata["normal_average_wait_seconds"]
     r32 = None

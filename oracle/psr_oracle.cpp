@@ -35,6 +35,14 @@
 #include <vector>
 #ifdef _OPENMP
 #include <omp.h>
+#include <parallel/algorithm>
+// Loops whose iterations write disjoint outputs and read nothing another
+// iteration writes (per row, per query, per aggregate): the OpenMP build
+// runs them on all threads with unchanged per-iteration arithmetic, so its
+// arrays equal the serial build's bit for bit.
+#define PSR_PARALLEL_FOR _Pragma("omp parallel for schedule(dynamic, 512)")
+#else
+#define PSR_PARALLEL_FOR
 #endif
 
 namespace {
@@ -98,7 +106,12 @@ inline int64_t morton(I3 c, int depth) {
 std::vector<int> stable_order(const std::vector<int64_t>& keys) {
   std::vector<int> ids(keys.size());
   std::iota(ids.begin(), ids.end(), 0);
+#ifdef _OPENMP
+  // ties broken by position: the one stable order, from a parallel sort
+  __gnu_parallel::sort(ids.begin(), ids.end(), [&](int a, int b) { return keys[a] < keys[b] || (keys[a] == keys[b] && a < b); });
+#else
   std::stable_sort(ids.begin(), ids.end(), [&](int a, int b) { return keys[a] < keys[b]; });
+#endif
   return ids;
 }
 
@@ -483,6 +496,7 @@ template <class T> void build_density(State<T>& s) {
     std::vector<int64_t> keys(27 * m, KMAX);
     std::vector<V4<T>> vals(27 * m, vzero4<T>());
     int n = 1 << d;
+    PSR_PARALLEL_FOR
     for (size_t i = 0; i < m; ++i) {  // density.py:74-106 (_splat)
       int j = order[i];
       I3 c = coords(src.keys[j]);
@@ -505,6 +519,7 @@ template <class T> void build_density(State<T>& s) {
       }
     }
     s.density_levels.push_back(reduce_records(keys, vals));
+    PSR_PARALLEL_FOR
     for (size_t i = 0; i < G; ++i)  // density.py:138-150 (_evaluate)
       s.densities[i * (kd + 1) + d] =
           density_value(s.density_levels.back(), point_div(s.position_sums[i], s.weights[i]), d);
@@ -513,6 +528,7 @@ template <class T> void build_density(State<T>& s) {
   const T target = T(s.prm.samples_per_node);
   s.dw.assign(2 * G, T(0));
   const double two_ln2 = LN2_D * 2.0;
+  PSR_PARALLEL_FOR
   for (size_t i = 0; i < G; ++i) {
     int d = kd;
     T density = s.densities[i * (kd + 1) + d];
@@ -546,6 +562,7 @@ template <class T> void build_normal_field(State<T>& s) {
   s.normal_order = order;
   std::vector<int64_t> keys(54 * G, KMAX);
   std::vector<V4<T>> vals(54 * G, vzero4<T>());
+  PSR_PARALLEL_FOR
   for (size_t i = 0; i < G; ++i) {  // normals.py:46-104 (_splats)
     int src = order[i];
     T w = s.weights[src];
@@ -724,24 +741,36 @@ CSR<double> assemble_stiffness(const std::vector<int64_t>& row_keys, const std::
   for (int i = 0; i <= res; ++i)
     for (int k = 0; k < 3; ++k) table[i * 3 + k] = hat_integrals(depth, i, trial_depth, (i >> shift) + k - 1);
   m.offsets.assign(m.rows + 1, 0);
-  for (int i = 0; i < m.rows; ++i) {
+  // present neighbours of row i in slot order; a count pass, then a fill pass
+  auto row = [&](int i, bool fill) {
     I3 c = coords(row_keys[i]);
     I3 center = {{c[0] >> shift, c[1] >> shift, c[2] >> shift}};
+    int e = fill ? m.offsets[i] : 0;
     for (int k = 0; k < 27; ++k) {
       I3 q = {{center[0] + k / 9 - 1, center[1] + (k / 3) % 3 - 1, center[2] + k % 3 - 1}};
       int64_t index = -1;
       if (std::min(std::min(q[0], q[1]), q[2]) >= 0 && std::max(std::max(q[0], q[1]), q[2]) <= (1 << trial_depth))
         index = find_key(col_keys, basis_key(trial_depth, q[0], q[1], q[2]));
       if (index >= 0) {
-        const D2& x = table[c[0] * 3 + k / 9];
-        const D2& y = table[c[1] * 3 + (k / 3) % 3];
-        const D2& z = table[c[2] * 3 + k % 3];
-        m.values.push_back(x.v[1] * y.v[0] * z.v[0] + x.v[0] * y.v[1] * z.v[0] + x.v[0] * y.v[0] * z.v[1]);
-        m.columns.push_back((int)index);
+        if (fill) {
+          const D2& x = table[c[0] * 3 + k / 9];
+          const D2& y = table[c[1] * 3 + (k / 3) % 3];
+          const D2& z = table[c[2] * 3 + k % 3];
+          m.values[e] = x.v[1] * y.v[0] * z.v[0] + x.v[0] * y.v[1] * z.v[0] + x.v[0] * y.v[0] * z.v[1];
+          m.columns[e] = (int)index;
+        }
+        ++e;
       }
     }
-    m.offsets[i + 1] = (int)m.columns.size();
-  }
+    return e;
+  };
+  PSR_PARALLEL_FOR
+  for (int i = 0; i < m.rows; ++i) m.offsets[i + 1] = row(i, false);
+  for (int i = 0; i < m.rows; ++i) m.offsets[i + 1] += m.offsets[i];
+  m.columns.resize(m.offsets[m.rows]);
+  m.values.resize(m.offsets[m.rows]);
+  PSR_PARALLEL_FOR
+  for (int i = 0; i < m.rows; ++i) row(i, true);
   return m;
 }
 
@@ -827,6 +856,7 @@ std::vector<double> add_screening(CSR<double>& A, const std::vector<int64_t>& ro
                                   const Points<T>& pts, int depth, int trial_depth) {
   std::vector<double> load(row_keys.size(), 0.0);
   const int n = 1 << depth;
+  PSR_PARALLEL_FOR
   for (size_t i = 0; i < row_keys.size(); ++i) {
     I3 c = coords(row_keys[i]);
     const int o0 = A.offsets[i], o1 = A.offsets[i + 1];
@@ -928,6 +958,7 @@ std::vector<double> normal_stencil_load(const std::vector<int64_t>& keys, const 
     for (int k = 0; k < width; ++k)
       table[i * width + k] = normal_integral(test_depth, i, normal_depth, normal_center(i, test_depth, normal_depth) + first + k);
   const int lim = 1 << normal_depth;
+  PSR_PARALLEL_FOR
   for (size_t i = 0; i < keys.size(); ++i) {
     I3 c = coords(keys[i]);
     I3 center = {{normal_center(c[0], test_depth, normal_depth), normal_center(c[1], test_depth, normal_depth),
@@ -1181,12 +1212,14 @@ template <class T> CSR<T> cast_csr(const CSR<double>& a) {
   CSR<T> m;
   m.rows = a.rows; m.cols = a.cols; m.offsets = a.offsets; m.columns = a.columns;
   m.values.resize(a.values.size());
+  PSR_PARALLEL_FOR
   for (size_t i = 0; i < a.values.size(); ++i) m.values[i] = T(a.values[i]);
   return m;
 }
 
 // warp.sparse.bsr_mv semantics: v = sum_k A_k x[col_k] (CSR order); v *= alpha; v += beta*y
 template <class T> void spmv(const CSR<T>& A, const std::vector<T>& x, std::vector<T>& y, T alpha, T beta) {
+  PSR_PARALLEL_FOR
   for (int i = 0; i < A.rows; ++i) {
     T v = T(0);
     for (int e = A.offsets[i]; e < A.offsets[i + 1]; ++e) v = v + A.values[e] * x[A.columns[e]];
@@ -1271,7 +1304,13 @@ Schedule row_schedule(const std::vector<int64_t>& keys, int depth, bool sliced) 
 }
 
 template <class T> void relax_group(const CSR<T>& A, const std::vector<T>& b, std::vector<T>& x, const Schedule& s, int group) {
+  // rows of one colour group are not coupled: their updates commute
+  const int m = s.offsets[group + 1] - s.offsets[group];
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 256) if (m > 2048)
+#endif
   for (int j = s.offsets[group]; j < s.offsets[group + 1]; ++j) gs_row(A, b, x, s.rows[j]);
+  (void)m;
 }
 
 // relaxation.py:261-271 (8-colour)
@@ -1285,6 +1324,31 @@ template <class T> void relax_scheduled(const CSR<T>& A, const std::vector<T>& b
 template <class T> void relax_sliced(const CSR<T>& A, const std::vector<T>& b, std::vector<T>& x, const Schedule& s,
                                      int depth, int iters) {
   const int n = 1 << depth;
+#ifdef _OPENMP
+  if (omp_get_max_threads() > 1) {
+    // relaxation.py:128-143 (wavefront_groups): the slices of one wave are two
+    // apart and uncoupled, so a colour of all of them runs as one parallel
+    // loop; every row keeps its inputs (the reference's own wavefront proof)
+    std::vector<int> rows;
+    for (int q = 0; q <= n + 2 * (iters - 1); ++q) {
+      const int dq = q - n + 1;
+      const int first = std::max(0, dq >= 0 ? dq / 2 : -((-dq + 1) / 2)), last = std::min(iters - 1, q / 2);
+      if (last < first) continue;
+      const int z0 = n - q + 2 * first;
+      for (int color = 0; color < 4; ++color) {
+        rows.clear();
+        for (int t = 0; t <= last - first; ++t) {
+          const int g = 4 * (z0 + 2 * t) + color;
+          for (int j = s.offsets[g]; j < s.offsets[g + 1]; ++j) rows.push_back(s.rows[j]);
+        }
+        const int m = (int)rows.size();
+#pragma omp parallel for schedule(dynamic, 256) if (m > 2048)
+        for (int k = 0; k < m; ++k) gs_row(A, b, x, rows[k]);
+      }
+    }
+    return;
+  }
+#endif
   for (int front = n; front > -iters; --front)
     for (int z = std::max(front, 0); z < std::min(front + iters, n + 1); ++z)
       for (int color = 0; color < 4; ++color) relax_group(A, b, x, s, z * 4 + color);
@@ -1603,6 +1667,7 @@ template <class T> void cumulative_coefficients(State<T>& s) {
   s.cumulative = s.coefficients;
   const int64_t K = (int64_t)s.keys.size();
   for (int d = 1; d <= s.prm.depth; ++d) {
+    PSR_PARALLEL_FOR
     for (int64_t i = 0; i < K; ++i) {
       int64_t key = s.keys[i];
       if (key_depth(key) != d) continue;
@@ -2013,6 +2078,7 @@ int oracle_evaluate(void* handle, const void* queries, int64_t m, void* values, 
       tr[r] = s->model_to_cube[r * 4 + 3];
     }
     const T* q = (const T*)queries;
+    PSR_PARALLEL_FOR
     for (int64_t i = 0; i < m; ++i) {
       V3<T> qq = {{q[3 * i], q[3 * i + 1], q[3 * i + 2]}};
       int ok;
