@@ -221,6 +221,17 @@ template <class T> bool export_array(Result<T>& r, const std::string& name, int 
   return false;
 }
 
+// A number for the metadata JSON: non-finite values as the NaN / Infinity
+// tokens Python's json module reads (a degenerate field's isovalue is NaN).
+struct JsonNumber {
+  double v;
+};
+inline std::ostream& operator<<(std::ostream& o, JsonNumber n) {
+  if (std::isnan(n.v)) return o << "NaN";
+  if (std::isinf(n.v)) return o << (n.v > 0 ? "Infinity" : "-Infinity");
+  return o << n.v;
+}
+
 template <class T> std::string metadata_json(const Result<T>& r) {
   std::ostringstream o;
   o.precision(17);
@@ -233,7 +244,7 @@ template <class T> std::string metadata_json(const Result<T>& r) {
   o << ", \"cube_to_model\": [";
   for (int i = 0; i < 4; ++i) {
     o << (i ? ", " : "") << "[";
-    for (int j = 0; j < 4; ++j) o << (j ? ", " : "") << double(r.cube_to_model[4 * i + j]);
+    for (int j = 0; j < 4; ++j) o << (j ? ", " : "") << JsonNumber{double(r.cube_to_model[4 * i + j])};
     o << "]";
   }
   const Params& p = r.prm;
@@ -258,7 +269,7 @@ template <class T> std::string metadata_json(const Result<T>& r) {
     << ((p.base_depth == 0 && p.point_weight == 0) ? "reference root mean term" : "reference solve without an added mean term")
     << "\", \"exact_interpolation_behavior\": \""
     << (p.exact_interpolation ? "empty constraints in pinned reference (issue #16)" : "level-clustered approximate constraints")
-    << "\", \"iso_value\": " << double(r.iso) << "}";
+    << "\", \"iso_value\": " << JsonNumber{double(r.iso)} << "}";
   return o.str();
 }
 
