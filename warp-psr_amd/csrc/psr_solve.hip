@@ -1316,9 +1316,10 @@ __global__ void k_diag_sched(const int* __restrict__ order, int R, const int* __
 // interior starting at 4) grouped per tile by category -- rows computed for colour
 // 0..3 (categories 0..3; tiles without interior rows have none), then the
 // staged values of the tile's slice (4) -- and per tile 12 ints: the five
-// category starts, the end, and the staged ranges of the tiles at the same
+// category starts, the end, the staged ranges of the tiles at the same
 // (x, y) in slices z + 1 and z - 1 (empty when absent), which are exactly the
-// values the tile needs from those slices.
+// values the tile needs from those slices, and the tile's first computed-row
+// record.
 constexpr int TILE_DESC_INTS = 12;
 
 __device__ __forceinline__ int ring_of(int v, int t0, int ts) {
@@ -1382,7 +1383,7 @@ __device__ __forceinline__ int lower_bound_u32(const uint32_t* __restrict__ a, i
 // staged values (all three slices).
 __global__ void k_tile_desc(const uint32_t* __restrict__ skeys, int E, int tiles, const int* __restrict__ dense_of,
                             const int* __restrict__ flags, const int* __restrict__ compact, int64_t per_slice,
-                            int64_t ndense, int* __restrict__ desc, int* __restrict__ stats) {
+                            int64_t ndense, int* __restrict__ desc, int* __restrict__ computed, int* __restrict__ stats) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= tiles) return;
   int* o = desc + TILE_DESC_INTS * t;
@@ -1402,6 +1403,7 @@ __global__ void k_tile_desc(const uint32_t* __restrict__ skeys, int E, int tiles
     staged += hi - lo;
   }
   o[10] = o[11] = 0;
+  computed[t] = o[4] - o[0];
   atomicMax(&stats[0], o[4] - o[0]);
   atomicMax(&stats[1], staged);
 }
@@ -1421,24 +1423,58 @@ __global__ void k_tile_entries(const uint32_t* __restrict__ skeys, const int* __
   entries[e] = make_int2(j, lx | (ly << 8) | (interior << 16));
 }
 
+// desc[10]: the tile's first computed-row record, from the exclusive scan of
+// the computed-row counts
+__global__ void k_tile_record_base(const int* __restrict__ base, int tiles, int* __restrict__ desc) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < tiles) desc[TILE_DESC_INTS * t + 10] = base[t];
+}
+
 // first tile of each slice z (z = 0..n + 1) from the exclusive scan of the flags
 __global__ void k_tile_slice_start(const int* __restrict__ compact, int n, int per_slice, int* __restrict__ out) {
   const int z = blockIdx.x * blockDim.x + threadIdx.x;
   if (z <= n + 1) out[z] = compact[(int64_t)z * per_slice];
 }
 
+// One computed row of a tile: its 27 operator slots, diagonal flag, inverse
+// diagonal and right-hand side, stored per computed tile entry (tile by tile,
+// entry order), so a workgroup reads its rows as consecutive 16-byte words
+// addressed by the tile descriptor alone.  A row computed by several tiles
+// has a record in each.
+template <class T, class V> struct alignas(16) TiledOp {
+  V v[27];
+  int has;
+  T inv, b;
+};
+
 template <class T, class V> struct TiledArgs {
   const int2* entries;
   const int* desc;
-  const V* vals;
-  const T* b;
-  const T* inv_diag;
-  const unsigned char* has_diag;
+  const TiledOp<T, V>* ops;
   T* x0;
   T* x1;
-  int R;
   int g;  // grid side, ts + 8
 };
+
+// The computed-row records of every tile, from the schedule-ordered operator
+// (slot k of row j at vals[k * R + j]), diagonal factors and right-hand side.
+template <class T, class V>
+__global__ void k_tile_ops(const int2* __restrict__ entries, const int* __restrict__ desc, const V* __restrict__ vals,
+                           int R, const T* __restrict__ b, const T* __restrict__ inv_diag,
+                           const unsigned char* __restrict__ has_diag, TiledOp<T, V>* __restrict__ ops) {
+  const int t = blockIdx.x;
+  const int c0 = desc[TILE_DESC_INTS * t], stop = desc[TILE_DESC_INTS * t + 4], base = desc[TILE_DESC_INTS * t + 10];
+  for (int e = c0 + (int)threadIdx.x; e < stop; e += (int)blockDim.x) {
+    const unsigned j = (unsigned)entries[e].x;
+    TiledOp<T, V> o;
+#pragma unroll
+    for (int k = 0; k < 27; ++k) o.v[k] = vals[(size_t)k * R + j];
+    o.has = has_diag[j];
+    o.inv = inv_diag[j];
+    o.b = b[j];
+    ops[base + (e - c0)] = o;
+  }
+}
 
 // The slices of one wave: tiles [start[s], end[s]) of slice z0 + 2s; bit s of
 // flip selects buffer 1 as the slice's read buffer.
@@ -1457,28 +1493,33 @@ __device__ __forceinline__ int grid_index(int lx, int ly, int gh) {
   return (((ly & 1) << 1) | (lx & 1)) * gh * gh + (lx >> 1) * gh + (ly >> 1);
 }
 
-// One computed row: operator values, right-hand side and diagonal factor;
-// base = (lx >> 1) * gh + (ly >> 1), the row's offset inside its plane.
+// One computed row: its record and position; base = (lx >> 1) * gh +
+// (ly >> 1), the row's offset inside its plane.
 template <class T, class V> struct TiledRow {
   int j, base, interior;
-  V v[27];
-  T b, inv;
-  unsigned char has;
+  TiledOp<T, V> op;
 };
 
+typedef int tiled_word __attribute__((ext_vector_type(4)));
+
 template <class T, class V>
-__device__ __forceinline__ void tiled_fetch(const TiledArgs<T, V>& a, int2 en, int gh, TiledRow<T, V>& w) {
-  const int lx = en.y & 255, ly = (en.y >> 8) & 255;
-  w.j = en.x;
-  w.base = (lx >> 1) * gh + (ly >> 1);
-  w.interior = (en.y >> 16) & 1;
-  const size_t R = (size_t)a.R;
-  const unsigned j = (unsigned)en.x;
+__device__ __forceinline__ void tiled_load(const TiledArgs<T, V>& a, int record, TiledRow<T, V>& w) {
+  constexpr int words = (int)(sizeof(TiledOp<T, V>) / sizeof(tiled_word));
+  union {
+    tiled_word q[words];
+    TiledOp<T, V> r;
+  } u;
+  const tiled_word* src = (const tiled_word*)(a.ops + record);
 #pragma unroll
-  for (int k = 0; k < 27; ++k) w.v[k] = (a.vals + k * R)[j];
-  w.b = a.b[j];
-  w.inv = a.inv_diag[j];
-  w.has = a.has_diag[j];
+  for (int i = 0; i < words; ++i) u.q[i] = src[i];
+  w.op = u.r;
+}
+
+__device__ __forceinline__ void tiled_place(int2 en, int gh, int& j, int& base, int& interior) {
+  const int lx = en.y & 255, ly = (en.y >> 8) & 255;
+  j = en.x;
+  base = (lx >> 1) * gh + (ly >> 1);
+  interior = (en.y >> 16) & 1;
 }
 
 // The update of a row of colour C (x parity C & 1, y parity C >> 1):
@@ -1488,18 +1529,18 @@ template <int C, class T, class V>
 __device__ __forceinline__ void tiled_update(T* grid, int gh, T* Q, const TiledRow<T, V>& w) {
   constexpr int px = C & 1, py = C >> 1;
   const int pl = gh * gh, layer = 4 * pl;
-  T r = w.b;
+  T r = w.op.b;
 #pragma unroll
   for (int k = 0; k < 27; ++k) {
     const int dx = k / 9 - 1, dy = (k / 3) % 3 - 1, dz = k % 3 - 1;
     const int qx = px + dx + 2, qy = py + dy + 2;  // shifted to stay nonnegative
     const int plane = ((qy & 1) << 1) | (qx & 1);
     const int off = (dz + 1) * layer + plane * pl + ((qx >> 1) - 1) * gh + ((qy >> 1) - 1);
-    r = r - grid[off + w.base] * T(w.v[k]);
+    r = r - grid[off + w.base] * T(w.op.v[k]);
   }
   const int self = layer + (py * 2 + px) * pl + w.base;
   const T xi = grid[self];
-  const T nv = w.has ? xi + r * w.inv : xi;
+  const T nv = w.op.has ? xi + r * w.op.inv : xi;
   grid[self] = nv;
   if (w.interior) Q[w.j] = nv;
 }
@@ -1507,8 +1548,8 @@ __device__ __forceinline__ void tiled_update(T* grid, int gh, T* Q, const TiledR
 // BS threads per tile: thread i computes the tile's i-th row (rows of colour
 // c are consecutive, so a colour's rows sit on consecutive threads) and
 // stages grid values i, i + BS, ...  Loads go out in dependency steps:
-// descriptor; entries; operator rows and staged values.  Tiles with more
-// than BS computed rows take further passes for the rest.
+// descriptor; entries and row records; staged values.  Tiles with more than
+// BS computed rows take further passes for the rest.
 constexpr int TILED_STAGE_UNROLL = 4;
 
 template <class T, class V>
@@ -1541,39 +1582,50 @@ __global__ void __launch_bounds__(BS) k_relax_tiled(TiledArgs<T, V> a, WaveTiles
   // (layer 2, read buffer of the write side) and slice z - 1 [d2.x, d2.y)
   // (layer 0), as one index range
   const int n_own = d1.y - d1.x, n_up = d1.w - d1.z, n_all = n_own + n_up + (d2.y - d2.x);
-  for (int i = tid; i < 3 * layer; i += BS) grid[i] = T(0);
   const int e = c0 + tid;
   const bool have = e < stage0;
   const int colour = e < c1 ? 0 : (e < c2 ? 1 : (e < c3 ? 2 : 3));
+  // Loads are issued unconditionally, at clamped indices (the tile has
+  // computed rows and staged values of its own slice, so c0 and d1.x are
+  // entries of it), and only the stores are predicated: with no branch
+  // around them, each dependency step waits once for all of its loads.
+  const int ec = have ? e : c0;
   TiledRow<T, V> w;
   int2 sen[TILED_STAGE_UNROLL];
   int slay[TILED_STAGE_UNROLL];
   T sval[TILED_STAGE_UNROLL];
-  const int2 en = have ? a.entries[e] : make_int2(0, 0);
-  auto staged = [&](int i, int& lay) {
-    if (i < n_own) { lay = 1; return d1.x + i; }
-    if (i < n_own + n_up) { lay = 2; return d1.z + (i - n_own); }
-    lay = 0;
-    return d2.x + (i - n_own - n_up);
+  auto stage_entries = [&](int first) {
+#pragma unroll
+    for (int u = 0; u < TILED_STAGE_UNROLL; ++u) {
+      const int i = first + u * BS;
+      const int ic = i < n_all ? i : 0;
+      const int lay = ic < n_own ? 1 : (ic < n_own + n_up ? 2 : 0);
+      const int idx = lay == 1 ? d1.x + ic : (lay == 2 ? d1.z + (ic - n_own) : d2.x + (ic - n_own - n_up));
+      slay[u] = i < n_all ? lay : -1;
+      sen[u] = a.entries[idx];
+    }
   };
+  auto stage_values = [&]() {
 #pragma unroll
-  for (int u = 0; u < TILED_STAGE_UNROLL; ++u) {
-    const int i = tid + u * BS;
-    slay[u] = -1;
-    if (i < n_all) sen[u] = a.entries[staged(i, slay[u])];
-  }
-  if (have) tiled_fetch(a, en, gh, w);
+    for (int u = 0; u < TILED_STAGE_UNROLL; ++u) sval[u] = (slay[u] == 2 ? (const T*)Q : P)[(unsigned)sen[u].x];
+  };
+  auto stage_store = [&]() {
 #pragma unroll
-  for (int u = 0; u < TILED_STAGE_UNROLL; ++u)
-    if (slay[u] >= 0) sval[u] = (slay[u] == 2 ? (const T*)Q : P)[(unsigned)sen[u].x];
+    for (int u = 0; u < TILED_STAGE_UNROLL; ++u)
+      if (slay[u] >= 0) grid[slay[u] * layer + grid_index(sen[u].y & 255, (sen[u].y >> 8) & 255, gh)] = sval[u];
+  };
+  stage_entries(tid);
+  const int2 en = a.entries[ec];
+  tiled_load(a, d2.z + (ec - c0), w);
+  stage_values();
+  for (int i = tid; i < 3 * layer; i += BS) grid[i] = T(0);
+  tiled_place(en, gh, w.j, w.base, w.interior);
   __syncthreads();  // grid cleared
-#pragma unroll
-  for (int u = 0; u < TILED_STAGE_UNROLL; ++u)
-    if (slay[u] >= 0) grid[slay[u] * layer + grid_index(sen[u].y & 255, (sen[u].y >> 8) & 255, gh)] = sval[u];
-  for (int i = tid + TILED_STAGE_UNROLL * BS; i < n_all; i += BS) {
-    int lay;
-    const int2 ge = a.entries[staged(i, lay)];
-    grid[lay * layer + grid_index(ge.y & 255, (ge.y >> 8) & 255, gh)] = (lay == 2 ? (const T*)Q : P)[(unsigned)ge.x];
+  stage_store();
+  for (int first = tid + TILED_STAGE_UNROLL * BS; first - tid < n_all; first += TILED_STAGE_UNROLL * BS) {
+    stage_entries(first);
+    stage_values();
+    stage_store();
   }
   __syncthreads();  // grid staged
   // Further rows of a thread come after its first row in entry order, so
@@ -1584,7 +1636,8 @@ __global__ void __launch_bounds__(BS) k_relax_tiled(TiledArgs<T, V> a, WaveTiles
     if (have && colour == phase) tiled_phase(phase, grid, gh, Q, w);
     for (int e2 = c0 + tid + BS; e2 < stage0; e2 += BS) {
       if (e2 < cstart[phase] || e2 >= cstart[phase + 1]) continue;
-      tiled_fetch(a, a.entries[e2], gh, w);
+      tiled_load(a, d2.z + (e2 - c0), w);
+      tiled_place(a.entries[e2], gh, w.j, w.base, w.interior);
       tiled_phase(phase, grid, gh, Q, w);
     }
     __syncthreads();
@@ -1837,6 +1890,7 @@ struct TileTable {
   int max_tiles = 0;             // most tiles of one slice
   int max_rows = 0;              // most computed rows of one tile
   int max_staged = 0;            // most staged values of one tile
+  int computed = 0;              // computed-row records of all tiles
 };
 
 // Tile side for a level: the run's relax_tile (8..40, which keeps the three
@@ -1889,13 +1943,20 @@ static void build_tile_table(const int64_t* keys, const int* order, int R, int d
                      (const int*)compact.data(), ndense, dense_of.data());
   KERNEL_CHECK();
   // slice starts and stats in one buffer, read once
-  DArray<int> small(n + 2 + 2, s);
-  HIP_CHECK(hipMemsetAsync(small.data() + n + 2, 0, 2 * sizeof(int), s));
+  DArray<int> small(n + 2 + 3, s);
+  HIP_CHECK(hipMemsetAsync(small.data() + n + 2, 0, 3 * sizeof(int), s));
   tt.desc.reset(TILE_DESC_INTS * (size_t)std::max(tt.tiles, 1), s);
+  DArray<int> computed(std::max(tt.tiles, 1), s);
   hipLaunchKernelGGL(k_tile_desc, dim3(blocks_for(tt.tiles, 256)), dim3(256), 0, s, (const uint32_t*)skeys.data(), E, tt.tiles,
                      (const int*)dense_of.data(), (const int*)flags.data(), (const int*)compact.data(), per_slice, ndense,
-                     tt.desc.data(), small.data() + n + 2);
+                     tt.desc.data(), computed.data(), small.data() + n + 2);
   KERNEL_CHECK();
+  if (tt.tiles) {
+    scan_exclusive_i32(computed.data(), computed.data(), tt.tiles, small.data() + n + 4, s);
+    hipLaunchKernelGGL(k_tile_record_base, dim3(blocks_for(tt.tiles, 256)), dim3(256), 0, s, (const int*)computed.data(),
+                       tt.tiles, tt.desc.data());
+    KERNEL_CHECK();
+  }
   tt.entries.reset(std::max(E, 1), s);
   hipLaunchKernelGGL(k_tile_entries, dim3(blocks_for(E, 256)), dim3(256), 0, s, (const uint32_t*)skeys.data(),
                      (const int*)svals.data(), E, (const int*)dense_of.data(), order, keys, ts, tt.nt, tt.entries.data());
@@ -1903,11 +1964,12 @@ static void build_tile_table(const int64_t* keys, const int* order, int R, int d
   hipLaunchKernelGGL(k_tile_slice_start, dim3(blocks_for(n + 2, 256)), dim3(256), 0, s, (const int*)compact.data(), n,
                      (int)per_slice, small.data());
   KERNEL_CHECK();
-  std::vector<int> h(n + 4);
-  read_bytes(h.data(), small.data(), sizeof(int) * (n + 4), s);
+  std::vector<int> h(n + 5);
+  read_bytes(h.data(), small.data(), sizeof(int) * (n + 5), s);
   tt.slice_start.assign(h.begin(), h.begin() + n + 2);
   tt.max_rows = h[n + 2];
   tt.max_staged = h[n + 3];
+  tt.computed = h[n + 4];
   tt.max_tiles = 0;
   for (int z = 0; z <= n; ++z) tt.max_tiles = std::max(tt.max_tiles, tt.slice_start[z + 1] - tt.slice_start[z]);
 }
@@ -1971,6 +2033,12 @@ static std::unique_ptr<TiledGraph> build_tiled_graph(int n, int iters, int width
 // absent slots must hold +0 (k_level_operator stores them so).
 template <class T, class V>
 int relax_tiled_graph(const GsOperator<T, V>& op, const TileTable& tt, T* x, T* x1, int depth, int iters, hipStream_t s) {
+  DArray<TiledOp<T, V>> ops(std::max(tt.computed, 1), s);
+  if (tt.tiles > 0) {
+    hipLaunchKernelGGL((k_tile_ops<T, V>), dim3(tt.tiles), dim3(64), 0, s, tt.entries.data(), tt.desc.data(),
+                       op.vals.data(), op.R, op.b.data(), op.inv_diag.data(), op.has_diag.data(), ops.data());
+    KERNEL_CHECK();
+  }
   static std::mutex mu;
   static std::map<std::tuple<int, int, int, int, unsigned, int, hipStream_t>, std::unique_ptr<TiledGraph>> cache;
   const int n = 1 << depth;
@@ -1994,8 +2062,7 @@ int relax_tiled_graph(const GsOperator<T, V>& op, const TileTable& tt, T* x, T* 
   if (iters % 2) HIP_CHECK(hipMemcpyAsync(x1, x, sizeof(T) * op.R, hipMemcpyDeviceToDevice, s));
   std::lock_guard<std::mutex> guard(g->mu);
   HIP_CHECK(hipEventSynchronize(g->done));
-  TiledArgs<T, V> a{tt.entries.data(), tt.desc.data(), op.vals.data(), op.b.data(), op.inv_diag.data(),
-                    op.has_diag.data(), x, x1, op.R, gside};
+  TiledArgs<T, V> a{tt.entries.data(), tt.desc.data(), ops.data(), x, x1, gside};
   WaveTiles wv;
   void* args[2] = {&a, &wv};
   hipKernelNodeParams kp = g->params;
