@@ -380,9 +380,10 @@ __global__ void __launch_bounds__(128) k_screening(const int64_t* __restrict__ r
 // levels, in one pass per schedule position j: the 27 stiffness entries,
 // plus (SCREEN) the row's screening sums, rounded once to the stored type V
 // -- the same value the separate stiffness, screening and cast passes give.
-// For the level's own operator (inv != nullptr) the columns are also written
-// as schedule positions, with the diagonal factors 1/A_ii and the count of
-// present slots taken from the same registers.
+// For the level's own operator (inv_diag != nullptr) the diagonal factors
+// 1/A_ii and the count of present slots are taken from the same registers,
+// and with sched_cols (the colour-per-launch relaxation's input) the columns
+// are also written as schedule positions (inv: the schedule's inverse).
 template <class T, class V, bool SCREEN>
 __global__ void __launch_bounds__(128) k_level_operator(const int64_t* __restrict__ rows, int R, const int* __restrict__ row_order,
                                                         const int64_t* __restrict__ cols, int64_t C, ColumnTable ct,
@@ -408,13 +409,13 @@ __global__ void __launch_bounds__(128) k_level_operator(const int64_t* __restric
       const V stored = V(found && SCREEN ? v + acc[k] : v);
       out_cols[(int64_t)k * R + j] = (int)index;
       out_vals[(int64_t)k * R + j] = stored;
-      if (inv) {
-        sched_cols[(int64_t)k * R + j] = found ? inv[index] : -1;
+      if (inv_diag) {
+        if (sched_cols) sched_cols[(int64_t)k * R + j] = found ? inv[index] : -1;
         if (index == i) diagonal = T(stored);
         count += found ? 1 : 0;
       }
     });
-    if (inv) {
+    if (inv_diag) {
       has_diag[j] = diagonal != T(0);
       inv_diag[j] = diagonal != T(0) ? T(1) / diagonal : T(0);
     }
@@ -2224,7 +2225,7 @@ static void level_operator(const LevelState<T>& rowsL, const LevelState<T>& cols
   out.cols.reset(27 * (size_t)R, s);
   out.vals.reset(27 * (size_t)R, s);
   if (op) {
-    op->cols.reset(27 * (size_t)R, s);
+    if (inv) op->cols.reset(27 * (size_t)R, s);
     op->inv_diag.reset(R, s);
     op->has_diag.reset(R, s);
   }
@@ -2235,7 +2236,7 @@ static void level_operator(const LevelState<T>& rowsL, const LevelState<T>& cols
   KERNEL_CHECK();
   const PointBins<T> pb = pts ? point_bins(*pts) : PointBins<T>{};
   const int* dense = colsL.dense_dim ? (const int*)colsL.dense.data() : (const int*)nullptr;
-  int* sched = op ? op->cols.data() : nullptr;
+  int* sched = op && inv ? op->cols.data() : nullptr;
   T* inv_diag = op ? op->inv_diag.data() : nullptr;
   unsigned char* has_diag = op ? op->has_diag.data() : nullptr;
   if (pts)
@@ -2787,11 +2788,15 @@ static void prepare_finer(Result<T>& r, int d, Stencil<V>& Aop, Stencil<V>& Cop,
   if (!screened) L.point_load.zero();
   PointSet<T> pts;
   if (screened) pts = point_constraints(r, d, s, false);
-  // x is relaxed in schedule order: columns become schedule positions, so
-  // a phase writes contiguous ranges and gathers from nearby ranges
-  DArray<int> inv(R, s);
-  hipLaunchKernelGGL(k_inverse_permutation, dim3(blocks_for(R, 256)), dim3(256), 0, s, ord, R, inv.data());
-  KERNEL_CHECK();
+  // x is relaxed in schedule order: for the colour-per-launch form columns
+  // become schedule positions, so a phase writes contiguous ranges and
+  // gathers from nearby ranges (the tiled form reads no columns)
+  DArray<int> inv;
+  if (!f.tiles.tiles) {
+    inv.reset(R, s);
+    hipLaunchKernelGGL(k_inverse_permutation, dim3(blocks_for(R, 256)), dim3(256), 0, s, ord, R, inv.data());
+    KERNEL_CHECK();
+  }
   f.nnz.reset(1, s);
   f.nnz.zero();
   GsOperator<T, V>& op = f.op;
