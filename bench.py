@@ -50,26 +50,32 @@ def measured_traffic(kernel):
     return doc["bytes_per_launch"] if doc.get("kernel") == kernel else None
 
 
+def relax_bytes(st, v, s, tiled):
+    """Algorithmic bytes of one level's relaxation: per sweep, the operator's
+    stored values (nnz * v), the right-hand side, and x read and written once
+    per row (3 * R * s), plus the row structure -- one int per row for the
+    tiled form (its grid position; neighbours are found by offset), the CSR
+    row extents and nnz column indices for the colour-per-launch form."""
+    per_sweep = st["nnz"] * v + 3 * st["rows"] * s
+    per_sweep += st["rows"] * 4 if tiled else (st["rows"] + 1) * 4 + st["nnz"] * 4
+    return per_sweep * st["sweeps"]
+
+
 def relax_roofline(metadata, value_bytes, vector_bytes, kernel=None):
     """Sliced Gauss-Seidel relaxation of every finer level: algorithmic bytes
-    over the HIP-event span of its launches on the reconstruction stream.
-    The tiled form (k_relax_tiled) launches once per wave; the
-    colour-per-launch form (k_relax_wave_args) four times.
-
-    Per level and sweep: nnz*(4 + v) (column index + operator value) +
-    (R+1)*4 (row extent) + R*s (b) + 2*R*s (x read and written), v = operator
-    value bytes, s = vector bytes; times the sweeps, summed over levels.  (The
-    tiled form reads no column indices; the CSR-equivalent count is kept so
-    both forms are measured against the same work.)
+    (relax_bytes) over the HIP-event span of its launches on the
+    reconstruction stream.  The tiled form (k_relax_tiled) launches once per
+    wave, after one k_tile_ops per level (inside the span); the
+    colour-per-launch form (k_relax_wave_args) four times per wave.
     """
     stats = metadata["relaxation"]
     v, s = value_bytes, vector_bytes
-    total = sum((st["nnz"] * (4 + v) + (st["rows"] + 1) * 4 + 3 * st["rows"] * s) * st["sweeps"] for st in stats)
+    tiled = all(st.get("tile", 0) > 0 for st in stats)
+    total = sum(relax_bytes(st, v, s, tiled) for st in stats)
     seconds = sum(st["seconds"] for st in stats)
     launches = sum(st["launches"] for st in stats)
     achieved = total / seconds / 1e9
     fine = max(stats, key=lambda st: st["rows"])
-    tiled = all(st.get("tile", 0) > 0 for st in stats)
     name = "k_relax_tiled" if tiled else "k_relax_wave_args"
     return {
         "kernel": f"{name} (sliced Gauss-Seidel, all finer levels)",
@@ -89,8 +95,7 @@ def relax_roofline(metadata, value_bytes, vector_bytes, kernel=None):
         "finest": {"depth": fine["depth"], "rows": fine["rows"], "nnz": fine["nnz"], "launches": fine["launches"],
                    "seconds": fine["seconds"]},
         "per_level": [{"depth": st["depth"], "launches": st["launches"], "us_per_launch": round(st["seconds"] / st["launches"] * 1e6, 3),
-                       "GBps": round((st["nnz"] * (4 + v) + (st["rows"] + 1) * 4 + 3 * st["rows"] * s) * st["sweeps"]
-                                     / st["seconds"] / 1e9, 1), "tile": st.get("tile", 0),
+                       "GBps": round(relax_bytes(st, v, s, tiled) / st["seconds"] / 1e9, 1), "tile": st.get("tile", 0),
                        "tiles": st.get("tiles", 0), "tile_entries": st.get("tile_entries", 0)} for st in stats],
     }
 
