@@ -376,14 +376,31 @@ __global__ void __launch_bounds__(128) k_screening(const int64_t* __restrict__ r
     if (scols[(int64_t)k * R + j] >= 0) svals[(int64_t)k * R + j] += acc[k];
 }
 
+// One row of a finer level's operator for the tiled relaxation: its 27 slot
+// values, diagonal flag and 1/A_ii, at schedule position j, so a row's
+// operator is a few consecutive 16-byte words.
+template <class T, class V> struct alignas(16) RowRecord {
+  V v[27];
+  int has;
+  T inv;
+};
+
+typedef int record_word __attribute__((ext_vector_type(4)));
+
+template <class T, class V> union RowRecordWords {
+  record_word q[sizeof(RowRecord<T, V>) / sizeof(record_word)];
+  RowRecord<T, V> r;
+};
+
 // stiffness.py:38-109 with screening_stencils.py:34-70 for the sliced finer
 // levels, in one pass per schedule position j: the 27 stiffness entries,
 // plus (SCREEN) the row's screening sums, rounded once to the stored type V
 // -- the same value the separate stiffness, screening and cast passes give.
 // For the level's own operator (inv_diag != nullptr) the diagonal factors
-// 1/A_ii and the count of present slots are taken from the same registers,
-// and with sched_cols (the colour-per-launch relaxation's input) the columns
-// are also written as schedule positions (inv: the schedule's inverse).
+// 1/A_ii and the count of present slots are taken from the same registers;
+// with sched_cols (the colour-per-launch relaxation's input) the columns are
+// also written as schedule positions (inv: the schedule's inverse), and with
+// records (the tiled relaxation's input) each row's record.
 template <class T, class V, bool SCREEN>
 __global__ void __launch_bounds__(128) k_level_operator(const int64_t* __restrict__ rows, int R, const int* __restrict__ row_order,
                                                         const int64_t* __restrict__ cols, int64_t C, ColumnTable ct,
@@ -392,6 +409,7 @@ __global__ void __launch_bounds__(128) k_level_operator(const int64_t* __restric
                                                         int* __restrict__ out_cols, V* __restrict__ out_vals,
                                                         const int* __restrict__ inv, int* __restrict__ sched_cols,
                                                         T* __restrict__ inv_diag, unsigned char* __restrict__ has_diag,
+                                                        RowRecord<T, V>* __restrict__ records,
                                                         unsigned long long* __restrict__ present) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;  // output position
   int count = 0;
@@ -404,11 +422,13 @@ __global__ void __launch_bounds__(128) k_level_operator(const int64_t* __restric
       if (load) load[j] = rhs;
     }
     T diagonal = T(0);
+    RowRecordWords<T, V> rec;
     stiffness_row(key, cols, C, ct, dense, dense_dim, depth, trial_depth, table, [&](int k, int64_t index, double v) {
       const bool found = index >= 0;
       const V stored = V(found && SCREEN ? v + acc[k] : v);
       out_cols[(int64_t)k * R + j] = (int)index;
       out_vals[(int64_t)k * R + j] = stored;
+      rec.r.v[k] = stored;
       if (inv_diag) {
         if (sched_cols) sched_cols[(int64_t)k * R + j] = found ? inv[index] : -1;
         if (index == i) diagonal = T(stored);
@@ -418,6 +438,13 @@ __global__ void __launch_bounds__(128) k_level_operator(const int64_t* __restric
     if (inv_diag) {
       has_diag[j] = diagonal != T(0);
       inv_diag[j] = diagonal != T(0) ? T(1) / diagonal : T(0);
+      if (records) {
+        rec.r.has = diagonal != T(0);
+        rec.r.inv = diagonal != T(0) ? T(1) / diagonal : T(0);
+        record_word* dst = (record_word*)(records + j);
+#pragma unroll
+        for (int q = 0; q < (int)(sizeof(rec.q) / sizeof(record_word)); ++q) dst[q] = rec.q[q];
+      }
     }
   }
   if (present) {
@@ -818,6 +845,8 @@ __device__ __forceinline__ void gs_update(int j, int R, const int* __restrict__ 
                                           const unsigned char* __restrict__ has_diag, T* __restrict__ x) {
   // All column loads, then all value/x loads in flight, then the subtraction
   // chain in slot order; absent slots are skipped by a select (no rounding).
+  // An absent slot loads x[0], which exists: row j belongs to this level, so
+  // the level (and x) holds at least one row.
   int c[27];
 #pragma unroll
   for (int k = 0; k < 27; ++k) c[k] = cols[(int64_t)k * R + j];
@@ -1319,8 +1348,7 @@ __global__ void k_diag_sched(const int* __restrict__ order, int R, const int* __
 // staged values of the tile's slice (4) -- and per tile 12 ints: the five
 // category starts, the end, the staged ranges of the tiles at the same
 // (x, y) in slices z + 1 and z - 1 (empty when absent), which are exactly the
-// values the tile needs from those slices, and the tile's first computed-row
-// record.
+// values the tile needs from those slices.
 constexpr int TILE_DESC_INTS = 12;
 
 __device__ __forceinline__ int ring_of(int v, int t0, int ts) {
@@ -1384,7 +1412,7 @@ __device__ __forceinline__ int lower_bound_u32(const uint32_t* __restrict__ a, i
 // staged values (all three slices).
 __global__ void k_tile_desc(const uint32_t* __restrict__ skeys, int E, int tiles, const int* __restrict__ dense_of,
                             const int* __restrict__ flags, const int* __restrict__ compact, int64_t per_slice,
-                            int64_t ndense, int* __restrict__ desc, int* __restrict__ computed, int* __restrict__ stats) {
+                            int64_t ndense, int* __restrict__ desc, int* __restrict__ stats) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= tiles) return;
   int* o = desc + TILE_DESC_INTS * t;
@@ -1404,7 +1432,6 @@ __global__ void k_tile_desc(const uint32_t* __restrict__ skeys, int E, int tiles
     staged += hi - lo;
   }
   o[10] = o[11] = 0;
-  computed[t] = o[4] - o[0];
   atomicMax(&stats[0], o[4] - o[0]);
   atomicMax(&stats[1], staged);
 }
@@ -1424,58 +1451,21 @@ __global__ void k_tile_entries(const uint32_t* __restrict__ skeys, const int* __
   entries[e] = make_int2(j, lx | (ly << 8) | (interior << 16));
 }
 
-// desc[10]: the tile's first computed-row record, from the exclusive scan of
-// the computed-row counts
-__global__ void k_tile_record_base(const int* __restrict__ base, int tiles, int* __restrict__ desc) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < tiles) desc[TILE_DESC_INTS * t + 10] = base[t];
-}
-
 // first tile of each slice z (z = 0..n + 1) from the exclusive scan of the flags
 __global__ void k_tile_slice_start(const int* __restrict__ compact, int n, int per_slice, int* __restrict__ out) {
   const int z = blockIdx.x * blockDim.x + threadIdx.x;
   if (z <= n + 1) out[z] = compact[(int64_t)z * per_slice];
 }
 
-// One computed row of a tile: its 27 operator slots, diagonal flag, inverse
-// diagonal and right-hand side, stored per computed tile entry (tile by tile,
-// entry order), so a workgroup reads its rows as consecutive 16-byte words
-// addressed by the tile descriptor alone.  A row computed by several tiles
-// has a record in each.
-template <class T, class V> struct alignas(16) TiledOp {
-  V v[27];
-  int has;
-  T inv, b;
-};
-
 template <class T, class V> struct TiledArgs {
   const int2* entries;
   const int* desc;
-  const TiledOp<T, V>* ops;
+  const RowRecord<T, V>* records;  // by schedule position
+  const T* b;                      // right-hand side by schedule position
   T* x0;
   T* x1;
   int g;  // grid side, ts + 8
 };
-
-// The computed-row records of every tile, from the schedule-ordered operator
-// (slot k of row j at vals[k * R + j]), diagonal factors and right-hand side.
-template <class T, class V>
-__global__ void k_tile_ops(const int2* __restrict__ entries, const int* __restrict__ desc, const V* __restrict__ vals,
-                           int R, const T* __restrict__ b, const T* __restrict__ inv_diag,
-                           const unsigned char* __restrict__ has_diag, TiledOp<T, V>* __restrict__ ops) {
-  const int t = blockIdx.x;
-  const int c0 = desc[TILE_DESC_INTS * t], stop = desc[TILE_DESC_INTS * t + 4], base = desc[TILE_DESC_INTS * t + 10];
-  for (int e = c0 + (int)threadIdx.x; e < stop; e += (int)blockDim.x) {
-    const unsigned j = (unsigned)entries[e].x;
-    TiledOp<T, V> o;
-#pragma unroll
-    for (int k = 0; k < 27; ++k) o.v[k] = vals[(size_t)k * R + j];
-    o.has = has_diag[j];
-    o.inv = inv_diag[j];
-    o.b = b[j];
-    ops[base + (e - c0)] = o;
-  }
-}
 
 // The slices of one wave: tiles [start[s], end[s]) of slice z0 + 2s; bit s of
 // flip selects buffer 1 as the slice's read buffer.
@@ -1494,33 +1484,31 @@ __device__ __forceinline__ int grid_index(int lx, int ly, int gh) {
   return (((ly & 1) << 1) | (lx & 1)) * gh * gh + (lx >> 1) * gh + (ly >> 1);
 }
 
-// One computed row: its record and position; base = (lx >> 1) * gh +
-// (ly >> 1), the row's offset inside its plane.
+// One computed row: its record, right-hand side and position; base =
+// (lx >> 1) * gh + (ly >> 1), the row's offset inside its plane.
 template <class T, class V> struct TiledRow {
   int j, base, interior;
-  TiledOp<T, V> op;
+  RowRecord<T, V> op;
+  T b;
 };
 
-typedef int tiled_word __attribute__((ext_vector_type(4)));
-
 template <class T, class V>
-__device__ __forceinline__ void tiled_load(const TiledArgs<T, V>& a, int record, TiledRow<T, V>& w) {
-  constexpr int words = (int)(sizeof(TiledOp<T, V>) / sizeof(tiled_word));
-  union {
-    tiled_word q[words];
-    TiledOp<T, V> r;
-  } u;
-  const tiled_word* src = (const tiled_word*)(a.ops + record);
-#pragma unroll
-  for (int i = 0; i < words; ++i) u.q[i] = src[i];
-  w.op = u.r;
+__device__ __forceinline__ void tiled_place(int2 en, int gh, TiledRow<T, V>& w) {
+  const int lx = en.y & 255, ly = (en.y >> 8) & 255;
+  w.j = en.x;
+  w.base = (lx >> 1) * gh + (ly >> 1);
+  w.interior = (en.y >> 16) & 1;
 }
 
-__device__ __forceinline__ void tiled_place(int2 en, int gh, int& j, int& base, int& interior) {
-  const int lx = en.y & 255, ly = (en.y >> 8) & 255;
-  j = en.x;
-  base = (lx >> 1) * gh + (ly >> 1);
-  interior = (en.y >> 16) & 1;
+// Row w.j's record and right-hand side.
+template <class T, class V>
+__device__ __forceinline__ void tiled_load(const TiledArgs<T, V>& a, TiledRow<T, V>& w) {
+  RowRecordWords<T, V> u;
+  const record_word* src = (const record_word*)(a.records + (unsigned)w.j);
+#pragma unroll
+  for (int i = 0; i < (int)(sizeof(u.q) / sizeof(record_word)); ++i) u.q[i] = src[i];
+  w.op = u.r;
+  w.b = a.b[(unsigned)w.j];
 }
 
 // The update of a row of colour C (x parity C & 1, y parity C >> 1):
@@ -1530,7 +1518,7 @@ template <int C, class T, class V>
 __device__ __forceinline__ void tiled_update(T* grid, int gh, T* Q, const TiledRow<T, V>& w) {
   constexpr int px = C & 1, py = C >> 1;
   const int pl = gh * gh, layer = 4 * pl;
-  T r = w.op.b;
+  T r = w.b;
 #pragma unroll
   for (int k = 0; k < 27; ++k) {
     const int dx = k / 9 - 1, dy = (k / 3) % 3 - 1, dz = k % 3 - 1;
@@ -1617,10 +1605,13 @@ __global__ void __launch_bounds__(BS) k_relax_tiled(TiledArgs<T, V> a, WaveTiles
   };
   stage_entries(tid);
   const int2 en = a.entries[ec];
-  tiled_load(a, d2.z + (ec - c0), w);
+  // a thread's second row (tiles of more than BS computed rows): its entry
+  // now, so its record loads need no further step
+  const int2 en2 = a.entries[e + BS < stage0 ? e + BS : c0];
   stage_values();
+  tiled_place(en, gh, w);
+  tiled_load(a, w);
   for (int i = tid; i < 3 * layer; i += BS) grid[i] = T(0);
-  tiled_place(en, gh, w.j, w.base, w.interior);
   __syncthreads();  // grid cleared
   stage_store();
   for (int first = tid + TILED_STAGE_UNROLL * BS; first - tid < n_all; first += TILED_STAGE_UNROLL * BS) {
@@ -1635,10 +1626,10 @@ __global__ void __launch_bounds__(BS) k_relax_tiled(TiledArgs<T, V> a, WaveTiles
   const int cstart[5] = {c0, c1, c2, c3, stage0};
   for (int phase = 0; phase < 4; ++phase) {
     if (have && colour == phase) tiled_phase(phase, grid, gh, Q, w);
-    for (int e2 = c0 + tid + BS; e2 < stage0; e2 += BS) {
+    for (int e2 = e + BS; e2 < stage0; e2 += BS) {
       if (e2 < cstart[phase] || e2 >= cstart[phase + 1]) continue;
-      tiled_load(a, d2.z + (e2 - c0), w);
-      tiled_place(a.entries[e2], gh, w.j, w.base, w.interior);
+      tiled_place(e2 == e + BS ? en2 : a.entries[e2], gh, w);
+      tiled_load(a, w);
       tiled_phase(phase, grid, gh, Q, w);
     }
     __syncthreads();
@@ -1891,7 +1882,6 @@ struct TileTable {
   int max_tiles = 0;             // most tiles of one slice
   int max_rows = 0;              // most computed rows of one tile
   int max_staged = 0;            // most staged values of one tile
-  int computed = 0;              // computed-row records of all tiles
 };
 
 // Tile side for a level: the run's relax_tile (8..40, which keeps the three
@@ -1944,20 +1934,13 @@ static void build_tile_table(const int64_t* keys, const int* order, int R, int d
                      (const int*)compact.data(), ndense, dense_of.data());
   KERNEL_CHECK();
   // slice starts and stats in one buffer, read once
-  DArray<int> small(n + 2 + 3, s);
-  HIP_CHECK(hipMemsetAsync(small.data() + n + 2, 0, 3 * sizeof(int), s));
+  DArray<int> small(n + 2 + 2, s);
+  HIP_CHECK(hipMemsetAsync(small.data() + n + 2, 0, 2 * sizeof(int), s));
   tt.desc.reset(TILE_DESC_INTS * (size_t)std::max(tt.tiles, 1), s);
-  DArray<int> computed(std::max(tt.tiles, 1), s);
   hipLaunchKernelGGL(k_tile_desc, dim3(blocks_for(tt.tiles, 256)), dim3(256), 0, s, (const uint32_t*)skeys.data(), E, tt.tiles,
                      (const int*)dense_of.data(), (const int*)flags.data(), (const int*)compact.data(), per_slice, ndense,
-                     tt.desc.data(), computed.data(), small.data() + n + 2);
+                     tt.desc.data(), small.data() + n + 2);
   KERNEL_CHECK();
-  if (tt.tiles) {
-    scan_exclusive_i32(computed.data(), computed.data(), tt.tiles, small.data() + n + 4, s);
-    hipLaunchKernelGGL(k_tile_record_base, dim3(blocks_for(tt.tiles, 256)), dim3(256), 0, s, (const int*)computed.data(),
-                       tt.tiles, tt.desc.data());
-    KERNEL_CHECK();
-  }
   tt.entries.reset(std::max(E, 1), s);
   hipLaunchKernelGGL(k_tile_entries, dim3(blocks_for(E, 256)), dim3(256), 0, s, (const uint32_t*)skeys.data(),
                      (const int*)svals.data(), E, (const int*)dense_of.data(), order, keys, ts, tt.nt, tt.entries.data());
@@ -1965,12 +1948,11 @@ static void build_tile_table(const int64_t* keys, const int* order, int R, int d
   hipLaunchKernelGGL(k_tile_slice_start, dim3(blocks_for(n + 2, 256)), dim3(256), 0, s, (const int*)compact.data(), n,
                      (int)per_slice, small.data());
   KERNEL_CHECK();
-  std::vector<int> h(n + 5);
-  read_bytes(h.data(), small.data(), sizeof(int) * (n + 5), s);
+  std::vector<int> h(n + 4);
+  read_bytes(h.data(), small.data(), sizeof(int) * (n + 4), s);
   tt.slice_start.assign(h.begin(), h.begin() + n + 2);
   tt.max_rows = h[n + 2];
   tt.max_staged = h[n + 3];
-  tt.computed = h[n + 4];
   tt.max_tiles = 0;
   for (int z = 0; z <= n; ++z) tt.max_tiles = std::max(tt.max_tiles, tt.slice_start[z + 1] - tt.slice_start[z]);
 }
@@ -2033,13 +2015,8 @@ static std::unique_ptr<TiledGraph> build_tiled_graph(int n, int iters, int width
 // and receives the result; x1 is a second buffer of R values.  The operator's
 // absent slots must hold +0 (k_level_operator stores them so).
 template <class T, class V>
-int relax_tiled_graph(const GsOperator<T, V>& op, const TileTable& tt, T* x, T* x1, int depth, int iters, hipStream_t s) {
-  DArray<TiledOp<T, V>> ops(std::max(tt.computed, 1), s);
-  if (tt.tiles > 0) {
-    hipLaunchKernelGGL((k_tile_ops<T, V>), dim3(tt.tiles), dim3(64), 0, s, tt.entries.data(), tt.desc.data(),
-                       op.vals.data(), op.R, op.b.data(), op.inv_diag.data(), op.has_diag.data(), ops.data());
-    KERNEL_CHECK();
-  }
+int relax_tiled_graph(const GsOperator<T, V>& op, const TileTable& tt, const RowRecord<T, V>* records, T* x, T* x1,
+                      int depth, int iters, hipStream_t s) {
   static std::mutex mu;
   static std::map<std::tuple<int, int, int, int, unsigned, int, hipStream_t>, std::unique_ptr<TiledGraph>> cache;
   const int n = 1 << depth;
@@ -2063,7 +2040,7 @@ int relax_tiled_graph(const GsOperator<T, V>& op, const TileTable& tt, T* x, T* 
   if (iters % 2) HIP_CHECK(hipMemcpyAsync(x1, x, sizeof(T) * op.R, hipMemcpyDeviceToDevice, s));
   std::lock_guard<std::mutex> guard(g->mu);
   HIP_CHECK(hipEventSynchronize(g->done));
-  TiledArgs<T, V> a{tt.entries.data(), tt.desc.data(), ops.data(), x, x1, gside};
+  TiledArgs<T, V> a{tt.entries.data(), tt.desc.data(), records, op.b.data(), x, x1, gside};
   WaveTiles wv;
   void* args[2] = {&a, &wv};
   hipKernelNodeParams kp = g->params;
@@ -2218,7 +2195,7 @@ template <class T> static Stencil<T> cast_stencil(Stencil<double>&& m, hipStream
 template <class T, class V>
 static void level_operator(const LevelState<T>& rowsL, const LevelState<T>& colsL, int depth, int trial_depth,
                            const PointSet<T>* pts, double* load, const int* ord, const int* inv, Stencil<V>& out,
-                           GsOperator<T, V>* op, unsigned long long* present, hipStream_t s) {
+                           GsOperator<T, V>* op, RowRecord<T, V>* records, unsigned long long* present, hipStream_t s) {
   const int R = rowsL.basis_count;
   out.rows = R;
   out.cols_count = colsL.basis_count;
@@ -2244,13 +2221,13 @@ static void level_operator(const LevelState<T>& rowsL, const LevelState<T>& cols
                        (const int64_t*)rowsL.basis_keys.data(), R, ord, (const int64_t*)colsL.basis_keys.data(),
                        (int64_t)colsL.basis_count, colsL.columns.view(), dense, colsL.dense_dim, depth, trial_depth,
                        (const D2*)table.data(), pb, load, out.cols.data(), out.vals.data(), inv, sched, inv_diag, has_diag,
-                       present);
+                       records, present);
   else
     hipLaunchKernelGGL((k_level_operator<T, V, false>), dim3(blocks_for(R, 128)), dim3(128), 0, s,
                        (const int64_t*)rowsL.basis_keys.data(), R, ord, (const int64_t*)colsL.basis_keys.data(),
                        (int64_t)colsL.basis_count, colsL.columns.view(), dense, colsL.dense_dim, depth, trial_depth,
                        (const D2*)table.data(), pb, load, out.cols.data(), out.vals.data(), inv, sched, inv_diag, has_diag,
-                       present);
+                       records, present);
   KERNEL_CHECK();
 }
 
@@ -2773,6 +2750,7 @@ template <class T, class V> struct FinerLevel {
   GsOperator<T, V> op;  // cols (schedule positions), vals, inv_diag, has_diag; b is set by the solve
   DArray<unsigned long long> nnz;  // stored nonzeros, read after the run
   TileTable tiles;                 // tiled relaxation (empty when the colour-per-launch form runs)
+  DArray<RowRecord<T, V>> records;  // the tiled relaxation's row records
 };
 
 template <class T, class V>
@@ -2801,12 +2779,15 @@ static void prepare_finer(Result<T>& r, int d, Stencil<V>& Aop, Stencil<V>& Cop,
   f.nnz.zero();
   GsOperator<T, V>& op = f.op;
   op.R = R;
-  level_operator<T, V>(L, L, d, d, screened ? &pts : nullptr, L.point_load.data(), ord, inv.data(), Aop, &op, f.nnz.data(), s);
+  if (f.tiles.tiles > 0) f.records.reset(R, s);
+  level_operator<T, V>(L, L, d, d, screened ? &pts : nullptr, L.point_load.data(), ord, inv.data(), Aop, &op,
+                       f.records.data(), f.nnz.data(), s);
   op.vals = std::move(Aop.vals);
   // the cross operator keeps canonical rows: its product with the coarser
   // cumulative solution gathers from neighbouring parents in key order
   level_operator<T, V>(L, r.levels[d - 1], d, d - 1, screened ? &pts : nullptr, (double*)nullptr, (const int*)nullptr,
-                       (const int*)nullptr, Cop, (GsOperator<T, V>*)nullptr, (unsigned long long*)nullptr, s);
+                       (const int*)nullptr, Cop, (GsOperator<T, V>*)nullptr, (RowRecord<T, V>*)nullptr,
+                       (unsigned long long*)nullptr, s);
 }
 
 // One finer level's solve (solver.py:343-360): right-hand side, b -= cross *
@@ -2852,7 +2833,8 @@ static void solve_finer(Result<T>& r, int d, Stencil<V>& Aop, Stencil<V>& Cop, F
   HIP_CHECK(hipEventCreate(&st.end));
   HIP_CHECK(hipEventRecord(st.start, s));
   if (tiled)
-    st.launches = relax_tiled_graph(op, f.tiles, xs.data(), xs1.data(), d, r.prm.iters, s);
+    st.launches = relax_tiled_graph(op, f.tiles, (const RowRecord<T, V>*)f.records.data(), xs.data(), xs1.data(), d,
+                                    r.prm.iters, s);
   else
     st.launches = relax_sliced_any(op, f.sc, xs.data(), d, r.prm.iters, s);
   HIP_CHECK(hipEventRecord(st.end, s));
