@@ -321,7 +321,7 @@ template <class T> void stage_field(Result<T>& r) {
   }
   r.offsets_dev.reset(D + 2, s);
   HIP_CHECK(hipMemcpyAsync(r.offsets_dev.data(), r.level_offsets.data(), sizeof(int) * (D + 2), hipMemcpyHostToDevice, s));
-  if (D <= FIELD_TABLE_MAX_DEPTH && K) {
+  if (D <= field_table_max_depth() && K) {
     const size_t cells = (size_t)column_base(D + 1);
     r.column_start.reset(cells, s);
     r.column_end.reset(cells, s);

@@ -17,7 +17,6 @@ struct FieldIndex {
 
 // first cell of level d's column table: sum over e < d of (2^e + 1)^2
 PSR_HD int64_t column_base(int d) { return (((int64_t)1 << (2 * d)) - 1) / 3 + 2 * (((int64_t)1 << d) - 1) + d; }
-constexpr int FIELD_TABLE_MAX_DEPTH = 11;
 
 // Absolute indices of keys (d, x, y, z0 + k), k < len (-1 when absent).
 template <int MAXLEN>

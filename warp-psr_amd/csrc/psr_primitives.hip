@@ -6,6 +6,8 @@
 // pipeline its reference summation order.
 #include "psr_runtime.h"
 
+#include <cstdlib>
+
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -439,9 +441,22 @@ __global__ void k_column_index(const int64_t* __restrict__ keys, int64_t n, int 
 }
 }  // namespace
 
+static int64_t env_limit(const char* name, int64_t fallback, int64_t lo, int64_t hi) {
+  const char* v = std::getenv(name);
+  if (!v || !*v) return fallback;
+  char* end = nullptr;
+  const long long x = std::strtoll(v, &end, 10);
+  if (*end) return fallback;
+  return x < lo ? lo : (x > hi ? hi : (int64_t)x);
+}
+
+int64_t dense_index_max_cells() { return env_limit("PSR_DENSE_INDEX_MAX_CELLS", (int64_t)1 << 28, 0, (int64_t)1 << 28); }
+int column_table_max_dim() { return (int)env_limit("PSR_COLUMN_TABLE_MAX_DIM", 4097, 0, 4097); }
+int field_table_max_depth() { return (int)env_limit("PSR_FIELD_TABLE_MAX_DEPTH", 11, -1, 11); }
+
 void build_column_index(const int64_t* keys, int64_t n, int dim, ColumnIndex& out, hipStream_t s) {
   out.dim = 0;
-  if (dim <= 0 || dim > COLUMN_TABLE_MAX_DIM || n <= 0) {
+  if (dim <= 0 || dim > column_table_max_dim() || n <= 0) {
     out.start.free();
     out.end.free();
     return;

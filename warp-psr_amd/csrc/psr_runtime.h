@@ -190,7 +190,16 @@ struct ColumnIndex {
   }
 };
 void build_column_index(const int64_t* keys, int64_t n, int dim, ColumnIndex& out, hipStream_t s);
-constexpr int COLUMN_TABLE_MAX_DIM = 4097;  // depth 12 basis coordinates
+// Size limits of the lookup tables.  A level beyond a limit takes the next
+// method -- dense index, then (x, y) column table, then a search of the
+// sorted keys -- with the same results, slower and in less memory.  The
+// defaults (2^28 dense cells: depth 9; column tables up to 4097 per side:
+// depth 12; the field's column table up to depth 11) can be lowered with
+// PSR_DENSE_INDEX_MAX_CELLS, PSR_COLUMN_TABLE_MAX_DIM and
+// PSR_FIELD_TABLE_MAX_DEPTH (read at each use), e.g. to cap memory.
+int64_t dense_index_max_cells();
+int column_table_max_dim();
+int field_table_max_depth();
 
 inline int bits_for(uint64_t maxval) {
   int b = 0;

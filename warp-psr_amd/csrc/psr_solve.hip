@@ -1641,8 +1641,6 @@ inline int floordiv(int a, int b) { return (a >= 0) ? a / b : -((-a + b - 1) / b
 }  // namespace
 
 // ---------------------------------------------------------------- host side
-// dense z-major index for levels up to depth 9 (513^3 ints = 541 MB)
-constexpr int64_t DENSE_INDEX_MAX_CELLS = (int64_t)1 << 28;
 
 template <class T> struct Schedule {
   DArray<int> order, offsets;
@@ -2136,7 +2134,7 @@ template <class T> static PointSet<T> point_constraints(Result<T>& r, int d, hip
     sort_pairs_u64(ps.bins.data(), ps.ids.data(), ps.P, 3 * d, s);
     const int n = 1 << d;
     const int64_t cells = (int64_t)n * n * n;
-    if (d >= 5 && cells <= DENSE_INDEX_MAX_CELLS) {
+    if (d >= 5 && cells <= dense_index_max_cells()) {
       ps.cell_start.reset((size_t)cells, s);
       ps.cell_end.reset((size_t)cells, s);
       ps.cell_start.zero();
@@ -2144,7 +2142,7 @@ template <class T> static PointSet<T> point_constraints(Result<T>& r, int d, hip
       hipLaunchKernelGGL(k_bin_cells, dim3(blocks_for(ps.P, 256)), dim3(256), 0, s, (const uint64_t*)ps.bins.data(), ps.P, d,
                          ps.cell_start.data(), ps.cell_end.data());
       KERNEL_CHECK();
-    } else if (n <= COLUMN_TABLE_MAX_DIM) {
+    } else if (n <= column_table_max_dim()) {
       ps.columns.start.reset((size_t)n * n, s);
       ps.columns.end.reset((size_t)n * n, s);
       ps.columns.start.zero();
@@ -2547,7 +2545,7 @@ static void normal_loads_base(Result<T>& r, std::vector<DArray<int>>& restr, Nor
     // dense (x, y, z)-major index of the level's records for the tests taken
     // in canonical order (consecutive z)
     const int64_t rdim = (1 << d) + 1;
-    if (r1 > r0 && d >= 5 && rdim * rdim * rdim <= DENSE_INDEX_MAX_CELLS) {
+    if (r1 > r0 && d >= 5 && rdim * rdim * rdim <= dense_index_max_cells()) {
       plan.rdense[d].reset((size_t)(rdim * rdim * rdim), s);
       HIP_CHECK(hipMemsetAsync(plan.rdense[d].data(), 0xff, sizeof(int) * (size_t)(rdim * rdim * rdim), s));
       hipLaunchKernelGGL(k_dense_fill_xyz, dim3(blocks_for(r1 - r0, 256)), dim3(256), 0, s, nk, r1 - r0, (int)rdim,
@@ -3064,7 +3062,7 @@ template <class T> void stage_solve(Result<T>& r) {
     LevelState<T>& L = r.levels[d];
     build_column_index((const int64_t*)L.basis_keys.data(), L.basis_count, (1 << d) + 1, L.columns, s);
     const int64_t dim = (1 << d) + 1;
-    if (L.basis_count && d >= 5 && dim * dim * dim <= DENSE_INDEX_MAX_CELLS) {
+    if (L.basis_count && d >= 5 && dim * dim * dim <= dense_index_max_cells()) {
       L.dense.reset((size_t)(dim * dim * dim), s);
       HIP_CHECK(hipMemsetAsync(L.dense.data(), 0xff, sizeof(int) * (size_t)(dim * dim * dim), s));  // -1
       hipLaunchKernelGGL(k_dense_fill, dim3(blocks_for(L.basis_count, 256)), dim3(256), 0, s, (const int64_t*)L.basis_keys.data(),
