@@ -13,6 +13,7 @@
 #include <map>
 #include <sstream>
 #include <atomic>
+#include <chrono>
 #include <thread>
 
 using namespace psr;
@@ -407,6 +408,7 @@ int psr_reconstruct_batch(const void* const* points, const void* const* normals,
     std::atomic<int> next(0);
     std::vector<int> codes(batch, 0);
     std::vector<std::string> messages(batch);
+    const auto start = std::chrono::steady_clock::now();
     auto work = [&](int w) {
       if (hipSetDevice(dev) != hipSuccess) return;
       for (int i = next++; i < batch; i = next++) {
@@ -414,6 +416,14 @@ int psr_reconstruct_batch(const void* const* points, const void* const* normals,
           out[i] = reconstruct_one(points[i], normals[i], counts[i], dtype, prm[i], streams[w]);
           out[i]->sync_on_free = true;
           HIP_CHECK(hipStreamSynchronize(streams[w]));
+          // the cloud's completion time since the batch began, seen by the
+          // host after its stream drained (the metadata's last key)
+          const double done = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
+          std::ostringstream o;
+          o.precision(9);
+          o << ", \"batch_completion_seconds\": " << done << "}";
+          std::string& m = out[i]->metadata;
+          if (!m.empty() && m.back() == '}') m.replace(m.size() - 1, 1, o.str());
         } catch (const PsrError& e) {
           codes[i] = e.code;
           messages[i] = e.what();
