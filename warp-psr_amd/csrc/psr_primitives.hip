@@ -62,8 +62,11 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_tile_sums(const int* __restric
   if (threadIdx.x == 0) sums[blockIdx.x] = total;
 }
 
+// Exclusive scan of one 4,096-value tile plus its offset; the last tile also
+// writes the total (its offset plus its sum) when total is non-null.  The
+// tile is read into LDS before anything is written, so out may alias in.
 __global__ void __launch_bounds__(SCAN_THREADS) k_scan_tiles(const int* __restrict__ in, int* __restrict__ out, int64_t n,
-                                                             const int* __restrict__ tile_offsets) {
+                                                             const int* __restrict__ tile_offsets, int* __restrict__ total_out) {
   __shared__ int tile[SCAN_TILE];
   const int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
   for (int r = 0; r < SCAN_ITEMS; ++r) {
@@ -81,6 +84,7 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_tiles(const int* __restri
   int total;
   int prefix = block_exclusive_scan(s, &total);
   const int off = tile_offsets ? tile_offsets[blockIdx.x] : 0;
+  if (total_out && threadIdx.x == 0 && blockIdx.x == gridDim.x - 1) *total_out = off + total;
 #pragma unroll
   for (int k = 0; k < SCAN_ITEMS; ++k) tile[threadIdx.x * SCAN_ITEMS + k] = local[k] + prefix + off;
   __syncthreads();
@@ -88,11 +92,6 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_tiles(const int* __restri
     int64_t i = base + r * SCAN_THREADS + threadIdx.x;
     if (i < n) out[i] = tile[r * SCAN_THREADS + threadIdx.x];
   }
-}
-
-// total = exclusive prefix of the last element + the last input value
-__global__ void k_scan_total(const int* __restrict__ last_input, const int* __restrict__ out, int64_t n, int* total) {
-  *total = out[n - 1] + *last_input;
 }
 
 // ------------------------------------------------------------ radix sort
@@ -338,13 +337,8 @@ void scan_exclusive_i32(const int* in, int* out, int64_t n, int* total_dev, hipS
     return;
   }
   const int64_t tiles = (n + SCAN_TILE - 1) / SCAN_TILE;
-  DArray<int> last;
-  if (total_dev) {  // keep the last input: out may alias in
-    last.reset(1, s);
-    HIP_CHECK(hipMemcpyAsync(last.data(), in + (n - 1), sizeof(int), hipMemcpyDeviceToDevice, s));
-  }
   if (tiles == 1) {
-    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(SCAN_THREADS), 0, s, in, out, n, (const int*)nullptr);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(SCAN_THREADS), 0, s, in, out, n, (const int*)nullptr, total_dev);
     KERNEL_CHECK();
   } else {
     DArray<int> sums(tiles, s);
@@ -352,11 +346,7 @@ void scan_exclusive_i32(const int* in, int* out, int64_t n, int* total_dev, hipS
     KERNEL_CHECK();
     scan_exclusive_i32(sums.data(), sums.data(), tiles, nullptr, s);
     hipLaunchKernelGGL(k_scan_tiles, dim3((unsigned)tiles), dim3(SCAN_THREADS), 0, s, in, out, n,
-                       (const int*)sums.data());
-    KERNEL_CHECK();
-  }
-  if (total_dev) {
-    hipLaunchKernelGGL(k_scan_total, dim3(1), dim3(1), 0, s, (const int*)last.data(), (const int*)out, n, total_dev);
+                       (const int*)sums.data(), total_dev);
     KERNEL_CHECK();
   }
 }

@@ -652,7 +652,8 @@ __global__ void k_normal_splat_reduce(const K* __restrict__ keys, const int* __r
   out_vals[3 * g] = -a; out_vals[3 * g + 1] = -b; out_vals[3 * g + 2] = -c;
 }
 
-// normals.py:111-163: contributions in first-index order, then one serial chain.
+// normals.py:111-163: contributions in first-index order, then one serial
+// chain; stored component-major (component c of aggregate i at c * G + i).
 template <class T>
 __global__ void k_average_contributions(const int* __restrict__ order, int G, const T* __restrict__ nsums,
                                         const T* __restrict__ weights, const T* __restrict__ dw, T* __restrict__ out) {
@@ -667,53 +668,70 @@ __global__ void k_average_contributions(const int* __restrict__ order, int G, co
     v1 = dw[2 * s + 1] * w;
     v2 = w;
   }
-  out[3 * i] = v0; out[3 * i + 1] = v1; out[3 * i + 2] = v2;
+  out[i] = v0; out[G + i] = v1; out[2 * (int64_t)G + i] = v2;
 }
 
-// One component chain per wave (three waves on three SIMDs).  A skipped
-// contribution is exactly (0,0,0); the running sums start at +0 and never
-// become -0, so adding it is the identity and the reference's `> 0` test
-// (normals.py:160-163) needs no branch.  Each wave holds AVG_ROUND x 64
-// contributions in registers (lane j holds elements base + u*64 + j), loads
-// the next round while the chain reads the current one with readlane, in
-// element order.
-template <class T> __device__ __forceinline__ T lane_value(T v, int lane);
-template <> __device__ __forceinline__ float lane_value<float>(float v, int lane) {
-  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
-}
-template <> __device__ __forceinline__ double lane_value<double>(double v, int lane) {
-  const long long bits = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_readlane((int)(bits & 0xffffffff), lane);
-  const int hi = __builtin_amdgcn_readlane((int)(bits >> 32), lane);
-  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
-}
-
-constexpr int AVG_ROUND = 8;
+// One component chain per wave (three waves on three SIMDs), in element
+// order.  A skipped contribution is exactly (0,0,0); the running sums start at
+// +0 and never become -0, so adding it (or the +0 padding of the last chunk)
+// is the identity and the reference's `> 0` test (normals.py:160-163) needs no
+// branch.  Each wave stages AVG_CHUNK values of its component in LDS and reads
+// them back with one broadcast LDS load per value, so a step of the chain is
+// that load and one add; the next chunk's global loads are in flight
+// meanwhile.
+constexpr int AVG_PER_LANE = 16;
+constexpr int AVG_CHUNK = 64 * AVG_PER_LANE;
 
 template <class T>
 __global__ void __launch_bounds__(192) k_average_serial(const T* __restrict__ contrib, int G, T* __restrict__ out) {
+  __shared__ T stage[2][3][AVG_CHUNK];
   __shared__ T totals[3];
   const int comp = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  T acc = T(0);
-  T cur[AVG_ROUND], nxt[AVG_ROUND];
-#pragma unroll
-  for (int u = 0; u < AVG_ROUND; ++u) {
-    const int e = u * 64 + lane;
-    cur[u] = e < G ? contrib[3 * (int64_t)e + comp] : T(0);
+  if (G <= 0) {
+    if (threadIdx.x == 0) out[0] = out[1] = out[2] = T(0);
+    return;
   }
-  for (int base = 0; base < G; base += 64 * AVG_ROUND) {
-    const int next = base + 64 * AVG_ROUND;
+  const T* __restrict__ src = contrib + (int64_t)comp * G;
+  T pre[AVG_PER_LANE];
+  auto load = [&](int first) {  // clamped addresses (G >= 1)
 #pragma unroll
-    for (int u = 0; u < AVG_ROUND; ++u) {
-      const int e = next + u * 64 + lane;
-      nxt[u] = e < G ? contrib[3 * (int64_t)e + comp] : T(0);
+    for (int k = 0; k < AVG_PER_LANE; ++k) {
+      const int e = first + k * 64 + lane;
+      pre[k] = src[e < G ? e : G - 1];
     }
+  };
+  auto keep = [&](int b, int first) {  // +0 past the end
 #pragma unroll
-    for (int u = 0; u < AVG_ROUND; ++u)
+    for (int k = 0; k < AVG_PER_LANE; ++k) stage[b][comp][k * 64 + lane] = first + k * 64 + lane < G ? pre[k] : T(0);
+  };
+  T acc = T(0);
+  load(0);
+  keep(0, 0);
+  __syncthreads();
+  int b = 0;
+  for (int first = 0; first < G; first += AVG_CHUNK) {
+    const bool more = first + AVG_CHUNK < G;
+    if (more) load(first + AVG_CHUNK);
+    const T* chunk = stage[b][comp];
+    // the next 16 values are read while the current 16 are added
+    T cur[16], nxt[16];
 #pragma unroll
-      for (int j = 0; j < 64; ++j) acc = acc + lane_value(cur[u], j);
+    for (int u = 0; u < 16; ++u) cur[u] = chunk[u];
+    for (int k = 0; k < AVG_CHUNK; k += 16) {
+      if (k + 16 < AVG_CHUNK) {
 #pragma unroll
-    for (int u = 0; u < AVG_ROUND; ++u) cur[u] = nxt[u];
+        for (int u = 0; u < 16; ++u) nxt[u] = chunk[k + 16 + u];
+      }
+#pragma unroll
+      for (int u = 0; u < 16; ++u) acc = acc + cur[u];
+#pragma unroll
+      for (int u = 0; u < 16; ++u) cur[u] = nxt[u];
+    }
+    // the staging stores wait for the next chunk's loads: keep them after the chain
+    __asm__ __volatile__("" ::: "memory");
+    if (more) keep(b ^ 1, first + AVG_CHUNK);
+    __syncthreads();
+    b ^= 1;
   }
   if (lane == 0) totals[comp] = acc;
   __syncthreads();
