@@ -1451,6 +1451,138 @@ __global__ void k_tile_entries(const uint32_t* __restrict__ skeys, const int* __
   entries[e] = make_int2(j, lx | (ly << 8) | (interior << 16));
 }
 
+// Tile tables from a level's dense z-major index, tile by tile: the
+// workgroup of tile t scans its (ts + 8)^2 window of its slice in window
+// order (lx-major) and files each row it finds under its categories -- the
+// computed colour (ring <= 3 - colour) and the staged values (every row of
+// the window).  A count pass gives the per-(tile, category) sizes, one scan
+// their offsets, and an emit pass writes the entries in window order, so no
+// sort is needed; rows of one category are independent, so their order
+// within the category does not change any result.
+constexpr int TILE_SCAN_THREADS = 256;
+
+struct TileCell {
+  int j, packed, cat;  // schedule position, lx | ly << 8 | interior << 16, computed colour or -1
+  bool present;
+};
+
+template <bool EMIT>
+__device__ __forceinline__ TileCell tile_cell(const int* __restrict__ dense, int dim, int n, int z, int ox, int oy, int ts,
+                                              int g, int w, const int* __restrict__ inv) {
+  TileCell c{0, 0, -1, false};
+  if (w >= g * g) return c;
+  const int lx = w / g, ly = w - (w / g) * g;
+  const int x = ox + lx, y = oy + ly;
+  if (x < 0 || y < 0 || x > n || y > n) return c;
+  const int i = dense[((int64_t)z * dim + x) * dim + y];
+  if (i < 0) return c;
+  c.present = true;
+  const int r = max(ring_of(x, ox + 4, ts), ring_of(y, oy + 4, ts));
+  const int colour = (x & 1) + 2 * (y & 1);
+  c.cat = r <= 3 - colour ? colour : -1;
+  if (EMIT) {
+    c.j = inv[i];
+    c.packed = lx | (ly << 8);
+  }
+  return c;
+}
+
+__global__ void __launch_bounds__(TILE_SCAN_THREADS) k_tile_scan_count(const int* __restrict__ dense, int dim, int n,
+                                                                        const int* __restrict__ dense_of, int nt, int ts,
+                                                                        int* __restrict__ counts) {
+  __shared__ int total[5];
+  const int t = blockIdx.x, tid = threadIdx.x;
+  if (tid < 5) total[tid] = 0;
+  __syncthreads();
+  const int d = dense_of[t];
+  const int tx = d % nt, ty = (d / nt) % nt, z = d / (nt * nt);
+  const int ox = tx * ts - 4, oy = ty * ts - 4, g = ts + 8;
+  int cnt[5] = {0, 0, 0, 0, 0};
+  for (int w = tid; w < g * g; w += TILE_SCAN_THREADS) {
+    const TileCell c = tile_cell<false>(dense, dim, n, z, ox, oy, ts, g, w, nullptr);
+    if (!c.present) continue;
+    if (c.cat >= 0) ++cnt[c.cat];
+    ++cnt[4];
+  }
+#pragma unroll
+  for (int k = 0; k < 5; ++k)
+    if (cnt[k]) atomicAdd(&total[k], cnt[k]);
+  __syncthreads();
+  if (tid < 5) counts[5 * (int64_t)t + tid] = total[tid];
+}
+
+__global__ void __launch_bounds__(TILE_SCAN_THREADS) k_tile_scan_emit(const int* __restrict__ dense, int dim, int n,
+                                                                       const int* __restrict__ dense_of, int nt, int ts,
+                                                                       const int* __restrict__ inv, const int* __restrict__ offs,
+                                                                       int2* __restrict__ entries) {
+  constexpr int WAVES = TILE_SCAN_THREADS / 64;
+  __shared__ int base[5];
+  __shared__ int wave_total[5][WAVES];
+  const int t = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  if (tid < 5) base[tid] = offs[5 * (int64_t)t + tid];
+  const int d = dense_of[t];
+  const int tx = d % nt, ty = (d / nt) % nt, z = d / (nt * nt);
+  const int ox = tx * ts - 4, oy = ty * ts - 4, g = ts + 8;
+  const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
+  __syncthreads();
+  for (int w0 = 0; w0 < g * g; w0 += TILE_SCAN_THREADS) {
+    const TileCell c = tile_cell<true>(dense, dim, n, z, ox, oy, ts, g, w0 + tid, inv);
+    int pre[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      const bool f = k < 4 ? c.cat == k : c.present;
+      const unsigned long long m = __ballot(f);
+      pre[k] = __popcll(m & below);
+      if (lane == 0) wave_total[k][wave] = __popcll(m);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      const bool f = k < 4 ? c.cat == k : c.present;
+      if (!f) continue;
+      int off = base[k] + pre[k];
+      for (int v = 0; v < wave; ++v) off += wave_total[k][v];
+      const int lx = c.packed & 255, ly = c.packed >> 8;
+      const int interior = (k < 4 && lx >= 4 && lx < 4 + ts && ly >= 4 && ly < 4 + ts) ? 1 : 0;
+      entries[off] = make_int2(c.j, c.packed | (interior << 16));
+    }
+    __syncthreads();
+    if (tid < 5) {
+      int sum = 0;
+      for (int v = 0; v < WAVES; ++v) sum += wave_total[tid][v];
+      base[tid] += sum;
+    }
+    __syncthreads();
+  }
+}
+
+// desc of tile t from the category offsets (offs[5 t + k], offs[5 tiles] = E)
+__global__ void k_tile_desc_offs(const int* __restrict__ offs, int tiles, const int* __restrict__ dense_of,
+                                 const int* __restrict__ flags, const int* __restrict__ compact, int64_t per_slice,
+                                 int64_t ndense, int* __restrict__ desc, int* __restrict__ stats) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= tiles) return;
+  int* o = desc + TILE_DESC_INTS * t;
+  for (int c = 0; c < 6; ++c) o[c] = offs[5 * (int64_t)t + c];
+  const int64_t d = dense_of[t];
+  int staged = o[5] - o[4];
+  for (int side = 0; side < 2; ++side) {  // slice z + 1, then z - 1
+    const int64_t nd = side == 0 ? d + per_slice : d - per_slice;
+    int lo = 0, hi = 0;
+    if (nd >= 0 && nd < ndense && flags[nd]) {
+      const int64_t tn = compact[nd];
+      lo = offs[5 * tn + 4];
+      hi = offs[5 * tn + 5];
+    }
+    o[6 + 2 * side] = lo;
+    o[7 + 2 * side] = hi;
+    staged += hi - lo;
+  }
+  o[10] = o[11] = 0;
+  atomicMax(&stats[0], o[4] - o[0]);
+  atomicMax(&stats[1], staged);
+}
+
 // first tile of each slice z (z = 0..n + 1) from the exclusive scan of the flags
 __global__ void k_tile_slice_start(const int* __restrict__ compact, int n, int per_slice, int* __restrict__ out) {
   const int z = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1894,8 +2026,11 @@ static int relax_tile_size(int requested, int depth) {
   return 16;
 }
 
-static void build_tile_table(const int64_t* keys, const int* order, int R, int depth, int ts, TileTable& tt,
-                             hipStream_t s) {
+// With the level's dense z-major index (level_dense, side dim) the entries
+// come tile by tile from the index (k_tile_scan_*); without it, row by row
+// and a sort of the (tile, category) keys.
+static void build_tile_table(const int64_t* keys, const int* order, int R, int depth, int ts, const int* level_dense,
+                             int dim, TileTable& tt, hipStream_t s) {
   const int n = 1 << depth;
   tt.ts = ts;
   tt.nt = (n + 1 + ts - 1) / ts;
@@ -1908,6 +2043,51 @@ static void build_tile_table(const int64_t* keys, const int* order, int R, int d
                      (const int*)nullptr, (int*)nullptr, (uint32_t*)nullptr, (int*)nullptr);
   KERNEL_CHECK();
   scan_exclusive_i32(flags.data(), compact.data(), ndense + 1, total.data(), s);
+  if (level_dense && dim == n + 1) {
+    tt.tiles = read_int(total.data(), s);
+    const int tiles = tt.tiles;
+    DArray<int> dense_of(std::max(tiles, 1), s);
+    hipLaunchKernelGGL(k_tile_dense_of, dim3(blocks_for(ndense, 256)), dim3(256), 0, s, (const int*)flags.data(),
+                       (const int*)compact.data(), ndense, dense_of.data());
+    KERNEL_CHECK();
+    DArray<int> inv(std::max(R, 1), s), offs(5 * (size_t)tiles + 1, s), etotal(1, s);
+    hipLaunchKernelGGL(k_inverse_permutation, dim3(blocks_for(R, 256)), dim3(256), 0, s, order, R, inv.data());
+    KERNEL_CHECK();
+    HIP_CHECK(hipMemsetAsync(offs.data() + 5 * (size_t)tiles, 0, sizeof(int), s));
+    if (tiles) {
+      hipLaunchKernelGGL(k_tile_scan_count, dim3(tiles), dim3(TILE_SCAN_THREADS), 0, s, level_dense, dim, n,
+                         (const int*)dense_of.data(), tt.nt, ts, offs.data());
+      KERNEL_CHECK();
+    }
+    scan_exclusive_i32(offs.data(), offs.data(), 5 * (int64_t)tiles + 1, etotal.data(), s);
+    const int E = read_int(etotal.data(), s);
+    tt.entries_count = E;
+    tt.entries.reset(std::max(E, 1), s);
+    DArray<int> small(n + 2 + 2, s);
+    HIP_CHECK(hipMemsetAsync(small.data() + n + 2, 0, 2 * sizeof(int), s));
+    tt.desc.reset(TILE_DESC_INTS * (size_t)std::max(tiles, 1), s);
+    if (tiles) {
+      hipLaunchKernelGGL(k_tile_scan_emit, dim3(tiles), dim3(TILE_SCAN_THREADS), 0, s, level_dense, dim, n,
+                         (const int*)dense_of.data(), tt.nt, ts, (const int*)inv.data(), (const int*)offs.data(),
+                         tt.entries.data());
+      KERNEL_CHECK();
+      hipLaunchKernelGGL(k_tile_desc_offs, dim3(blocks_for(tiles, 256)), dim3(256), 0, s, (const int*)offs.data(), tiles,
+                         (const int*)dense_of.data(), (const int*)flags.data(), (const int*)compact.data(), per_slice,
+                         ndense, tt.desc.data(), small.data() + n + 2);
+      KERNEL_CHECK();
+    }
+    hipLaunchKernelGGL(k_tile_slice_start, dim3(blocks_for(n + 2, 256)), dim3(256), 0, s, (const int*)compact.data(), n,
+                       (int)per_slice, small.data());
+    KERNEL_CHECK();
+    std::vector<int> h(n + 4);
+    read_bytes(h.data(), small.data(), sizeof(int) * (n + 4), s);
+    tt.slice_start.assign(h.begin(), h.begin() + n + 2);
+    tt.max_rows = h[n + 2];
+    tt.max_staged = h[n + 3];
+    tt.max_tiles = 0;
+    for (int z = 0; z <= n; ++z) tt.max_tiles = std::max(tt.max_tiles, tt.slice_start[z + 1] - tt.slice_start[z]);
+    return;
+  }
   DArray<int> count(R + 1, s), pos(R + 1, s), etotal(1, s);
   hipLaunchKernelGGL(k_tile_members<1>, dim3(blocks_for(R, 256)), dim3(256), 0, s, order, keys, R, ts, tt.nt, flags.data(),
                      (const int*)compact.data(), count.data(), (uint32_t*)nullptr, (int*)nullptr);
@@ -2759,7 +2939,9 @@ static void prepare_finer(Result<T>& r, int d, Stencil<V>& Aop, Stencil<V>& Cop,
   f.sc = make_schedule<T>(L.basis_keys, R, d, true, s);
   const int* ord = f.sc.order.data();
   const int ts = relax_tile_size(r.prm.relax_tile, d);
-  if (ts && R && r.prm.iters <= RELAX_MAX_SLICES) build_tile_table(L.basis_keys.data(), ord, R, d, ts, f.tiles, s);
+  if (ts && R && r.prm.iters <= RELAX_MAX_SLICES)
+    build_tile_table(L.basis_keys.data(), ord, R, d, ts, L.dense_dim ? (const int*)L.dense.data() : (const int*)nullptr,
+                     L.dense_dim, f.tiles, s);
   L.point_load.reset(R, s);
   if (!screened) L.point_load.zero();
   PointSet<T> pts;
