@@ -106,6 +106,11 @@ template <class T> class DArray {
     p_ = nullptr;
     n_ = 0;
   }
+  // keep only the first n elements (n <= size()); the block is unchanged and
+  // returns to the pool whole
+  void shrink(size_t n) {
+    if (n < n_) n_ = n;
+  }
   T* data() const { return p_; }
   size_t size() const { return n_; }
   bool empty() const { return n_ == 0; }

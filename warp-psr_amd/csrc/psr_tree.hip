@@ -145,6 +145,10 @@ template <class T> void stage_tree(Result<T>& r) {
   r.levels.resize(depth + 1);
   DArray<int64_t> broods, requests;
   bool have_requests = false;
+  // per-level basis counts stay on the device until the loop ends (one read):
+  // each level's basis keys are allocated for all 8 * broods candidates
+  DArray<int> basis_counts(depth + 1, s);
+  basis_counts.zero();
   for (int d = depth; d >= 0; --d) {
     const int64_t r0 = r.normal_offsets[d], r1 = r.normal_offsets[d + 1];
     const int64_t count = r1 - r0;
@@ -209,23 +213,23 @@ template <class T> void stage_tree(Result<T>& r) {
     }
     // compact_basis (tree.py:126-143)
     const int64_t nt = 8 * nb;
+    L.basis_keys.reset(nt, s);
     if (nt) {
-      DArray<int> marks(nt, s), prefix(nt, s), cnt(1, s);
+      DArray<int> marks(nt, s), prefix(nt, s);
       hipLaunchKernelGGL(k_basis_marks, dim3(blocks_for(nt, 256)), dim3(256), 0, s, (const int64_t*)L.tree_keys.data(), nt, d,
                          marks.data());
       KERNEL_CHECK();
-      scan_exclusive_i32(marks.data(), prefix.data(), nt, cnt.data(), s);
-      L.basis_count = read_int(cnt.data(), s);
-      L.basis_keys.reset(L.basis_count, s);
-      if (L.basis_count) {
-        hipLaunchKernelGGL(k_basis_fill, dim3(blocks_for(nt, 256)), dim3(256), 0, s, (const int64_t*)L.tree_keys.data(),
-                           (const int*)marks.data(), (const int*)prefix.data(), nt, d, L.basis_keys.data());
-        KERNEL_CHECK();
-      }
-    } else {
-      L.basis_count = 0;
-      L.basis_keys.reset(0, s);
+      scan_exclusive_i32(marks.data(), prefix.data(), nt, basis_counts.data() + d, s);
+      hipLaunchKernelGGL(k_basis_fill, dim3(blocks_for(nt, 256)), dim3(256), 0, s, (const int64_t*)L.tree_keys.data(),
+                         (const int*)marks.data(), (const int*)prefix.data(), nt, d, L.basis_keys.data());
+      KERNEL_CHECK();
     }
+  }
+  std::vector<int> counts(depth + 1);
+  read_bytes(counts.data(), basis_counts.data(), sizeof(int) * (depth + 1), s);
+  for (int d = 0; d <= depth; ++d) {
+    r.levels[d].basis_count = counts[d];
+    r.levels[d].basis_keys.shrink((size_t)counts[d]);
   }
 }
 
