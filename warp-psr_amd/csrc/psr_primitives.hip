@@ -331,6 +331,42 @@ __global__ void k_compact_heads(const int* __restrict__ heads, const int* __rest
 
 }  // namespace
 
+// Exclusive scan of up to SCAN_SMALL_TILES tiles in one workgroup, tile after
+// tile with a running carry (one launch where the tiled form takes three).
+constexpr int SCAN_SMALL_TILES = 4;
+
+__global__ void __launch_bounds__(SCAN_THREADS) k_scan_small(const int* __restrict__ in, int* __restrict__ out, int64_t n,
+                                                             int* __restrict__ total_out) {
+  __shared__ int tile[SCAN_TILE];
+  int carry = 0;
+  for (int64_t base = 0; base < n; base += SCAN_TILE) {
+    for (int r = 0; r < SCAN_ITEMS; ++r) {
+      const int64_t i = base + r * SCAN_THREADS + threadIdx.x;
+      tile[r * SCAN_THREADS + threadIdx.x] = i < n ? in[i] : 0;
+    }
+    __syncthreads();
+    int local[SCAN_ITEMS];
+    int s = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+      local[k] = s;
+      s += tile[threadIdx.x * SCAN_ITEMS + k];
+    }
+    int total;
+    const int prefix = block_exclusive_scan(s, &total);
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k) tile[threadIdx.x * SCAN_ITEMS + k] = local[k] + prefix + carry;
+    __syncthreads();
+    for (int r = 0; r < SCAN_ITEMS; ++r) {
+      const int64_t i = base + r * SCAN_THREADS + threadIdx.x;
+      if (i < n) out[i] = tile[r * SCAN_THREADS + threadIdx.x];
+    }
+    carry += total;
+    __syncthreads();
+  }
+  if (total_out && threadIdx.x == 0) *total_out = carry;
+}
+
 void scan_exclusive_i32(const int* in, int* out, int64_t n, int* total_dev, hipStream_t s) {
   if (n <= 0) {
     if (total_dev) HIP_CHECK(hipMemsetAsync(total_dev, 0, sizeof(int), s));
@@ -339,6 +375,9 @@ void scan_exclusive_i32(const int* in, int* out, int64_t n, int* total_dev, hipS
   const int64_t tiles = (n + SCAN_TILE - 1) / SCAN_TILE;
   if (tiles == 1) {
     hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(SCAN_THREADS), 0, s, in, out, n, (const int*)nullptr, total_dev);
+    KERNEL_CHECK();
+  } else if (tiles <= SCAN_SMALL_TILES) {
+    hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(SCAN_THREADS), 0, s, in, out, n, total_dev);
     KERNEL_CHECK();
   } else {
     DArray<int> sums(tiles, s);
