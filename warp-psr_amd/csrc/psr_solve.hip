@@ -2241,6 +2241,56 @@ int relax_tiled_graph(const GsOperator<T, V>& op, const TileTable& tt, const Row
   return (int)g->nodes.size();
 }
 
+// Levels of at most 1,024 rows (one per thread): each thread loads its
+// schedule position's 27 columns and values, b, 1/A_ii and x position once
+// and keeps them in registers for every sweep, so a colour phase is only LDS
+// gathers, the row's update and a barrier.  Same updates in the same order
+// as k_relax_block_lds.
+constexpr int RELAX_BLOCK_REGS_ROWS = 1024;
+
+template <class T>
+__global__ void __launch_bounds__(RELAX_BLOCK_REGS_ROWS) k_relax_block_regs(const int* __restrict__ order, const int* __restrict__ goff,
+                                                                            const int* __restrict__ cols, const T* __restrict__ vals,
+                                                                            const T* __restrict__ b, const T* __restrict__ inv_diag,
+                                                                            const unsigned char* __restrict__ has_diag,
+                                                                            T* __restrict__ x, int R, int iters, int forward) {
+  extern __shared__ unsigned char lds_raw[];
+  T* xs = (T*)lds_raw;
+  if (R <= 0) return;
+  const int j = (int)threadIdx.x;
+  if (j < R) xs[j] = x[j];
+  const int jj = j < R ? j : 0;  // clamped loads (R >= 1)
+  int cc[27];
+  T v[27];
+#pragma unroll
+  for (int k = 0; k < 27; ++k) {
+    cc[k] = cols[(int64_t)k * R + jj];
+    v[k] = vals[(int64_t)k * R + jj];
+  }
+  const T bj = b[jj], invj = inv_diag[jj];
+  const int oj = order[jj];
+  const bool has = j < R && has_diag[jj] != 0;
+  int group = -1;
+#pragma unroll
+  for (int g = 0; g < 8; ++g) group = j >= goff[g] && j < goff[g + 1] ? g : group;
+  __syncthreads();
+  for (int it = 0; it < iters; ++it) {
+    for (int c = 0; c < 8; ++c) {
+      if (group == (forward ? c : 7 - c)) {
+        T r = bj;
+#pragma unroll
+        for (int k = 0; k < 27; ++k) {
+          const T t = r - xs[cc[k] >= 0 ? cc[k] : 0] * v[k];
+          r = cc[k] >= 0 ? t : r;
+        }
+        if (has) xs[oj] = xs[oj] + r * invj;
+      }
+      __syncthreads();
+    }
+  }
+  if (j < R) x[j] = xs[j];
+}
+
 // x of up to 8,192 double (16,384 float) rows stays in LDS (64 KiB)
 constexpr size_t RELAX_BLOCK_LDS_BYTES = 65536;
 
@@ -2248,6 +2298,14 @@ template <class T>
 void relax_colored(const GsOperator<T>& op, const Schedule<T>& sc, T* x, int iters, bool forward, hipStream_t s) {
   if (!op.R) return;
   const size_t lds = sizeof(T) * (size_t)op.R;
+  if (op.update_order && op.R <= RELAX_BLOCK_REGS_ROWS) {
+    hipLaunchKernelGGL(k_relax_block_regs<T>, dim3(1), dim3(RELAX_BLOCK_REGS_ROWS), lds, s, op.update_order,
+                       (const int*)sc.offsets.data(), (const int*)op.cols.data(), (const T*)op.vals.data(),
+                       (const T*)op.b.data(), (const T*)op.inv_diag.data(), (const unsigned char*)op.has_diag.data(), x,
+                       op.R, iters, forward ? 1 : 0);
+    KERNEL_CHECK();
+    return;
+  }
   if (op.update_order && lds <= RELAX_BLOCK_LDS_BYTES) {
     hipLaunchKernelGGL(k_relax_block_lds<T>, dim3(1), dim3(1024), lds, s, op.update_order, (const int*)sc.offsets.data(),
                        (const int*)op.cols.data(), (const T*)op.vals.data(), (const T*)op.b.data(),
