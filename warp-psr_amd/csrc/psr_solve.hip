@@ -768,7 +768,7 @@ template <class T>
 __global__ void __launch_bounds__(128) k_galerkin(const int* __restrict__ rcols, int C, const int* __restrict__ acols,
                                                   const T* __restrict__ avals, int F, const int* __restrict__ pcols,
                                                   const T* __restrict__ pvals, int* __restrict__ out_cols, T* __restrict__ out_vals,
-                                                  const int64_t* __restrict__ ckeys, int64_t Ccount, int coarse_depth) {
+                                                  const int64_t* __restrict__ ckeys) {
   // one workgroup per coarse row r.  (R A)_rt for the 125 fine offsets t:
   // thread t sums its contributions in ascending R-row order (one A slot per
   // R entry reaches t), then thread s sums row s of (R A) P over t ascending
@@ -776,60 +776,77 @@ __global__ void __launch_bounds__(128) k_galerkin(const int* __restrict__ rcols,
   __shared__ T ra[125];
   __shared__ int rak[125];
   __shared__ int cs[125][8];
+  __shared__ int cn[125][8];  // the coarse column a contribution lands in
   __shared__ T cv[125][8];
   const int r = blockIdx.x;
   const int t = threadIdx.x;
   const int64_t rkey = ckeys[r];
   const int rx = key_x(rkey), ry = key_y(rkey), rz = key_z(rkey);
   if (t < 125) {
+    // Each dependent step (R columns, A columns, A values; P columns, coarse
+    // keys, P values) is issued for every slot before any is used, then the
+    // sums run in the sequential order.
     const int tx = t / 25, ty = (t / 5) % 5, tz = t % 5;
-    T acc = T(0);
-    int col = -1;
+    int ac[27];
+    int64_t at[27];
+#pragma unroll
     for (int kr = 0; kr < 27; ++kr) {  // fine k = 2r + o_kr, ascending
       const int k = rcols[(int64_t)kr * C + r];
-      if (k < 0) continue;
       const int jx = tx - kr / 9, jy = ty - (kr / 3) % 3, jz = tz - kr % 3;
-      if ((unsigned)jx > 2u || (unsigned)jy > 2u || (unsigned)jz > 2u) continue;
-      const int j = jx * 9 + jy * 3 + jz;
-      const int c = acols[(int64_t)j * F + k];
-      if (c < 0) continue;
-      acc = acc + T(restriction_weight(kr)) * avals[(int64_t)j * F + k];
-      col = c;
+      const bool in = k >= 0 && (unsigned)jx <= 2u && (unsigned)jy <= 2u && (unsigned)jz <= 2u;
+      at[kr] = in ? (int64_t)(jx * 9 + jy * 3 + jz) * F + k : -1;
+    }
+#pragma unroll
+    for (int kr = 0; kr < 27; ++kr) ac[kr] = at[kr] >= 0 ? acols[at[kr]] : -1;
+    T av[27];
+#pragma unroll
+    for (int kr = 0; kr < 27; ++kr) av[kr] = ac[kr] >= 0 ? avals[at[kr]] : T(0);
+    T acc = T(0);
+    int col = -1;
+#pragma unroll
+    for (int kr = 0; kr < 27; ++kr) {
+      if (ac[kr] < 0) continue;
+      acc = acc + T(restriction_weight(kr)) * av[kr];
+      col = ac[kr];
     }
     ra[t] = acc;
     rak[t] = col;
+    int pc[8];
+#pragma unroll
+    for (int kp = 0; kp < 8; ++kp) pc[kp] = col >= 0 ? pcols[(int64_t)kp * F + col] : -1;
+    int64_t pkey[8];
+#pragma unroll
+    for (int kp = 0; kp < 8; ++kp) pkey[kp] = pc[kp] >= 0 ? ckeys[pc[kp]] : int64_t(0);
+#pragma unroll
     for (int kp = 0; kp < 8; ++kp) {
       int sidx = -1;
       T pv = T(0);
-      if (col >= 0) {
-        const int pc = pcols[(int64_t)kp * F + col];
-        if (pc >= 0) {
-          const int64_t pkey = ckeys[pc];
-          const int ox = key_x(pkey) - rx + 1, oy = key_y(pkey) - ry + 1, oz = key_z(pkey) - rz + 1;
-          if ((unsigned)ox <= 2u && (unsigned)oy <= 2u && (unsigned)oz <= 2u) {  // always for trilinear P
-            sidx = ox * 9 + oy * 3 + oz;
-            pv = pvals[(int64_t)kp * F + col];
-          }
+      if (pc[kp] >= 0) {
+        const int ox = key_x(pkey[kp]) - rx + 1, oy = key_y(pkey[kp]) - ry + 1, oz = key_z(pkey[kp]) - rz + 1;
+        if ((unsigned)ox <= 2u && (unsigned)oy <= 2u && (unsigned)oz <= 2u) {  // always for trilinear P
+          sidx = ox * 9 + oy * 3 + oz;
+          pv = pvals[(int64_t)kp * F + col];
         }
       }
       cs[t][kp] = sidx;
+      cn[t][kp] = pc[kp];
       cv[t][kp] = pv;
     }
   }
   __syncthreads();
   if (t < 27) {
+    // the column of slot t is the coarse index whose key sits at offset t
+    // from row r (keys are unique), read from any contribution landing there
     T out = T(0);
-    bool present = false;
+    int col = -1;
     for (int u = 0; u < 125; ++u) {
       if (rak[u] < 0) continue;
       for (int kp = 0; kp < 8; ++kp)
         if (cs[u][kp] == t) {
           out = out + ra[u] * cv[u][kp];
-          present = true;
+          col = cn[u][kp];
         }
     }
-    int col = -1;
-    if (present) col = (int)find_key(ckeys, Ccount, basis_key(coarse_depth, rx + t / 9 - 1, ry + (t / 3) % 3 - 1, rz + t % 3 - 1));
     out_cols[(int64_t)t * C + r] = col;
     out_vals[(int64_t)t * C + r] = out;
   }
@@ -1780,7 +1797,11 @@ template <class T> struct Schedule {
   int max_group = 0;
 };
 
-template <class T> static Schedule<T> make_schedule(const DArray<int64_t>& keys, int R, int depth, bool sliced, hipStream_t s) {
+// host: also copy the group offsets to the host (host_offsets, max_group),
+// which the launch-per-group relaxations size their grids by; the
+// single-workgroup relaxations read them on the device only.
+template <class T> static Schedule<T> make_schedule(const DArray<int64_t>& keys, int R, int depth, bool sliced, hipStream_t s,
+                                                     bool host = true) {
   Schedule<T> sc;
   const int groups = sliced ? 4 * ((1 << depth) + 1) : 8;
   DArray<uint64_t> g(R, s);
@@ -1795,6 +1816,7 @@ template <class T> static Schedule<T> make_schedule(const DArray<int64_t>& keys,
   hipLaunchKernelGGL(k_group_offsets, dim3(blocks_for(groups + 1, 256)), dim3(256), 0, s, (const uint64_t*)g.data(), R, groups,
                      sc.offsets.data());
   KERNEL_CHECK();
+  if (!host) return sc;
   sc.host_offsets.resize(groups + 1);
   read_bytes(sc.host_offsets.data(), sc.offsets.data(), sizeof(int) * (groups + 1), s);
   for (int t = 0; t < groups; ++t) sc.max_group = std::max(sc.max_group, sc.host_offsets[t + 1] - sc.host_offsets[t]);
@@ -2909,7 +2931,7 @@ template <class T> static void regular_multigrid(Result<T>& r, std::vector<DArra
       hipLaunchKernelGGL(k_galerkin<T>, dim3(C), dim3(128), 0, s, (const int*)restr[d - 1].data(), C,
                          (const int*)mcols[d].data(), (const T*)mvals[d].data(), F, (const int*)r.levels[d].P.cols.data(),
                          (const T*)r.levels[d].P.vals.data(), mcols[d - 1].data(), mvals[d - 1].data(),
-                         (const int64_t*)r.levels[d - 1].basis_keys.data(), (int64_t)C, d - 1);
+                         (const int64_t*)r.levels[d - 1].basis_keys.data());
       KERNEL_CHECK();
     }
   }
@@ -2924,7 +2946,7 @@ template <class T> static void regular_multigrid(Result<T>& r, std::vector<DArra
   std::vector<Schedule<T>> sched;
   std::vector<GsOperator<T>> ops;
   for (int d = 0; d <= base; ++d) {
-    sched.push_back(make_schedule<T>(r.levels[d].basis_keys, Rn[d], d, false, s));
+    sched.push_back(make_schedule<T>(r.levels[d].basis_keys, Rn[d], d, false, s, false));
     ops.push_back(gather_operator<T>(sched[d], mcols[d], mvals[d], rhs[d].data(), Rn[d], s));
   }
   const int dc = (base == 0 && r.prm.point_weight == 0.0) ? 1 : 0;
@@ -4205,7 +4227,7 @@ template <class T> void solve_backward(Result<T>& r, const double* gcoef, double
           hipLaunchKernelGGL(k_galerkin<T>, dim3(C), dim3(128), 0, s, (const int*)r.levels[d - 1].restriction.data(),
                              C, (const int*)mcols[d].data(), (const T*)mvals[d].data(), F, (const int*)r.levels[d].P.cols.data(),
                              (const T*)r.levels[d].P.vals.data(), mcols[d - 1].data(), mvals[d - 1].data(),
-                             (const int64_t*)r.levels[d - 1].basis_keys.data(), (int64_t)C, d - 1);
+                             (const int64_t*)r.levels[d - 1].basis_keys.data());
           KERNEL_CHECK();
         }
       }
@@ -4215,7 +4237,7 @@ template <class T> void solve_backward(Result<T>& r, const double* gcoef, double
       std::vector<Schedule<T>> sched;
       std::vector<GsOperator<T>> ops;
       for (int d = 0; d <= base; ++d) {
-        sched.push_back(make_schedule<T>(r.levels[d].basis_keys, Rn[d], d, false, s));
+        sched.push_back(make_schedule<T>(r.levels[d].basis_keys, Rn[d], d, false, s, false));
         ops.push_back(gather_operator<T>(sched[d], mcols[d], mvals[d], zero_b.data(), Rn[d], s));
       }
       const int iters = r.prm.iters, cycles = r.prm.base_v_cycles;
