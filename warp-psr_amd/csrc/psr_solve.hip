@@ -778,10 +778,12 @@ __global__ void __launch_bounds__(128) k_galerkin(const int* __restrict__ rcols,
   __shared__ int cs[125][8];
   __shared__ int cn[125][8];  // the coarse column a contribution lands in
   __shared__ T cv[125][8];
+  __shared__ int landed[128];  // contributions per fine offset
   const int r = blockIdx.x;
   const int t = threadIdx.x;
   const int64_t rkey = ckeys[r];
   const int rx = key_x(rkey), ry = key_y(rkey), rz = key_z(rkey);
+  int n = 0;
   if (t < 125) {
     // Each dependent step (R columns, A columns, A values; P columns, coarse
     // keys, P values) is issued for every slot before any is used, then the
@@ -831,24 +833,67 @@ __global__ void __launch_bounds__(128) k_galerkin(const int* __restrict__ rcols,
       cs[t][kp] = sidx;
       cn[t][kp] = pc[kp];
       cv[t][kp] = pv;
+      n += sidx >= 0 ? 1 : 0;
     }
   }
+  if (t < 128) landed[t] = n;
   __syncthreads();
-  if (t < 27) {
-    // the column of slot t is the coarse index whose key sits at offset t
-    // from row r (keys are unique), read from any contribution landing there
+  if (t < 64) {
+    // Slot t (coarse offset o per axis) is reached only from fine offsets
+    // 2o-1..2o+1 per axis (trilinear P: fine f feeds coarse f/2 when f is
+    // even, (f-1)/2 and (f+1)/2 when odd), so each slot scans those 27 in
+    // ascending order -- the order of the full scan.  Every contribution is
+    // counted once: if the candidates miss any, the full scan runs instead.
+    // The column of slot t is the coarse index of any contribution landing
+    // there (keys are unique).
+    int total = landed[t] + landed[t + 64];
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) total += __shfl_xor(total, m);
     T out = T(0);
-    int col = -1;
-    for (int u = 0; u < 125; ++u) {
-      if (rak[u] < 0) continue;
-      for (int kp = 0; kp < 8; ++kp)
-        if (cs[u][kp] == t) {
-          out = out + ra[u] * cv[u][kp];
-          col = cn[u][kp];
+    int col = -1, found = 0;
+    if (t < 27) {
+      const int ox = t / 9, oy = (t / 3) % 3, oz = t % 3;
+      for (int a = 0; a < 3; ++a) {
+        const int ux = 2 * ox - 1 + a;
+        if (ux < 0 || ux > 4) continue;
+        for (int b = 0; b < 3; ++b) {
+          const int uy = 2 * oy - 1 + b;
+          if (uy < 0 || uy > 4) continue;
+          for (int c = 0; c < 3; ++c) {
+            const int uz = 2 * oz - 1 + c;
+            if (uz < 0 || uz > 4) continue;
+            const int u = ux * 25 + uy * 5 + uz;
+            if (rak[u] < 0) continue;
+#pragma unroll
+            for (int kp = 0; kp < 8; ++kp)
+              if (cs[u][kp] == t) {
+                out = out + ra[u] * cv[u][kp];
+                col = cn[u][kp];
+                ++found;
+              }
+          }
         }
+      }
     }
-    out_cols[(int64_t)t * C + r] = col;
-    out_vals[(int64_t)t * C + r] = out;
+    int all = found;
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) all += __shfl_xor(all, m);
+    if (all != total && t < 27) {
+      out = T(0);
+      col = -1;
+      for (int u = 0; u < 125; ++u) {
+        if (rak[u] < 0) continue;
+        for (int kp = 0; kp < 8; ++kp)
+          if (cs[u][kp] == t) {
+            out = out + ra[u] * cv[u][kp];
+            col = cn[u][kp];
+          }
+      }
+    }
+    if (t < 27) {
+      out_cols[(int64_t)t * C + r] = col;
+      out_vals[(int64_t)t * C + r] = out;
+    }
   }
 }
 
