@@ -102,3 +102,37 @@ def test_batch_matches_single_reconstructions(cuda):
         for name in ("keys", "coefficients", "cumulative"):
             assert same(b.diagnostics.array(name), single.diagnostics.array(name)), name
         assert b.iso_value == single.iso_value
+
+
+@pytest.mark.parametrize("precision", ["double", "float"])
+def test_grouped_batch_relaxation_bit_exact(cuda, monkeypatch, precision):
+    """With PSR_BATCH_GROUPS=1 batch groups relax each finer level in one
+    launch sequence over all members' tiles.  Groups of 4 here mix depths 6
+    and 5, first finer levels 4 and 3, and (double inputs) mixed and float64
+    operators; every cloud equals its own reconstruct() bit for bit, and the
+    batch without groups (PSR_BATCH_GROUPS=0, the default)."""
+    import torch
+    from warp_psr_amd import reconstruct, reconstruct_batch
+    shapes = [("torus", 8192), ("sphere", 2048), ("cluster", 1024), ("plane", 1024), ("disconnected", 1024)]
+    clouds, opts = [], []
+    for i in range(10):
+        shape, count = shapes[i % 5]
+        p, n = fixture_arrays(shape, count, precision)
+        clouds.append((torch.from_numpy(p).to(cuda), torch.from_numpy(n).to(cuda)))
+        depth = 6 if i % 2 == 0 else 5
+        params = dict(depth=depth, full_depth=3 if depth == 6 else 2, point_weight=2.0)
+        if precision == "double" and i % 3 == 0:
+            params["mixed"] = True
+        opts.append(params)
+    print(f"grouped batch: {len(clouds)} {precision} clouds, concurrency 4", flush=True)
+    monkeypatch.setenv("PSR_BATCH_GROUPS", "1")
+    batch = reconstruct_batch(clouds, per_cloud=opts, max_concurrency=4)
+    monkeypatch.setenv("PSR_BATCH_GROUPS", "0")
+    print("grouped batch: the same clouds without groups", flush=True)
+    plain = reconstruct_batch(clouds, per_cloud=opts, max_concurrency=4)
+    for (p, n), params, b, q in zip(clouds, opts, batch, plain):
+        single = reconstruct(p, n, **params)
+        for name in ("keys", "coefficients", "cumulative"):
+            assert same(b.diagnostics.array(name), single.diagnostics.array(name)), name
+            assert same(q.diagnostics.array(name), single.diagnostics.array(name)), name
+        assert b.iso_value == single.iso_value == q.iso_value

@@ -8,8 +8,11 @@
 #include "psr_mesh.h"
 #include "psr_meta.h"
 
+#include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <map>
 #include <sstream>
 #include <atomic>
@@ -405,33 +408,69 @@ int psr_reconstruct_batch(const void* const* points, const void* const* normals,
     HIP_CHECK(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
     HIP_CHECK(hipEventRecord(ready, (hipStream_t)stream));
     for (hipStream_t ws : streams) HIP_CHECK(hipStreamWaitEvent(ws, ready, 0));
-    std::atomic<int> next(0);
     std::vector<int> codes(batch, 0);
     std::vector<std::string> messages(batch);
     const auto start = std::chrono::steady_clock::now();
-    auto work = [&](int w) {
-      if (hipSetDevice(dev) != hipSuccess) return;
-      for (int i = next++; i < batch; i = next++) {
-        try {
-          out[i] = reconstruct_one(points[i], normals[i], counts[i], dtype, prm[i], streams[w]);
-          out[i]->sync_on_free = true;
-          HIP_CHECK(hipStreamSynchronize(streams[w]));
-          // the cloud's completion time since the batch began, seen by the
-          // host after its stream drained (the metadata's last key)
-          const double done = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
-          std::ostringstream o;
-          o.precision(9);
-          o << ", \"batch_completion_seconds\": " << done << "}";
-          std::string& m = out[i]->metadata;
-          if (!m.empty() && m.back() == '}') m.replace(m.size() - 1, 1, o.str());
-        } catch (const PsrError& e) {
-          codes[i] = e.code;
-          messages[i] = e.what();
-        } catch (const std::exception& e) {
-          codes[i] = 1;
-          messages[i] = e.what();
-        }
+    auto run_cloud = [&](int i, hipStream_t ws) {
+      try {
+        out[i] = reconstruct_one(points[i], normals[i], counts[i], dtype, prm[i], ws);
+        out[i]->sync_on_free = true;
+        HIP_CHECK(hipStreamSynchronize(ws));
+        // the cloud's completion time since the batch began, seen by the
+        // host after its stream drained (the metadata's last key)
+        const double done = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
+        std::ostringstream o;
+        o.precision(9);
+        o << ", \"batch_completion_seconds\": " << done << "}";
+        std::string& m = out[i]->metadata;
+        if (!m.empty() && m.back() == '}') m.replace(m.size() - 1, 1, o.str());
+      } catch (const PsrError& e) {
+        codes[i] = e.code;
+        messages[i] = e.what();
+      } catch (const std::exception& e) {
+        codes[i] = 1;
+        messages[i] = e.what();
       }
+    };
+    // By default the workers take clouds from a shared counter, so one
+    // cloud's latency-bound relaxation overlaps other clouds' stages.  With
+    // PSR_BATCH_GROUPS=1 (2 to 8 workers) the clouds, deepest and then
+    // largest first, form groups of `workers`; worker w runs member w of
+    // every group, and a group's members relax each finer level in one
+    // launch sequence, in lockstep.
+    const char* env = std::getenv("PSR_BATCH_GROUPS");
+    const bool grouped = workers > 1 && workers <= 8 && env && std::string(env) == "1";
+    std::vector<int> order(batch);
+    for (int i = 0; i < batch; ++i) order[i] = i;
+    std::vector<std::unique_ptr<RelaxGroup, void (*)(RelaxGroup*)>> groups;
+    if (grouped) {
+      std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        if (prm[a].depth != prm[b].depth) return prm[a].depth > prm[b].depth;
+        return counts[a] > counts[b];
+      });
+      for (int g = 0; g * workers < batch; ++g)
+        groups.emplace_back(relax_group_create(std::min(workers, batch - g * workers)), relax_group_destroy);
+    }
+    std::atomic<int> next(0);
+    auto work = [&](int w) {
+      const bool device_ok = hipSetDevice(dev) == hipSuccess;
+      if (grouped) {
+        for (size_t g = 0; g < groups.size(); ++g) {
+          const int idx = (int)g * workers + w;
+          if (idx >= batch) break;
+          relax_group_bind(groups[g].get(), w);
+          if (device_ok) {
+            run_cloud(order[idx], streams[w]);
+          } else {
+            codes[order[idx]] = 1;
+            messages[order[idx]] = "hipSetDevice failed on a worker thread";
+          }
+          relax_group_release();  // also when the cloud failed: the group stops waiting for it
+        }
+        return;
+      }
+      if (!device_ok) return;
+      for (int i = next++; i < batch; i = next++) run_cloud(i, streams[w]);
     };
     std::vector<std::thread> threads;
     for (int w = 1; w < workers; ++w) threads.emplace_back(work, w);

@@ -221,6 +221,15 @@ void relax_scheduled_csr(const int64_t* keys, int R, const int* offsets, const i
 template <class T> void sort_records_u64(DArray<uint64_t>& keys, DArray<int>& ids, int64_t n, int bits, hipStream_t s);
 
 // Host-side transform (samples.py:54-88) -- scalar work done once per call.
+// Batched reconstructions relax in groups of up to 8: a worker thread binds
+// itself to a member slot for one reconstruction and releases it after (also
+// on failure); see RelaxGroup in psr_solve.hip.
+struct RelaxGroup;
+RelaxGroup* relax_group_create(int members);
+void relax_group_destroy(RelaxGroup* g);
+void relax_group_bind(RelaxGroup* g, int member);
+void relax_group_release();
+
 template <class T> int compute_transform(const T* b, double scale, T* t);
 template <class T> void reference_inverse(const T* m, T* out);
 

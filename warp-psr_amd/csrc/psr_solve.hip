@@ -1745,12 +1745,10 @@ __device__ __forceinline__ void tiled_phase(int phase, T* grid, int gh, T* Q, co
   }
 }
 
+// One tile of slice s of a wave: the bx-th tile of the slice's range.
 template <class T, class V, int BS>
-__global__ void __launch_bounds__(BS) k_relax_tiled(TiledArgs<T, V> a, WaveTiles wv) {
-  extern __shared__ unsigned char lds_raw[];
-  T* grid = (T*)lds_raw;
-  const int s = blockIdx.y;
-  const int tile = wv.start[s] + (int)blockIdx.x;
+__device__ __forceinline__ void relax_tile(const TiledArgs<T, V>& a, const WaveTiles& wv, int bx, int s, T* grid) {
+  const int tile = wv.start[s] + bx;
   if (tile >= wv.end[s]) return;
   const bool flip = (wv.flip >> s) & 1u;
   const T* P = flip ? a.x1 : a.x0;
@@ -1828,6 +1826,33 @@ __global__ void __launch_bounds__(BS) k_relax_tiled(TiledArgs<T, V> a, WaveTiles
     }
     __syncthreads();
   }
+}
+
+template <class T, class V, int BS>
+__global__ void __launch_bounds__(BS) k_relax_tiled(TiledArgs<T, V> a, WaveTiles wv) {
+  extern __shared__ unsigned char lds_raw[];
+  relax_tile<T, V, BS>(a, wv, (int)blockIdx.x, (int)blockIdx.y, (T*)lds_raw);
+}
+
+// Several reconstructions' tiles of the same wave in one launch: members'
+// blocks are consecutive ranges of x, [first[m], first[m + 1]); the
+// reconstructions share no data, so each tile's updates are those of its
+// own reconstruction's launch.
+constexpr int RELAX_GROUP_MAX = 8;
+
+template <class T, class V> struct TiledGroup {
+  int members;
+  int first[RELAX_GROUP_MAX + 1];
+  TiledArgs<T, V> a[RELAX_GROUP_MAX];
+  WaveTiles wv[RELAX_GROUP_MAX];
+};
+
+template <class T, class V, int BS>
+__global__ void __launch_bounds__(BS) k_relax_tiled_group(TiledGroup<T, V> g) {
+  int m = 0;
+  for (int k = 1; k < g.members; ++k) m = (int)blockIdx.x >= g.first[k] ? k : m;
+  extern __shared__ unsigned char lds_raw[];
+  relax_tile<T, V, BS>(g.a[m], g.wv[m], (int)blockIdx.x - g.first[m], (int)blockIdx.y, (T*)lds_raw);
 }
 
 inline int floordiv(int a, int b) { return (a >= 0) ? a / b : -((-a + b - 1) / b); }
@@ -2224,15 +2249,18 @@ template <class T, class V, int BS> static void* tiled_kernel() {
   return reinterpret_cast<void*>(&k_relax_tiled<T, V, BS>);
 }
 
-template <class T, class V>
-static std::unique_ptr<TiledGraph> build_tiled_graph(int n, int iters, int width, unsigned bx, size_t lds) {
+template <class T, class V, int BS> static void* tiled_group_kernel() {
+  return reinterpret_cast<void*>(&k_relax_tiled_group<T, V, BS>);
+}
+
+// One kernel node per wave of a level of side n with iters sweeps, chained;
+// args points at placeholder arguments of the kernel func.
+static std::unique_ptr<TiledGraph> build_wave_graph(int n, int iters, void* func, int width, unsigned bx, size_t lds,
+                                                     void** args) {
   auto g = std::make_unique<TiledGraph>();
   HIP_CHECK(hipGraphCreate(&g->graph, 0));
-  TiledArgs<T, V> a{};
-  WaveTiles wv{};
-  void* args[2] = {&a, &wv};
   hipKernelNodeParams& kp = g->params;
-  kp.func = width == 64 ? tiled_kernel<T, V, 64>() : width == 128 ? tiled_kernel<T, V, 128>() : tiled_kernel<T, V, 256>();
+  kp.func = func;
   kp.blockDim = dim3(width);
   kp.sharedMemBytes = (unsigned)lds;
   kp.extra = nullptr;
@@ -2254,6 +2282,15 @@ static std::unique_ptr<TiledGraph> build_tiled_graph(int n, int iters, int width
   HIP_CHECK(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
   HIP_CHECK(hipEventCreateWithFlags(&g->done, hipEventDisableTiming));
   return g;
+}
+
+template <class T, class V>
+static std::unique_ptr<TiledGraph> build_tiled_graph(int n, int iters, int width, unsigned bx, size_t lds) {
+  TiledArgs<T, V> a{};
+  WaveTiles wv{};
+  void* args[2] = {&a, &wv};
+  void* func = width == 64 ? tiled_kernel<T, V, 64>() : width == 128 ? tiled_kernel<T, V, 128>() : tiled_kernel<T, V, 256>();
+  return build_wave_graph(n, iters, func, width, bx, lds, args);
 }
 
 // relax_sliced on the tile tables: x (schedule order) holds the start values
@@ -2306,6 +2343,243 @@ int relax_tiled_graph(const GsOperator<T, V>& op, const TileTable& tt, const Row
   HIP_CHECK(hipGraphLaunch(g->exec, s));
   HIP_CHECK(hipEventRecord(g->done, s));
   return (int)g->nodes.size();
+}
+
+// ---------------------------------------------------------------- grouped relaxation
+// Reconstructions of a batch run in groups; at each finer level every member
+// offers its tiled relaxation and one of them launches a single graph whose
+// wave launches cover all members' tiles (k_relax_tiled_group), so a group
+// of m reconstructions issues the launches of one.  The levels run in
+// lockstep: a level's graph starts once every member that will relax it has
+// offered it.
+
+// What a member offers at a level: its arguments, tile table and the stream
+// (with an event) its inputs are ready on; valid is false for a level it
+// relaxes on its own.
+struct GroupOffer {
+  bool valid = false;
+  int value_bytes = 0, vector_bytes = 0, gside = 0, width = 0, iters = 0;
+  const void* entries = nullptr;
+  const void* desc = nullptr;
+  const void* records = nullptr;
+  const void* b = nullptr;
+  void* x0 = nullptr;
+  void* x1 = nullptr;
+  const TileTable* tt = nullptr;
+  hipEvent_t ready = nullptr;
+};
+
+struct RelaxGroup {
+  explicit RelaxGroup(int n) : members(n), first(n, -1), arrived(n, -1), finished(n, 0) {}
+  ~RelaxGroup() {
+    for (auto& kv : levels) {
+      for (const GroupOffer& o : kv.second.offers)
+        if (o.ready) (void)hipEventDestroy(o.ready);
+      for (hipEvent_t e : kv.second.events) (void)hipEventDestroy(e);
+    }
+  }
+  struct Level {
+    std::vector<GroupOffer> offers;  // by member
+    std::vector<hipEvent_t> done;    // by member: the launch it waits for, or null
+    std::vector<hipEvent_t> events;  // owned
+    bool launching = false, launched = false;
+    int launches = 0;
+    std::string error;
+  };
+  // every member that will relax level L has offered it
+  bool complete(int L) const {
+    for (int m = 0; m < members; ++m)
+      if (!finished[m] && arrived[m] < L && !(first[m] >= 0 && first[m] > L)) return false;
+    return true;
+  }
+  std::mutex mu;
+  std::condition_variable cv;
+  const int members;
+  std::vector<int> first;    // first finer level, -1 until declared
+  std::vector<int> arrived;  // last level offered
+  std::vector<char> finished;
+  std::map<int, Level> levels;
+};
+
+static thread_local RelaxGroup* t_group = nullptr;
+static thread_local int t_member = -1;
+
+RelaxGroup* relax_group_create(int members) {
+  if (members < 1 || members > RELAX_GROUP_MAX) throw PsrError(2, "a relaxation group holds 1 to 8 reconstructions");
+  return new RelaxGroup(members);
+}
+void relax_group_destroy(RelaxGroup* g) { delete g; }
+void relax_group_bind(RelaxGroup* g, int member) {
+  t_group = g;
+  t_member = member;
+}
+
+// The bound member relaxes no further level (its cascade ended or failed).
+static void group_finish() {
+  if (!t_group) return;
+  {
+    std::lock_guard<std::mutex> lk(t_group->mu);
+    t_group->finished[t_member] = 1;
+  }
+  t_group->cv.notify_all();
+}
+
+void relax_group_release() {
+  group_finish();
+  t_group = nullptr;
+  t_member = -1;
+}
+
+static void group_declare(int first_level) {
+  if (!t_group) return;
+  {
+    std::lock_guard<std::mutex> lk(t_group->mu);
+    t_group->first[t_member] = first_level;
+  }
+  t_group->cv.notify_all();
+}
+
+// One graph over the members ms (same precision, tile side and sweeps) on s;
+// returns the event recorded after it and the launch count.
+template <class T, class V>
+static std::pair<hipEvent_t, int> launch_tiled_group(const std::vector<const GroupOffer*>& ms, int depth, hipStream_t s) {
+  static std::mutex mu;
+  static std::map<std::tuple<int, int, int, int, int, int, hipStream_t>, std::unique_ptr<TiledGraph>> cache;
+  const int n = 1 << depth, M = (int)ms.size(), iters = ms[0]->iters, gside = ms[0]->gside;
+  int width = 64;
+  for (const GroupOffer* o : ms) width = std::max(width, o->width);
+  const size_t lds = sizeof(T) * 3 * (size_t)gside * gside;
+  int dev = 0;
+  HIP_CHECK(hipGetDevice(&dev));
+  TiledGraph* g = nullptr;
+  {
+    std::lock_guard<std::mutex> guard(mu);
+    std::unique_ptr<TiledGraph>& slot = cache[std::make_tuple((int)sizeof(V), depth, iters, width, M, dev, s)];
+    if (!slot) {
+      TiledGroup<T, V> tg{};
+      void* args[1] = {&tg};
+      void* func = width == 64    ? tiled_group_kernel<T, V, 64>()
+                   : width == 128 ? tiled_group_kernel<T, V, 128>()
+                                  : tiled_group_kernel<T, V, 256>();
+      slot = build_wave_graph(n, iters, func, width, 1, lds, args);
+    }
+    g = slot.get();
+  }
+  for (const GroupOffer* o : ms) HIP_CHECK(hipStreamWaitEvent(s, o->ready, 0));
+  std::lock_guard<std::mutex> guard(g->mu);
+  HIP_CHECK(hipEventSynchronize(g->done));
+  TiledGroup<T, V> tg{};
+  tg.members = M;
+  for (int m = 0; m < M; ++m) {
+    const GroupOffer& o = *ms[m];
+    tg.a[m] = TiledArgs<T, V>{(const int2*)o.entries, (const int*)o.desc, (const RowRecord<T, V>*)o.records, (const T*)o.b,
+                              (T*)o.x0, (T*)o.x1, gside};
+  }
+  void* args[1] = {&tg};
+  hipKernelNodeParams kp = g->params;
+  kp.kernelParams = args;
+  kp.sharedMemBytes = (unsigned)lds;
+  for (size_t i = 0; i < g->nodes.size(); ++i) {
+    int total = 0;
+    for (int m = 0; m < M; ++m) {
+      const TileTable& tt = *ms[m]->tt;
+      WaveTiles& wv = tg.wv[m];
+      for (int t = 0; t < RELAX_MAX_SLICES; ++t) wv.start[t] = wv.end[t] = 0;
+      wv.flip = 0;
+      int blocks = 0;
+      for (int k = 0; k < g->slices[i]; ++k) {
+        const int z = g->first_z[i] + 2 * k, t = g->first_t[i] + k;
+        wv.start[k] = tt.slice_start[z];
+        wv.end[k] = tt.slice_start[z + 1];
+        blocks = std::max(blocks, wv.end[k] - wv.start[k]);
+        if ((iters - t) % 2) wv.flip |= 1u << k;
+      }
+      tg.first[m] = total;
+      total += blocks;
+    }
+    tg.first[M] = total;
+    kp.gridDim = dim3((unsigned)std::max(1, total), (unsigned)g->slices[i]);
+    HIP_CHECK(hipGraphExecKernelNodeSetParams(g->exec, g->nodes[i], &kp));
+  }
+  HIP_CHECK(hipGraphLaunch(g->exec, s));
+  HIP_CHECK(hipEventRecord(g->done, s));
+  hipEvent_t ev = nullptr;
+  HIP_CHECK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  HIP_CHECK(hipEventRecord(ev, s));
+  return {ev, (int)g->nodes.size()};
+}
+
+// Launches level L for every offer, one graph per kind of offer, on s.
+static void group_launch_level(RelaxGroup::Level& lv, int L, hipStream_t s) {
+  std::vector<char> taken(lv.offers.size(), 0);
+  for (size_t i = 0; i < lv.offers.size(); ++i) {
+    const GroupOffer& oi = lv.offers[i];
+    if (!oi.valid || taken[i]) continue;
+    std::vector<const GroupOffer*> ms;
+    std::vector<size_t> who;
+    for (size_t j = i; j < lv.offers.size(); ++j) {
+      const GroupOffer& oj = lv.offers[j];
+      if (!oj.valid || taken[j] || oj.value_bytes != oi.value_bytes || oj.vector_bytes != oi.vector_bytes ||
+          oj.gside != oi.gside || oj.iters != oi.iters)
+        continue;
+      taken[j] = 1;
+      ms.push_back(&oj);
+      who.push_back(j);
+    }
+    std::pair<hipEvent_t, int> r;
+    if (oi.vector_bytes == 8 && oi.value_bytes == 4) r = launch_tiled_group<double, float>(ms, L, s);
+    else if (oi.vector_bytes == 8) r = launch_tiled_group<double, double>(ms, L, s);
+    else r = launch_tiled_group<float, float>(ms, L, s);
+    lv.events.push_back(r.first);
+    for (size_t j : who) lv.done[j] = r.first;
+    lv.launches = r.second;
+  }
+}
+
+// The bound member's offer at level L; returns the launch count when its
+// relaxation ran in a group launch (s then waits for it), else 0.
+static int group_relax(int L, GroupOffer offer, hipStream_t s) {
+  RelaxGroup* g = t_group;
+  const int me = t_member;
+  if (offer.valid) {
+    HIP_CHECK(hipEventCreateWithFlags(&offer.ready, hipEventDisableTiming));
+    HIP_CHECK(hipEventRecord(offer.ready, s));
+  }
+  std::unique_lock<std::mutex> lk(g->mu);
+  RelaxGroup::Level& lv = g->levels[L];
+  if (lv.offers.empty()) {
+    lv.offers.resize(g->members);
+    lv.done.assign(g->members, nullptr);
+  }
+  lv.offers[me] = offer;
+  g->arrived[me] = L;
+  g->cv.notify_all();
+  while (!lv.launched) {
+    if (!lv.launching && g->complete(L)) {
+      lv.launching = true;
+      lk.unlock();
+      std::string error;
+      try {
+        group_launch_level(lv, L, s);
+      } catch (const std::exception& e) {
+        error = e.what();
+      }
+      lk.lock();
+      lv.error = error;
+      lv.launched = true;
+      g->cv.notify_all();
+      break;
+    }
+    g->cv.wait(lk);
+  }
+  const hipEvent_t done = lv.done[me];
+  const std::string error = lv.error;
+  const int launches = lv.launches;
+  lk.unlock();
+  if (!error.empty()) throw PsrError(1, "grouped relaxation at depth " + std::to_string(L) + ": " + error);
+  if (!done) return 0;
+  HIP_CHECK(hipStreamWaitEvent(s, done, 0));
+  return launches;
 }
 
 // Levels of at most 1,024 rows (one per thread): each thread loads its
@@ -3137,7 +3411,30 @@ static void solve_finer(Result<T>& r, int d, Stencil<V>& Aop, Stencil<V>& Cop, F
   HIP_CHECK(hipEventCreate(&st.start));
   HIP_CHECK(hipEventCreate(&st.end));
   HIP_CHECK(hipEventRecord(st.start, s));
-  if (tiled)
+  int grouped = 0;
+  if (t_group) {  // a batch member: the level's tiles join its group's launch
+    GroupOffer o;
+    if (tiled) {
+      if (r.prm.iters % 2) HIP_CHECK(hipMemcpyAsync(xs1.data(), xs.data(), sizeof(T) * R, hipMemcpyDeviceToDevice, s));
+      o.valid = true;
+      o.value_bytes = (int)sizeof(V);
+      o.vector_bytes = (int)sizeof(T);
+      o.gside = f.tiles.ts + 8;
+      o.width = tiled_width(f.tiles);
+      o.iters = r.prm.iters;
+      o.entries = f.tiles.entries.data();
+      o.desc = f.tiles.desc.data();
+      o.records = f.records.data();
+      o.b = op.b.data();
+      o.x0 = xs.data();
+      o.x1 = xs1.data();
+      o.tt = &f.tiles;
+    }
+    grouped = group_relax(d, o, s);
+  }
+  if (grouped)
+    st.launches = grouped;
+  else if (tiled)
     st.launches = relax_tiled_graph(op, f.tiles, (const RowRecord<T, V>*)f.records.data(), xs.data(), xs1.data(), d,
                                     r.prm.iters, s);
   else
@@ -3353,12 +3650,14 @@ template <class T, class V>
 static void solve_levels(Result<T>& r, std::vector<DArray<int>>& restr, bool screened, hipStream_t s) {
   const int base = r.prm.base_depth;
   NormalLoadPlan<T> plan;  // outlives the assembly, whose stream reads its arrays
+  group_declare(base + 1);
   {
     FinerAssembly<T, V> finer(r, base + 1, r.prm.solve_depth, screened, s);
     normal_loads_base(r, restr, plan, s);
     finer.set_loads(&plan);
     solve_cascade<T, V>(r, restr, screened, finer, s);
   }  // the solve stream is ordered after the assembly stream here
+  group_finish();
   normal_loads_rest(r, plan, s);
 }
 
