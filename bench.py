@@ -65,7 +65,7 @@ def relax_roofline(metadata, value_bytes, vector_bytes, kernel=None):
     """Sliced Gauss-Seidel relaxation of every finer level: algorithmic bytes
     (relax_bytes) over the HIP-event span of its launches on the
     reconstruction stream.  The tiled form (k_relax_tiled) launches once per
-    wave, after one k_tile_ops per level (inside the span); the
+    wave, over row records the level's operator pass wrote; the
     colour-per-launch form (k_relax_wave_args) four times per wave.
     """
     stats = metadata["relaxation"]
