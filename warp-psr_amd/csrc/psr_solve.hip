@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <condition_variable>
 #include <exception>
+#include <functional>
 #include <thread>
 #include <cmath>
 #include <cstdlib>
@@ -3229,7 +3230,12 @@ template <class T> static void normal_loads_rest(Result<T>& r, NormalLoadPlan<T>
 }
 
 // solver.py:241-317 (regular_multigrid)
-template <class T> static void regular_multigrid(Result<T>& r, std::vector<DArray<int>>& restr, hipStream_t s) {
+// rhs_ready (optional) runs on the host once the levels' operators and
+// schedules are enqueued, before the base right-hand side is read: the
+// caller makes it ready there (it may depend on work of another stream).
+template <class T>
+static void regular_multigrid(Result<T>& r, std::vector<DArray<int>>& restr, hipStream_t s,
+                              const std::function<void()>& rhs_ready = {}) {
   const int base = r.prm.base_depth;
   LevelState<T>& B = r.levels[base];
   std::vector<DArray<int>> mcols(base + 1);
@@ -3261,13 +3267,14 @@ template <class T> static void regular_multigrid(Result<T>& r, std::vector<DArra
     x[d].reset(Rn[d], s);
     x[d].zero();
   }
-  if (Rn[base]) HIP_CHECK(hipMemcpyAsync(rhs[base].data(), B.rhs.data(), sizeof(T) * Rn[base], hipMemcpyDeviceToDevice, s));
   std::vector<Schedule<T>> sched;
   std::vector<GsOperator<T>> ops;
-  for (int d = 0; d <= base; ++d) {
+  for (int d = 0; d <= base; ++d) {  // (the gathered b is replaced before each relaxation)
     sched.push_back(make_schedule<T>(r.levels[d].basis_keys, Rn[d], d, false, s, false));
     ops.push_back(gather_operator<T>(sched[d], mcols[d], mvals[d], rhs[d].data(), Rn[d], s));
   }
+  if (rhs_ready) rhs_ready();
+  if (Rn[base]) HIP_CHECK(hipMemcpyAsync(rhs[base].data(), B.rhs.data(), sizeof(T) * Rn[base], hipMemcpyDeviceToDevice, s));
   const int dc = (base == 0 && r.prm.point_weight == 0.0) ? 1 : 0;
   DArray<T> pad_x(64, s), work(256, s), residual;
   pad_x.zero();
@@ -3612,7 +3619,7 @@ template <class T, class V> class FinerAssembly {
 // The base solve, then each finer level once its operators and load are ready.
 template <class T, class V>
 static void solve_cascade(Result<T>& r, std::vector<DArray<int>>& restr, bool screened, FinerAssembly<T, V>& finer,
-                          hipStream_t s) {
+                          hipEvent_t loads_ready, hipStream_t s) {
   const int base = r.prm.base_depth;
   DArray<T> cumulative;
   for (int d = base; d <= r.prm.solve_depth; ++d) {
@@ -3628,11 +3635,15 @@ static void solve_cascade(Result<T>& r, std::vector<DArray<int>>& restr, bool sc
         PointSet<T> pts = point_constraints(r, d, s);
         add_screening(A, L.basis_keys, pts, d, d, L.point_load.data(), s);
       }
-      hipLaunchKernelGGL(k_rhs<T>, dim3(blocks_for(R, 256)), dim3(256), 0, s, (const double*)L.normal_load.data(),
-                         (const double*)L.point_load.data(), R, L.rhs.data());
-      KERNEL_CHECK();
       L.A = cast_stencil<T>(std::move(A), s);
-      regular_multigrid(r, restr, s);
+      // the base operators are set up while the normal loads finish on
+      // their own stream; the right-hand side waits for them
+      regular_multigrid(r, restr, s, [&] {
+        HIP_CHECK(hipStreamWaitEvent(s, loads_ready, 0));
+        hipLaunchKernelGGL(k_rhs<T>, dim3(blocks_for(R, 256)), dim3(256), 0, s, (const double*)L.normal_load.data(),
+                           (const double*)L.point_load.data(), R, L.rhs.data());
+        KERNEL_CHECK();
+      });
       cumulative.reset(R, s);
       HIP_CHECK(hipMemcpyAsync(cumulative.data(), L.correction.data(), sizeof(T) * R, hipMemcpyDeviceToDevice, s));
       finer.open_gate();
@@ -3651,13 +3662,27 @@ static void solve_levels(Result<T>& r, std::vector<DArray<int>>& restr, bool scr
   const int base = r.prm.base_depth;
   NormalLoadPlan<T> plan;  // outlives the assembly, whose stream reads its arrays
   group_declare(base + 1);
+  // The normal loads the base level needs run on a loads stream beside the
+  // base operators' setup on s; loads_ready orders their readers after them.
+  int dev = 0;
+  HIP_CHECK(hipGetDevice(&dev));
+  const hipStream_t h = companion_stream(dev, s, 2);
+  hipEvent_t start = nullptr, loads_ready = nullptr;
+  HIP_CHECK(hipEventCreateWithFlags(&start, hipEventDisableTiming));
+  HIP_CHECK(hipEventCreateWithFlags(&loads_ready, hipEventDisableTiming));
+  HIP_CHECK(hipEventRecord(start, s));
+  HIP_CHECK(hipStreamWaitEvent(h, start, 0));
   {
     FinerAssembly<T, V> finer(r, base + 1, r.prm.solve_depth, screened, s);
-    normal_loads_base(r, restr, plan, s);
+    normal_loads_base(r, restr, plan, h);
+    HIP_CHECK(hipEventRecord(loads_ready, h));
     finer.set_loads(&plan);
-    solve_cascade<T, V>(r, restr, screened, finer, s);
+    solve_cascade<T, V>(r, restr, screened, finer, loads_ready, s);
   }  // the solve stream is ordered after the assembly stream here
   group_finish();
+  HIP_CHECK(hipStreamWaitEvent(s, loads_ready, 0));  // also when the cascade stopped before the base level
+  (void)hipEventDestroy(start);
+  (void)hipEventDestroy(loads_ready);
   normal_loads_rest(r, plan, s);
 }
 
