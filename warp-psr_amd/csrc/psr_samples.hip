@@ -526,34 +526,53 @@ template <class T> void stage_density(Result<T>& r) {
   }
   r.density_scale = density_scale_host<T>();
   r.densities.reset((size_t)G * (kd + 1), s);
-  for (int d = 0; d <= kd; ++d) {  // density.py:245-271
+  // density.py:245-271.  The levels are independent: every level's records
+  // are sorted and their heads counted first, the counts come to the host in
+  // one read, then each level is reduced and evaluated.
+  struct Splat {
+    int64_t m = 0, total_records = 0;
+    DArray<int> order, rids, heads, prefix;
+    DArray<uint64_t> rkeys;
+  };
+  std::vector<Splat> lv(kd + 1);
+  DArray<int> totals(kd + 1, s);
+  for (int d = 0; d <= kd; ++d) {
     LevelState<T>& L = r.levels[d];
-    const int64_t m = (int64_t)L.agg_keys.size();
-    DArray<uint64_t> codes(m, s);
-    DArray<int> order(m, s);
-    hipLaunchKernelGGL(k_morton, dim3(blocks_for(m, 256)), dim3(256), 0, s, (const int64_t*)L.agg_keys.data(), m, d,
-                       codes.data(), order.data());
+    Splat& v = lv[d];
+    v.m = (int64_t)L.agg_keys.size();
+    DArray<uint64_t> codes(v.m, s);
+    v.order.reset(v.m, s);
+    hipLaunchKernelGGL(k_morton, dim3(blocks_for(v.m, 256)), dim3(256), 0, s, (const int64_t*)L.agg_keys.data(), v.m, d,
+                       codes.data(), v.order.data());
     KERNEL_CHECK();
-    sort_pairs_u64(codes.data(), order.data(), m, 3 * d, s);
-    const int64_t total_records = 27 * m;
-    DArray<uint64_t> rkeys(total_records, s);
-    DArray<int> rids(total_records, s), heads(total_records, s), prefix(total_records, s), total(1, s);
-    hipLaunchKernelGGL(k_density_splat_keys, dim3(blocks_for(total_records, 256)), dim3(256), 0, s,
-                       (const int64_t*)L.agg_keys.data(), (const int*)order.data(), m, d, rkeys.data(), rids.data());
+    sort_pairs_u64(codes.data(), v.order.data(), v.m, 3 * d, s);
+    v.total_records = 27 * v.m;
+    v.rkeys.reset(v.total_records, s);
+    v.rids.reset(v.total_records, s);
+    v.heads.reset(v.total_records, s);
+    v.prefix.reset(v.total_records, s);
+    hipLaunchKernelGGL(k_density_splat_keys, dim3(blocks_for(v.total_records, 256)), dim3(256), 0, s,
+                       (const int64_t*)L.agg_keys.data(), (const int*)v.order.data(), v.m, d, v.rkeys.data(), v.rids.data());
     KERNEL_CHECK();
     const uint64_t sentinel = (uint64_t)1 << (3 * d);
-    sort_pairs_u64(rkeys.data(), rids.data(), total_records, 3 * d + 1, s);
-    mark_heads_u64(rkeys.data(), total_records, sentinel, heads.data(), s);
-    scan_exclusive_i32(heads.data(), prefix.data(), total_records, total.data(), s);
-    const int count = read_int(total.data(), s);
+    sort_pairs_u64(v.rkeys.data(), v.rids.data(), v.total_records, 3 * d + 1, s);
+    mark_heads_u64(v.rkeys.data(), v.total_records, sentinel, v.heads.data(), s);
+    scan_exclusive_i32(v.heads.data(), v.prefix.data(), v.total_records, totals.data() + d, s);
+  }
+  std::vector<int> counts(kd + 1);
+  read_bytes(counts.data(), totals.data(), sizeof(int) * (kd + 1), s);
+  for (int d = 0; d <= kd; ++d) {
+    LevelState<T>& L = r.levels[d];
+    Splat& v = lv[d];
+    const int count = counts[d];
     DArray<int> head_pos(count, s);
-    compact_heads(heads.data(), prefix.data(), total_records, head_pos.data(), s);
+    compact_heads(v.heads.data(), v.prefix.data(), v.total_records, head_pos.data(), s);
     L.dens_keys.reset(count, s);
     L.dens_vals.reset(count, s);
     hipLaunchKernelGGL(k_density_splat_reduce<T>, dim3(blocks_for(count, 256)), dim3(256), 0, s,
-                       (const uint64_t*)rkeys.data(), (const int*)rids.data(), (const int*)head_pos.data(), count,
-                       total_records, d, (const int64_t*)L.agg_keys.data(), (const T*)L.agg_vals.data(),
-                       (const int*)order.data(), r.density_scale, L.dens_keys.data(), L.dens_vals.data());
+                       (const uint64_t*)v.rkeys.data(), (const int*)v.rids.data(), (const int*)head_pos.data(), count,
+                       v.total_records, d, (const int64_t*)L.agg_keys.data(), (const T*)L.agg_vals.data(),
+                       (const int*)v.order.data(), r.density_scale, L.dens_keys.data(), L.dens_vals.data());
     KERNEL_CHECK();
     ColumnIndex columns;
     build_column_index((const int64_t*)L.dens_keys.data(), count, (1 << d) + 1, columns, s);
