@@ -683,51 +683,37 @@ __global__ void __launch_bounds__(128) k_normal_load_dense(const int64_t* __rest
 #pragma unroll
   for (int k = 0; k < W; ++k) iz[k] = table[c2 * W + k];
   const int64_t dim = dn.dim;
-  // YC rows of y at a time: their dense-index entries, then their records,
-  // go out as one load step each; the sum keeps the x, y, z order
-  constexpr int YC = W <= 4 ? W : 2;
-  static_assert(W % YC == 0, "y chunks divide the stencil width");
   double value = 0.0;
   for (int x = 0; x < W; ++x) {
     const int nx = e0 + x + first;
     if (nx < 0 || nx >= lim) continue;
     const D2 ix = table[c0 * W + x];
-    for (int y0 = 0; y0 < W; y0 += YC) {
-      int jn[YC][W];
+    for (int y = 0; y < W; ++y) {
+      const int ny = e1 + y + first;
+      if (ny < 0 || ny >= lim) continue;
+      const D2 iy = table[c1 * W + y];
+      int jn[W];
 #pragma unroll
-      for (int y = 0; y < YC; ++y) {
-        const int ny = e1 + y0 + y + first;
-        const bool yin = ny >= 0 && ny < lim;
-#pragma unroll
-        for (int z = 0; z < W; ++z) {
-          const int nz = e2 + z + first;
-          const bool in = yin && nz >= zlo && nz <= zhi;
-          int64_t cell = dn.zmajor ? ((int64_t)nz * dim + nx) * dim + ny : ((int64_t)nx * dim + ny) * dim + nz;
-          cell = in ? cell : 0;
-          jn[y][z] = in ? dn.idx[cell] : -1;
-        }
+      for (int z = 0; z < W; ++z) {
+        const int nz = e2 + z + first;
+        const bool in = nz >= zlo && nz <= zhi;
+        int64_t cell = dn.zmajor ? ((int64_t)nz * dim + nx) * dim + ny : ((int64_t)nx * dim + ny) * dim + nz;
+        cell = in ? cell : 0;
+        jn[z] = in ? dn.idx[cell] : -1;
       }
-      V v[YC][W][3];
+      V v[W][3];
 #pragma unroll
-      for (int y = 0; y < YC; ++y)
+      for (int z = 0; z < W; ++z) {
+        const int64_t j = jn[z] >= 0 ? jn[z] : 0;
 #pragma unroll
-        for (int z = 0; z < W; ++z) {
-          const int64_t j = jn[y][z] >= 0 ? jn[y][z] : 0;
+        for (int c = 0; c < 3; ++c) v[z][c] = jn[z] >= 0 ? nvals[(int64_t)nstride * j + c] : V(0);
+      }
 #pragma unroll
-          for (int c = 0; c < 3; ++c) v[y][z][c] = jn[y][z] >= 0 ? nvals[(int64_t)nstride * j + c] : V(0);
-        }
-#pragma unroll
-      for (int y = 0; y < YC; ++y) {
-        const int ny = e1 + y0 + y + first;
-        if (ny < 0 || ny >= lim) continue;
-        const D2 iy = table[c1 * W + y0 + y];
-#pragma unroll
-        for (int z = 0; z < W; ++z) {
-          if (jn[y][z] < 0) continue;
-          value += double(v[y][z][0]) * ix.v1 * iy.v0 * iz[z].v0;
-          value += double(v[y][z][1]) * ix.v0 * iy.v1 * iz[z].v0;
-          value += double(v[y][z][2]) * ix.v0 * iy.v0 * iz[z].v1;
-        }
+      for (int z = 0; z < W; ++z) {
+        if (jn[z] < 0) continue;
+        value += double(v[z][0]) * ix.v1 * iy.v0 * iz[z].v0;
+        value += double(v[z][1]) * ix.v0 * iy.v1 * iz[z].v0;
+        value += double(v[z][2]) * ix.v0 * iy.v0 * iz[z].v1;
       }
     }
   }
