@@ -2294,17 +2294,30 @@ static std::unique_ptr<TiledGraph> build_tiled_graph(int n, int iters, int width
   return build_wave_graph(n, iters, func, width, bx, lds, args);
 }
 
-// relax_sliced on the tile tables: x (schedule order) holds the start values
-// and receives the result; x1 is a second buffer of R values.  The operator's
-// absent slots must hold +0 (k_level_operator stores them so).
+// A cached tiled graph with this call's node arguments written, held until
+// tiled_launch.
+struct TiledLaunch {
+  TiledGraph* g = nullptr;
+  std::unique_lock<std::mutex> lk;
+  int launches = 0;
+};
+
+// relax_sliced on the tile tables, in two steps so that a level's node
+// arguments can be written on the host while the GPU still relaxes the level
+// before: tiled_prepare writes them (x in schedule order holds the start
+// values and receives the result; x1 is a second buffer of R values; b is the
+// right-hand side by schedule position), tiled_launch copies the start values
+// where the first visit reads them and launches.  The operator's absent
+// slots must hold +0 (k_level_operator stores them so).
 template <class T, class V>
-int relax_tiled_graph(const GsOperator<T, V>& op, const TileTable& tt, const RowRecord<T, V>* records, T* x, T* x1,
-                      int depth, int iters, hipStream_t s) {
+static TiledLaunch tiled_prepare(const TileTable& tt, const RowRecord<T, V>* records, const T* b, T* x, T* x1, int R, int depth,
+                                 int iters, hipStream_t s) {
   static std::mutex mu;
   static std::map<std::tuple<int, int, int, int, unsigned, int, hipStream_t>, std::unique_ptr<TiledGraph>> cache;
+  TiledLaunch t;
   const int n = 1 << depth;
   if (iters > RELAX_MAX_SLICES) throw PsrError(1, "tiled relaxation supports at most 16 sweeps");
-  if (op.R <= 0 || tt.tiles <= 0) return 0;
+  if (R <= 0 || tt.tiles <= 0) return t;
   const int width = tiled_width(tt);
   unsigned bx = 1;
   while (bx < (unsigned)std::max(1, tt.max_tiles)) bx <<= 1;
@@ -2319,31 +2332,47 @@ int relax_tiled_graph(const GsOperator<T, V>& op, const TileTable& tt, const Row
     if (!slot) slot = build_tiled_graph<T, V>(n, iters, width, bx, lds);
     g = slot.get();
   }
-  // the first visit reads buffer iters % 2
-  if (iters % 2) HIP_CHECK(hipMemcpyAsync(x1, x, sizeof(T) * op.R, hipMemcpyDeviceToDevice, s));
-  std::lock_guard<std::mutex> guard(g->mu);
+  t.lk = std::unique_lock<std::mutex>(g->mu);
   HIP_CHECK(hipEventSynchronize(g->done));
-  TiledArgs<T, V> a{tt.entries.data(), tt.desc.data(), records, op.b.data(), x, x1, gside};
+  TiledArgs<T, V> a{tt.entries.data(), tt.desc.data(), records, b, x, x1, gside};
   WaveTiles wv;
   void* args[2] = {&a, &wv};
   hipKernelNodeParams kp = g->params;
   kp.kernelParams = args;
   kp.sharedMemBytes = (unsigned)lds;
   for (size_t i = 0; i < g->nodes.size(); ++i) {
-    for (int t = 0; t < RELAX_MAX_SLICES; ++t) wv.start[t] = wv.end[t] = 0;
+    for (int q = 0; q < RELAX_MAX_SLICES; ++q) wv.start[q] = wv.end[q] = 0;
     wv.flip = 0;
     for (int k = 0; k < g->slices[i]; ++k) {
-      const int z = g->first_z[i] + 2 * k, t = g->first_t[i] + k;
+      const int z = g->first_z[i] + 2 * k, q = g->first_t[i] + k;
       wv.start[k] = tt.slice_start[z];
       wv.end[k] = tt.slice_start[z + 1];
-      if ((iters - t) % 2) wv.flip |= 1u << k;
+      if ((iters - q) % 2) wv.flip |= 1u << k;
     }
     kp.gridDim = dim3(bx, (unsigned)g->slices[i]);
     HIP_CHECK(hipGraphExecKernelNodeSetParams(g->exec, g->nodes[i], &kp));
   }
-  HIP_CHECK(hipGraphLaunch(g->exec, s));
-  HIP_CHECK(hipEventRecord(g->done, s));
-  return (int)g->nodes.size();
+  t.g = g;
+  t.launches = (int)g->nodes.size();
+  return t;
+}
+
+template <class T> static int tiled_launch(TiledLaunch& t, const T* x, T* x1, int R, int iters, hipStream_t s) {
+  if (!t.g) return 0;
+  // the first visit reads buffer iters % 2
+  if (iters % 2) HIP_CHECK(hipMemcpyAsync(x1, x, sizeof(T) * R, hipMemcpyDeviceToDevice, s));
+  HIP_CHECK(hipGraphLaunch(t.g->exec, s));
+  HIP_CHECK(hipEventRecord(t.g->done, s));
+  t.lk.unlock();
+  t.g = nullptr;
+  return t.launches;
+}
+
+template <class T, class V>
+int relax_tiled_graph(const GsOperator<T, V>& op, const TileTable& tt, const RowRecord<T, V>* records, T* x, T* x1,
+                      int depth, int iters, hipStream_t s) {
+  TiledLaunch t = tiled_prepare<T, V>(tt, records, op.b.data(), x, x1, op.R, depth, iters, s);
+  return tiled_launch<T>(t, x, x1, op.R, iters, s);
 }
 
 // ---------------------------------------------------------------- grouped relaxation
@@ -3379,14 +3408,39 @@ static void prepare_finer(Result<T>& r, int d, Stencil<V>& Aop, Stencil<V>& Cop,
 // One finer level's solve (solver.py:343-360): right-hand side, b -= cross *
 // cumulative, sliced Gauss-Seidel from zero, then the cumulative solution is
 // prolonged and accumulated.
+// A finer level's right-hand side and x buffers, and (tiled form, single
+// reconstructions) its graph's node arguments: prepared on the host while
+// the GPU still works on the level before, so the level's relaxation
+// launches as soon as its right-hand side is enqueued.
+template <class T> struct PreparedLevel {
+  bool ready = false;
+  DArray<T> rhs, xs, xs1;
+  TiledLaunch launch;
+};
+
+template <class T, class V>
+static PreparedLevel<T> finer_prepare(Result<T>& r, int d, FinerLevel<T, V>& f, hipStream_t s) {
+  PreparedLevel<T> p;
+  const int R = r.levels[d].basis_count;
+  const bool tiled = f.tiles.tiles > 0;
+  p.rhs.reset(R, s);
+  p.xs.reset(R, s);
+  if (tiled) p.xs1.reset(R, s);
+  if (tiled && !t_group)
+    p.launch = tiled_prepare<T, V>(f.tiles, (const RowRecord<T, V>*)f.records.data(), p.rhs.data(), p.xs.data(),
+                                   p.xs1.data(), R, d, r.prm.iters, s);
+  p.ready = true;
+  return p;
+}
+
 template <class T, class V>
 static void solve_finer(Result<T>& r, int d, Stencil<V>& Aop, Stencil<V>& Cop, FinerLevel<T, V>& f, DArray<T>& cumulative,
-                        hipStream_t s) {
+                        PreparedLevel<T>& prep, hipStream_t s) {
   LevelState<T>& L = r.levels[d];
   const int R = L.basis_count;
   const int* ord = f.sc.order.data();
   GsOperator<T, V>& op = f.op;
-  L.rhs.reset(R, s);
+  L.rhs = std::move(prep.rhs);
   hipLaunchKernelGGL(k_rhs_sched<T>, dim3(blocks_for(R, 256)), dim3(256), 0, s, (const double*)L.normal_load.data(),
                      (const double*)L.point_load.data(), ord, R, L.rhs.data());
   KERNEL_CHECK();
@@ -3401,14 +3455,13 @@ static void solve_finer(Result<T>& r, int d, Stencil<V>& Aop, Stencil<V>& Cop, F
     KERNEL_CHECK();
   }
   op.b = std::move(L.rhs);
-  DArray<T> xs(R, s);
+  DArray<T> xs = std::move(prep.xs);
   xs.zero();  // the level's correction starts at zero (solver.py:343)
   r.relax_nnz.push_back(std::move(f.nnz));
   typename Result<T>::RelaxStat st{d, R, 0, 0, nullptr, nullptr, 0.f};
   const bool tiled = f.tiles.tiles > 0;
-  DArray<T> xs1;
+  DArray<T> xs1 = std::move(prep.xs1);
   if (tiled) {
-    xs1.reset(R, s);
     st.tile = f.tiles.ts;
     st.width = tiled_width(f.tiles);
     st.tiles = f.tiles.tiles;
@@ -3441,6 +3494,8 @@ static void solve_finer(Result<T>& r, int d, Stencil<V>& Aop, Stencil<V>& Cop, F
   }
   if (grouped)
     st.launches = grouped;
+  else if (tiled && prep.launch.g)
+    st.launches = tiled_launch<T>(prep.launch, xs.data(), xs1.data(), R, r.prm.iters, s);
   else if (tiled)
     st.launches = relax_tiled_graph(op, f.tiles, (const RowRecord<T, V>*)f.records.data(), xs.data(), xs1.data(), d,
                                     r.prm.iters, s);
@@ -3622,6 +3677,7 @@ static void solve_cascade(Result<T>& r, std::vector<DArray<int>>& restr, bool sc
                           hipEvent_t loads_ready, hipStream_t s) {
   const int base = r.prm.base_depth;
   DArray<T> cumulative;
+  PreparedLevel<T> next;  // the coming finer level's buffers and graph
   for (int d = base; d <= r.prm.solve_depth; ++d) {
     LevelState<T>& L = r.levels[d];
     const int R = L.basis_count;
@@ -3648,7 +3704,13 @@ static void solve_cascade(Result<T>& r, std::vector<DArray<int>>& restr, bool sc
       HIP_CHECK(hipMemcpyAsync(cumulative.data(), L.correction.data(), sizeof(T) * R, hipMemcpyDeviceToDevice, s));
       finer.open_gate();
     } else {
-      solve_finer<T, V>(r, d, finer_operator<V>(L, false), finer_operator<V>(L, true), finer.wait(d), cumulative, s);
+      FinerLevel<T, V>& f = finer.wait(d);
+      PreparedLevel<T> cur = next.ready ? std::move(next) : finer_prepare<T, V>(r, d, f, s);
+      next = PreparedLevel<T>();
+      solve_finer<T, V>(r, d, finer_operator<V>(L, false), finer_operator<V>(L, true), f, cumulative, cur, s);
+      // the next level's node arguments are written while this one relaxes
+      if (d + 1 <= r.prm.solve_depth && r.levels[d + 1].basis_count)
+        next = finer_prepare<T, V>(r, d + 1, finer.wait(d + 1), s);
     }
     L.solved = true;
   }
