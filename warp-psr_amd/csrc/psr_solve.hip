@@ -2108,15 +2108,18 @@ struct TileTable {
 };
 
 // Tile side for a level: the run's relax_tile (8..40, which keeps the three
-// double grids within 64 KiB of LDS) or, for 0, the library default; -1 (or
+// double grids within 64 KiB of LDS) or, for 0, the library default by depth
+// (12 up to depth 6, 16 for depths 7-8, 18 beyond); -1 (or
 // the PSR_RELAX environment choice of a colour-per-launch form) selects no
 // tiles.
 constexpr int RELAX_TILE_MAX = 40;
 static int relax_tile_size(int requested, int depth) {
   if (requested < 0 || relax_mode() != 0) return 0;
   if (requested >= 8 && requested <= RELAX_TILE_MAX && requested % 2 == 0) return requested;
-  (void)depth;
-  return 16;
+  // measured on the 800k-point depth-9 workload: the small levels gain from
+  // smaller tiles (more workgroups per launch despite more recomputed ring
+  // rows), the finest level from 18
+  return depth <= 6 ? 12 : depth <= 8 ? 16 : 18;
 }
 
 // With the level's dense z-major index (level_dense, side dim) the entries
