@@ -483,6 +483,15 @@ int64_t dense_index_max_cells() { return env_limit("PSR_DENSE_INDEX_MAX_CELLS", 
 int column_table_max_dim() { return (int)env_limit("PSR_COLUMN_TABLE_MAX_DIM", 4097, 0, 4097); }
 int field_table_max_depth() { return (int)env_limit("PSR_FIELD_TABLE_MAX_DEPTH", 11, -1, 11); }
 
+// both arrays of a column index cleared in one launch
+__global__ void k_zero_pair(int* __restrict__ a, int* __restrict__ b, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    a[i] = 0;
+    b[i] = 0;
+  }
+}
+
 void build_column_index(const int64_t* keys, int64_t n, int dim, ColumnIndex& out, hipStream_t s) {
   out.dim = 0;
   if (dim <= 0 || dim > column_table_max_dim() || n <= 0) {
@@ -493,8 +502,9 @@ void build_column_index(const int64_t* keys, int64_t n, int dim, ColumnIndex& ou
   const size_t cells = (size_t)dim * dim;
   out.start.reset(cells, s);
   out.end.reset(cells, s);
-  out.start.zero();
-  out.end.zero();
+  hipLaunchKernelGGL(k_zero_pair, dim3(blocks_for((int64_t)cells, 256)), dim3(256), 0, s, out.start.data(), out.end.data(),
+                     (int64_t)cells);
+  KERNEL_CHECK();
   hipLaunchKernelGGL(k_column_index, dim3(blocks_for(n, 256)), dim3(256), 0, s, keys, n, dim, out.start.data(),
                      out.end.data());
   KERNEL_CHECK();

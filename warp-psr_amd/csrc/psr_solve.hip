@@ -1331,24 +1331,25 @@ __global__ void k_root_extract(const int64_t* __restrict__ keys, int R, const T*
   x[i] = pad_x[cz * 16 + (cx & 1) + 2 * (cy & 1) + 2 * (cx & 2) + 4 * (cy & 2)];
 }
 
-// b[j] = (sum[order[j]] * -1) + 1 * b[j]: k_spmv's alpha = -1, beta = 1 step
-// for a sum computed over canonical rows
+// A finer level's right-hand side at schedule position j (solver.py:119-126
+// and :343-350): b_j = load_i + 0.5 * point_load_j rounded to T (i the row
+// at j), then -1 * (cross * cumulative)_i + 1 * b_j as k_spmv combines with
+// alpha = -1, beta = 1; and x_j = 0, the level's start value.
 template <class T>
-__global__ void k_subtract_coupling(const int* __restrict__ order, int R, const T* __restrict__ sum, T* __restrict__ b) {
+__global__ void k_level_rhs(const double* __restrict__ load, const double* __restrict__ point_load_sched,
+                            const int* __restrict__ order, int R, const T* __restrict__ sum, T* __restrict__ b,
+                            T* __restrict__ x) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= R) return;
-  T v = sum[order[j]] * T(-1);
-  v = v + T(1) * b[j];
+  const int i = order[j];
+  const T bj = T(load[i]) + T(0.5) * T(point_load_sched[j]);
+  T v = sum[i] * T(-1);
+  v = v + T(1) * bj;
   b[j] = v;
+  x[j] = T(0);
 }
 
-// solver.py:119-126 (_rhs) written in schedule order
-template <class T>
-__global__ void k_rhs_sched(const double* __restrict__ load, const double* __restrict__ point_load_sched,
-                            const int* __restrict__ order, int R, T* __restrict__ b) {
-  int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j < R) b[j] = T(load[order[j]]) + T(0.5) * T(point_load_sched[j]);
-}
+
 
 __global__ void k_inverse_permutation(const int* __restrict__ order, int R, int* __restrict__ inv) {
   int j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -3444,22 +3445,19 @@ static void solve_finer(Result<T>& r, int d, Stencil<V>& Aop, Stencil<V>& Cop, F
   const int* ord = f.sc.order.data();
   GsOperator<T, V>& op = f.op;
   L.rhs = std::move(prep.rhs);
-  hipLaunchKernelGGL(k_rhs_sched<T>, dim3(blocks_for(R, 256)), dim3(256), 0, s, (const double*)L.normal_load.data(),
-                     (const double*)L.point_load.data(), ord, R, L.rhs.data());
-  KERNEL_CHECK();
-  {  // b -= cross * cumulative(d-1): the canonical-row sum, then k_spmv's
-     // alpha = -1, beta = 1 combination at each schedule position
+  DArray<T> xs = std::move(prep.xs);
+  {  // b = rhs - cross * cumulative(d-1): the canonical-row sum, then k_spmv's
+     // alpha = -1, beta = 1 combination at each schedule position; the
+     // level's correction starts at zero (solver.py:343)
     DArray<T> coupling(R, s);
     hipLaunchKernelGGL((k_spmv<T, V>), dim3(blocks_for(R, 256)), dim3(256), 0, s, (const int*)Cop.cols.data(),
                        (const V*)Cop.vals.data(), R, 27, (const T*)cumulative.data(), coupling.data(), T(1), T(0));
     KERNEL_CHECK();
-    hipLaunchKernelGGL(k_subtract_coupling<T>, dim3(blocks_for(R, 256)), dim3(256), 0, s, ord, R, (const T*)coupling.data(),
-                       L.rhs.data());
+    hipLaunchKernelGGL(k_level_rhs<T>, dim3(blocks_for(R, 256)), dim3(256), 0, s, (const double*)L.normal_load.data(),
+                       (const double*)L.point_load.data(), ord, R, (const T*)coupling.data(), L.rhs.data(), xs.data());
     KERNEL_CHECK();
   }
   op.b = std::move(L.rhs);
-  DArray<T> xs = std::move(prep.xs);
-  xs.zero();  // the level's correction starts at zero (solver.py:343)
   r.relax_nnz.push_back(std::move(f.nnz));
   typename Result<T>::RelaxStat st{d, R, 0, 0, nullptr, nullptr, 0.f};
   const bool tiled = f.tiles.tiles > 0;
