@@ -585,7 +585,7 @@ __device__ __forceinline__ T stencil_row_sum(const int* __restrict__ cols, const
 
 template <class T, class V = T>
 __global__ void k_spmv(const int* __restrict__ cols, const V* __restrict__ vals, int R, int slots, const T* __restrict__ x,
-                       T* __restrict__ y, T alpha, T beta) {
+                       T* __restrict__ y, T alpha, T beta, const T* __restrict__ y_in) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= R) return;
   T v = T(0);
@@ -600,7 +600,7 @@ __global__ void k_spmv(const int* __restrict__ cols, const V* __restrict__ vals,
     }
   }
   v = v * alpha;
-  if (beta != T(0)) v = v + beta * y[i];
+  if (beta != T(0)) v = v + beta * (y_in ? y_in[i] : y[i]);  // y_in: the beta term's input when not y
   y[i] = v;
 }
 
@@ -3318,9 +3318,9 @@ static void regular_multigrid(Result<T>& r, std::vector<DArray<int>>& restr, hip
       refresh_rhs(ops[d], sched[d], rhs[d].data(), s);
       relax_colored(ops[d], sched[d], x[d].data(), iters, true, s);
       residual.reset(Rn[d], s);
-      HIP_CHECK(hipMemcpyAsync(residual.data(), rhs[d].data(), sizeof(T) * Rn[d], hipMemcpyDeviceToDevice, s));
       hipLaunchKernelGGL(k_spmv<T>, dim3(blocks_for(Rn[d], 256)), dim3(256), 0, s, (const int*)mcols[d].data(),
-                         (const T*)mvals[d].data(), Rn[d], 27, (const T*)x[d].data(), residual.data(), T(-1), T(1));
+                         (const T*)mvals[d].data(), Rn[d], 27, (const T*)x[d].data(), residual.data(), T(-1), T(1),
+                         (const T*)rhs[d].data());
       KERNEL_CHECK();
       if (Rn[d - 1]) {
         hipLaunchKernelGGL(k_restrict_apply<T>, dim3(blocks_for(Rn[d - 1], 256)), dim3(256), 0, s,
@@ -3343,7 +3343,8 @@ static void regular_multigrid(Result<T>& r, std::vector<DArray<int>>& restr, hip
     if (read_int(ran.data(), s)) r.cg_runs.push_back(cycle);
     for (int d = 1; d <= base; ++d) {
       hipLaunchKernelGGL(k_spmv<T>, dim3(blocks_for(Rn[d], 256)), dim3(256), 0, s, (const int*)r.levels[d].P.cols.data(),
-                         (const T*)r.levels[d].P.vals.data(), Rn[d], 8, (const T*)x[d - 1].data(), x[d].data(), T(1), T(1));
+                         (const T*)r.levels[d].P.vals.data(), Rn[d], 8, (const T*)x[d - 1].data(), x[d].data(), T(1), T(1),
+                         (const T*)nullptr);
       KERNEL_CHECK();
       x[d - 1].zero();
       relax_colored(ops[d], sched[d], x[d].data(), iters, false, s);
@@ -3451,7 +3452,8 @@ static void solve_finer(Result<T>& r, int d, Stencil<V>& Aop, Stencil<V>& Cop, F
      // level's correction starts at zero (solver.py:343)
     DArray<T> coupling(R, s);
     hipLaunchKernelGGL((k_spmv<T, V>), dim3(blocks_for(R, 256)), dim3(256), 0, s, (const int*)Cop.cols.data(),
-                       (const V*)Cop.vals.data(), R, 27, (const T*)cumulative.data(), coupling.data(), T(1), T(0));
+                       (const V*)Cop.vals.data(), R, 27, (const T*)cumulative.data(), coupling.data(), T(1), T(0),
+                       (const T*)nullptr);
     KERNEL_CHECK();
     hipLaunchKernelGGL(k_level_rhs<T>, dim3(blocks_for(R, 256)), dim3(256), 0, s, (const double*)L.normal_load.data(),
                        (const double*)L.point_load.data(), ord, R, (const T*)coupling.data(), L.rhs.data(), xs.data());
@@ -3513,9 +3515,9 @@ static void solve_finer(Result<T>& r, int d, Stencil<V>& Aop, Stencil<V>& Cop, F
   L.order = std::move(f.sc.order);
   L.scheduled = true;
   DArray<T> next(R, s);
-  HIP_CHECK(hipMemcpyAsync(next.data(), L.correction.data(), sizeof(T) * R, hipMemcpyDeviceToDevice, s));
   hipLaunchKernelGGL(k_spmv<T>, dim3(blocks_for(R, 256)), dim3(256), 0, s, (const int*)L.P.cols.data(),
-                     (const T*)L.P.vals.data(), R, 8, (const T*)cumulative.data(), next.data(), T(1), T(1));
+                     (const T*)L.P.vals.data(), R, 8, (const T*)cumulative.data(), next.data(), T(1), T(1),
+                     (const T*)L.correction.data());
   KERNEL_CHECK();
   cumulative = std::move(next);
 }
@@ -4576,7 +4578,8 @@ template <class T> void solve_backward(Result<T>& r, const double* gcoef, double
       if (R) HIP_CHECK(hipMemcpyAsync(cum[d].data(), r.levels[d].correction.data(), sizeof(T) * R, hipMemcpyDeviceToDevice, s));
       if (d > base && R) {
         hipLaunchKernelGGL(k_spmv<T>, dim3(blocks_for(R, 256)), dim3(256), 0, s, (const int*)r.levels[d].P.cols.data(),
-                           (const T*)r.levels[d].P.vals.data(), R, 8, (const T*)cum[d - 1].data(), cum[d].data(), T(1), T(1));
+                           (const T*)r.levels[d].P.vals.data(), R, 8, (const T*)cum[d - 1].data(), cum[d].data(), T(1), T(1),
+                           (const T*)nullptr);
         KERNEL_CHECK();
       }
     }
@@ -4698,9 +4701,9 @@ template <class T> void solve_backward(Result<T>& r, const double* gcoef, double
             snap_down[cycle][d].reset((size_t)(iters + 1) * Rn[d], s);
             relax_colored_snap(ops[d], sched[d], x[d].data(), iters, true, snap_down[cycle][d].data(), s);
             residual.reset(Rn[d], s);
-            HIP_CHECK(hipMemcpyAsync(residual.data(), rhs[d].data(), sizeof(T) * Rn[d], hipMemcpyDeviceToDevice, s));
             hipLaunchKernelGGL(k_spmv<T>, dim3(blocks_for(Rn[d], 256)), dim3(256), 0, s, (const int*)mcols[d].data(),
-                               (const T*)mvals[d].data(), Rn[d], 27, (const T*)x[d].data(), residual.data(), T(-1), T(1));
+                               (const T*)mvals[d].data(), Rn[d], 27, (const T*)x[d].data(), residual.data(), T(-1), T(1),
+                               (const T*)rhs[d].data());
             KERNEL_CHECK();
             if (Rn[d - 1]) {
               hipLaunchKernelGGL(k_restrict_apply<T>, dim3(blocks_for(Rn[d - 1], 256)), dim3(256), 0, s,
@@ -4716,7 +4719,8 @@ template <class T> void solve_backward(Result<T>& r, const double* gcoef, double
           KERNEL_CHECK();
           for (int d = 1; d <= base; ++d) {
             hipLaunchKernelGGL(k_spmv<T>, dim3(blocks_for(Rn[d], 256)), dim3(256), 0, s, (const int*)r.levels[d].P.cols.data(),
-                               (const T*)r.levels[d].P.vals.data(), Rn[d], 8, (const T*)x[d - 1].data(), x[d].data(), T(1), T(1));
+                               (const T*)r.levels[d].P.vals.data(), Rn[d], 8, (const T*)x[d - 1].data(), x[d].data(), T(1), T(1),
+                               (const T*)nullptr);
             KERNEL_CHECK();
             x[d - 1].zero();
             snap_up[cycle][d].reset((size_t)(iters + 1) * Rn[d], s);
