@@ -3123,7 +3123,10 @@ template <class T> struct NormalLoadPlan {
   int chain = 0;                            // next level of the chain
   bool has_cumulative = false;
   int next_level = 0;                       // next level whose load is still to be set
-  hipEvent_t ready = nullptr;               // solve stream, after normal_loads_base
+  std::vector<DArray<double>> from_child;   // normal_loads_base_levels' per-level loads
+  DArray<double> same_base;
+  bool levels_done = false;
+  hipEvent_t ready = nullptr;               // loads stream, after normal_loads_base
   NormalLoadPlan() = default;
   NormalLoadPlan(const NormalLoadPlan&) = delete;
   NormalLoadPlan& operator=(const NormalLoadPlan&) = delete;
@@ -3132,24 +3135,22 @@ template <class T> struct NormalLoadPlan {
   }
 };
 
-template <class T>
-static void normal_loads_base(Result<T>& r, std::vector<DArray<int>>& restr, NormalLoadPlan<T>& plan, hipStream_t s) {
+// The per-level part of normal_loads_base: each level's records indexed and
+// the loads of the tests one level coarser (from_child) and of the base level
+// (same level), which need only the tree; s may run it before the
+// restriction gathers exist.
+template <class T> static void normal_loads_base_levels(Result<T>& r, NormalLoadPlan<T>& plan, hipStream_t s) {
   const int D = r.prm.depth, base = r.prm.base_depth;
   plan.base = base;
   plan.depth = D;
-  if (std::is_same<T, float>::value && base == 0) {
-    ordered_normal_loads(r, s);
-    plan.active = false;
-    return;
-  }
+  if (std::is_same<T, float>::value && base == 0) return;  // the ordered loads (normal_loads_base)
   plan.active = true;
   plan.records.resize(D + 1);
   plan.rdense.resize(D + 1);
   plan.rdn.assign(D + 1, DenseIndex{});
   plan.finer.resize(D + 1);
   plan.from_parent.resize(D + 1);
-  std::vector<DArray<double>> from_child(D + 1);
-  DArray<double> same_base;
+  plan.from_child.resize(D + 1);
   for (int d = 0; d <= D; ++d) {
     const int64_t r0 = r.normal_offsets[d], r1 = r.normal_offsets[d + 1];
     const int64_t* nk = r.normal_keys.data() + r0;
@@ -3168,15 +3169,33 @@ static void normal_loads_base(Result<T>& r, std::vector<DArray<int>>& restr, Nor
       plan.rdn[d].dim = (int)rdim;
     }
     if (d == base) {
-      same_base.reset(r.levels[d].basis_count, s);
-      normal_stencil_load(r.levels[d], d, nk, r1 - r0, plan.records[d].view(), plan.rdn[d], nv, d, same_base.data(), s);
+      plan.same_base.reset(r.levels[d].basis_count, s);
+      normal_stencil_load(r.levels[d], d, nk, r1 - r0, plan.records[d].view(), plan.rdn[d], nv, d, plan.same_base.data(), s);
     }
     if (d > base) {
-      from_child[d - 1].reset(r.levels[d - 1].basis_count, s);
+      plan.from_child[d - 1].reset(r.levels[d - 1].basis_count, s);
       normal_stencil_load(r.levels[d - 1], d - 1, nk, r1 - r0, plan.records[d].view(), plan.rdn[d], nv, d,
-                          from_child[d - 1].data(), s);
+                          plan.from_child[d - 1].data(), s);
     }
   }
+  plan.levels_done = true;
+}
+
+// The loads the base level needs: normal_loads_base_levels (unless it ran
+// already), then every finer level's loads restricted down to the base.
+template <class T>
+static void normal_loads_base(Result<T>& r, std::vector<DArray<int>>& restr, NormalLoadPlan<T>& plan, hipStream_t s) {
+  const int D = r.prm.depth, base = r.prm.base_depth;
+  plan.base = base;
+  plan.depth = D;
+  if (std::is_same<T, float>::value && base == 0) {
+    ordered_normal_loads(r, s);
+    plan.active = false;
+    return;
+  }
+  if (!plan.levels_done) normal_loads_base_levels(r, plan, s);
+  std::vector<DArray<double>>& from_child = plan.from_child;
+  DArray<double> same_base = std::move(plan.same_base);
   DArray<double> finer(r.levels[D].basis_count, s);
   finer.zero();
   for (int d = D; d > base; --d) {
@@ -3723,9 +3742,9 @@ static void solve_cascade(Result<T>& r, std::vector<DArray<int>>& restr, bool sc
 // (solver.py:332-360); the finer levels' operators and loads are assembled
 // alongside.
 template <class T, class V>
-static void solve_levels(Result<T>& r, std::vector<DArray<int>>& restr, bool screened, hipStream_t s) {
+static void solve_levels(Result<T>& r, std::vector<DArray<int>>& restr, bool screened, NormalLoadPlan<T>& plan,
+                         hipStream_t s) {
   const int base = r.prm.base_depth;
-  NormalLoadPlan<T> plan;  // outlives the assembly, whose stream reads its arrays
   group_declare(base + 1);
   // The normal loads the base level needs run on a loads stream beside the
   // base operators' setup on s; loads_ready orders their readers after them.
@@ -3754,6 +3773,21 @@ static void solve_levels(Result<T>& r, std::vector<DArray<int>>& restr, bool scr
 template <class T> void stage_solve(Result<T>& r) {
   hipStream_t s = r.s;
   const int D = r.prm.depth;
+  // The base level's per-level normal loads need only the tree: they start
+  // on the loads stream now, beside the index, prolongation and restriction
+  // set-up below.  The plan outlives the assembly, whose stream reads it.
+  NormalLoadPlan<T> plan;
+  {
+    int dev = 0;
+    HIP_CHECK(hipGetDevice(&dev));
+    const hipStream_t h = companion_stream(dev, s, 2);
+    hipEvent_t tree_done = nullptr;
+    HIP_CHECK(hipEventCreateWithFlags(&tree_done, hipEventDisableTiming));
+    HIP_CHECK(hipEventRecord(tree_done, s));
+    HIP_CHECK(hipStreamWaitEvent(h, tree_done, 0));
+    (void)hipEventDestroy(tree_done);
+    normal_loads_base_levels(r, plan, h);
+  }
   for (int d = 0; d <= D; ++d) {
     LevelState<T>& L = r.levels[d];
     build_column_index((const int64_t*)L.basis_keys.data(), L.basis_count, (1 << d) + 1, L.columns, s);
@@ -3797,9 +3831,9 @@ template <class T> void stage_solve(Result<T>& r) {
   }
   const bool screened = r.prm.point_weight > 0 && !r.prm.exact_interpolation;
   if (r.prm.float_operators)
-    solve_levels<T, float>(r, restr, screened, s);
+    solve_levels<T, float>(r, restr, screened, plan, s);
   else
-    solve_levels<T, T>(r, restr, screened, s);
+    solve_levels<T, T>(r, restr, screened, plan, s);
   for (int d = 0; d < D; ++d) r.levels[d].restriction = std::move(restr[d]);
   for (int d = 0; d <= D; ++d) {
     r.levels[d].dense.free();
