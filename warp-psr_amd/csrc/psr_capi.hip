@@ -109,6 +109,14 @@ template <class T> Result<T>* run(const T* pts, const T* nrm, int64_t n, const P
       if (k < r->relax_nnz.size()) read_bytes(&nnz, r->relax_nnz[k].data(), sizeof(nnz), s);
       st.nnz = (int64_t)nnz;
     }
+    // the base V-cycles whose root conjugate gradient ran (flags written on
+    // the device, read here where the run has synchronised anyway)
+    if (r->cg_ran.size()) {
+      std::vector<int> ran(r->cg_ran.size());
+      read_bytes(ran.data(), r->cg_ran.data(), sizeof(int) * ran.size(), s);
+      for (size_t c = 0; c < ran.size(); ++c)
+        if (ran[c]) r->cg_runs.push_back((int)c);
+    }
   } catch (...) {
     for (auto& e : ev) (void)hipEventDestroy(e);
     throw;

@@ -97,6 +97,7 @@ template <class T> struct Result {
   // tree + solve per level
   std::vector<LevelState<T>> levels;
   std::vector<int> cg_runs;
+  DArray<int> cg_ran;  // per base V-cycle: 1 when the root CG ran; read into cg_runs once the run has synchronised
   // root solve inputs per base V-cycle (right-hand side, initial x), kept for
   // the reverse pass
   std::vector<DArray<T>> root_b, root_x;

@@ -3330,7 +3330,7 @@ static void regular_multigrid(Result<T>& r, std::vector<DArray<int>>& restr, hip
   const int dc = (base == 0 && r.prm.point_weight == 0.0) ? 1 : 0;
   DArray<T> pad_x(64, s), work(256, s), residual;
   pad_x.zero();
-  DArray<int> ran(1, s);
+  r.cg_ran.reset((size_t)std::max(0, r.prm.base_v_cycles), s);
   const int iters = r.prm.iters;
   for (int cycle = 0; cycle < r.prm.base_v_cycles; ++cycle) {
     for (int d = base; d > 0; --d) {
@@ -3357,9 +3357,8 @@ static void regular_multigrid(Result<T>& r, std::vector<DArray<int>>& restr, hip
     }
     hipLaunchKernelGGL(k_root_solve<T>, dim3(1), dim3(64), 0, s, (const int*)mcols[0].data(), (const T*)mvals[0].data(), Rn[0],
                        (const int64_t*)r.levels[0].basis_keys.data(), (const T*)rhs[0].data(), x[0].data(), pad_x.data(),
-                       work.data(), dc, base, cycle, r.prm.cg_accuracy, Rn[0], ran.data());
+                       work.data(), dc, base, cycle, r.prm.cg_accuracy, Rn[0], r.cg_ran.data() + cycle);
     KERNEL_CHECK();
-    if (read_int(ran.data(), s)) r.cg_runs.push_back(cycle);
     for (int d = 1; d <= base; ++d) {
       hipLaunchKernelGGL(k_spmv<T>, dim3(blocks_for(Rn[d], 256)), dim3(256), 0, s, (const int*)r.levels[d].P.cols.data(),
                          (const T*)r.levels[d].P.vals.data(), Rn[d], 8, (const T*)x[d - 1].data(), x[d].data(), T(1), T(1),
