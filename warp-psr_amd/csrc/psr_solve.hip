@@ -403,7 +403,7 @@ template <class T, class V> union RowRecordWords {
 // also written as schedule positions (inv: the schedule's inverse), and with
 // records (the tiled relaxation's input) each row's record.
 template <class T, class V, bool SCREEN>
-__global__ void __launch_bounds__(128) k_level_operator(const int64_t* __restrict__ rows, int R, const int* __restrict__ row_order,
+__global__ void __launch_bounds__(256) k_level_operator(const int64_t* __restrict__ rows, int R, const int* __restrict__ row_order,
                                                         const int64_t* __restrict__ cols, int64_t C, ColumnTable ct,
                                                         const int* __restrict__ dense, int dense_dim, int depth, int trial_depth,
                                                         const D2* __restrict__ table, PointBins<T> pb, double* __restrict__ load,
@@ -2828,13 +2828,13 @@ static void level_operator(const LevelState<T>& rowsL, const LevelState<T>& cols
   T* inv_diag = op ? op->inv_diag.data() : nullptr;
   unsigned char* has_diag = op ? op->has_diag.data() : nullptr;
   if (pts)
-    hipLaunchKernelGGL((k_level_operator<T, V, true>), dim3(blocks_for(R, 128)), dim3(128), 0, s,
+    hipLaunchKernelGGL((k_level_operator<T, V, true>), dim3(blocks_for(R, 256)), dim3(256), 0, s,
                        (const int64_t*)rowsL.basis_keys.data(), R, ord, (const int64_t*)colsL.basis_keys.data(),
                        (int64_t)colsL.basis_count, colsL.columns.view(), dense, colsL.dense_dim, depth, trial_depth,
                        (const D2*)table.data(), pb, load, out.cols.data(), out.vals.data(), inv, sched, inv_diag, has_diag,
                        records, present);
   else
-    hipLaunchKernelGGL((k_level_operator<T, V, false>), dim3(blocks_for(R, 128)), dim3(128), 0, s,
+    hipLaunchKernelGGL((k_level_operator<T, V, false>), dim3(blocks_for(R, 256)), dim3(256), 0, s,
                        (const int64_t*)rowsL.basis_keys.data(), R, ord, (const int64_t*)colsL.basis_keys.data(),
                        (int64_t)colsL.basis_count, colsL.columns.view(), dense, colsL.dense_dim, depth, trial_depth,
                        (const D2*)table.data(), pb, load, out.cols.data(), out.vals.data(), inv, sched, inv_diag, has_diag,
